@@ -1,0 +1,114 @@
+/*
+ * mh_oracle.h -- CPU oracle for the seed-anchoring + inter-anchor DP path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithms, used by tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg as the checker.  The product library (libmauvehip.so)
+ * never links, loads or calls anything in this directory.
+ *
+ * Every function cites the reference lines it follows (paths relative to
+ * mauve/src/: LM = libMems/libMems, MU = muscle/libMUSCLE).
+ *
+ * Parity pins (what checks each function beyond its own restatement):
+ *   seeds (or_get_seed & co)      : known-answer tests on LM/SeedMasks.h tables
+ *   or_translate32 / keys / sort  : oracle/_ref harness built from the reference
+ *                                   SortedMerList.cpp + DNAMemorySML.cpp, golden
+ *                                   vectors under tests/golden, brute force
+ *   or_nwsmall                    : oracle/_ref harness built from libMUSCLE
+ *                                   (ProfileProfile -> NWSmall), golden vectors,
+ *                                   brute-force path-score optimality
+ *   or_find_matches (SearchRange, HashMatch, ExtendMatch, AddHashEntry replay):
+ *                                   MatchFinder/MemHash need boost (absent), so
+ *                                   no reference build; pinned by brute force
+ *                                   (all unique shared seeds, naive maximal
+ *                                   extension) and invariants -- see tests.
+ */
+#ifndef MH_ORACLE_H
+#define MH_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_SEQ 4
+
+/* ---- seed patterns (LM/SeedMasks.h) ---- */
+int64_t  or_get_seed(int weight, int seed_rank);
+int      or_get_seed_length(int64_t seed);
+int      or_get_seed_weight(int64_t seed);
+uint32_t or_default_seed_weight(uint64_t avg_sequence_length);
+
+/* ---- 2-bit packing and keys (LM/SortedMerList.cpp) ---- */
+uint64_t or_packed_words(uint64_t len);
+int      or_translate32(const char *src, uint64_t len, uint32_t *dest);
+uint64_t or_get_mer(const uint32_t *seq, uint64_t pos, uint64_t mer_mask);
+uint64_t or_revcomp_mer(uint64_t mer, int mer_length);
+uint64_t or_get_seed_mer(const uint32_t *seq, uint64_t pos, uint64_t seed,
+                         int seed_length, int seed_weight);
+uint64_t or_get_dna_seed_mer(const uint32_t *seq, uint64_t pos, uint64_t seed,
+                             int seed_length, int seed_weight);
+void     or_fill_dna_sml(const uint32_t *seq, uint64_t len, int seed_length,
+                         uint64_t *keys_out);
+void     or_fill_dna_seed_sml(const uint32_t *seq, uint64_t n, uint64_t seed,
+                              int seed_length, int seed_weight, uint64_t *keys_out);
+
+/* Build a sorted mer list: packed words, keys sorted ascending (ties by
+ * position) and the matching positions.  Returns number of entries, or -1. */
+int64_t  or_sml_build(const char *ascii, uint64_t len, uint64_t seed,
+                      uint32_t *packed_out, uint64_t *keys_out, uint32_t *pos_out);
+
+/* ---- seed matching (LM/MatchFinder.cpp, LM/MemHash.cpp, ...) ---- */
+typedef struct {
+    int64_t  start[OR_MAX_SEQ];   /* 1-based, <0 = reverse strand, 0 = none */
+    uint64_t length;
+} or_match;
+
+typedef struct {
+    int64_t  hash_calls;      /* HashMatch invocations (seed tuples)      */
+    int64_t  collisions;      /* AddHashEntry collisions (dropped seeds)  */
+    int64_t  restarts;        /* repeat-limit SearchRange restarts        */
+    int64_t  matches;         /* entries in the final match list          */
+} or_match_stats;
+
+/* Full pairwise match finding over k genomes (k <= OR_MAX_SEQ):
+ * SML build + SearchRange + PairwiseMatchFinder::EnumerateMatches +
+ * HashMatch + AddHashEntry/ExtendMatch + GetMatchList order.
+ * Returns number of matches written (<= max_out) or -1 on error. */
+int64_t or_find_matches_pairwise(int k, const char *const *seqs, const uint64_t *lens,
+                                 uint64_t seed, or_match *out, int64_t max_out,
+                                 or_match_stats *stats);
+
+/* The seed tuples in HashMatch call order (after SetDirection), with their
+ * generalized offsets.  Returns count or -1. */
+int64_t or_seed_tuples_pairwise(int k, const char *const *seqs, const uint64_t *lens,
+                                uint64_t seed, or_match *out, int64_t *offsets,
+                                int64_t max_out);
+
+/* ExtendMatch applied to one seed (starts, length = seed length). */
+int or_extend_seed(int k, const char *const *seqs, const uint64_t *lens, uint64_t seed,
+                   or_match *m);
+
+/* ---- inter-anchor DP (MU/nwsmall.cpp, MU/bittraceback.cpp) ---- */
+/* A, B: letter codes 0..3 (A,C,G,T).  path_out receives LA+LB bytes max of
+ * 'M'/'D'/'I' in alignment order; returns path length, or -1 if LA==0 or
+ * LB==0 (the reference Quit()s there).  *score_out gets the best terminal
+ * score of the chosen edge (float, integer-valued). */
+int64_t or_nwsmall(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
+                   char *path_out, float *score_out);
+
+/* Same DP with explicit gap parameters (open is the full open penalty, split
+ * in half between open and close as MU/profilefrommsa.cpp:291-292 does). */
+int64_t or_nwsmall_params(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
+                          float gap_open, float gap_extend,
+                          char *path_out, float *score_out);
+
+/* Score of a given path under the same scheme (used by tests). */
+float or_path_score(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
+                    const char *path, int64_t plen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

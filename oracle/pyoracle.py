@@ -1,0 +1,226 @@
+"""ctypes view of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product package never
+imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+OR_MAX_SEQ = 4
+CODE = {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}
+
+
+class OrMatch(C.Structure):
+    _fields_ = [("start", C.c_int64 * OR_MAX_SEQ), ("length", C.c_uint64)]
+
+
+class OrStats(C.Structure):
+    _fields_ = [("hash_calls", C.c_int64), ("collisions", C.c_int64),
+                ("restarts", C.c_int64), ("matches", C.c_int64)]
+
+
+def build():
+    """Compile liboracle.so in place (gcc, a few seconds)."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.or_get_seed.restype = C.c_int64
+        L.or_get_seed.argtypes = [C.c_int, C.c_int]
+        L.or_get_seed_length.argtypes = [C.c_int64]
+        L.or_get_seed_weight.argtypes = [C.c_int64]
+        L.or_default_seed_weight.restype = C.c_uint32
+        L.or_default_seed_weight.argtypes = [C.c_uint64]
+        L.or_packed_words.restype = C.c_uint64
+        L.or_packed_words.argtypes = [C.c_uint64]
+        L.or_translate32.argtypes = [C.c_char_p, C.c_uint64, C.c_void_p]
+        L.or_get_dna_seed_mer.restype = C.c_uint64
+        L.or_get_dna_seed_mer.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_int]
+        L.or_revcomp_mer.restype = C.c_uint64
+        L.or_revcomp_mer.argtypes = [C.c_uint64, C.c_int]
+        L.or_fill_dna_sml.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]
+        L.or_fill_dna_seed_sml.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_void_p]
+        L.or_sml_build.restype = C.c_int64
+        L.or_sml_build.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_find_matches_pairwise.restype = C.c_int64
+        L.or_find_matches_pairwise.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
+                                               C.c_uint64, C.POINTER(OrMatch), C.c_int64, C.POINTER(OrStats)]
+        L.or_seed_tuples_pairwise.restype = C.c_int64
+        L.or_seed_tuples_pairwise.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
+                                              C.c_uint64, C.POINTER(OrMatch), C.c_void_p, C.c_int64]
+        L.or_extend_seed.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
+                                     C.c_uint64, C.POINTER(OrMatch)]
+        L.or_nwsmall.restype = C.c_int64
+        L.or_nwsmall.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_char_p,
+                                 C.POINTER(C.c_float)]
+        L.or_path_score.restype = C.c_float
+        L.or_path_score.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_char_p, C.c_int64]
+        _LIB = L
+    return _LIB
+
+
+def _b(s):
+    return s if isinstance(s, bytes) else s.encode()
+
+
+def get_seed(weight, rank=0):
+    return lib().or_get_seed(weight, rank)
+
+
+def seed_length(seed):
+    return lib().or_get_seed_length(seed)
+
+
+def seed_weight(seed):
+    return lib().or_get_seed_weight(seed)
+
+
+def default_seed_weight(avg_len):
+    return lib().or_default_seed_weight(avg_len)
+
+
+def revcomp_mer(mer, length):
+    return lib().or_revcomp_mer(mer, length)
+
+
+def packed(seq):
+    seq = _b(seq)
+    nw = lib().or_packed_words(len(seq))
+    out = np.zeros(nw, dtype=np.uint32)
+    rc = lib().or_translate32(seq, len(seq), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("gap character in genome sequence")
+    return out
+
+
+def keys_by_position(seq, seed):
+    """GetDnaSeedMer at every SML position, in position order."""
+    seq = _b(seq)
+    words = packed(seq)
+    sl, sw = seed_length(seed), seed_weight(seed)
+    n = len(seq) - sl + 1
+    out = np.zeros(max(n, 0), dtype=np.uint64)
+    if n > 0:
+        lib().or_fill_dna_seed_sml(words.ctypes.data, n, seed, sl, sw, out.ctypes.data)
+    return out
+
+
+def fill_dna_sml(seq, seed):
+    """The incremental solid-seed fill (FillDnaSML) in position order."""
+    seq = _b(seq)
+    words = packed(seq)
+    sl = seed_length(seed)
+    n = len(seq) - sl + 1
+    out = np.zeros(max(n, 0), dtype=np.uint64)
+    if n > 0:
+        lib().or_fill_dna_sml(words.ctypes.data, len(seq), sl, out.ctypes.data)
+    return out
+
+
+def sml_build(seq, seed):
+    """Returns (packed words, sorted keys, sorted positions)."""
+    seq = _b(seq)
+    L = len(seq)
+    words = np.zeros(lib().or_packed_words(L), dtype=np.uint32)
+    n = max(L - seed_length(seed) + 1, 0)
+    keys = np.zeros(max(n, 1), dtype=np.uint64)
+    pos = np.zeros(max(n, 1), dtype=np.uint32)
+    r = lib().or_sml_build(seq, L, seed, words.ctypes.data, keys.ctypes.data, pos.ctypes.data)
+    if r < 0:
+        raise ValueError("SML build failed")
+    return words, keys[:r], pos[:r]
+
+
+def _seq_args(seqs):
+    bs = [_b(s) for s in seqs]
+    arr = (C.c_char_p * len(bs))(*bs)
+    lens = (C.c_uint64 * len(bs))(*[len(s) for s in bs])
+    return bs, arr, lens
+
+
+def find_matches(seqs, seed, max_out=None):
+    """Pairwise match list in reference order: (starts[k] int64 array, lengths)."""
+    k = len(seqs)
+    bs, arr, lens = _seq_args(seqs)
+    cap = max_out or (max(len(s) for s in bs) + 16)
+    out = (OrMatch * cap)()
+    st = OrStats()
+    n = lib().or_find_matches_pairwise(k, arr, lens, seed, out, cap, C.byref(st))
+    if n < 0:
+        raise ValueError("match finding failed")
+    starts = np.array([[out[i].start[s] for s in range(k)] for i in range(n)], dtype=np.int64).reshape(n, k)
+    lengths = np.array([out[i].length for i in range(n)], dtype=np.uint64)
+    return starts, lengths, st
+
+
+def seed_tuples(seqs, seed):
+    k = len(seqs)
+    bs, arr, lens = _seq_args(seqs)
+    n = lib().or_seed_tuples_pairwise(k, arr, lens, seed, None, None, 0)
+    out = (OrMatch * max(n, 1))()
+    off = np.zeros(max(n, 1), dtype=np.int64)
+    n2 = lib().or_seed_tuples_pairwise(k, arr, lens, seed, out, off.ctypes.data, n)
+    starts = np.array([[out[i].start[s] for s in range(k)] for i in range(n2)], dtype=np.int64).reshape(n2, k)
+    return starts, off[:n2]
+
+
+def extend_seed(seqs, seed, starts, length):
+    k = len(seqs)
+    bs, arr, lens = _seq_args(seqs)
+    m = OrMatch()
+    for s in range(k):
+        m.start[s] = int(starts[s])
+    m.length = length
+    lib().or_extend_seed(k, arr, lens, seed, C.byref(m))
+    return [m.start[s] for s in range(k)], m.length
+
+
+def codes(seq):
+    return np.frombuffer(_b(seq).upper().translate(bytes.maketrans(b"ACGT", b"\x00\x01\x02\x03")),
+                         dtype=np.uint8).copy()
+
+
+def nwsmall(a, b):
+    """Path string ('M'/'D'/'I') and terminal score for one job (A, B strings or code arrays)."""
+    A = codes(a) if isinstance(a, (str, bytes)) else np.ascontiguousarray(a, dtype=np.uint8)
+    B = codes(b) if isinstance(b, (str, bytes)) else np.ascontiguousarray(b, dtype=np.uint8)
+    buf = C.create_string_buffer(len(A) + len(B) + 1)
+    sc = C.c_float()
+    n = lib().or_nwsmall(A.ctypes.data, len(A), B.ctypes.data, len(B), buf, C.byref(sc))
+    if n < 0:
+        raise ValueError("NWSmall: length=0")
+    return buf.raw[:n].decode(), sc.value
+
+
+def path_score(a, b, path):
+    A = codes(a) if isinstance(a, (str, bytes)) else np.ascontiguousarray(a, dtype=np.uint8)
+    B = codes(b) if isinstance(b, (str, bytes)) else np.ascontiguousarray(b, dtype=np.uint8)
+    return lib().or_path_score(A.ctypes.data, len(A), B.ctypes.data, len(B), _b(path), len(path))
+
+
+def rows_from_path(a, b, path):
+    """Aligned rows (as AlignTwoMSAsGivenPath lays them out) from a path."""
+    a = a.decode() if isinstance(a, bytes) else a
+    b = b.decode() if isinstance(b, bytes) else b
+    ra, rb, i, j = [], [], 0, 0
+    for c in path:
+        if c == "M":
+            ra.append(a[i]); rb.append(b[j]); i += 1; j += 1
+        elif c == "D":
+            ra.append(a[i]); rb.append("-"); i += 1
+        else:
+            ra.append("-"); rb.append(b[j]); j += 1
+    return "".join(ra), "".join(rb)

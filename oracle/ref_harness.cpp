@@ -1,0 +1,150 @@
+// ref_harness.cpp -- driver around the reference's own SortedMerList and
+// libMUSCLE code, compiled from the sources under the reference tree by
+// oracle/Makefile into oracle/_ref/.  TEST INFRASTRUCTURE ONLY: it is used
+// to write golden vectors and to cross-check the plain-C oracle.  Nothing in
+// the product library links it.
+//
+//   ref_harness keys <seed> <in.seq> <out.bin>
+//       Runs DNAMemorySML::Create (LM/MemorySML.cpp:45-60) on the sequence
+//       and writes: n_words, packed words (LM/SortedMerList.cpp:306-317),
+//       n, fill keys in position order (FillDnaSML or FillDnaSeedSML),
+//       GetDnaSeedMer per position (LM/SortedMerList.cpp:764-769), and the
+//       sorted positions with their keys as the reference std::sort left them.
+//
+//   ref_harness nw <in.txt> <out.txt>
+//       For each line "A B" runs ProfileProfile (MU/profile.cpp:68-93) on two
+//       one-sequence MSAs with the globals set as MuscleInterface::
+//       ProfileAlignFast sets them (LM/MuscleInterface.cpp:1090-1106), and
+//       writes the two aligned rows.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <fstream>
+#include <sstream>
+#include <iostream>
+
+#include "libGenome/gnSequence.h"
+#include "libMems/DNAMemorySML.h"
+
+#include "libMUSCLE/muscle.h"
+#include "libMUSCLE/msa.h"
+#include "libMUSCLE/profile.h"
+#include "libMUSCLE/params.h"
+
+using namespace std;
+
+namespace {
+
+// Exposes the protected state of the reference SML class to the driver.
+class ProbeSML : public mems::DNAMemorySML {
+public:
+    const uint32* words() const { return sequence; }
+    gnSeqI nwords() const { return binary_seq_len; }
+    void fill(const genome::gnSequence& seq, vector<mems::bmer>& arr) {
+        if (header.seed_length != header.seed_weight) FillDnaSeedSML(seq, arr);
+        else FillSML(seq, arr);
+    }
+    uint64 seedmer(gnSeqI p) const { return GetDnaSeedMer(p); }
+    gnSeqI npos() const { return positions.size(); }
+    gnSeqI posat(gnSeqI i) const { return positions[i]; }
+};
+
+void put64(FILE* f, uint64_t v) { fwrite(&v, 8, 1, f); }
+
+int do_keys(uint64_t seed, const char* in, const char* out)
+{
+    ifstream is(in);
+    string s((istreambuf_iterator<char>(is)), istreambuf_iterator<char>());
+    while (!s.empty() && (s.back() == '\n' || s.back() == '\r')) s.pop_back();
+    genome::gnSequence seq(s);
+    ProbeSML sml;
+    sml.Create(seq, seed);
+    FILE* f = fopen(out, "wb");
+    if (!f) return 2;
+    uint64_t nw = sml.nwords();
+    // the reference leaves the padding words past the data uninitialised; the
+    // driver reports only the words that hold characters
+    uint64_t used = (s.size() * 2 + 31) / 32;
+    put64(f, used);
+    for (uint64_t i = 0; i < used; ++i) { uint32_t w = sml.words()[i]; fwrite(&w, 4, 1, f); }
+    (void)nw;
+    vector<mems::bmer> arr;
+    sml.fill(seq, arr);
+    uint64_t n = arr.size();
+    put64(f, n);
+    for (uint64_t i = 0; i < n; ++i) put64(f, arr[i].mer);
+    for (uint64_t i = 0; i < n; ++i) put64(f, sml.seedmer(i));
+    uint64_t np = sml.npos();
+    put64(f, np);
+    for (uint64_t i = 0; i < np; ++i) {
+        uint32_t p = (uint32_t)sml.posat(i);
+        fwrite(&p, 4, 1, f);
+    }
+    fclose(f);
+    return 0;
+}
+
+void msa_from_string(muscle::MSA& msa, const string& s, unsigned id_base)
+{
+    // same steps as msaFromSeqTable (LM/MuscleInterface.cpp:808-820)
+    msa.SetSize(1, s.size());
+    char name[32];
+    snprintf(name, sizeof(name), "seq%u", 0u);
+    msa.SetSeqName(0, name);
+    msa.SetSeqId(0, id_base);
+    for (size_t i = 0; i < s.size(); ++i) msa.SetChar(0, i, s[i]);
+}
+
+int do_nw(const char* in, const char* out)
+{
+    using namespace muscle;
+    ifstream is(in);
+    FILE* f = fopen(out, "w");
+    if (!f) return 2;
+    string line;
+    while (getline(is, line)) {
+        istringstream ls(line);
+        string a, b;
+        if (!(ls >> a >> b)) continue;
+        g_SeqType.get() = SEQTYPE_DNA;
+        g_uMaxIters.get() = 1;
+        g_bStable.get() = true;
+        g_bQuiet.get() = true;
+        g_SeqWeight1.get() = SEQWEIGHT_ClustalW;
+        SetMaxIters(g_uMaxIters.get());
+        SetSeqWeightMethod(g_SeqWeight1.get());
+        MSA::SetIdCount(2);
+        MSA msa1, msa2, msaOut;
+        msa_from_string(msa1, a, 0);
+        msa_from_string(msa2, b, 1);
+        SetAlpha(ALPHA_DNA);
+        msa1.FixAlpha();
+        msa2.FixAlpha();
+        SetPPScore(PPSCORE_SPN);
+        ProfileProfile(msa1, msa2, msaOut);
+        // ProfileProfile leaves both ids at 0 (ProfileFromMSALocal renumbers
+        // each input); rows come out as msa1's sequence then msa2's
+        // (AlignTwoMSAsGivenPath), so read them by row index.
+        for (unsigned r = 0; r < msaOut.GetSeqCount(); ++r) {
+            const char* buf = msaOut.GetSeqBuffer(r);
+            string row(buf, msaOut.GetColCount());
+            fprintf(f, "%s%c", row.c_str(), r + 1 < msaOut.GetSeqCount() ? ' ' : '\n');
+        }
+    }
+    fclose(f);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc >= 5 && !strcmp(argv[1], "keys"))
+        return do_keys(strtoull(argv[2], 0, 0), argv[3], argv[4]);
+    if (argc >= 4 && !strcmp(argv[1], "nw"))
+        return do_nw(argv[2], argv[3]);
+    fprintf(stderr, "usage: ref_harness keys <seed> <in.seq> <out.bin> | nw <in.txt> <out.txt>\n");
+    return 1;
+}
