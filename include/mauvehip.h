@@ -1,0 +1,175 @@
+/*
+ * mauvehip.h -- C ABI of libmauvehip.so, the MI355X (gfx950) implementation of
+ * progressiveMauve's seed-anchoring and inter-anchor gapped-DP path.
+ *
+ * Plain C types only (pointers, sizes, fixed-width integers).  Every entry
+ * point returns MH_OK (0) or a negative MH_ERR_* code; mh_last_error() gives
+ * the message of the calling thread's last failure.  A context owns one HIP
+ * stream and its device scratch; calls on one context must not overlap.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * file:line relative to mauve/src/ (LM = libMems/libMems,
+ * MU = muscle/libMUSCLE).
+ */
+#ifndef MAUVEHIP_H
+#define MAUVEHIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_OK                 0
+#define MH_ERR_INVALID       -1   /* bad argument                                   */
+#define MH_ERR_DEVICE        -2   /* HIP runtime error                              */
+#define MH_ERR_NOMEM         -3   /* device or host allocation failed               */
+#define MH_ERR_GAP_CHAR      -4   /* '-' in a genome (LM/SortedMerList.cpp:433-437) */
+#define MH_ERR_ZERO_LENGTH   -5   /* empty DP operand (MU/nwsmall.cpp:503-504 Quit) */
+#define MH_ERR_UNSUPPORTED   -6   /* input outside what this build implements       */
+
+#define MH_MAX_SEQ 4              /* genomes per pairwise match search (the CLI uses
+                                     PairwiseMatchFinder for <= 4 genomes,
+                                     MA/progressiveMauve.cpp:493-504)                */
+
+typedef struct mh_context mh_context;
+typedef struct mh_sml mh_sml;
+
+/* ---- context ---------------------------------------------------------- */
+int         mh_create(int device, mh_context **out);
+void        mh_destroy(mh_context *ctx);
+const char *mh_last_error(void);
+/* Synchronise the context stream. */
+int         mh_synchronize(mh_context *ctx);
+/* Average device time (ms) and launch count of a named kernel since the last
+ * reset, measured with HIP events on the context stream.  Names:
+ * "dp_forward", "dp_traceback", "sml_keys", "sml_sort", "join", "extend". */
+int         mh_kernel_time(mh_context *ctx, const char *kernel, double *avg_ms, int64_t *launches);
+void        mh_kernel_time_reset(mh_context *ctx);
+int         mh_set_timing(mh_context *ctx, int enabled);
+
+/* ---- seed patterns: getSeed / getSeedLength / getSeedWeight /
+ *      getDefaultSeedWeight, LM/SeedMasks.h:298-401 ----------------------- */
+int64_t  mh_get_seed(int weight, int seed_rank);
+int      mh_get_seed_length(int64_t seed);
+int      mh_get_seed_weight(int64_t seed);
+uint32_t mh_default_seed_weight(uint64_t avg_sequence_length);
+
+/* ---- sorted mer list ---------------------------------------------------- */
+
+/* Same 16-byte layout as struct bmer, LM/SortedMerList.h:43-46. */
+typedef struct {
+    uint32_t position;
+    uint64_t mer;
+} mh_bmer;
+
+/* The SMLHeader fields that carry meaning (LM/SortedMerList.h:48-63). */
+typedef struct {
+    uint32_t version;          /* 5, LM/DNAFileSML.h:58-62   */
+    uint32_t alphabet_bits;    /* 2                          */
+    uint64_t seed;
+    uint32_t seed_length;
+    uint32_t seed_weight;
+    uint64_t length;           /* sequence length            */
+    uint64_t sml_length;       /* SMLLength()                */
+    uint8_t  circular;         /* always 0 in this build     */
+} mh_sml_header;
+
+/* Replaces FileSML::Create / MemorySML::Create (LM/FileSML.cpp:401-459,
+ * LM/MemorySML.cpp:45-60): 2-bit packing, canonical seed keys and the sort by
+ * 64-bit key.  The list stays resident on the device.  Keys with equal value
+ * are ordered by ascending position (the reference's std::sort leaves them in
+ * an unspecified order).  Host ASCII input. */
+int mh_sml_create(mh_context *ctx, const char *seq, uint64_t len, uint64_t seed, mh_sml **out);
+/* As mh_sml_create, with the ASCII sequence already in device memory. */
+int mh_sml_create_device(mh_context *ctx, const void *d_seq, uint64_t len, uint64_t seed, mh_sml **out);
+void mh_sml_free(mh_sml *sml);
+/* SortedMerList::GetHeader / Length / SMLLength, LM/SortedMerList.h:181-212. */
+int mh_sml_get_header(const mh_sml *sml, mh_sml_header *out);
+/* FileSML::Read, LM/FileSML.cpp:470-500: entries [offset, offset+count) of the
+ * sorted list, clipped to SMLLength; *n_read gets the number written. */
+int mh_sml_read(mh_context *ctx, const mh_sml *sml, uint64_t offset, uint64_t count,
+                mh_bmer *out, uint64_t *n_read);
+/* GetDnaSeedMer at each given position, LM/SortedMerList.cpp:764-769. */
+int mh_sml_get_seed_mers(mh_context *ctx, const mh_sml *sml, const uint32_t *positions,
+                         uint64_t n, uint64_t *out);
+/* SortedMerList::FindMer, LM/SortedMerList.cpp:170-179 (same bsearch probe
+ * order, so the same index on a miss). */
+int mh_sml_find_mer(mh_context *ctx, const mh_sml *sml, uint64_t mer, uint64_t *index, int *found);
+/* The packed sequence words (translate32, LM/SortedMerList.cpp:425-460);
+ * n_words = ceil(2L/32)+2, trailing words zero. */
+int mh_sml_get_sequence(mh_context *ctx, const mh_sml *sml, uint32_t *words, uint64_t n_words);
+
+/* ---- seed matching ---------------------------------------------------- */
+
+/* A match as the reference's Match (UngappedLocalAlignment<HybridAbstractMatch>):
+ * 1-based starts, negative = reverse strand, 0 = absent (LM/AbstractMatch.h:27). */
+typedef struct {
+    int64_t  start[MH_MAX_SEQ];
+    uint64_t length;
+} mh_match;
+
+typedef struct {
+    int64_t seed_tuples;     /* HashMatch calls                              */
+    int64_t collisions;      /* seeds dropped by AddHashEntry                */
+    int64_t matches;         /* entries in the match list                    */
+    int64_t repeat_runs;     /* runs above MER_REPEAT_LIMIT                  */
+} mh_match_stats;
+
+/* Replaces PairwiseMatchFinder/MemHash::FindMatches (LM/MemHash.cpp:109-127,
+ * LM/PairwiseMatchFinder.cpp:37-71, LM/MatchFinder.cpp:172-340,
+ * LM/MemHash.cpp:167-251, LM/MatchFinder.h:218-374): all matches between the
+ * k sorted mer lists, in GetMatchList order (LM/MemHash.h:182-203).
+ * This build supports k == 2.  If *n_out exceeds cap, only cap entries are
+ * written and the call still returns MH_OK with the full count. */
+int mh_find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k,
+                             mh_match *out, uint64_t cap, uint64_t *n_out,
+                             mh_match_stats *stats);
+
+/* ---- inter-anchor gapped DP ----------------------------------------------- */
+
+/* One DP job: A and B as letter codes 0..3 (A,C,G,T) at the given offsets of
+ * the A and B code buffers. */
+typedef struct {
+    uint64_t a_off;
+    uint64_t b_off;
+    uint32_t a_len;
+    uint32_t b_len;
+} mh_dp_job;
+
+typedef struct {
+    int32_t  score;        /* terminal score of the chosen edge (NWSmall 641-656) */
+    uint32_t path_len;     /* number of edges                                     */
+    uint64_t path_start;   /* offset of the first edge in the path buffer         */
+} mh_dp_result;
+
+/* Replaces GlobalAlign -> NWSmall -> BitTraceBack (MU/glbalign.cpp:69-81,
+ * MU/nwsmall.cpp:500-670, MU/bittraceback.cpp:138-209) for a batch of
+ * one-sequence profiles scored as ProfileAlignFast sets them up (NUC_SP,
+ * gap open -400 split as -200/-200, extend 0, half terminal gaps).
+ * Job j's edges ('M','D','I', in alignment order) are written inside the slot
+ * [sum_{i<j}(a_len+b_len), +a_len+b_len) of `paths`; results[j].path_start
+ * gives the first edge.  Host buffers. */
+int mh_dp_align(mh_context *ctx, const uint8_t *a_codes, uint64_t a_bytes,
+                const uint8_t *b_codes, uint64_t b_bytes,
+                const mh_dp_job *jobs, uint64_t njobs,
+                uint8_t *paths, mh_dp_result *results);
+
+/* Device-resident variant: d_a, d_b, d_paths, d_results are device pointers;
+ * jobs is a host array (the scheduler reads it).  Runs on the context stream
+ * and returns without synchronising. */
+int mh_dp_align_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
+                       const mh_dp_job *jobs, uint64_t njobs,
+                       uint8_t *d_paths, mh_dp_result *d_results);
+
+/* Device memory helpers for callers without their own allocator. */
+int mh_device_alloc(mh_context *ctx, uint64_t bytes, void **d_ptr);
+int mh_device_free(mh_context *ctx, void *d_ptr);
+int mh_memcpy_h2d(mh_context *ctx, void *d_dst, const void *h_src, uint64_t bytes);
+int mh_memcpy_d2h(mh_context *ctx, void *h_dst, const void *d_src, uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAUVEHIP_H */

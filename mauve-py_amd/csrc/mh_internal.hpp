@@ -1,0 +1,104 @@
+// mh_internal.hpp -- shared host-side state of libmauvehip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <map>
+#include "mauvehip.h"
+
+namespace mh {
+
+void set_error(const std::string &msg);
+
+#define MH_HIP(call)                                                            \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            ::mh::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return MH_ERR_DEVICE;                                               \
+        }                                                                       \
+    } while (0)
+
+#define MH_CHECK(cond, code, msg)                  \
+    do {                                           \
+        if (!(cond)) {                             \
+            ::mh::set_error(msg);                  \
+            return (code);                         \
+        }                                          \
+    } while (0)
+
+// A growable device buffer owned by a context.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);   // MH_OK or MH_ERR_NOMEM
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+struct KernelTimer {
+    double total_ms = 0;
+    int64_t launches = 0;
+};
+
+}  // namespace mh
+
+struct mh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::map<std::string, mh::KernelTimer> timers;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    // DP scratch
+    mh::DevBuf dp_tb, dp_bnd, dp_jobs, dp_a, dp_b, dp_paths, dp_res;
+    // seed-matching scratch
+    mh::DevBuf mf_scratch[8];
+    // scratch for sorting
+    mh::DevBuf sort_tmp[4];
+};
+
+struct mh_sml {
+    mh_context *ctx = nullptr;
+    uint64_t len = 0;          // sequence length
+    uint64_t n = 0;            // SMLLength
+    uint64_t seed = 0;
+    int seed_length = 0, seed_weight = 0;
+    uint64_t n_words = 0;
+    uint32_t *d_words = nullptr;     // packed sequence
+    uint64_t *d_keys_pos = nullptr;  // GetDnaSeedMer per position (position order)
+    uint64_t *d_keys = nullptr;      // sorted keys
+    uint32_t *d_pos = nullptr;       // sorted positions
+};
+
+namespace mh {
+
+// Kernel timing: record a start event before and a stop event after a launch
+// on the context stream; totals are folded in by collect_timers().
+struct TimedLaunch {
+    mh_context *ctx;
+    std::string name;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(mh_context *c, const char *n);
+    ~TimedLaunch();
+};
+int collect_timers(mh_context *ctx);
+
+// Launchers (return MH_OK or an error code); all run on ctx->stream.
+int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
+           uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results);
+
+int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed, mh_sml *s);
+int sml_seed_mers(mh_context *ctx, const mh_sml *s, const uint32_t *d_pos, uint64_t n, uint64_t *d_out);
+
+int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k,
+                          std::vector<mh_match> &out, mh_match_stats *st);
+
+// host-side seed helpers (LM/SeedMasks.h)
+int64_t get_seed(int weight, int rank);
+int seed_length(int64_t seed);
+int seed_weight(int64_t seed);
+
+}  // namespace mh

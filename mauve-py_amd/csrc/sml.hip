@@ -1,0 +1,396 @@
+// sml.hip -- sorted mer list build on gfx950.
+//
+// K1a  pack:  ASCII -> 2-bit words, 16 characters per uint32, first character
+//             in the two most significant bits (translate32,
+//             LM/SortedMerList.cpp:425-460; BasicDNATable 29-47).
+// K1b  keys:  canonical seed key per position (GetDnaSeedMer,
+//             LM/SortedMerList.cpp:726-769): the pattern-selected characters
+//             left-aligned in 64 bits, or the reverse complement with the LSB
+//             strand flag set (RevCompMer 597-614), whichever is smaller.
+//             Keys are also written in compact form for sorting:
+//             (selected chars << 1) | flag, 2w+1 significant bits, which
+//             orders exactly like the 64-bit key.
+// K2   sort:  LSD radix sort of (compact key, position) by 8-bit digits.
+//             Each pass: per-tile digit histograms, one exclusive scan over
+//             the digit-major histogram table, then a scatter that ranks the
+//             tile stably in LDS (wave ballots per digit bit) and writes every
+//             digit's run out contiguously.  Positions enter in ascending
+//             order and every pass is stable, so equal keys stay in position
+//             order.
+#include "mh_internal.hpp"
+#include <algorithm>
+#include <cstring>
+
+namespace mh {
+namespace {
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;   // 4096 entries per tile
+constexpr int RADIX = 256;
+
+struct SeedPattern {
+    int sl, sw, spaced;
+    int off[32];          // offsets of the selected characters, first to last
+};
+
+__device__ __forceinline__ uint32_t dna_code(uint32_t c)
+{
+    // BasicDNATable: c/C/b/B/y/Y -> 1, g/G/s/S/k/K -> 2, t/T -> 3, else 0.
+    const uint32_t l = c | 0x20u;   // exact fold for letters; other bytes stay distinct
+    if (l == 'c' || l == 'b' || l == 'y') return 1;
+    if (l == 'g' || l == 's' || l == 'k') return 2;
+    if (l == 't') return 3;
+    return 0;
+}
+
+__global__ void pack_kernel(const uint8_t *__restrict__ ascii, uint64_t len, uint64_t n_words,
+                            uint32_t *__restrict__ words, int *__restrict__ err, int aligned16)
+{
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_words) return;
+    const uint64_t base = w * 16;
+    uint32_t out = 0;
+    if (aligned16 && base + 16 <= len) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(ascii + base);
+        const uint32_t d[4] = {v.x, v.y, v.z, v.w};
+        bool gap = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t c = (d[q] >> (8 * b)) & 0xFFu;
+                gap |= (c == '-');
+                out |= dna_code(c) << (30 - 2 * (4 * q + b));
+            }
+        if (gap) atomicOr(err, 1);
+    } else {
+        for (int k = 0; k < 16; ++k) {
+            const uint64_t p = base + k;
+            if (p < len) {
+                const uint32_t c = ascii[p];
+                if (c == '-') atomicOr(err, 1);
+                out |= dna_code(c) << (30 - 2 * k);
+            }
+        }
+    }
+    words[w] = out;
+}
+
+__device__ __forceinline__ uint64_t get_mer(const uint32_t *__restrict__ words, uint64_t pos)
+{
+    const uint64_t w = pos >> 4;
+    const uint32_t bit = (uint32_t)(pos & 15) * 2;
+    uint64_t m = ((uint64_t)words[w] << 32) | words[w + 1];
+    if (bit) m = (m << bit) | (words[w + 2] >> (32 - bit));
+    return m;
+}
+
+__global__ void keys_kernel(const uint32_t *__restrict__ words, uint64_t n, SeedPattern sp,
+                            uint64_t *__restrict__ keys_pos, uint64_t *__restrict__ ckeys,
+                            uint32_t *__restrict__ pos_out)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint64_t mer = get_mer(words, p);
+    uint64_t fwd = 0, rc = 0;
+    if (!sp.spaced) {
+        fwd = mer >> (64 - 2 * sp.sl);
+#pragma unroll 4
+        for (int k = 0; k < sp.sl; ++k) rc |= (uint64_t)(3u - ((mer >> (62 - 2 * k)) & 3u)) << (2 * k);
+    } else {
+        for (int k = 0; k < sp.sw; ++k) {
+            const uint32_t ch = (uint32_t)(mer >> (62 - 2 * sp.off[k])) & 3u;
+            fwd = (fwd << 2) | ch;
+            rc |= (uint64_t)(3u - ch) << (2 * k);
+        }
+    }
+    const int sh = 64 - 2 * sp.sw;
+    const uint64_t f64 = fwd << sh;
+    const uint64_t r64 = (rc << sh) | 1ull;
+    const bool use_r = r64 < f64;
+    keys_pos[p] = use_r ? r64 : f64;
+    ckeys[p] = use_r ? ((rc << 1) | 1ull) : (fwd << 1);
+    pos_out[p] = (uint32_t)p;
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void hist_kernel(const uint64_t *__restrict__ keys, uint64_t n,
+                                                           int shift, uint32_t *__restrict__ counts,
+                                                           uint32_t ntiles)
+{
+    __shared__ uint32_t h[RADIX];
+    const int t = threadIdx.x;
+    h[t] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        const uint64_t x = base + (uint64_t)r * SORT_THREADS + t;
+        if (x < n) atomicAdd(&h[(uint32_t)(keys[x] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    counts[(uint64_t)t * ntiles + blockIdx.x] = h[t];
+}
+
+// Exclusive scan of the digit-major table (RADIX * ntiles entries) by one
+// workgroup: each thread scans a contiguous chunk, then the chunk totals.
+__global__ __launch_bounds__(1024) void scan_kernel(uint32_t *__restrict__ counts, uint64_t total)
+{
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const uint64_t chunk = (total + 1023) / 1024;
+    const uint64_t lo = (uint64_t)t * chunk, hi = min(lo + chunk, total);
+    uint32_t s = 0;
+    for (uint64_t x = lo; x < hi; ++x) s += counts[x];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (uint64_t x = lo; x < hi; ++x) {
+        const uint32_t c = counts[x];
+        counts[x] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *__restrict__ kin,
+                                                              const uint32_t *__restrict__ vin,
+                                                              uint64_t *__restrict__ kout,
+                                                              uint32_t *__restrict__ vout, uint64_t n,
+                                                              int shift, const uint32_t *__restrict__ offs,
+                                                              uint32_t ntiles)
+{
+    __shared__ uint64_t sk[SORT_TILE];
+    __shared__ uint32_t sv[SORT_TILE];
+    __shared__ uint32_t wcnt[SORT_THREADS / 64][RADIX];
+    __shared__ uint32_t run[RADIX];
+    __shared__ uint32_t start[RADIX];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
+    const uint64_t lane_lt = (1ull << lane) - 1ull;
+    uint64_t key[SORT_ITEMS];
+    uint32_t val[SORT_ITEMS], rank[SORT_ITEMS];
+    run[t] = 0;
+    for (int w = 0; w < SORT_THREADS / 64; ++w) wcnt[w][t] = 0;
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        const uint64_t x = base + (uint64_t)r * SORT_THREADS + t;
+        key[r] = x < n ? kin[x] : 0ull;
+        val[r] = x < n ? vin[x] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        const bool valid = base + (uint64_t)r * SORT_THREADS + t < n;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const uint32_t below = (uint32_t)__popcll(peers & lane_lt);
+        if (valid && below == 0) wcnt[wv][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {   // thread t owns digit t: wave offsets for this round
+            uint32_t acc = run[t];
+            for (int w = 0; w < SORT_THREADS / 64; ++w) {
+                const uint32_t c = wcnt[w][t];
+                wcnt[w][t] = acc;
+                acc += c;
+            }
+            run[t] = acc;
+        }
+        __syncthreads();
+        rank[r] = valid ? wcnt[wv][d] + below : 0u;
+        __syncthreads();
+        for (int w = 0; w < SORT_THREADS / 64; ++w) wcnt[w][t] = 0;
+        __syncthreads();
+    }
+    // exclusive scan of the tile's digit counts -> start of each digit run
+    {
+        const uint32_t c = run[t];
+        start[t] = c;
+        __syncthreads();
+        for (int off = 1; off < RADIX; off <<= 1) {
+            const uint32_t v = t >= off ? start[t - off] : 0u;
+            __syncthreads();
+            start[t] += v;
+            __syncthreads();
+        }
+        const uint32_t incl = start[t];
+        __syncthreads();
+        start[t] = incl - c;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        if (base + (uint64_t)r * SORT_THREADS + t < n) {
+            const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
+            const uint32_t p = start[d] + rank[r];
+            sk[p] = key[r];
+            sv[p] = val[r];
+        }
+    }
+    __syncthreads();
+    const uint32_t tile_n = (uint32_t)min<uint64_t>(SORT_TILE, n - base);
+    for (uint32_t p = t; p < tile_n; p += SORT_THREADS) {
+        const uint64_t k = sk[p];
+        const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
+        const uint64_t g = (uint64_t)offs[(uint64_t)d * ntiles + blockIdx.x] + (p - start[d]);
+        kout[g] = k;
+        vout[g] = sv[p];
+    }
+}
+
+__global__ void find_mer_kernel(const uint64_t *__restrict__ ckeys, uint64_t last, uint64_t query_c,
+                                uint64_t *__restrict__ out)
+{
+    // SortedMerList::bsearch, LM/SortedMerList.cpp:380-394 (one thread)
+    uint64_t start = 0, end = last;
+    for (;;) {
+        const uint64_t mid = (start + end) / 2;
+        const uint64_t mm = ckeys[mid];
+        if (mm == query_c) { out[0] = mid; return; }
+        if (mm < query_c && mid < end) { start = mid + 1; continue; }
+        if (mm > query_c && start < mid) { end = mid - 1; continue; }
+        out[0] = mid;
+        return;
+    }
+}
+
+__global__ void gather_keys_kernel(const uint64_t *__restrict__ keys_pos, uint64_t n,
+                                   const uint32_t *__restrict__ pos, uint64_t m, uint64_t *__restrict__ out)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= m) return;
+    const uint32_t p = pos[x];
+    out[x] = p < n ? keys_pos[p] : ~0ull;
+}
+
+SeedPattern make_pattern(uint64_t seed)
+{
+    SeedPattern sp{};
+    sp.sl = seed_length((int64_t)seed);
+    sp.sw = seed_weight((int64_t)seed);
+    sp.spaced = sp.sl != sp.sw;
+    // characters are taken from the seed's most significant set bit down
+    // (GetSeedMer walks char_mask from bit seed_length-1 to 0)
+    const int low = __builtin_ctzll(seed);
+    int k = 0;
+    for (int q = 0; q < sp.sl; ++q)
+        if ((seed >> (low + sp.sl - 1 - q)) & 1ull) sp.off[k++] = q;
+    return sp;
+}
+
+}  // namespace
+
+// compact key <-> 64-bit key
+static inline uint64_t full_key(uint64_t c, int sw) { return ((c >> 1) << (64 - 2 * sw)) | (c & 1ull); }
+static inline uint64_t compact_key(uint64_t k, int sw) { return ((k >> (64 - 2 * sw)) << 1) | (k & 1ull); }
+
+int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed, mh_sml *s)
+{
+    const int sl = seed_length((int64_t)seed), sw = seed_weight((int64_t)seed);
+    MH_CHECK(sl > 0, MH_ERR_INVALID, "Can't have 0 seed length");
+    MH_CHECK(sl <= 31, MH_ERR_INVALID, "Mer size is too large");    // LM/DNAFileSML.cpp:51-53
+    MH_CHECK(len < (1ull << 32), MH_ERR_UNSUPPORTED, "sequence longer than 2^32 (32-bit SML positions)");
+    s->ctx = ctx;
+    s->len = len;
+    s->seed = seed;
+    s->seed_length = sl;
+    s->seed_weight = sw;
+    s->n = len >= (uint64_t)sl ? len - sl + 1 : 0;
+    s->n_words = (len * 2 + 31) / 32 + 2;
+    const uint64_t n = s->n;
+    MH_HIP(hipMalloc(&s->d_words, s->n_words * 4));
+    MH_HIP(hipMalloc(&s->d_keys_pos, (n ? n : 1) * 8));
+    MH_HIP(hipMalloc(&s->d_keys, (n ? n : 1) * 8));
+    MH_HIP(hipMalloc(&s->d_pos, (n ? n : 1) * 4));
+    int rc;
+    if ((rc = ctx->sort_tmp[0].ensure((n ? n : 1) * 8 + 64)) != MH_OK) return rc;
+    if ((rc = ctx->sort_tmp[1].ensure((n ? n : 1) * 4 + 64)) != MH_OK) return rc;
+    const uint32_t ntiles = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
+    if ((rc = ctx->sort_tmp[2].ensure((uint64_t)RADIX * (ntiles ? ntiles : 1) * 4 + 64)) != MH_OK) return rc;
+    if ((rc = ctx->sort_tmp[3].ensure(64)) != MH_OK) return rc;
+    int *d_err = ctx->sort_tmp[3].as<int>();
+    MH_HIP(hipMemsetAsync(d_err, 0, 4, ctx->stream));
+    {
+        TimedLaunch tl(ctx, "sml_pack");
+        const uint64_t blocks = (s->n_words + 255) / 256;
+        hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                           static_cast<const uint8_t *>(d_ascii), len, s->n_words, s->d_words, d_err,
+                           (int)(((uintptr_t)d_ascii & 15u) == 0));
+        MH_HIP(hipGetLastError());
+    }
+    int herr = 0;
+    MH_HIP(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (n == 0) {
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+        MH_CHECK(herr == 0, MH_ERR_GAP_CHAR, "Gap in genome sequence");
+        return MH_OK;
+    }
+    const SeedPattern sp = make_pattern(seed);
+    uint64_t *k0 = s->d_keys, *k1 = ctx->sort_tmp[0].as<uint64_t>();
+    uint32_t *v0 = s->d_pos, *v1 = ctx->sort_tmp[1].as<uint32_t>();
+    {
+        TimedLaunch tl(ctx, "sml_keys");
+        hipLaunchKernelGGL(keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, s->d_words, n,
+                           sp, s->d_keys_pos, k0, v0);
+        MH_HIP(hipGetLastError());
+    }
+    const int kbits = 2 * sw + 1;
+    const int passes = (kbits + 7) / 8;
+    uint32_t *counts = ctx->sort_tmp[2].as<uint32_t>();
+    {
+        TimedLaunch tl(ctx, "sml_sort");
+        for (int pass = 0; pass < passes; ++pass) {
+            const int shift = 8 * pass;
+            hipLaunchKernelGGL(hist_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, n, shift, counts,
+                               ntiles);
+            hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, counts, (uint64_t)RADIX * ntiles);
+            hipLaunchKernelGGL(scatter_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, v0, k1, v1, n,
+                               shift, counts, ntiles);
+            std::swap(k0, k1);
+            std::swap(v0, v1);
+        }
+        MH_HIP(hipGetLastError());
+    }
+    if (k0 != s->d_keys) {   // odd number of passes: result is in the scratch pair
+        MH_HIP(hipMemcpyAsync(s->d_keys, k0, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        MH_HIP(hipMemcpyAsync(s->d_pos, v0, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    MH_CHECK(herr == 0, MH_ERR_GAP_CHAR, "Gap in genome sequence");
+    return MH_OK;
+}
+
+int sml_seed_mers(mh_context *ctx, const mh_sml *s, const uint32_t *d_pos, uint64_t m, uint64_t *d_out)
+{
+    if (!m) return MH_OK;
+    hipLaunchKernelGGL(gather_keys_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream,
+                       s->d_keys_pos, s->n, d_pos, m, d_out);
+    MH_HIP(hipGetLastError());
+    return MH_OK;
+}
+
+int sml_find_mer_dev(mh_context *ctx, const mh_sml *s, uint64_t mer, uint64_t *index)
+{
+    uint64_t *d = ctx->sort_tmp[3].as<uint64_t>();
+    hipLaunchKernelGGL(find_mer_kernel, dim3(1), dim3(1), 0, ctx->stream, s->d_keys, s->len - s->seed_length,
+                       compact_key(mer, s->seed_weight), d);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipMemcpyAsync(index, d, 8, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    return MH_OK;
+}
+
+uint64_t sml_full_key(uint64_t c, int sw) { return full_key(c, sw); }
+uint64_t sml_compact_key(uint64_t k, int sw) { return compact_key(k, sw); }
+
+}  // namespace mh

@@ -1,0 +1,254 @@
+"""mauvehip -- Python host layer over libmauvehip.so (gfx950).
+
+Mirrors the reference's interfaces for the seed-anchoring and inter-anchor DP
+path:
+
+* ``SortedMerList``      -- DNAFileSML / DNAMemorySML (LM/SortedMerList.h:69-282)
+* ``find_matches``       -- PairwiseMatchFinder::FindMatches (LM/MemHash.cpp:109-127)
+* ``global_align``       -- GlobalAlign / NWSmall for a batch of jobs
+                            (MU/glbalign.cpp:69-81, MU/nwsmall.cpp:500-670)
+* ``get_seed`` & co      -- LM/SeedMasks.h:298-401
+
+Every call goes through the C ABI declared in include/mauvehip.h.  There is
+no CPU fallback: if the library is missing or no GPU is visible, the calls
+raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmauvehip.so")
+MH_MAX_SEQ = 4
+
+MH_OK = 0
+ERRORS = {
+    -1: "MH_ERR_INVALID",
+    -2: "MH_ERR_DEVICE",
+    -3: "MH_ERR_NOMEM",
+    -4: "MH_ERR_GAP_CHAR",
+    -5: "MH_ERR_ZERO_LENGTH",
+    -6: "MH_ERR_UNSUPPORTED",
+}
+
+# Exported symbols of the C ABI (checked by the CPU test suite).
+EXPORTS = [
+    "mh_create", "mh_destroy", "mh_last_error", "mh_synchronize", "mh_kernel_time",
+    "mh_kernel_time_reset", "mh_set_timing", "mh_get_seed", "mh_get_seed_length",
+    "mh_get_seed_weight", "mh_default_seed_weight", "mh_sml_create", "mh_sml_create_device",
+    "mh_sml_free", "mh_sml_get_header", "mh_sml_read", "mh_sml_get_seed_mers", "mh_sml_find_mer",
+    "mh_sml_get_sequence", "mh_find_matches_pairwise", "mh_dp_align", "mh_dp_align_device",
+    "mh_device_alloc", "mh_device_free", "mh_memcpy_h2d", "mh_memcpy_d2h",
+]
+
+
+class MhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class DpJob(C.Structure):
+    _fields_ = [("a_off", C.c_uint64), ("b_off", C.c_uint64), ("a_len", C.c_uint32), ("b_len", C.c_uint32)]
+
+
+class DpResult(C.Structure):
+    _fields_ = [("score", C.c_int32), ("path_len", C.c_uint32), ("path_start", C.c_uint64)]
+
+
+class Bmer(C.Structure):
+    _fields_ = [("position", C.c_uint32), ("mer", C.c_uint64)]
+
+
+class SmlHeader(C.Structure):
+    _fields_ = [("version", C.c_uint32), ("alphabet_bits", C.c_uint32), ("seed", C.c_uint64),
+                ("seed_length", C.c_uint32), ("seed_weight", C.c_uint32), ("length", C.c_uint64),
+                ("sml_length", C.c_uint64), ("circular", C.c_uint8)]
+
+
+class Match(C.Structure):
+    _fields_ = [("start", C.c_int64 * MH_MAX_SEQ), ("length", C.c_uint64)]
+
+
+class MatchStats(C.Structure):
+    _fields_ = [("seed_tuples", C.c_int64), ("collisions", C.c_int64), ("matches", C.c_int64),
+                ("repeat_runs", C.c_int64)]
+
+
+DP_JOB_DTYPE = np.dtype([("a_off", "<u8"), ("b_off", "<u8"), ("a_len", "<u4"), ("b_len", "<u4")])
+DP_RESULT_DTYPE = np.dtype([("score", "<i4"), ("path_len", "<u4"), ("path_start", "<u8")])
+MATCH_DTYPE = np.dtype([("start", "<i8", (MH_MAX_SEQ,)), ("length", "<u8")])
+BMER_DTYPE = np.dtype([("position", "<u4"), ("pad", "<u4"), ("mer", "<u8")])
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libmauvehip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"libmauvehip.so not built ({path}); run __graft_entry__.build()")
+    L = C.CDLL(path)
+    vp, u64, i64 = C.c_void_p, C.c_uint64, C.c_int64
+    sig = {
+        "mh_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "mh_destroy": (None, [vp]),
+        "mh_last_error": (C.c_char_p, []),
+        "mh_synchronize": (C.c_int, [vp]),
+        "mh_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)]),
+        "mh_kernel_time_reset": (None, [vp]),
+        "mh_set_timing": (C.c_int, [vp, C.c_int]),
+        "mh_get_seed": (i64, [C.c_int, C.c_int]),
+        "mh_get_seed_length": (C.c_int, [i64]),
+        "mh_get_seed_weight": (C.c_int, [i64]),
+        "mh_default_seed_weight": (C.c_uint32, [u64]),
+        "mh_sml_create": (C.c_int, [vp, C.c_char_p, u64, u64, C.POINTER(vp)]),
+        "mh_sml_create_device": (C.c_int, [vp, vp, u64, u64, C.POINTER(vp)]),
+        "mh_sml_free": (None, [vp]),
+        "mh_sml_get_header": (C.c_int, [vp, C.POINTER(SmlHeader)]),
+        "mh_sml_read": (C.c_int, [vp, vp, u64, u64, vp, C.POINTER(u64)]),
+        "mh_sml_get_seed_mers": (C.c_int, [vp, vp, vp, u64, vp]),
+        "mh_sml_find_mer": (C.c_int, [vp, vp, u64, C.POINTER(u64), C.POINTER(C.c_int)]),
+        "mh_sml_get_sequence": (C.c_int, [vp, vp, vp, u64]),
+        "mh_find_matches_pairwise": (C.c_int, [vp, C.POINTER(vp), C.c_int, vp, u64, C.POINTER(u64),
+                                               C.POINTER(MatchStats)]),
+        "mh_dp_align": (C.c_int, [vp, vp, u64, vp, u64, vp, u64, vp, vp]),
+        "mh_dp_align_device": (C.c_int, [vp, vp, vp, vp, u64, vp, vp]),
+        "mh_device_alloc": (C.c_int, [vp, u64, C.POINTER(vp)]),
+        "mh_device_free": (C.c_int, [vp, vp]),
+        "mh_memcpy_h2d": (C.c_int, [vp, vp, vp, u64]),
+        "mh_memcpy_d2h": (C.c_int, [vp, vp, vp, u64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != MH_OK:
+        msg = load_library().mh_last_error()
+        raise MhError(rc, msg.decode() if msg else "")
+    return rc
+
+
+# ---- seeds (LM/SeedMasks.h) -------------------------------------------------
+
+def get_seed(weight, rank=0):
+    return load_library().mh_get_seed(weight, rank)
+
+
+def get_seed_length(seed):
+    return load_library().mh_get_seed_length(seed)
+
+
+def get_seed_weight(seed):
+    return load_library().mh_get_seed_weight(seed)
+
+
+def get_default_seed_weight(avg_len):
+    return load_library().mh_default_seed_weight(avg_len)
+
+
+CODING_SEED = 3   # LM/SeedMasks.h:262
+
+
+class Context:
+    """One device and stream (mh_context)."""
+
+    def __init__(self, device=0):
+        L = load_library()
+        h = C.c_void_p()
+        _check(L.mh_create(device, C.byref(h)))
+        self.h = h
+        self.L = L
+
+    def close(self):
+        if self.h:
+            self.L.mh_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(self.L.mh_synchronize(self.h))
+
+    def set_timing(self, on=True):
+        _check(self.L.mh_set_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(), C.c_int64()
+        _check(self.L.mh_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def reset_kernel_times(self):
+        self.L.mh_kernel_time_reset(self.h)
+
+    # ---- device memory helpers ----
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        _check(self.L.mh_device_alloc(self.h, nbytes, C.byref(p)))
+        return p.value
+
+    def free(self, ptr):
+        _check(self.L.mh_device_free(self.h, ptr))
+
+    def h2d(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        _check(self.L.mh_memcpy_h2d(self.h, dptr, arr.ctypes.data, arr.nbytes))
+
+    def d2h(self, arr, dptr):
+        _check(self.L.mh_memcpy_d2h(self.h, arr.ctypes.data, dptr, arr.nbytes))
+        return arr
+
+
+# ---- DP ----------------------------------------------------------------------
+
+def make_jobs(a_off, a_len, b_off, b_len):
+    jobs = np.zeros(len(a_len), dtype=DP_JOB_DTYPE)
+    jobs["a_off"], jobs["a_len"] = a_off, a_len
+    jobs["b_off"], jobs["b_len"] = b_off, b_len
+    return jobs
+
+
+def global_align(ctx, a_codes, b_codes, jobs):
+    """Batch GlobalAlign.  Returns (paths buffer uint8, results structured array).
+
+    a_codes / b_codes: uint8 letter codes 0..3; jobs: DP_JOB_DTYPE array.
+    Job j's path is paths[r['path_start']: r['path_start'] + r['path_len']].
+    """
+    a = np.ascontiguousarray(a_codes, dtype=np.uint8)
+    b = np.ascontiguousarray(b_codes, dtype=np.uint8)
+    jobs = np.ascontiguousarray(jobs, dtype=DP_JOB_DTYPE)
+    total = int((jobs["a_len"].astype(np.uint64) + jobs["b_len"]).sum())
+    paths = np.zeros(max(total, 1), dtype=np.uint8)
+    res = np.zeros(len(jobs), dtype=DP_RESULT_DTYPE)
+    _check(ctx.L.mh_dp_align(ctx.h, a.ctypes.data, a.nbytes, b.ctypes.data, b.nbytes, jobs.ctypes.data,
+                             len(jobs), paths.ctypes.data, res.ctypes.data))
+    return paths, res
+
+
+def path_of(paths, res, j):
+    r = res[j]
+    s = int(r["path_start"])
+    return bytes(paths[s:s + int(r["path_len"])]).decode()
+
+
+def global_align_device(ctx, d_a, d_b, jobs, d_paths, d_results):
+    jobs = np.ascontiguousarray(jobs, dtype=DP_JOB_DTYPE)
+    _check(ctx.L.mh_dp_align_device(ctx.h, d_a, d_b, jobs.ctypes.data, len(jobs), d_paths, d_results))

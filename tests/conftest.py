@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "mauve-py_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libmauvehip.so)")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    import mauvehip
+    c = mauvehip.Context(0)
+    yield c
+    c.close()
