@@ -127,6 +127,14 @@ int mh_find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k,
                              mh_match *out, uint64_t cap, uint64_t *n_out,
                              mh_match_stats *stats);
 
+/* Same search, result kept in a library-owned list (one call, no size guess). */
+typedef struct mh_match_list mh_match_list;
+int mh_find_matches_pairwise_list(mh_context *ctx, const mh_sml *const *smls, int k,
+                                  mh_match_list **out, mh_match_stats *stats);
+uint64_t        mh_match_list_size(const mh_match_list *list);
+const mh_match *mh_match_list_data(const mh_match_list *list);
+void            mh_match_list_free(mh_match_list *list);
+
 /* ---- inter-anchor gapped DP ----------------------------------------------- */
 
 /* One DP job: A and B as letter codes 0..3 (A,C,G,T) at the given offsets of
@@ -162,6 +170,21 @@ int mh_dp_align(mh_context *ctx, const uint8_t *a_codes, uint64_t a_bytes,
 int mh_dp_align_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
                        const mh_dp_job *jobs, uint64_t njobs,
                        uint8_t *d_paths, mh_dp_result *d_results);
+
+/* ASCII genome (device) -> letter codes 0..3 (A,C,G,T, either case) for the DP.
+ * Any other byte is refused with MH_ERR_UNSUPPORTED (the DP implements the
+ * four-letter alphabet only). */
+int mh_ascii_to_codes_device(mh_context *ctx, const void *d_ascii, uint64_t len, uint8_t *d_codes);
+
+/* Inter-anchor DP jobs from a match list.  No counterpart in the reference:
+ * there the jobs come from the LCB pass and MUSCLE's anchored ranges
+ * (LM/ProgressiveAligner.cpp:1119-1290, MU/anchoredpp.cpp:443-552), which this
+ * library does not implement.  This producer takes the heaviest chain of
+ * non-overlapping collinear forward matches and emits one job per gap whose
+ * sides are both in 1..max_gap.  chain (optional) receives the chained match
+ * indices; jobs are offsets into the two genomes' code arrays. */
+int mh_chain_gap_jobs(const mh_match *matches, uint64_t n, uint64_t len_a, uint64_t len_b, uint32_t max_gap,
+                      uint32_t *chain, uint64_t *n_chain, mh_dp_job *jobs, uint64_t cap, uint64_t *n_jobs);
 
 /* Device memory helpers for callers without their own allocator. */
 int mh_device_alloc(mh_context *ctx, uint64_t bytes, void **d_ptr);

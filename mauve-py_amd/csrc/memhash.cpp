@@ -1,14 +1,360 @@
-// memhash.cpp -- placeholder C-ABI for SML/match entry points
+// memhash.cpp -- MemHash insertion replay and the SML / match-finding C ABI.
+//
+// The reference inserts every seed into one of 40000 buckets, chosen by the
+// seed's generalized offset, with std::lower_bound under MheCompare; a seed
+// equivalent to the entry found there (containment either way) is dropped
+// as a collision, otherwise it is extended and inserted
+// (LM/MemHash.cpp:209-251).  MheCompare is not a strict weak order, so the
+// result depends on the exact probe sequence of lower_bound and on the
+// insertion history of the bucket; both are reproduced here literally.
+// Buckets never interact, so they are replayed in parallel host threads,
+// each in the seeds' HashMatch order.  Extensions were computed on the GPU.
 #include "mh_internal.hpp"
-using namespace mh;
-extern "C" {
-int mh_sml_create(mh_context*, const char*, uint64_t, uint64_t, mh_sml**) { set_error("not built"); return MH_ERR_UNSUPPORTED; }
-int mh_sml_create_device(mh_context*, const void*, uint64_t, uint64_t, mh_sml**) { set_error("not built"); return MH_ERR_UNSUPPORTED; }
-void mh_sml_free(mh_sml*) {}
-int mh_sml_get_header(const mh_sml*, mh_sml_header*) { return MH_ERR_UNSUPPORTED; }
-int mh_sml_read(mh_context*, const mh_sml*, uint64_t, uint64_t, mh_bmer*, uint64_t*) { return MH_ERR_UNSUPPORTED; }
-int mh_sml_get_seed_mers(mh_context*, const mh_sml*, const uint32_t*, uint64_t, uint64_t*) { return MH_ERR_UNSUPPORTED; }
-int mh_sml_find_mer(mh_context*, const mh_sml*, uint64_t, uint64_t*, int*) { return MH_ERR_UNSUPPORTED; }
-int mh_sml_get_sequence(mh_context*, const mh_sml*, uint32_t*, uint64_t) { return MH_ERR_UNSUPPORTED; }
-int mh_find_matches_pairwise(mh_context*, const mh_sml* const*, int, mh_match*, uint64_t, uint64_t*, mh_match_stats*) { return MH_ERR_UNSUPPORTED; }
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <cstring>
+
+namespace mh {
+
+namespace {
+
+constexpr uint32_t TABLE_SIZE = 40000;   // DEFAULT_MEM_TABLE_SIZE, LM/MemHash.h:30
+
+// A two-genome MatchHashEntry: both starts always present in pairwise seeds.
+struct Ent {
+    int64_t s0, s1;
+    uint64_t len;
+    int64_t off;
+    uint64_t mersize;   // seed length for candidates, 0 for stored copies
+};
+
+// MatchHashEntry::Contains (a contains b), LM/MatchHashEntry.cpp:164-200
+inline bool contains(const Ent &a, const Ent &b)
+{
+    if (a.off != b.off) return false;
+    const int64_t diff = b.s0 - a.s0;
+    if (diff < 0 || a.len < b.len + (uint64_t)diff) return false;
+    const int64_t diff_rc = (int64_t)b.len - (int64_t)a.len + diff;
+    const int64_t di = b.s1 - a.s1;
+    if (b.s1 < 0 && diff_rc == di) return true;
+    return diff == di;
 }
+
+// strict_start_lessthan_ptr, LM/MatchHashEntry.cpp:48-67
+inline bool strict_less(const Ent &a, const Ent &b)
+{
+    int64_t as = a.s0, bs = b.s0;
+    if (as < 0) as = (int64_t)((uint64_t)(-as) + a.len - a.mersize);
+    if (bs < 0) bs = (int64_t)((uint64_t)(-bs) + b.len - b.mersize);
+    if (as != bs) return as < bs;
+    as = a.s1;
+    bs = b.s1;
+    if (as < 0) as = (int64_t)((uint64_t)(-as) + a.len - a.mersize);
+    if (bs < 0) bs = (int64_t)((uint64_t)(-bs) + b.len - b.mersize);
+    if (as != bs) return as < bs;
+    return false;
+}
+
+// MheCompare, LM/MatchHashEntry.h:121-143, with both genomes present in both
+// entries (the FirstStart and genome-presence tests never decide)
+inline bool mhe_less(const Ent &a, const Ent &b)
+{
+    if (contains(a, b) || contains(b, a)) return false;
+    return strict_less(a, b);
+}
+
+// libstdc++ std::lower_bound probe sequence
+inline size_t lower_bound_ent(const std::vector<Ent> &v, const Ent &q)
+{
+    size_t first = 0, len = v.size();
+    while (len > 0) {
+        const size_t half = len >> 1, mid = first + half;
+        if (mhe_less(v[mid], q)) {
+            first = mid + 1;
+            len = len - half - 1;
+        } else {
+            len = half;
+        }
+    }
+    return first;
+}
+
+}  // namespace
+
+int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std::vector<mh_match> &out,
+                          mh_match_stats *st)
+{
+    MH_CHECK(k == 2, MH_ERR_UNSUPPORTED, "this build matches exactly two genomes");
+    const mh_sml *a = smls[0], *b = smls[1];
+    MH_CHECK(a && b, MH_ERR_INVALID, "null sorted mer list");
+    MH_CHECK(a->seed == b->seed, MH_ERR_INVALID, "Different seed patterns.");   // LM/MatchFinder.cpp:190-194
+    std::vector<SeedRec> seeds;
+    uint32_t reps = 0;
+    int rc = join_and_extend(ctx, a, b, seeds, &reps);
+    if (rc != MH_OK) return rc;
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->repeat_runs = reps;
+    }
+    MH_CHECK(reps == 0, MH_ERR_UNSUPPORTED,
+             "a seed run exceeds MER_REPEAT_LIMIT (1000); the reference's restart after such runs "
+             "(LM/MatchFinder.cpp:253-276) is not implemented");
+    const int64_t sl = a->seed_length;
+    // HashMatch order, grouped by bucket
+    std::vector<uint32_t> bucket_of(seeds.size());
+    std::vector<uint32_t> bcount(TABLE_SIZE + 1, 0);
+    for (size_t x = 0; x < seeds.size(); ++x) {
+        const SeedRec &r = seeds[x];
+        const int64_t off = (int64_t)r.s1 - r.s0 - (r.s1 < 0 ? sl : 0);   // CalculateOffset
+        const uint32_t bk = (uint32_t)(((off % (int64_t)TABLE_SIZE) + TABLE_SIZE) % TABLE_SIZE);
+        bucket_of[x] = bk;
+        bcount[bk + 1]++;
+    }
+    for (uint32_t q = 0; q < TABLE_SIZE; ++q) bcount[q + 1] += bcount[q];
+    std::vector<uint32_t> order(seeds.size());
+    {
+        std::vector<uint32_t> fill(bcount.begin(), bcount.end() - 1);
+        for (size_t x = 0; x < seeds.size(); ++x) order[fill[bucket_of[x]]++] = (uint32_t)x;
+    }
+    std::vector<std::vector<Ent>> table(TABLE_SIZE);
+    std::vector<uint32_t> busy;
+    for (uint32_t q = 0; q < TABLE_SIZE; ++q)
+        if (bcount[q + 1] > bcount[q]) busy.push_back(q);
+    std::sort(busy.begin(), busy.end(),
+              [&](uint32_t x, uint32_t y) { return bcount[x + 1] - bcount[x] > bcount[y + 1] - bcount[y]; });
+    std::atomic<size_t> next{0};
+    std::atomic<int64_t> collisions{0};
+    auto work = [&]() {
+        int64_t coll = 0;
+        for (;;) {
+            const size_t w = next.fetch_add(1);
+            if (w >= busy.size()) break;
+            const uint32_t q = busy[w];
+            std::vector<Ent> &v = table[q];
+            for (uint32_t y = bcount[q]; y < bcount[q + 1]; ++y) {
+                const SeedRec &r = seeds[order[y]];
+                Ent c;
+                c.s0 = r.s0;
+                c.s1 = r.s1;
+                c.len = (uint64_t)sl;
+                c.off = (int64_t)r.s1 - r.s0 - (r.s1 < 0 ? sl : 0);
+                c.mersize = (uint64_t)sl;
+                size_t it = lower_bound_ent(v, c);
+                if (it != v.size() && !mhe_less(c, v[it])) {   // !comp(*it, c) holds at lower_bound
+                    ++coll;
+                    continue;
+                }
+                Ent e;
+                e.s0 = r.e0;
+                e.s1 = r.e1;
+                e.len = r.elen;
+                e.off = c.off;       // the offset is not recomputed after ExtendMatch
+                e.mersize = 0;       // MatchHashEntry::operator= zeroes m_mersize
+                it = lower_bound_ent(v, e);
+                v.insert(v.begin() + (ptrdiff_t)it, e);
+            }
+        }
+        collisions += coll;
+    };
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    if (nt > 16) nt = 16;
+    if (nt > busy.size()) nt = (unsigned)std::max<size_t>(1, busy.size());
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    // GetMatchList order: bucket by bucket, LM/MemHash.h:182-203
+    out.clear();
+    for (uint32_t q = 0; q < TABLE_SIZE; ++q)
+        for (const Ent &e : table[q]) {
+            mh_match m;
+            std::memset(&m, 0, sizeof(m));
+            m.start[0] = e.s0;
+            m.start[1] = e.s1;
+            m.length = e.len;
+            out.push_back(m);
+        }
+    if (st) {
+        st->seed_tuples = (int64_t)seeds.size();
+        st->collisions = collisions.load();
+        st->matches = (int64_t)out.size();
+    }
+    return MH_OK;
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+extern "C" {
+
+int mh_sml_create_device(mh_context *ctx, const void *d_seq, uint64_t len, uint64_t seed, mh_sml **out)
+{
+    MH_CHECK(ctx && out && (d_seq || len == 0), MH_ERR_INVALID, "null argument");
+    MH_HIP(hipSetDevice(ctx->device));
+    mh_sml *s = new mh_sml();
+    int rc = sml_build(ctx, d_seq, len, seed, s);
+    if (rc != MH_OK) {
+        mh_sml_free(s);
+        return rc;
+    }
+    *out = s;
+    return MH_OK;
+}
+
+int mh_sml_create(mh_context *ctx, const char *seq, uint64_t len, uint64_t seed, mh_sml **out)
+{
+    MH_CHECK(ctx && out && (seq || len == 0), MH_ERR_INVALID, "null argument");
+    MH_HIP(hipSetDevice(ctx->device));
+    void *d = nullptr;
+    MH_HIP(hipMalloc(&d, len + 16));
+    hipError_t e = hipMemcpyAsync(d, seq, len, hipMemcpyHostToDevice, ctx->stream);
+    int rc = e == hipSuccess ? mh_sml_create_device(ctx, d, len, seed, out) : MH_ERR_DEVICE;
+    if (e != hipSuccess) set_error(std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    return rc;
+}
+
+void mh_sml_free(mh_sml *s)
+{
+    if (!s) return;
+    if (s->d_words) (void)hipFree(s->d_words);
+    if (s->d_keys_pos) (void)hipFree(s->d_keys_pos);
+    if (s->d_keys) (void)hipFree(s->d_keys);
+    if (s->d_pos) (void)hipFree(s->d_pos);
+    delete s;
+}
+
+int mh_sml_get_header(const mh_sml *s, mh_sml_header *h)
+{
+    MH_CHECK(s && h, MH_ERR_INVALID, "null argument");
+    std::memset(h, 0, sizeof(*h));
+    h->version = 5;
+    h->alphabet_bits = 2;
+    h->seed = s->seed;
+    h->seed_length = (uint32_t)s->seed_length;
+    h->seed_weight = (uint32_t)s->seed_weight;
+    h->length = s->len;
+    h->sml_length = s->n;
+    h->circular = 0;
+    return MH_OK;
+}
+
+int mh_sml_read(mh_context *ctx, const mh_sml *s, uint64_t offset, uint64_t count, mh_bmer *out, uint64_t *n_read)
+{
+    MH_CHECK(ctx && s && n_read && (out || count == 0), MH_ERR_INVALID, "null argument");
+    *n_read = 0;
+    if (offset >= s->n) return MH_OK;
+    const uint64_t m = std::min<uint64_t>(count, s->n - offset);
+    std::vector<uint64_t> ck(m);
+    std::vector<uint32_t> ps(m);
+    MH_HIP(hipMemcpyAsync(ck.data(), s->d_keys + offset, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(ps.data(), s->d_pos + offset, m * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t x = 0; x < m; ++x) {
+        out[x].position = ps[x];
+        out[x].mer = sml_full_key(ck[x], s->seed_weight);
+    }
+    *n_read = m;
+    return MH_OK;
+}
+
+int mh_sml_get_seed_mers(mh_context *ctx, const mh_sml *s, const uint32_t *positions, uint64_t n, uint64_t *out)
+{
+    MH_CHECK(ctx && s && (n == 0 || (positions && out)), MH_ERR_INVALID, "null argument");
+    for (uint64_t x = 0; x < n; ++x)
+        MH_CHECK(positions[x] < s->n, MH_ERR_INVALID, "position beyond SMLLength");
+    if (!n) return MH_OK;
+    void *dp = nullptr, *dk = nullptr;
+    MH_HIP(hipMalloc(&dp, n * 4));
+    if (hipMalloc(&dk, n * 8) != hipSuccess) {
+        (void)hipFree(dp);
+        set_error("device allocation failed");
+        return MH_ERR_NOMEM;
+    }
+    int rc = MH_OK;
+    if (hipMemcpyAsync(dp, positions, n * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) rc = MH_ERR_DEVICE;
+    if (rc == MH_OK) rc = sml_seed_mers(ctx, s, (const uint32_t *)dp, n, (uint64_t *)dk);
+    if (rc == MH_OK && hipMemcpyAsync(out, dk, n * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        rc = MH_ERR_DEVICE;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dp);
+    (void)hipFree(dk);
+    if (rc == MH_ERR_DEVICE) set_error("copy failed in mh_sml_get_seed_mers");
+    return rc;
+}
+
+int mh_sml_find_mer(mh_context *ctx, const mh_sml *s, uint64_t mer, uint64_t *index, int *found)
+{
+    MH_CHECK(ctx && s && index && found, MH_ERR_INVALID, "null argument");
+    *found = 0;
+    // LM/SortedMerList.cpp:173-175: an empty or too-short list reports "not found"
+    if (s->len == 0 || s->len < (uint64_t)s->seed_length) return MH_OK;
+    uint64_t idx = 0;
+    int rc = sml_find_mer_dev(ctx, s, mer, &idx);
+    if (rc != MH_OK) return rc;
+    uint64_t ck = 0;
+    MH_HIP(hipMemcpyAsync(&ck, s->d_keys + idx, 8, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    *index = idx;
+    *found = sml_full_key(ck, s->seed_weight) == mer;
+    return MH_OK;
+}
+
+int mh_sml_get_sequence(mh_context *ctx, const mh_sml *s, uint32_t *words, uint64_t n_words)
+{
+    MH_CHECK(ctx && s && words, MH_ERR_INVALID, "null argument");
+    const uint64_t m = std::min<uint64_t>(n_words, s->n_words);
+    std::memset(words, 0, n_words * 4);
+    MH_HIP(hipMemcpyAsync(words, s->d_words, m * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    return MH_OK;
+}
+
+int mh_find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, mh_match *out, uint64_t cap,
+                             uint64_t *n_out, mh_match_stats *stats)
+{
+    MH_CHECK(ctx && smls && n_out && (out || cap == 0), MH_ERR_INVALID, "null argument");
+    MH_CHECK(k >= 1 && k <= MH_MAX_SEQ, MH_ERR_INVALID, "genome count out of range");
+    MH_HIP(hipSetDevice(ctx->device));
+    std::vector<mh_match> v;
+    int rc = find_matches_pairwise(ctx, smls, k, v, stats);
+    if (rc != MH_OK) return rc;
+    *n_out = v.size();
+    const uint64_t m = std::min<uint64_t>(cap, v.size());
+    if (m) std::memcpy(out, v.data(), m * sizeof(mh_match));
+    return MH_OK;
+}
+
+}  // extern "C"
+
+struct mh_match_list {
+    std::vector<mh_match> v;
+};
+
+extern "C" {
+
+int mh_find_matches_pairwise_list(mh_context *ctx, const mh_sml *const *smls, int k, mh_match_list **out,
+                                  mh_match_stats *stats)
+{
+    MH_CHECK(ctx && smls && out, MH_ERR_INVALID, "null argument");
+    MH_CHECK(k >= 1 && k <= MH_MAX_SEQ, MH_ERR_INVALID, "genome count out of range");
+    MH_HIP(hipSetDevice(ctx->device));
+    mh_match_list *l = new mh_match_list();
+    int rc = find_matches_pairwise(ctx, smls, k, l->v, stats);
+    if (rc != MH_OK) {
+        delete l;
+        return rc;
+    }
+    *out = l;
+    return MH_OK;
+}
+
+uint64_t mh_match_list_size(const mh_match_list *l) { return l ? l->v.size() : 0; }
+const mh_match *mh_match_list_data(const mh_match_list *l) { return l && !l->v.empty() ? l->v.data() : nullptr; }
+void mh_match_list_free(mh_match_list *l) { delete l; }
+
+}  // extern "C"

@@ -96,6 +96,19 @@ int sml_seed_mers(mh_context *ctx, const mh_sml *s, const uint32_t *d_pos, uint6
 int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k,
                           std::vector<mh_match> &out, mh_match_stats *st);
 
+// One seed in HashMatch order (after SetDirection) with its ExtendMatch result.
+struct SeedRec {
+    int32_t s0, s1;        // seed starts, 1-based, genome 1 negative on the other strand
+    int32_t e0, e1;        // extended starts
+    uint32_t elen;         // extended length
+};
+int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, std::vector<SeedRec> &seeds,
+                    uint32_t *repeat_runs);
+int sml_find_mer_dev(mh_context *ctx, const mh_sml *s, uint64_t mer, uint64_t *index);
+uint64_t sml_full_key(uint64_t c, int sw);
+int ascii_to_codes(mh_context *ctx, const void *d_ascii, uint64_t len, uint8_t *d_codes);
+uint64_t sml_compact_key(uint64_t k, int sw);
+
 // host-side seed helpers (LM/SeedMasks.h)
 int64_t get_seed(int weight, int rank);
 int seed_length(int64_t seed);

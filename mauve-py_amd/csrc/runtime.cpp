@@ -270,6 +270,13 @@ int mh_memcpy_d2h(mh_context *ctx, void *h_dst, const void *d_src, uint64_t byte
     return MH_OK;
 }
 
+int mh_ascii_to_codes_device(mh_context *ctx, const void *d_ascii, uint64_t len, uint8_t *d_codes)
+{
+    MH_CHECK(ctx && ((d_ascii && d_codes) || len == 0), MH_ERR_INVALID, "null argument");
+    MH_HIP(hipSetDevice(ctx->device));
+    return ascii_to_codes(ctx, d_ascii, len, d_codes);
+}
+
 // ---- DP ------------------------------------------------------------------
 
 int mh_dp_align_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,

@@ -248,17 +248,20 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *_
     }
 }
 
-__global__ void find_mer_kernel(const uint64_t *__restrict__ ckeys, uint64_t last, uint64_t query_c,
+__global__ void find_mer_kernel(const uint64_t *__restrict__ ckeys, uint64_t last, uint64_t query, int sw,
                                 uint64_t *__restrict__ out)
 {
-    // SortedMerList::bsearch, LM/SortedMerList.cpp:380-394 (one thread)
+    // SortedMerList::bsearch, LM/SortedMerList.cpp:380-394 (one thread); the
+    // probed keys are expanded to 64 bits so any query compares as in the
+    // reference
     uint64_t start = 0, end = last;
     for (;;) {
         const uint64_t mid = (start + end) / 2;
-        const uint64_t mm = ckeys[mid];
-        if (mm == query_c) { out[0] = mid; return; }
-        if (mm < query_c && mid < end) { start = mid + 1; continue; }
-        if (mm > query_c && start < mid) { end = mid - 1; continue; }
+        const uint64_t c = ckeys[mid];
+        const uint64_t mm = ((c >> 1) << (64 - 2 * sw)) | (c & 1ull);
+        if (mm == query) { out[0] = mid; return; }
+        if (mm < query && mid < end) { start = mid + 1; continue; }
+        if (mm > query && start < mid) { end = mid - 1; continue; }
         out[0] = mid;
         return;
     }
@@ -271,6 +274,22 @@ __global__ void gather_keys_kernel(const uint64_t *__restrict__ keys_pos, uint64
     if (x >= m) return;
     const uint32_t p = pos[x];
     out[x] = p < n ? keys_pos[p] : ~0ull;
+}
+
+// ASCII -> DP letter codes (A,C,G,T either case -> 0..3; anything else flags)
+__global__ void codes_kernel(const uint8_t *__restrict__ ascii, uint64_t len, uint8_t *__restrict__ codes,
+                             int *__restrict__ bad)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= len) return;
+    const uint32_t c = ascii[p] | 0x20u;
+    uint32_t v = 255;
+    if (c == 'a') v = 0;
+    else if (c == 'c') v = 1;
+    else if (c == 'g') v = 2;
+    else if (c == 't') v = 3;
+    if (v == 255) atomicOr(bad, 1);
+    codes[p] = (uint8_t)v;
 }
 
 SeedPattern make_pattern(uint64_t seed)
@@ -382,11 +401,28 @@ int sml_seed_mers(mh_context *ctx, const mh_sml *s, const uint32_t *d_pos, uint6
 int sml_find_mer_dev(mh_context *ctx, const mh_sml *s, uint64_t mer, uint64_t *index)
 {
     uint64_t *d = ctx->sort_tmp[3].as<uint64_t>();
-    hipLaunchKernelGGL(find_mer_kernel, dim3(1), dim3(1), 0, ctx->stream, s->d_keys, s->len - s->seed_length,
-                       compact_key(mer, s->seed_weight), d);
+    hipLaunchKernelGGL(find_mer_kernel, dim3(1), dim3(1), 0, ctx->stream, s->d_keys, s->len - s->seed_length, mer,
+                       s->seed_weight, d);
     MH_HIP(hipGetLastError());
     MH_HIP(hipMemcpyAsync(index, d, 8, hipMemcpyDeviceToHost, ctx->stream));
     MH_HIP(hipStreamSynchronize(ctx->stream));
+    return MH_OK;
+}
+
+int ascii_to_codes(mh_context *ctx, const void *d_ascii, uint64_t len, uint8_t *d_codes)
+{
+    if (!len) return MH_OK;
+    int rc;
+    if ((rc = ctx->sort_tmp[3].ensure(64)) != MH_OK) return rc;
+    int *d_bad = ctx->sort_tmp[3].as<int>();
+    MH_HIP(hipMemsetAsync(d_bad, 0, 4, ctx->stream));
+    hipLaunchKernelGGL(codes_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, ctx->stream,
+                       static_cast<const uint8_t *>(d_ascii), len, d_codes, d_bad);
+    MH_HIP(hipGetLastError());
+    int bad = 0;
+    MH_HIP(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    MH_CHECK(bad == 0, MH_ERR_UNSUPPORTED, "residue other than A,C,G,T in a DP operand");
     return MH_OK;
 }
 

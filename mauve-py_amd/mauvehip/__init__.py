@@ -40,6 +40,8 @@ EXPORTS = [
     "mh_sml_free", "mh_sml_get_header", "mh_sml_read", "mh_sml_get_seed_mers", "mh_sml_find_mer",
     "mh_sml_get_sequence", "mh_find_matches_pairwise", "mh_dp_align", "mh_dp_align_device",
     "mh_device_alloc", "mh_device_free", "mh_memcpy_h2d", "mh_memcpy_d2h",
+    "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
+    "mh_match_list_size", "mh_match_list_data", "mh_match_list_free",
 ]
 
 
@@ -121,6 +123,14 @@ def load_library(path=LIB_PATH):
         "mh_device_free": (C.c_int, [vp, vp]),
         "mh_memcpy_h2d": (C.c_int, [vp, vp, vp, u64]),
         "mh_memcpy_d2h": (C.c_int, [vp, vp, vp, u64]),
+        "mh_ascii_to_codes_device": (C.c_int, [vp, vp, u64, vp]),
+        "mh_find_matches_pairwise_list": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(vp),
+                                                    C.POINTER(MatchStats)]),
+        "mh_match_list_size": (u64, [vp]),
+        "mh_match_list_data": (vp, [vp]),
+        "mh_match_list_free": (None, [vp]),
+        "mh_chain_gap_jobs": (C.c_int, [vp, u64, u64, u64, C.c_uint32, vp, C.POINTER(u64), vp, u64,
+                                        C.POINTER(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -252,3 +262,119 @@ def path_of(paths, res, j):
 def global_align_device(ctx, d_a, d_b, jobs, d_paths, d_results):
     jobs = np.ascontiguousarray(jobs, dtype=DP_JOB_DTYPE)
     _check(ctx.L.mh_dp_align_device(ctx.h, d_a, d_b, jobs.ctypes.data, len(jobs), d_paths, d_results))
+
+
+# ---- sorted mer lists (DNAFileSML / DNAMemorySML) -------------------------
+
+class SortedMerList:
+    """A device-resident sorted mer list (mh_sml).
+
+    ``SortedMerList(ctx, seq, seed)`` builds the list as FileSML::Create does
+    (LM/FileSML.cpp:401-459); ``seq`` is ASCII bytes / str / uint8 array.
+    """
+
+    def __init__(self, ctx, seq, seed):
+        if isinstance(seq, str):
+            seq = seq.encode()
+        if isinstance(seq, np.ndarray):
+            seq = np.ascontiguousarray(seq, dtype=np.uint8).tobytes()
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(ctx.L.mh_sml_create(ctx.h, seq, len(seq), seed, C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_device(cls, ctx, d_seq, length, seed):
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(ctx.L.mh_sml_create_device(ctx.h, d_seq, length, seed, C.byref(h)))
+        self.h = h
+        return self
+
+    def free(self):
+        if getattr(self, "h", None):
+            self.ctx.L.mh_sml_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def header(self):
+        hd = SmlHeader()
+        _check(self.ctx.L.mh_sml_get_header(self.h, C.byref(hd)))
+        return hd
+
+    def sml_length(self):
+        return self.header().sml_length
+
+    def read(self, offset=0, count=None):
+        """Entries [offset, offset+count): (positions uint32, mers uint64)."""
+        n = self.sml_length()
+        if count is None:
+            count = max(n - offset, 0)
+        buf = np.zeros(max(count, 1), dtype=BMER_DTYPE)
+        got = C.c_uint64()
+        _check(self.ctx.L.mh_sml_read(self.ctx.h, self.h, offset, count, buf.ctypes.data, C.byref(got)))
+        buf = buf[:got.value]
+        return buf["position"].copy(), buf["mer"].copy()
+
+    def get_seed_mers(self, positions):
+        pos = np.ascontiguousarray(positions, dtype=np.uint32)
+        out = np.zeros(len(pos), dtype=np.uint64)
+        _check(self.ctx.L.mh_sml_get_seed_mers(self.ctx.h, self.h, pos.ctypes.data, len(pos), out.ctypes.data))
+        return out
+
+    def find_mer(self, mer):
+        idx, found = C.c_uint64(), C.c_int()
+        _check(self.ctx.L.mh_sml_find_mer(self.ctx.h, self.h, mer, C.byref(idx), C.byref(found)))
+        return bool(found.value), idx.value
+
+    def sequence_words(self):
+        hd = self.header()
+        nw = (hd.length * 2 + 31) // 32 + 2
+        out = np.zeros(nw, dtype=np.uint32)
+        _check(self.ctx.L.mh_sml_get_sequence(self.ctx.h, self.h, out.ctypes.data, nw))
+        return out
+
+
+def find_matches(ctx, smls):
+    """PairwiseMatchFinder::FindMatches: (starts int64 (n, k), lengths uint64, MatchStats)."""
+    k = len(smls)
+    arr = (C.c_void_p * k)(*[s.h.value for s in smls])
+    st = MatchStats()
+    h = C.c_void_p()
+    _check(ctx.L.mh_find_matches_pairwise_list(ctx.h, arr, k, C.byref(h), C.byref(st)))
+    try:
+        n = ctx.L.mh_match_list_size(h)
+        out = np.zeros(n, dtype=MATCH_DTYPE)
+        if n:
+            C.memmove(out.ctypes.data, ctx.L.mh_match_list_data(h), n * MATCH_DTYPE.itemsize)
+    finally:
+        ctx.L.mh_match_list_free(h)
+    return out["start"][:, :k].copy(), out["length"].copy(), st
+
+
+def ascii_to_codes_device(ctx, d_ascii, length, d_codes):
+    _check(ctx.L.mh_ascii_to_codes_device(ctx.h, d_ascii, length, d_codes))
+
+
+def chain_gap_jobs(starts, lengths, len_a, len_b, max_gap=20000):
+    """Stand-in DP job producer (see mh_chain_gap_jobs): (chain indices, jobs)."""
+    L = load_library()
+    n = len(lengths)
+    m = np.zeros(max(n, 1), dtype=MATCH_DTYPE)
+    if n:
+        m["start"][:n, :starts.shape[1]] = starts
+        m["length"][:n] = lengths
+    chain = np.zeros(max(n, 1), dtype=np.uint32)
+    nc, nj = C.c_uint64(), C.c_uint64()
+    _check(L.mh_chain_gap_jobs(m.ctypes.data, n, len_a, len_b, max_gap, chain.ctypes.data, C.byref(nc), None, 0,
+                               C.byref(nj)))
+    jobs = np.zeros(max(nj.value, 1), dtype=DP_JOB_DTYPE)
+    _check(L.mh_chain_gap_jobs(m.ctypes.data, n, len_a, len_b, max_gap, chain.ctypes.data, C.byref(nc),
+                               jobs.ctypes.data, nj.value, C.byref(nj)))
+    return chain[:nc.value].copy(), jobs[:nj.value].copy()

@@ -151,68 +151,104 @@ def pair_recoded(length=4_600_000, seed_ref=201, seed_mut=202, coverage=0.88):
     return ref, mut, orfs
 
 
-def dp_jobs(n=100_000, seed=301, min_len=10, max_ratio=2000, div=(0.02, 0.30)):
-    """Batched DP job set as letter codes (0..3).
-
-    Returns (a_codes, a_off, a_len, b_codes, b_off, b_len) with all A
-    sequences concatenated in a_codes (uint8) and likewise for B.
-    """
-    rng = np.random.Generator(np.random.PCG64(seed))
-    u = rng.random(n)
-    la = np.floor(min_len * np.power(float(max_ratio), u)).astype(np.int64)
-    la = np.maximum(la, 1)
-    d = rng.uniform(div[0], div[1], size=n)
-    a_off = np.zeros(n, dtype=np.int64)
-    a_off[1:] = np.cumsum(la)[:-1]
-    A = rng.integers(0, 4, size=int(la.sum()), dtype=np.uint8)
-    pair_id = np.repeat(np.arange(n), la)
-    # per-base events: substitution 90%, indel 10% (half insertions)
-    ev = rng.random(len(A)) < d[pair_id]
-    kind = rng.random(len(A))
+def _derive_chunk(rng, A, la, d):
+    """B for a run of pairs whose A sequences are concatenated in A."""
+    n = len(la)
+    starts = np.zeros(n, dtype=np.int64)
+    starts[1:] = np.cumsum(la)[:-1]
+    ends = starts + la
+    pair_id = np.repeat(np.arange(n, dtype=np.int32), la)
+    u = rng.random(len(A), dtype=np.float32)
+    ev = u < d[pair_id].astype(np.float32)
+    kind = rng.random(len(A), dtype=np.float32)
     sub = ev & (kind < 0.9)
     ins = ev & (kind >= 0.9) & (kind < 0.95)
     dele = ev & (kind >= 0.95)
     B = A.copy()
     B[sub] = (B[sub] + rng.integers(1, 4, size=int(sub.sum()), dtype=np.uint8)) % 4
-    keep = np.ones(len(A), dtype=bool)
     dpos = np.nonzero(dele)[0]
+    keep = np.ones(len(A), dtype=bool)
     if len(dpos):
         glen = rng.geometric(0.3, size=len(dpos))
-        end_of_pair = (a_off + la)[pair_id[dpos]]
-        stop = np.minimum(dpos + glen, end_of_pair)
-        cover = np.zeros(len(A) + 1, dtype=np.int64)
-        np.add.at(cover, dpos, 1)
-        np.add.at(cover, stop, -1)
+        stop = np.minimum(dpos + glen, ends[pair_id[dpos]])
+        cover = np.bincount(dpos, minlength=len(A) + 1) - np.bincount(stop, minlength=len(A) + 1)
         keep = np.cumsum(cover[:-1]) == 0
-    ipos = np.nonzero(ins)[0]
     icount = np.zeros(len(A), dtype=np.int64)
+    ipos = np.nonzero(ins)[0]
     if len(ipos):
         icount[ipos] = rng.geometric(0.3, size=len(ipos))
     reps = keep.astype(np.int64) + icount
-    base_src = np.repeat(B, reps)
-    # inserted bases after a kept base: the first copy keeps the base, the rest are random
-    first = np.zeros(len(base_src), dtype=bool)
-    starts = np.cumsum(reps) - reps
-    first[starts[reps > 0]] = True
-    is_orig = first & np.repeat(keep, reps)
-    newb = rng.integers(0, 4, size=len(base_src), dtype=np.uint8)
-    Bout = np.where(is_orig, base_src, newb).astype(np.uint8)
+    src = np.repeat(B, reps)
+    is_orig = np.zeros(len(src), dtype=bool)
+    first = np.cumsum(reps) - reps
+    is_orig[first[keep]] = True
+    newb = rng.integers(0, 4, size=len(src), dtype=np.uint8)
+    Bout = np.where(is_orig, src, newb).astype(np.uint8)
     blen = np.bincount(pair_id, weights=reps, minlength=n).astype(np.int64)
-    # the reference aborts on an empty operand: give empty B one random base
-    empty = np.nonzero(blen == 0)[0]
+    return Bout, blen
+
+
+def _dp_chunk(args):
+    ss, la, d = args
+    rng = np.random.Generator(np.random.PCG64(ss))
+    A = rng.integers(0, 4, size=int(la.sum()), dtype=np.uint8)
+    B, blen = _derive_chunk(rng, A, la, d)
+    return A, B, blen
+
+
+def dp_jobs(n=100_000, seed=301, min_len=10, max_ratio=2000, div=(0.02, 0.30), chunk=2000, workers=None):
+    """Batched DP job set as letter codes (0..3).
+
+    len(A) = floor(min_len * max_ratio**u), u ~ U(0,1); B derived from A at
+    divergence d ~ U(div): per base an event with probability d, 90%
+    substitutions and 10% indels (half insertions, half deletions) of
+    Geom(0.3) length.  An empty B gets one random base (the DP rejects empty
+    operands, as the reference does).  Lengths and divergences come from
+    PCG64(seed); the sequences of each run of `chunk` pairs come from its own
+    spawned PCG64 stream, so chunks can be generated by parallel workers
+    (spawned processes; call this before opening a GPU context).
+
+    Returns (a_codes, a_off, a_len, b_codes, b_off, b_len); all A sequences
+    are concatenated in a_codes (uint8) and likewise for B.
+    """
+    ss = np.random.SeedSequence(seed)
+    rng = np.random.Generator(np.random.PCG64(ss))
+    u = rng.random(n)
+    la = np.maximum(np.floor(min_len * np.power(float(max_ratio), u)).astype(np.int64), 1)
+    d = rng.uniform(div[0], div[1], size=n)
+    bounds = list(range(0, n, chunk)) + [n]
+    kids = ss.spawn(len(bounds) - 1)
+    tasks = [(kids[i], la[bounds[i]:bounds[i + 1]], d[bounds[i]:bounds[i + 1]]) for i in range(len(bounds) - 1)]
+    if workers is None:
+        import os
+        workers = min(16, os.cpu_count() or 1) if n > 4 * chunk else 1
+    if workers > 1:
+        import concurrent.futures as cf
+        import multiprocessing as mp
+        with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+            parts = list(ex.map(_dp_chunk, tasks))
+    else:
+        parts = [_dp_chunk(t) for t in tasks]
+    A = np.concatenate([p[0] for p in parts])
+    B = np.concatenate([p[1] for p in parts])
+    blen = np.concatenate([p[2] for p in parts])
     b_off = np.zeros(n, dtype=np.int64)
     b_off[1:] = np.cumsum(blen)[:-1]
+    empty = np.nonzero(blen == 0)[0]
     if len(empty):
-        parts, prev = [], 0
-        for e in empty:
-            parts.append(Bout[prev:b_off[e]])
-            parts.append(rng.integers(0, 4, size=1, dtype=np.uint8))
+        fill = rng.integers(0, 4, size=len(empty), dtype=np.uint8)
+        pieces, prev = [], 0
+        for k, e in enumerate(empty):
+            pieces.append(B[prev:b_off[e]])
+            pieces.append(fill[k:k + 1])
             prev = b_off[e]
-        parts.append(Bout[prev:])
-        Bout = np.concatenate(parts)
+        pieces.append(B[prev:])
+        B = np.concatenate(pieces)
         blen[empty] = 1
         b_off[1:] = np.cumsum(blen)[:-1]
-    return A, a_off, la, Bout, b_off, blen
+    a_off = np.zeros(n, dtype=np.int64)
+    a_off[1:] = np.cumsum(la)[:-1]
+    return A, a_off, la, B, b_off, blen
 
 
 def ascii_to_codes(seq):

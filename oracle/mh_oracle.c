@@ -548,7 +548,8 @@ typedef struct {
     mhe_t **pool;          /* every stored entry, for freeing */
     size_t npool, cappool;
     or_match_stats st;
-    /* seed-tuple capture mode */
+    /* seed-tuple capture mode: record HashMatch calls instead of inserting */
+    int tup_mode;
     or_match *tup_out;
     int64_t *tup_off;
     int64_t tup_max, tup_n;
@@ -620,8 +621,8 @@ static void hash_match(memhash_t *h, const idmer_t *ent, int cnt)
                 m.start[s] = -m.start[s];
     mhe_calc_offset(&m, k);
     h->st.hash_calls++;
-    if (h->tup_out) {
-        if (h->tup_n < h->tup_max) {
+    if (h->tup_mode) {
+        if (h->tup_out && h->tup_n < h->tup_max) {
             for (int i = 0; i < OR_MAX_SEQ; ++i) h->tup_out[h->tup_n].start[i] = i < k ? m.start[i] : 0;
             h->tup_out[h->tup_n].length = m.length;
             if (h->tup_off) h->tup_off[h->tup_n] = m.offset;
@@ -875,6 +876,7 @@ int64_t or_seed_tuples_pairwise(int k, const char *const *seqs, const uint64_t *
     memset(&h, 0, sizeof(h));
     h.k = k;
     h.sm = sm;
+    h.tup_mode = 1;
     h.tup_out = out;
     h.tup_off = offsets;
     h.tup_max = max_out;

@@ -1,0 +1,83 @@
+"""GPU parity of pairwise match finding (join + extension + hash replay) against the oracle.
+
+Bar: identical match lists (starts, lengths, order) and identical seed/collision
+counts.  The oracle's SearchRange/HashMatch/ExtendMatch/AddHashEntry
+restatement is pinned by brute force in tests/test_oracle_match.py.
+"""
+import numpy as np
+import pytest
+
+import mauvehip
+from mauvehip import synth
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(ctx, a, b, seed):
+    sa = mauvehip.SortedMerList(ctx, a, seed)
+    sb = mauvehip.SortedMerList(ctx, b, seed)
+    gs, gl, gst = mauvehip.find_matches(ctx, [sa, sb])
+    os_, ol, ost = po.find_matches([a, b], seed)
+    assert gst.seed_tuples == ost.hash_calls
+    assert gst.collisions == ost.collisions
+    assert len(gl) == len(ol)
+    assert np.array_equal(gs, os_)
+    assert np.array_equal(gl, ol)
+    return len(gl)
+
+
+def _mutant(rng, ref, snp=0.01, indel=0.001, inv=5000):
+    m = ref.copy()
+    hit = rng.random(len(m)) < snp
+    m[hit] = synth.BASES[(np.searchsorted(synth.BASES, m[hit]) + rng.integers(1, 4, hit.sum())) % 4]
+    out, prev = [], 0
+    for p in np.nonzero(rng.random(len(m)) < indel)[0]:
+        if p < prev:
+            continue
+        out.append(m[prev:p])
+        if rng.random() < 0.5:
+            out.append(synth.BASES[rng.integers(0, 4, rng.integers(1, 21))])
+            prev = p
+        else:
+            prev = p + int(rng.integers(1, 21))
+    out.append(m[prev:])
+    m = np.concatenate(out)
+    if inv and len(m) > 2 * inv:
+        s = int(rng.integers(0, len(m) - inv))
+        m[s:s + inv] = synth.revcomp(m[s:s + inv])
+    return m
+
+
+@pytest.mark.parametrize("seed", [po.get_seed(15, 3), po.get_seed(17, 3), po.get_seed(11, 0), po.get_seed(13, 3)])
+def test_match_small_pairs(ctx, seed):
+    rng = np.random.default_rng(seed & 0xFFF)
+    for L in (300, 3000, 30000):
+        ref = synth.BASES[rng.integers(0, 4, size=L)]
+        mut = _mutant(rng, ref, inv=L // 6)
+        _compare(ctx, ref.tobytes(), mut.tobytes(), seed)
+
+
+def test_match_iid_200kb(ctx):
+    ref, mut = synth.pair_iid()
+    n = _compare(ctx, ref.tobytes(), mut.tobytes(), po.get_seed(15, 3))
+    assert n > 0
+
+
+def test_match_recoded_4_6Mb(ctx):
+    ref, mut, _ = synth.pair_recoded()
+    n = _compare(ctx, ref.tobytes(), mut.tobytes(), po.get_seed(17, 3))
+    assert n > 10000
+
+
+def test_match_repeat_limit_refused(ctx):
+    rng = np.random.default_rng(9)
+    ref = np.concatenate([synth.BASES[rng.integers(0, 4, 5000)], np.full(1200, ord("A"), np.uint8),
+                          synth.BASES[rng.integers(0, 4, 5000)]])
+    mut = ref.copy()
+    seed = po.get_seed(15, 3)
+    sa = mauvehip.SortedMerList(ctx, ref.tobytes(), seed)
+    sb = mauvehip.SortedMerList(ctx, mut.tobytes(), seed)
+    with pytest.raises(mauvehip.MhError) as e:
+        mauvehip.find_matches(ctx, [sa, sb])
+    assert e.value.code == -6
