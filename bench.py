@@ -63,7 +63,7 @@ def main():
     dp = None
     if args.dp_pairs > 0:
         t0 = time.time()
-        dp = synth.dp_jobs(n=args.dp_pairs, seed=301)
+        dp = synth.dp_jobs(n=args.dp_pairs, seed=301, workers=min(16, os.cpu_count() or 1))
         log(f"[bench] {args.dp_pairs} DP pairs generated in {time.time() - t0:.1f}s")
 
     import mauvehip
@@ -114,7 +114,8 @@ def main():
     w = mauvehip.get_seed_weight(seed)
     passes = (2 * w + 1 + 7) // 8
     kern = {}
-    for name in ("sml_pack", "sml_keys", "sml_sort", "join", "extend", "dp_forward", "dp_traceback"):
+    for name in ("sml_pack", "sml_keys", "sml_sort", "join", "extend", "partition", "dp_forward", "dp_traceback",
+                 "host_wait_seeds", "host_partition", "host_replay", "host_list"):
         ms, n = ctx.kernel_time(name)
         kern[name] = {"avg_ms": ms, "launches": n, "ms_per_step": ms * n / args.steps}
     nseeds = int(state["stats"].seed_tuples)

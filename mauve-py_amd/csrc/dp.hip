@@ -30,13 +30,14 @@
 #include "mh_internal.hpp"
 #include <algorithm>
 #include <numeric>
+#include <cstdlib>
+#include <utility>
+#include <type_traits>
 
 namespace mh {
 namespace {
 
 constexpr int LANES = 64;
-constexpr int COLS = 8;
-constexpr int STRIPE = LANES * COLS;
 constexpr int NEG = -(1 << 29);
 constexpr int SBIAS = 65;           // smallest NUC_SP entry is -125 + 60 = -65
 
@@ -47,7 +48,7 @@ struct DevJob {
     uint64_t bnd_off;   // int32 elements
     uint64_t path_off;  // bytes (slot start)
     uint32_t user;      // index in the caller's job array
-    uint32_t pad;
+    uint32_t cols;      // columns per lane (8 or 16): stripe width 64*cols
 };
 
 struct DpOut {
@@ -55,24 +56,48 @@ struct DpOut {
     int32_t edge;       // 'M', 'D' or 'I'
 };
 
-__device__ __forceinline__ int shr1(int v)
+__device__ __forceinline__ int shr1_old(int v, int old)
 {
-    // wave_shr:1 -- lane k receives lane k-1's value (lane 0 keeps its own)
-    return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false);
+    // wave_shr:1 -- lane k receives lane k-1's value, lane 0 receives `old`
+    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xF, 0xF, false);
 }
 
-__device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-
-__global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs,
-                                                 const uint8_t *__restrict__ A,
-                                                 const uint8_t *__restrict__ B,
-                                                 uint32_t *__restrict__ tb,
-                                                 int *__restrict__ bnd,
-                                                 DpOut *__restrict__ outs,
-                                                 uint32_t sr0, uint32_t sr1, uint32_t sr2, uint32_t sr3,
-                                                 int goh)
+template <int... Cs, class F>
+__device__ __forceinline__ void static_for(std::integer_sequence<int, Cs...>, F &&f)
 {
-    const DevJob jb = jobs[blockIdx.x];
+    (f(std::integral_constant<int, Cs>{}), ...);
+}
+
+__device__ __forceinline__ int max3i(int a, int b, int c)
+{
+    int r;
+    asm("v_max3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// One wavefront per job.  D and I are kept shifted by the close half (goh):
+// D' = D + goh and I' = I + goh, so the M recurrence reads them without an
+// add; every comparison is between two values shifted alike, so ties and
+// trace bits are unchanged.  Per row a lane runs two passes over its COLS
+// columns: M and D right to left (each needs only the previous row's values of
+// its own and the left column), then I left to right (it chains along the
+// row).  Both update the registers in place.
+//
+// Lanes outside their row range sit out the step (exec mask).  Trace bits are the
+// reference's 4-bit code (MU/types.h:26-46): bits 0-1 the M source (MM=0,
+// DM=1, IM=2), bit 2 MD, bit 3 MI; COLS/8 32-bit words per lane per step, so
+// a wavefront stores 64*COLS/2 contiguous bytes per step.  Cell (i, j) of lane
+// k, column c is in word (stripe*(LA+63) + i-1+k)*64 + k)*COLS/8 + c/8, bits
+// 4*(c%8) .. +3.
+template <int COLS>
+__device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restrict__ A,
+                                       const uint8_t *__restrict__ B, uint32_t *__restrict__ tb,
+                                       int *__restrict__ bnd, DpOut *__restrict__ out, uint32_t sr0,
+                                       uint32_t sr1, uint32_t sr2, uint32_t sr3, int goh, int (*lds_in)[64],
+                                       int (*lds_out)[64])
+{
+    constexpr int STRIPE_W = LANES * COLS;
+    constexpr int NW = COLS / 8;           // trace words per lane per step
     const int lane = threadIdx.x;
     const int la = (int)jb.la, lb = (int)jb.lb;
     const uint8_t *a = A + jb.a_off;
@@ -82,123 +107,162 @@ __global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs
     int *bD = bM + nr;
     int *bI = bD + nr;
     uint32_t *tbj = tb + jb.tb_off;
-    const int nstripes = (lb + STRIPE - 1) / STRIPE;
-    const int fin_lane = ((lb - 1) % STRIPE) / COLS;
+    const int nstripes = (lb + STRIPE_W - 1) / STRIPE_W;
+    const int fin_lane = ((lb - 1) % STRIPE_W) / COLS;
     const int fin_col = (lb - 1) % COLS;
+    const int go2 = 2 * goh;
     int finM = NEG, finD = NEG, finI = NEG;
 
     for (int s = 0; s < nstripes; ++s) {
-        const int j0 = s * STRIPE + 1;
+        const int j0 = s * STRIPE_W + 1;
         const int jl = j0 + lane * COLS;
         const int rem = lb - (jl - 1);
         const int ncols = rem < 0 ? 0 : (rem > COLS ? COLS : rem);
-        const int span = lb - s * STRIPE;
-        const int nl = span >= STRIPE ? LANES : (span + COLS - 1) / COLS;
+        const int span = lb - s * STRIPE_W;
+        const int nl = span >= STRIPE_W ? LANES : (span + COLS - 1) / COLS;
         const bool last_stripe = (s == nstripes - 1);
 
         uint32_t bsh[COLS];
 #pragma unroll
         for (int c = 0; c < COLS; ++c) bsh[c] = (c < ncols) ? 8u * (uint32_t)b[jl - 1 + c] : 0u;
 
+        // row-0 state: M = NEG, D' = NEG + goh, I' = 0 + goh; left column likewise,
+        // or column 0 (M(0,0) = 0, D' = I' = NEG + goh) for the first lane
+        const int iM0 = jl == 1 ? 0 : NEG, iI0 = jl == 1 ? NEG + goh : goh;
         int Mp[COLS], Dp[COLS], Ip[COLS];
 #pragma unroll
-        for (int c = 0; c < COLS; ++c) { Mp[c] = NEG; Dp[c] = NEG; Ip[c] = 0; }   // row 0, j >= 1
-        int lpM, lpD, lpI;                                                        // row i-1, column jl-1
-        if (jl == 1) { lpM = 0; lpD = NEG; lpI = NEG; }
-        else { lpM = NEG; lpD = NEG; lpI = 0; }
+        for (int c = 0; c < COLS; ++c) { Mp[c] = NEG; Dp[c] = NEG + goh; Ip[c] = goh; }
+        int lpM = iM0, lpD = NEG + goh, lpI = iI0;
         int outM = NEG, outD = NEG, outI = NEG, acode = 0;
+        const int ob2_0 = (jl == 1) ? goh : go2;      // open_B[j-1] + goh for the lane's first column
 
-        // 64-row chunks: A codes for lane 0 and (s > 0) the left boundary column
-        int cA = 0, nA = 0, cM = NEG, cD = NEG, cI = NEG, nM = NEG, nD = NEG, nI = NEG;
-        {
-            const int r0 = lane, r1 = 64 + lane;
-            cA = r0 < la ? a[r0] : 0;
-            nA = r1 < la ? a[r1] : 0;
-            if (s > 0) {
-                if (r0 < la) { cM = bM[r0]; cD = bD[r0]; cI = bI[r0]; }
-                if (r1 < la) { nM = bM[r1]; nD = bD[r1]; nI = bI[r1]; }
-            }
-        }
-        int capM = NEG, capD = NEG, capI = NEG;
+        // lane 0's inputs, 64 rows per chunk: column jl-1 of the previous stripe
+        // (or column 0: M = NEG, D' = 0 + goh, I' = NEG + goh) and the A codes;
+        // the next chunk is loaded one chunk ahead
+        int nM = NEG, nD = goh, nI = NEG + goh, nA = 0;
+        if (s > 0 && lane < la) { nM = bM[lane]; nD = bD[lane]; nI = bI[lane]; }
+        if (lane < la) nA = a[lane];
 
         const int nsteps = la + nl - 1;
         for (int t = 0; t < nsteps; ++t) {
             const int q = t & 63;
-            if (q == 0 && t > 0) {
-                cA = nA; cM = nM; cD = nD; cI = nI;
+            if (q == 0) {
+                lds_in[0][lane] = nM;
+                lds_in[1][lane] = nD;
+                lds_in[2][lane] = nI;
+                lds_in[3][lane] = nA;
                 const int r = t + 64 + lane;
-                nA = r < la ? a[r] : 0;
+                nM = NEG; nD = goh; nI = NEG + goh; nA = 0;
                 if (s > 0 && r < la) { nM = bM[r]; nD = bD[r]; nI = bI[r]; }
+                if (r < la) nA = a[r];
             }
-            int rM = shr1(outM), rD = shr1(outD), rI = shr1(outI), ra = shr1(acode);
-            const int a0 = rdlane(cA, q);
-            int l0M, l0D, l0I;
-            if (s == 0) { l0M = NEG; l0D = 0; l0I = NEG; }          // column 0 at row t+1
-            else { l0M = rdlane(cM, q); l0D = rdlane(cD, q); l0I = rdlane(cI, q); }
-            if (lane == 0) { rM = l0M; rD = l0D; rI = l0I; ra = a0; }
+            const int rM = shr1_old(outM, lds_in[0][q]);
+            const int rD = shr1_old(outD, lds_in[1][q]);
+            const int rI = shr1_old(outI, lds_in[2][q]);
+            const int ra = shr1_old(acode, lds_in[3][q]);
             acode = ra;
 
             const int i = t - lane + 1;
             if (i >= 1 && i <= la && lane < nl) {
-                const uint32_t srow = (ra & 2) ? ((ra & 1) ? sr3 : sr2) : ((ra & 1) ? sr1 : sr0);
-                const int oA = (i == 1) ? 0 : goh;
-                int mL = rM, iL = rI, dL = rD;
-                int dgM = lpM, dgD = lpD, dgI = lpI;
-                uint32_t word = 0;
-                // columns past LB (c >= ncols, last lane of the last stripe
-                // only) compute values nobody reads
+            const uint32_t srow = (ra & 2) ? ((ra & 1) ? sr3 : sr2) : ((ra & 1) ? sr1 : sr0);
+            const int oa2 = (i == 1) ? goh : go2;   // open_A[i-1] + goh
+            uint32_t word[NW];
 #pragma unroll
-                for (int c = 0; c < COLS; ++c) {
-                    const int oB = (c == 0 && jl == 1) ? 0 : goh;
-                    const int md = Mp[c] + oA;
-                    const uint32_t bitD = md >= Dp[c];
-                    const int Dn = bitD ? md : Dp[c];
-                    const int mi = mL + oB;
-                    const uint32_t bitI = mi >= iL;
-                    const int In = bitI ? mi : iL;
-                    const int sc = (int)__builtin_amdgcn_ubfe(srow, bsh[c], 8) - SBIAS;
-                    const int DM = dgD + goh, IM = dgI + goh;
-                    const int best = max(dgM, max(DM, IM));
-                    uint32_t src = (dgM == best) ? 0u : ((DM == best) ? 1u : 2u);
-                    int Mn = sc + best;
-                    if (c == 0 && la == 1 && jl == 1) { Mn = sc + goh; src = 1u; }
-                    dgM = Mp[c]; dgD = Dp[c]; dgI = Ip[c];
-                    Mp[c] = Mn; Dp[c] = Dn; Ip[c] = In;
-                    mL = Mn; iL = In; dL = Dn;
-                    word |= (src | (bitD << 2) | (bitI << 3)) << (4 * c);
+            for (int w = 0; w < NW; ++w) word[w] = 0;
+            // pass 1, right to left: D'(i,j) and M(i,j) from row i-1
+            static_for(std::make_integer_sequence<int, COLS>{}, [&](auto ic) {
+                constexpr int c = COLS - 1 - decltype(ic)::value;
+                constexpr uint32_t K = 1u << (4 * (c & 7));
+                const int md = Mp[c] + oa2;
+                const bool tD = md >= Dp[c];
+                const int Dn = max(md, Dp[c]);
+                const int dM = c ? Mp[c > 0 ? c - 1 : 0] : lpM;
+                const int dD = c ? Dp[c > 0 ? c - 1 : 0] : lpD;
+                const int dI = c ? Ip[c > 0 ? c - 1 : 0] : lpI;
+                const int best = max3i(dM, dD, dI);
+                uint32_t srcK = (dM != best) ? ((dD == best) ? K : 2u * K) : 0u;
+                const int sc = (int)__builtin_amdgcn_ubfe(srow, bsh[c], 8) - SBIAS;
+                int Mn = sc + best;
+                if (c == 0 && la == 1 && jl == 1 && i == 1) {
+                    // one-row quirk: with LA == 1, M(1,1) carries close_A[0] and a 'D' bit
+                    Mn = sc + goh;
+                    srcK = K;
                 }
-                outM = mL; outD = dL; outI = iL;
-                lpM = rM; lpD = rD; lpI = rI;
-                tbj[((size_t)s * (size_t)(la + 63) + (size_t)t) * 64 + lane] = word;
-                if (last_stripe && i == la && lane == fin_lane) {
+                Mp[c] = Mn;
+                Dp[c] = Dn;
+                word[c >> 3] |= srcK | (tD ? 4u * K : 0u);
+            });
+            // pass 2, left to right: I'(i,j) from I'(i,j-1) and M(i,j-1)
+            int iL = rI, mL = rM;
+            static_for(std::make_integer_sequence<int, COLS>{}, [&](auto ic) {
+                constexpr int c = decltype(ic)::value;
+                constexpr uint32_t K = 1u << (4 * (c & 7));
+                const int mi = mL + (c == 0 ? ob2_0 : go2);
+                const bool tI = mi >= iL;
+                iL = max(mi, iL);
+                Ip[c] = iL;
+                mL = Mp[c];
+                word[c >> 3] |= tI ? 8u * K : 0u;
+            });
+            outM = Mp[COLS - 1];
+            outD = Dp[COLS - 1];
+            outI = Ip[COLS - 1];
+            lpM = rM;
+            lpD = rD;
+            lpI = rI;
+            uint32_t *dst = tbj + (((size_t)s * (size_t)(la + 63) + (size_t)t) * 64 + lane) * NW;
+            if constexpr (NW == 2) {
+                *reinterpret_cast<uint2 *>(dst) = make_uint2(word[0], word[1]);
+            } else {
+                dst[0] = word[0];
+            }
+            if (last_stripe && i == la && lane == fin_lane) {
 #pragma unroll
-                    for (int c = 0; c < COLS; ++c)
-                        if (c == fin_col) { finM = Mp[c]; finD = Dp[c]; finI = Ip[c]; }
-                }
+                for (int c = 0; c < COLS; ++c)
+                    if (c == fin_col) { finM = Mp[c]; finD = Dp[c]; finI = Ip[c]; }
+            }
             }
             if (!last_stripe) {
-                // lane 63 finished row t-62 of the stripe's last column
+                // lane 63 just finished row t-62 of the stripe's last column
                 const int i63 = t - 62;
-                const int vM = rdlane(outM, 63), vD = rdlane(outD, 63), vI = rdlane(outI, 63);
-                if (i63 >= 1 && i63 <= la) {
+                if (lane == 63 && i63 >= 1 && i63 <= la) {
                     const int slot = (i63 - 1) & 63;
-                    if (lane == slot) { capM = vM; capD = vD; capI = vI; }
-                    if (slot == 63 || i63 == la) {
-                        const int r = ((i63 - 1) & ~63) + lane;
-                        if (r < la) { bM[r] = capM; bD[r] = capD; bI[r] = capI; }
-                    }
+                    lds_out[0][slot] = outM;
+                    lds_out[1][slot] = outD;
+                    lds_out[2][slot] = outI;
+                }
+                if (i63 >= 1 && i63 <= la && (((i63 - 1) & 63) == 63 || i63 == la)) {
+                    const int r = ((i63 - 1) & ~63) + lane;
+                    if (r < la) { bM[r] = lds_out[0][lane]; bD[r] = lds_out[1][lane]; bI[r] = lds_out[2][lane]; }
                 }
             }
         }
         __threadfence();
     }
     if (lane == fin_lane) {
+        // back to unshifted D and I for the final pick, MU/nwsmall.cpp:641-656
+        finD -= goh;
+        finI -= goh;
         int score = finM, edge = 'M';
         if (finD > score) { score = finD; edge = 'D'; }
         if (finI > score) { score = finI; edge = 'I'; }
-        outs[blockIdx.x].score = score;
-        outs[blockIdx.x].edge = edge;
+        out->score = score;
+        out->edge = edge;
     }
+}
+
+__global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs, const uint8_t *__restrict__ A,
+                                                 const uint8_t *__restrict__ B, uint32_t *__restrict__ tb,
+                                                 int *__restrict__ bnd, DpOut *__restrict__ outs, uint32_t sr0,
+                                                 uint32_t sr1, uint32_t sr2, uint32_t sr3, int goh)
+{
+    __shared__ int lds_in[4][64];
+    __shared__ int lds_out[3][64];
+    const DevJob jb = jobs[blockIdx.x];
+    if (jb.cols == 16)
+        dp_job<16>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+    else
+        dp_job<8>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
 }
 
 __device__ __forceinline__ int xchar(uint32_t bits, int t)
@@ -229,9 +293,10 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
         slot[--pos] = (uint8_t)type;
         uint32_t nib = 0;
         if (i > 0 && j > 0) {
-            const int s = (j - 1) / STRIPE, cc = (j - 1) % STRIPE, k = cc / COLS, c = cc % COLS;
-            const uint32_t w = t[((size_t)s * (size_t)(la + 63) + (size_t)(i - 1 + k)) * 64 + k];
-            nib = (w >> (4 * c)) & 15u;
+            const int cols = (int)jb.cols, wpl = cols / 8, sw = 64 * cols;
+            const int s = (j - 1) / sw, cc = (j - 1) % sw, k = cc / cols, c = cc % cols;
+            const uint32_t w = t[(((size_t)s * (size_t)(la + 63) + (size_t)(i - 1 + k)) * 64 + k) * wpl + (c >> 3)];
+            nib = (w >> (4 * (c & 7))) & 15u;
         }
         const int nx = xchar(nib, type);
         if (type == 'M') { if (i == 0 || j == 0) { ok = false; break; } --i; --j; }
@@ -285,19 +350,43 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
         return (uint64_t)jobs[x].a_len * jobs[x].b_len > (uint64_t)jobs[y].a_len * jobs[y].b_len;
     });
+    // columns per lane: 8 (512-column stripes) for narrow jobs, 16 (1024) for
+    // wide ones; MH_DP_COLS=8|16 forces one shape (used for tuning sweeps)
+    int force_cols = 0;
+    if (const char *e = getenv("MH_DP_COLS")) force_cols = atoi(e);
+    auto cols_of = [&](uint32_t j) -> uint32_t {
+        if (force_cols == 8 || force_cols == 16) return (uint32_t)force_cols;
+        return jobs[j].b_len > 512 ? 16u : 8u;
+    };
     auto tb_words = [&](uint32_t j) {
-        const uint64_t ns = (jobs[j].b_len + STRIPE - 1) / STRIPE;
-        return ns * (uint64_t)(jobs[j].a_len + 63) * 64;
+        const uint32_t c = cols_of(j);
+        const uint64_t ns = (jobs[j].b_len + 64 * c - 1) / (64 * c);
+        return ns * (uint64_t)(jobs[j].a_len + 63) * 64 * (c / 8);
     };
     auto bnd_ints = [&](uint32_t j) {
-        const uint64_t ns = (jobs[j].b_len + STRIPE - 1) / STRIPE;
+        const uint32_t c = cols_of(j);
+        const uint64_t ns = (jobs[j].b_len + 64 * c - 1) / (64 * c);
         return ns > 1 ? 3ull * ((jobs[j].a_len + 63) & ~63ull) : 0ull;
     };
+    // trace arena budget: 70% of the memory that is free or already held by
+    // this context's arena (at most 192 GB); once grown the arena is kept
     size_t free_b = 0, total_b = 0;
     MH_HIP(hipMemGetInfo(&free_b, &total_b));
-    uint64_t budget = (uint64_t)(free_b * 0.45);
-    const uint64_t cap = 96ull << 30;
+    uint64_t budget = (uint64_t)((free_b + ctx->dp_tb.bytes + ctx->dp_bnd.bytes) * 0.7);
+    const uint64_t cap = 192ull << 30;
     if (budget > cap) budget = cap;
+    if (const char *e = getenv("MH_DP_TB_BUDGET_GB")) budget = (uint64_t)(atof(e) * (1ull << 30));
+    {
+        // size the arena once for the whole call: the largest round fits the budget
+        uint64_t need = 0;
+        for (uint64_t j = 0; j < njobs; ++j) need += (tb_words(order[j]) + bnd_ints(order[j])) * 4;
+        if (need > budget) need = budget;
+        int rc0;
+        if (ctx->dp_tb.bytes < need) {
+            ctx->dp_bnd.release();
+            if ((rc0 = ctx->dp_tb.ensure(need + 256)) != MH_OK) return rc0;
+        }
+    }
     uint32_t sr[4];
     packed_rows(sr);
 
@@ -327,7 +416,7 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
             d.bnd_off = bo;
             d.path_off = slot[j];
             d.user = j;
-            d.pad = 0;
+            d.cols = cols_of(j);
             tw += tb_words(j);
             bo += bnd_ints(j);
         }

@@ -20,6 +20,7 @@
 // reference restart its merge from recomputed start points (253-276); such
 // runs are counted so the host can refuse inputs that contain them.
 #include "mh_internal.hpp"
+#include <cstring>
 
 namespace mh {
 
@@ -299,14 +300,260 @@ __global__ void seed_kernel(const uint32_t *__restrict__ pi, const uint32_t *__r
     out[x] = r;
 }
 
+// ---- solid odd-length seeds ----------------------------------------------
+// With a solid seed of odd length k, a window and its reverse complement are
+// never equal, so comparing canonical keys plus strand parity (ExtendMatch's
+// test) is exactly comparing the k-character windows themselves (forward, or
+// genome 0 against the reverse complement of genome 1).  The four-phase
+// extension then stops at the first mismatching character on each side, or at
+// a sequence end: the result is the maximal exact match through the seed.
+// It is computed here 32 characters per 64-bit XOR of packed words.
+
+__device__ __forceinline__ uint64_t win32(const uint32_t *__restrict__ w, uint64_t pos)
+{
+    // 32 characters starting at pos (first character in the top two bits)
+    const uint64_t q = pos >> 4;
+    const uint32_t bit = (uint32_t)(pos & 15) * 2;
+    uint64_t m = ((uint64_t)w[q] << 32) | w[q + 1];
+    if (bit) m = (m << bit) | (w[q + 2] >> (32 - bit));
+    return m;
+}
+
+// number of equal characters going right from (pa, pb), at most lim
+__device__ __forceinline__ int64_t match_right(const uint32_t *__restrict__ wa, int64_t pa,
+                                               const uint32_t *__restrict__ wb, int64_t pb, int64_t lim)
+{
+    int64_t n = 0;
+    while (n < lim) {
+        const uint64_t x = win32(wa, (uint64_t)(pa + n)) ^ win32(wb, (uint64_t)(pb + n));
+        const int64_t eq = x ? (int64_t)(__clzll(x) >> 1) : 32;
+        n += eq;
+        if (eq < 32) break;
+    }
+    return n < lim ? n : lim;
+}
+
+// number of equal characters going left from (pa-1, pb-1), at most lim
+__device__ __forceinline__ int64_t match_left(const uint32_t *__restrict__ wa, int64_t pa,
+                                              const uint32_t *__restrict__ wb, int64_t pb, int64_t lim)
+{
+    int64_t n = 0;
+    while (n < lim) {
+        // the 32 characters ending just before pa-n (clamped at 0)
+        const int64_t take = lim - n < 32 ? lim - n : 32;
+        const uint64_t xa = win32(wa, (uint64_t)(pa - n - take)) >> (64 - 2 * take);
+        const uint64_t xb = win32(wb, (uint64_t)(pb - n - take)) >> (64 - 2 * take);
+        const uint64_t x = xa ^ xb;
+        const int64_t eq = x ? (int64_t)(__ffsll((long long)x) - 1) >> 1 : take;
+        n += eq;
+        if (eq < take) break;
+    }
+    return n < lim ? n : lim;
+}
+
+__global__ void rc_pack_kernel(const uint32_t *__restrict__ w, uint64_t len, uint64_t n_words,
+                               uint32_t *__restrict__ out)
+{
+    // packed reverse complement: character q of the output is 3 - char(len-1-q)
+    const uint64_t ow = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ow >= n_words) return;
+    uint32_t v = 0;
+    for (int k = 0; k < 16; ++k) {
+        const uint64_t q = ow * 16 + k;
+        if (q < len) {
+            const uint64_t src = len - 1 - q;
+            const uint32_t c = (w[src >> 4] >> (30 - 2 * (src & 15))) & 3u;
+            v |= (3u - c) << (30 - 2 * k);
+        }
+    }
+    out[ow] = v;
+}
+
+__global__ void seed_kernel_solid(const uint32_t *__restrict__ pi, const uint32_t *__restrict__ pj, uint64_t ns,
+                                  const uint64_t *__restrict__ ck0, const uint32_t *__restrict__ pos0,
+                                  const uint64_t *__restrict__ ck1, const uint32_t *__restrict__ pos1,
+                                  const uint32_t *__restrict__ wa, const uint32_t *__restrict__ wb,
+                                  const uint32_t *__restrict__ wbrc, int64_t La, int64_t Lb, int k,
+                                  SeedRec *__restrict__ out)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ns) return;
+    const uint32_t i = pi[x], j = pj[x];
+    const int64_t p0 = pos0[i];
+    const int64_t p1f = pos1[j];
+    const bool rev = (ck0[i] & 1ull) != (ck1[j] & 1ull);
+    SeedRec r;
+    r.s0 = (int32_t)(p0 + 1);
+    if (!rev) {
+        const int64_t left = match_left(wa, p0, wb, p1f, p0 < p1f ? p0 : p1f);
+        const int64_t ra = La - (p0 + k), rb = Lb - (p1f + k);
+        const int64_t right = match_right(wa, p0 + k, wb, p1f + k, ra < rb ? ra : rb);
+        r.s1 = (int32_t)(p1f + 1);
+        r.e0 = (int32_t)(p0 - left + 1);
+        r.e1 = (int32_t)(p1f - left + 1);
+        r.elen = (uint32_t)(k + left + right);
+    } else {
+        // genome 1 window in reverse-complement coordinates
+        const int64_t p1 = Lb - p1f - k;
+        const int64_t left = match_left(wa, p0, wbrc, p1, p0 < p1 ? p0 : p1);
+        const int64_t ra = La - (p0 + k), rb = Lb - (p1 + k);
+        const int64_t right = match_right(wa, p0 + k, wbrc, p1 + k, ra < rb ? ra : rb);
+        const int64_t L = k + left + right;
+        r.s1 = (int32_t)(-(p1f + 1));
+        r.e0 = (int32_t)(p0 - left + 1);
+        r.e1 = (int32_t)(-(Lb - (p1 - left + L) + 1));
+        r.elen = (uint32_t)L;
+    }
+    out[x] = r;
+}
+
+
+// ---- bucket partition of the seeds ----------------------------------------
+// AddHashEntry puts a seed in bucket ((offset mod 40000) + 40000) mod 40000
+// (LM/MemHash.cpp:213).  In a bucket whose seeds are all on the forward
+// strand, share one offset and come from an odd solid seed, every extension is
+// the maximal exact match through its seed on that one diagonal, so two
+// extensions are disjoint or equal, and MheCompare (LM/MatchHashEntry.h:121-143)
+// restricted to the bucket orders entries by start and calls a seed equivalent
+// exactly to the extension containing it.  lower_bound then finds that
+// extension whenever it is stored, whatever its probe sequence: the first seed
+// of each extension inserts it, every later one collides, and the bucket ends
+// up holding the distinct extensions by start.  Such "clean" buckets are
+// resolved here by sorting on (bucket, extended start) and keeping the first of
+// each run; every other bucket's seeds go to the host for the literal replay.
+
+constexpr uint32_t TABLE_SIZE = 40000;   // DEFAULT_MEM_TABLE_SIZE, LM/MemHash.h:30
+
+__device__ __forceinline__ int64_t seed_offset(const SeedRec &r, int k)
+{
+    return (int64_t)r.s1 - r.s0 - (r.s1 < 0 ? k : 0);   // CalculateOffset, LM/MatchHashEntry.cpp:141-160
+}
+
+__device__ __forceinline__ uint32_t bucket_of(int64_t off)
+{
+    return (uint32_t)(((off % (int64_t)TABLE_SIZE) + TABLE_SIZE) % TABLE_SIZE);
+}
+
+__global__ void bucket_key_kernel(const SeedRec *__restrict__ seeds, uint64_t ns, int k, uint64_t *__restrict__ key,
+                                  uint32_t *__restrict__ val)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ns) return;
+    key[x] = bucket_of(seed_offset(seeds[x], k));
+    val[x] = (uint32_t)x;
+}
+
+__global__ void bucket_first_kernel(const uint64_t *__restrict__ skey, const uint32_t *__restrict__ sval,
+                                    const SeedRec *__restrict__ seeds, uint64_t ns, int k,
+                                    int64_t *__restrict__ first_off)
+{
+    const uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ns) return;
+    const uint64_t b = skey[y];
+    if (y == 0 || skey[y - 1] != b) first_off[b] = seed_offset(seeds[sval[y]], k);
+}
+
+__global__ void bucket_dirty_kernel(const uint64_t *__restrict__ skey, const uint32_t *__restrict__ sval,
+                                    const SeedRec *__restrict__ seeds, uint64_t ns, int k, int solid_odd,
+                                    const int64_t *__restrict__ first_off, uint32_t *__restrict__ dirty)
+{
+    const uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ns) return;
+    const uint64_t b = skey[y];
+    const SeedRec r = seeds[sval[y]];
+    if (!solid_odd || r.s1 < 0 || seed_offset(r, k) != first_off[b]) dirty[b] = 1u;
+}
+
+__global__ void key2_kernel(const uint64_t *__restrict__ skey, const uint32_t *__restrict__ sval,
+                            const SeedRec *__restrict__ seeds, uint64_t ns, const uint32_t *__restrict__ dirty,
+                            uint64_t *__restrict__ key2, uint32_t *__restrict__ val2, uint32_t *__restrict__ flag_dirty)
+{
+    const uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ns) return;
+    const uint64_t b = skey[y];
+    const uint32_t v = sval[y];
+    const bool d = dirty[b] != 0;
+    key2[y] = (b << 32) | (d ? 0xFFFFFFFFull : (uint64_t)(uint32_t)seeds[v].e0);
+    val2[y] = v;
+    flag_dirty[y] = d ? 1u : 0u;
+}
+
+__global__ void distinct_kernel(const uint64_t *__restrict__ key2, uint64_t ns, const uint32_t *__restrict__ dirty,
+                                uint32_t *__restrict__ flag)
+{
+    const uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= ns) return;
+    const uint64_t kz = key2[z];
+    flag[z] = (!dirty[kz >> 32] && (z == 0 || key2[z - 1] != kz)) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(JT) void compact_idx_kernel(const uint32_t *__restrict__ flag, uint64_t n,
+                                                         const uint32_t *__restrict__ tile_off,
+                                                         uint32_t *__restrict__ out_idx)
+{
+    __shared__ uint32_t sc[JT];
+    const int t = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * JTILE + (uint64_t)t * JITEMS;
+    uint32_t f[JITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int r = 0; r < JITEMS; ++r) {
+        const uint64_t x = base + r;
+        f[r] = x < n ? flag[x] : 0u;
+        s += f[r];
+    }
+    sc[t] = s;
+    __syncthreads();
+    for (int off = 1; off < JT; off <<= 1) {
+        const uint32_t w = t >= off ? sc[t - off] : 0u;
+        __syncthreads();
+        sc[t] += w;
+        __syncthreads();
+    }
+    uint32_t o = tile_off[blockIdx.x] + sc[t] - s;
+#pragma unroll
+    for (int r = 0; r < JITEMS; ++r)
+        if (f[r]) out_idx[o++] = (uint32_t)(base + r);
+}
+
+__global__ void gather_clean_kernel(const uint32_t *__restrict__ idx, uint64_t m, const uint64_t *__restrict__ key2,
+                                    const uint32_t *__restrict__ val2, const SeedRec *__restrict__ seeds,
+                                    CleanEnt *__restrict__ out)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= m) return;
+    const uint32_t z = idx[x];
+    const SeedRec r = seeds[val2[z]];
+    CleanEnt e;
+    e.bucket = (uint32_t)(key2[z] >> 32);
+    e.e0 = r.e0;
+    e.e1 = r.e1;
+    e.elen = r.elen;
+    out[x] = e;
+}
+
+__global__ void gather_dirty_kernel(const uint32_t *__restrict__ idx, uint64_t m, const uint64_t *__restrict__ skey,
+                                    const uint32_t *__restrict__ sval, const SeedRec *__restrict__ seeds,
+                                    DirtySeed *__restrict__ out)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= m) return;
+    const uint32_t y = idx[x];
+    DirtySeed d;
+    d.bucket = (uint32_t)skey[y];
+    d.r = seeds[sval[y]];
+    out[x] = d;
+}
+
 }  // namespace
 
 // Seeds in HashMatch order with their extensions; returns the count.
-int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, std::vector<SeedRec> &seeds,
+int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const SeedRec **seeds_out, uint64_t *ns_out,
                     uint32_t *repeat_runs)
 {
     const uint64_t n0 = a->n, n1 = b->n;
-    seeds.clear();
+    *seeds_out = nullptr;
+    *ns_out = 0;
     *repeat_runs = 0;
     if (n0 == 0 || n1 == 0) return MH_OK;
     MH_CHECK(a->len < (1ull << 31) && b->len < (1ull << 31), MH_ERR_UNSUPPORTED,
@@ -355,18 +602,128 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, std::vect
     env.L1 = (int64_t)b->len;
     env.seed_len = a->seed_length;
     env.mer_mask = a->seed_weight >= 32 ? ~0ull : (~0ull << (64 - 2 * a->seed_weight));
-    {
+    const bool solid_odd = a->seed_length == a->seed_weight && (a->seed_length & 1);
+    if (solid_odd) {
+        if ((rc = ctx->mf_scratch[7].ensure(b->n_words * 4 + 64)) != MH_OK) return rc;
+        uint32_t *wbrc = ctx->mf_scratch[7].as<uint32_t>();
+        TimedLaunch tl(ctx, "extend");
+        hipLaunchKernelGGL(rc_pack_kernel, dim3((unsigned)((b->n_words + 255) / 256)), dim3(256), 0, ctx->stream,
+                           b->d_words, b->len, b->n_words, wbrc);
+        hipLaunchKernelGGL(seed_kernel_solid, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, ctx->stream, oi, oj,
+                           ns, a->d_keys, a->d_pos, b->d_keys, b->d_pos, a->d_words, b->d_words, wbrc,
+                           (int64_t)a->len, (int64_t)b->len, a->seed_length, d_seeds);
+        MH_HIP(hipGetLastError());
+    } else {
         TimedLaunch tl(ctx, "extend");
         hipLaunchKernelGGL(seed_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, ctx->stream, oi, oj, ns,
                            a->d_keys, a->d_pos, b->d_keys, b->d_pos, env, d_seeds, cnt + 2);
         MH_HIP(hipGetLastError());
     }
-    seeds.resize(ns);
-    MH_HIP(hipMemcpyAsync(seeds.data(), d_seeds, ns * sizeof(SeedRec), hipMemcpyDeviceToHost, ctx->stream));
     uint32_t ext_err = 0;
     MH_HIP(hipMemcpyAsync(&ext_err, cnt + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+    const double t0 = now_ms();
     MH_HIP(hipStreamSynchronize(ctx->stream));
+    host_time(ctx, "host_wait_seeds", now_ms() - t0);
+    *seeds_out = d_seeds;
+    *ns_out = ns;
     MH_CHECK(ext_err == 0, MH_ERR_UNSUPPORTED, "seed extension left the sorted mer lists");
+    return MH_OK;
+}
+
+
+// flags -> compacted indices in order; returns the count (synchronises)
+static int compact_flags(mh_context *ctx, const uint32_t *flag, uint64_t n, uint32_t *out_idx, uint32_t *tiles,
+                         uint32_t *d_count, uint64_t *count)
+{
+    const uint64_t ntiles = (n + JTILE - 1) / JTILE;
+    hipLaunchKernelGGL(tile_count_kernel, dim3((unsigned)ntiles), dim3(JT), 0, ctx->stream, flag, n, tiles);
+    hipLaunchKernelGGL(excl_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, tiles, ntiles, d_count);
+    hipLaunchKernelGGL(compact_idx_kernel, dim3((unsigned)ntiles), dim3(JT), 0, ctx->stream, flag, n, tiles, out_idx);
+    MH_HIP(hipGetLastError());
+    uint32_t h = 0;
+    MH_HIP(hipMemcpyAsync(&h, d_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    *count = h;
+    return MH_OK;
+}
+
+int partition_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t ns, int k, bool solid_odd,
+                    std::vector<CleanEnt> &clean, std::vector<DirtySeed> &dirty, uint64_t *n_clean_seeds)
+{
+    clean.clear();
+    dirty.clear();
+    *n_clean_seeds = 0;
+    if (ns == 0) return MH_OK;
+    const double t0 = now_ms();
+    int rc;
+    // scratch: 8..15 of mf_scratch
+    DevBuf *m = ctx->mf_scratch;
+    if ((rc = m[8].ensure(ns * 8 + 64)) != MH_OK) return rc;     // key A
+    if ((rc = m[9].ensure(ns * 8 + 64)) != MH_OK) return rc;     // key B
+    if ((rc = m[10].ensure(ns * 4 + 64)) != MH_OK) return rc;    // val A
+    if ((rc = m[11].ensure(ns * 4 + 64)) != MH_OK) return rc;    // val B
+    if ((rc = m[12].ensure(TABLE_SIZE * 12 + 64)) != MH_OK) return rc;   // first_off + dirty
+    if ((rc = m[13].ensure(ns * 4 + 64)) != MH_OK) return rc;    // flags
+    if ((rc = m[14].ensure(ns * 4 + 64)) != MH_OK) return rc;    // compacted indices
+    if ((rc = m[15].ensure(((ns + JTILE - 1) / JTILE) * 4 + 64 + ns * (sizeof(DirtySeed) + 16))) != MH_OK) return rc;
+    uint64_t *kA = m[8].as<uint64_t>(), *kB = m[9].as<uint64_t>();
+    uint32_t *vA = m[10].as<uint32_t>(), *vB = m[11].as<uint32_t>();
+    int64_t *first_off = m[12].as<int64_t>();
+    uint32_t *dirty_b = reinterpret_cast<uint32_t *>(first_off + TABLE_SIZE);
+    uint32_t *flag = m[13].as<uint32_t>(), *idx = m[14].as<uint32_t>();
+    uint32_t *tiles = m[15].as<uint32_t>();
+    uint32_t *d_count = tiles + ((ns + JTILE - 1) / JTILE) + 4;
+    void *gather = reinterpret_cast<char *>(m[15].ptr) + (((ns + JTILE - 1) / JTILE) * 4 + 64);
+    const unsigned grid = (unsigned)((ns + 255) / 256);
+    MH_HIP(hipMemsetAsync(dirty_b, 0, TABLE_SIZE * 4, ctx->stream));
+    uint64_t *sk = nullptr, *s2 = nullptr;
+    uint32_t *sv = nullptr, *v2 = nullptr;
+    {
+        TimedLaunch tl(ctx, "partition");
+        hipLaunchKernelGGL(bucket_key_kernel, dim3(grid), dim3(256), 0, ctx->stream, d_seeds, ns, k, kA, vA);
+        if ((rc = radix_sort_kv(ctx, kA, vA, kB, vB, ns, 16, &sk, &sv)) != MH_OK) return rc;
+        hipLaunchKernelGGL(bucket_first_kernel, dim3(grid), dim3(256), 0, ctx->stream, sk, sv, d_seeds, ns, k,
+                           first_off);
+        hipLaunchKernelGGL(bucket_dirty_kernel, dim3(grid), dim3(256), 0, ctx->stream, sk, sv, d_seeds, ns, k,
+                           solid_odd ? 1 : 0, first_off, dirty_b);
+        // (sk, sv) now hold the bucket order; the other pair is free for the second sort
+        uint64_t *k2a = (sk == kA) ? kB : kA;
+        uint32_t *v2a = (sv == vA) ? vB : vA;
+        hipLaunchKernelGGL(key2_kernel, dim3(grid), dim3(256), 0, ctx->stream, sk, sv, d_seeds, ns, dirty_b, k2a, v2a,
+                           flag);
+        MH_HIP(hipGetLastError());
+        // dirty seeds in (bucket, call) order
+        uint64_t nd = 0;
+        if ((rc = compact_flags(ctx, flag, ns, idx, tiles, d_count, &nd)) != MH_OK) return rc;
+        DirtySeed *d_dirty = static_cast<DirtySeed *>(gather);
+        if (nd) {
+            hipLaunchKernelGGL(gather_dirty_kernel, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, ctx->stream, idx,
+                               nd, sk, sv, d_seeds, d_dirty);
+            dirty.resize(nd);
+            MH_HIP(hipMemcpyAsync(dirty.data(), d_dirty, nd * sizeof(DirtySeed), hipMemcpyDeviceToHost, ctx->stream));
+            MH_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        *n_clean_seeds = ns - nd;
+        if (nd < ns) {
+            // second sort: (bucket, extended start); the bucket-order pair is scratch now
+            uint64_t *k2b = (k2a == kA) ? kB : kA;
+            uint32_t *v2b = (v2a == vA) ? vB : vA;
+            if ((rc = radix_sort_kv(ctx, k2a, v2a, k2b, v2b, ns, 48, &s2, &v2)) != MH_OK) return rc;
+            hipLaunchKernelGGL(distinct_kernel, dim3(grid), dim3(256), 0, ctx->stream, s2, ns, dirty_b, flag);
+            uint64_t nc = 0;
+            if ((rc = compact_flags(ctx, flag, ns, idx, tiles, d_count, &nc)) != MH_OK) return rc;
+            CleanEnt *d_clean = static_cast<CleanEnt *>(gather);
+            if (nc) {
+                hipLaunchKernelGGL(gather_clean_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   idx, nc, s2, v2, d_seeds, d_clean);
+                clean.resize(nc);
+                MH_HIP(hipMemcpyAsync(clean.data(), d_clean, nc * sizeof(CleanEnt), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+                MH_HIP(hipStreamSynchronize(ctx->stream));
+            }
+        }
+    }
+    host_time(ctx, "host_partition", now_ms() - t0);
     return MH_OK;
 }
 

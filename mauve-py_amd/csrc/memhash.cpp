@@ -7,8 +7,10 @@
 // (LM/MemHash.cpp:209-251).  MheCompare is not a strict weak order, so the
 // result depends on the exact probe sequence of lower_bound and on the
 // insertion history of the bucket; both are reproduced here literally.
-// Buckets never interact, so they are replayed in parallel host threads,
-// each in the seeds' HashMatch order.  Extensions were computed on the GPU.
+// Buckets whose outcome is fixed in closed form (one diagonal, forward strand,
+// odd solid seed) were resolved on the GPU; the others never interact, so
+// they are replayed in parallel host threads, each in the seeds' HashMatch
+// order.  Extensions were computed on the GPU.
 #include "mh_internal.hpp"
 #include <algorithm>
 #include <atomic>
@@ -19,7 +21,7 @@ namespace mh {
 
 namespace {
 
-constexpr uint32_t TABLE_SIZE = 40000;   // DEFAULT_MEM_TABLE_SIZE, LM/MemHash.h:30
+constexpr uint32_t TABLE_SIZE = 40000;   // DEFAULT_MEM_TABLE_SIZE, LM/MemHash.h:30; bucket ids are below it
 
 // A two-genome MatchHashEntry: both starts always present in pairwise seeds.
 struct Ent {
@@ -89,9 +91,10 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
     const mh_sml *a = smls[0], *b = smls[1];
     MH_CHECK(a && b, MH_ERR_INVALID, "null sorted mer list");
     MH_CHECK(a->seed == b->seed, MH_ERR_INVALID, "Different seed patterns.");   // LM/MatchFinder.cpp:190-194
-    std::vector<SeedRec> seeds;
+    const SeedRec *d_seeds = nullptr;
+    uint64_t nseeds = 0;
     uint32_t reps = 0;
-    int rc = join_and_extend(ctx, a, b, seeds, &reps);
+    int rc = join_and_extend(ctx, a, b, &d_seeds, &nseeds, &reps);
     if (rc != MH_OK) return rc;
     if (st) {
         std::memset(st, 0, sizeof(*st));
@@ -101,28 +104,24 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
              "a seed run exceeds MER_REPEAT_LIMIT (1000); the reference's restart after such runs "
              "(LM/MatchFinder.cpp:253-276) is not implemented");
     const int64_t sl = a->seed_length;
-    // HashMatch order, grouped by bucket
-    std::vector<uint32_t> bucket_of(seeds.size());
-    std::vector<uint32_t> bcount(TABLE_SIZE + 1, 0);
-    for (size_t x = 0; x < seeds.size(); ++x) {
-        const SeedRec &r = seeds[x];
-        const int64_t off = (int64_t)r.s1 - r.s0 - (r.s1 < 0 ? sl : 0);   // CalculateOffset
-        const uint32_t bk = (uint32_t)(((off % (int64_t)TABLE_SIZE) + TABLE_SIZE) % TABLE_SIZE);
-        bucket_of[x] = bk;
-        bcount[bk + 1]++;
-    }
-    for (uint32_t q = 0; q < TABLE_SIZE; ++q) bcount[q + 1] += bcount[q];
-    std::vector<uint32_t> order(seeds.size());
-    {
-        std::vector<uint32_t> fill(bcount.begin(), bcount.end() - 1);
-        for (size_t x = 0; x < seeds.size(); ++x) order[fill[bucket_of[x]]++] = (uint32_t)x;
-    }
-    std::vector<std::vector<Ent>> table(TABLE_SIZE);
-    std::vector<uint32_t> busy;
-    for (uint32_t q = 0; q < TABLE_SIZE; ++q)
-        if (bcount[q + 1] > bcount[q]) busy.push_back(q);
+    const bool solid_odd = a->seed_length == a->seed_weight && (a->seed_length & 1);
+    std::vector<CleanEnt> clean;
+    std::vector<DirtySeed> dirty;
+    uint64_t n_clean_seeds = 0;
+    if ((rc = partition_seeds(ctx, d_seeds, nseeds, (int)sl, solid_odd, clean, dirty, &n_clean_seeds)) != MH_OK)
+        return rc;
+    const double tg1 = now_ms();
+    // the remaining buckets: runs of equal bucket in the dirty list, each in HashMatch order
+    std::vector<size_t> run_lo;
+    for (size_t y = 0; y < dirty.size(); ++y)
+        if (y == 0 || dirty[y].bucket != dirty[y - 1].bucket) run_lo.push_back(y);
+    const size_t nruns = run_lo.size();
+    run_lo.push_back(dirty.size());
+    std::vector<std::vector<Ent>> table(nruns);
+    std::vector<uint32_t> busy(nruns);
+    for (size_t q = 0; q < nruns; ++q) busy[q] = (uint32_t)q;
     std::sort(busy.begin(), busy.end(),
-              [&](uint32_t x, uint32_t y) { return bcount[x + 1] - bcount[x] > bcount[y + 1] - bcount[y]; });
+              [&](uint32_t x, uint32_t y) { return run_lo[x + 1] - run_lo[x] > run_lo[y + 1] - run_lo[y]; });
     std::atomic<size_t> next{0};
     std::atomic<int64_t> collisions{0};
     auto work = [&]() {
@@ -132,13 +131,13 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
             if (w >= busy.size()) break;
             const uint32_t q = busy[w];
             std::vector<Ent> &v = table[q];
-            for (uint32_t y = bcount[q]; y < bcount[q + 1]; ++y) {
-                const SeedRec &r = seeds[order[y]];
+            for (size_t y = run_lo[q]; y < run_lo[q + 1]; ++y) {
+                const SeedRec &r = dirty[y].r;
                 Ent c;
                 c.s0 = r.s0;
                 c.s1 = r.s1;
                 c.len = (uint64_t)sl;
-                c.off = (int64_t)r.s1 - r.s0 - (r.s1 < 0 ? sl : 0);
+                c.off = (int64_t)r.s1 - r.s0 - (r.s1 < 0 ? sl : 0);   // CalculateOffset
                 c.mersize = (uint64_t)sl;
                 size_t it = lower_bound_ent(v, c);
                 if (it != v.size() && !mhe_less(c, v[it])) {   // !comp(*it, c) holds at lower_bound
@@ -165,20 +164,31 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
     for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
     work();
     for (auto &th : pool) th.join();
-    // GetMatchList order: bucket by bucket, LM/MemHash.h:182-203
+    const double tr1 = now_ms();
+    host_time(ctx, "host_replay", tr1 - tg1);
+    // GetMatchList order: bucket by bucket (LM/MemHash.h:182-203); clean and
+    // replayed buckets are disjoint sets of buckets, both already in bucket order
     out.clear();
-    for (uint32_t q = 0; q < TABLE_SIZE; ++q)
-        for (const Ent &e : table[q]) {
-            mh_match m;
-            std::memset(&m, 0, sizeof(m));
-            m.start[0] = e.s0;
-            m.start[1] = e.s1;
-            m.length = e.len;
-            out.push_back(m);
-        }
+    out.reserve(clean.size() + dirty.size());
+    auto emit = [&](int64_t s0, int64_t s1, uint64_t len) {
+        mh_match m;
+        std::memset(&m, 0, sizeof(m));
+        m.start[0] = s0;
+        m.start[1] = s1;
+        m.length = len;
+        out.push_back(m);
+    };
+    size_t ci = 0;
+    for (size_t q = 0; q <= nruns; ++q) {
+        const uint32_t bq = q < nruns ? dirty[run_lo[q]].bucket : TABLE_SIZE;
+        for (; ci < clean.size() && clean[ci].bucket < bq; ++ci) emit(clean[ci].e0, clean[ci].e1, clean[ci].elen);
+        if (q < nruns)
+            for (const Ent &e : table[q]) emit(e.s0, e.s1, e.len);
+    }
+    host_time(ctx, "host_list", now_ms() - tr1);
     if (st) {
-        st->seed_tuples = (int64_t)seeds.size();
-        st->collisions = collisions.load();
+        st->seed_tuples = (int64_t)nseeds;
+        st->collisions = collisions.load() + (int64_t)(n_clean_seeds - clean.size());
         st->matches = (int64_t)out.size();
     }
     return MH_OK;

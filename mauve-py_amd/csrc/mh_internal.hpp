@@ -55,9 +55,12 @@ struct mh_context {
     // DP scratch
     mh::DevBuf dp_tb, dp_bnd, dp_jobs, dp_a, dp_b, dp_paths, dp_res;
     // seed-matching scratch
-    mh::DevBuf mf_scratch[8];
+    mh::DevBuf mf_scratch[16];
     // scratch for sorting
     mh::DevBuf sort_tmp[4];
+    // pinned host staging for device-to-host transfers
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
 };
 
 struct mh_sml {
@@ -85,6 +88,11 @@ struct TimedLaunch {
     ~TimedLaunch();
 };
 int collect_timers(mh_context *ctx);
+// Host-side stage timing folded into the same table (wall clock, ms).
+void host_time(mh_context *ctx, const char *name, double ms);
+double now_ms();
+// Pinned host staging buffer of at least `bytes` (owned by the context).
+int pinned(mh_context *ctx, size_t bytes, void **out);
 
 // Launchers (return MH_OK or an error code); all run on ctx->stream.
 int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
@@ -102,12 +110,35 @@ struct SeedRec {
     int32_t e0, e1;        // extended starts
     uint32_t elen;         // extended length
 };
-int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, std::vector<SeedRec> &seeds,
+// Seeds stay in device memory owned by the context (valid until the next
+// match search); *d_seeds points there.
+int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const SeedRec **d_seeds, uint64_t *ns,
                     uint32_t *repeat_runs);
+// A distinct extension of a clean bucket, and a seed of any other bucket.
+struct CleanEnt {
+    uint32_t bucket;
+    int32_t e0, e1;
+    uint32_t elen;
+};
+struct DirtySeed {
+    uint32_t bucket;
+    SeedRec r;
+};
+// Splits the seeds by MemHash bucket: clean buckets resolved on the device
+// (distinct extensions, bucket then start order), the rest returned in bucket
+// then HashMatch order for the literal replay.
+int partition_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t ns, int k, bool solid_odd,
+                    std::vector<CleanEnt> &clean, std::vector<DirtySeed> &dirty, uint64_t *n_clean_seeds);
 int sml_find_mer_dev(mh_context *ctx, const mh_sml *s, uint64_t mer, uint64_t *index);
 uint64_t sml_full_key(uint64_t c, int sw);
 int ascii_to_codes(mh_context *ctx, const void *d_ascii, uint64_t len, uint8_t *d_codes);
 uint64_t sml_compact_key(uint64_t k, int sw);
+
+// Stable LSD radix sort of (key, value) pairs on the low `bits` key bits, 8 bits
+// per pass, ping-ponging between (k0,v0) and (k1,v1); *out_k/*out_v get the
+// pair of buffers holding the result.  Runs on ctx->stream.
+int radix_sort_kv(mh_context *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, uint64_t n, int bits,
+                  uint64_t **out_k, uint32_t **out_v);
 
 // host-side seed helpers (LM/SeedMasks.h)
 int64_t get_seed(int weight, int rank);

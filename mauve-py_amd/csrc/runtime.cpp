@@ -3,6 +3,7 @@
 #include "mh_internal.hpp"
 #include <cstring>
 #include <cmath>
+#include <chrono>
 
 namespace mh {
 
@@ -78,6 +79,38 @@ int collect_timers(mh_context *ctx)
         ctx->event_pool.push_back(p.second.second);
     }
     ctx->pending.clear();
+    return MH_OK;
+}
+
+void host_time(mh_context *ctx, const char *name, double ms)
+{
+    if (!ctx->timing) return;
+    auto &t = ctx->timers[name];
+    t.total_ms += ms;
+    t.launches += 1;
+}
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int pinned(mh_context *ctx, size_t bytes, void **out)
+{
+    if (bytes > ctx->pinned_bytes) {
+        if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+        const size_t want = bytes + bytes / 4 + 4096;
+        if (hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->pinned = nullptr;
+            set_error("pinned host allocation failed");
+            return MH_ERR_NOMEM;
+        }
+        ctx->pinned_bytes = want;
+    }
+    *out = ctx->pinned;
     return MH_OK;
 }
 
@@ -179,6 +212,7 @@ void mh_destroy(mh_context *ctx)
     ctx->dp_res.release();
     for (auto &b : ctx->mf_scratch) b.release();
     for (auto &b : ctx->sort_tmp) b.release();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -133,28 +133,53 @@ __global__ __launch_bounds__(SORT_THREADS) void hist_kernel(const uint64_t *__re
 }
 
 // Exclusive scan of the digit-major table (RADIX * ntiles entries) by one
-// workgroup: each thread scans a contiguous chunk, then the chunk totals.
-__global__ __launch_bounds__(1024) void scan_kernel(uint32_t *__restrict__ counts, uint64_t total)
+// workgroup.  The table is streamed through LDS in chunks of 16K entries: a
+// coalesced load, a per-thread scan of 16 consecutive entries, a workgroup
+// scan of the thread totals, and a coalesced write-back.
+constexpr int SCAN_T = 1024, SCAN_K = 16, SCAN_CHUNK = SCAN_T * SCAN_K;
+
+__global__ __launch_bounds__(SCAN_T) void scan_kernel(uint32_t *__restrict__ counts, uint64_t total)
 {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t buf[SCAN_CHUNK];
+    __shared__ uint32_t part[SCAN_T];
+    __shared__ uint32_t carry;
     const int t = threadIdx.x;
-    const uint64_t chunk = (total + 1023) / 1024;
-    const uint64_t lo = (uint64_t)t * chunk, hi = min(lo + chunk, total);
-    uint32_t s = 0;
-    for (uint64_t x = lo; x < hi; ++x) s += counts[x];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = t >= off ? part[t - off] : 0u;
+    if (t == 0) carry = 0;
+    for (uint64_t base = 0; base < total; base += SCAN_CHUNK) {
+#pragma unroll
+        for (int r = 0; r < SCAN_K; ++r) {
+            const uint64_t x = base + (uint64_t)r * SCAN_T + t;
+            buf[r * SCAN_T + t] = x < total ? counts[x] : 0u;
+        }
         __syncthreads();
-        part[t] += v;
+        uint32_t v[SCAN_K], s = 0;
+#pragma unroll
+        for (int r = 0; r < SCAN_K; ++r) {
+            v[r] = buf[t * SCAN_K + r];
+            s += v[r];
+        }
+        part[t] = s;
         __syncthreads();
-    }
-    uint32_t run = part[t] - s;
-    for (uint64_t x = lo; x < hi; ++x) {
-        const uint32_t c = counts[x];
-        counts[x] = run;
-        run += c;
+        for (int off = 1; off < SCAN_T; off <<= 1) {
+            const uint32_t w = t >= off ? part[t - off] : 0u;
+            __syncthreads();
+            part[t] += w;
+            __syncthreads();
+        }
+        uint32_t run = carry + part[t] - s;
+#pragma unroll
+        for (int r = 0; r < SCAN_K; ++r) {
+            buf[t * SCAN_K + r] = run;
+            run += v[r];
+        }
+        __syncthreads();
+        if (t == SCAN_T - 1) carry += part[SCAN_T - 1];
+#pragma unroll
+        for (int r = 0; r < SCAN_K; ++r) {
+            const uint64_t x = base + (uint64_t)r * SCAN_T + t;
+            if (x < total) counts[x] = buf[r * SCAN_T + t];
+        }
+        __syncthreads();
     }
 }
 
@@ -334,8 +359,6 @@ int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed,
     int rc;
     if ((rc = ctx->sort_tmp[0].ensure((n ? n : 1) * 8 + 64)) != MH_OK) return rc;
     if ((rc = ctx->sort_tmp[1].ensure((n ? n : 1) * 4 + 64)) != MH_OK) return rc;
-    const uint32_t ntiles = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
-    if ((rc = ctx->sort_tmp[2].ensure((uint64_t)RADIX * (ntiles ? ntiles : 1) * 4 + 64)) != MH_OK) return rc;
     if ((rc = ctx->sort_tmp[3].ensure(64)) != MH_OK) return rc;
     int *d_err = ctx->sort_tmp[3].as<int>();
     MH_HIP(hipMemsetAsync(d_err, 0, 4, ctx->stream));
@@ -363,22 +386,14 @@ int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed,
                            sp, s->d_keys_pos, k0, v0);
         MH_HIP(hipGetLastError());
     }
-    const int kbits = 2 * sw + 1;
-    const int passes = (kbits + 7) / 8;
-    uint32_t *counts = ctx->sort_tmp[2].as<uint32_t>();
     {
         TimedLaunch tl(ctx, "sml_sort");
-        for (int pass = 0; pass < passes; ++pass) {
-            const int shift = 8 * pass;
-            hipLaunchKernelGGL(hist_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, n, shift, counts,
-                               ntiles);
-            hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, counts, (uint64_t)RADIX * ntiles);
-            hipLaunchKernelGGL(scatter_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, v0, k1, v1, n,
-                               shift, counts, ntiles);
-            std::swap(k0, k1);
-            std::swap(v0, v1);
-        }
-        MH_HIP(hipGetLastError());
+        uint64_t *rk = nullptr;
+        uint32_t *rv = nullptr;
+        int rc2 = radix_sort_kv(ctx, k0, v0, k1, v1, n, 2 * sw + 1, &rk, &rv);
+        if (rc2 != MH_OK) return rc2;
+        k0 = rk;
+        v0 = rv;
     }
     if (k0 != s->d_keys) {   // odd number of passes: result is in the scratch pair
         MH_HIP(hipMemcpyAsync(s->d_keys, k0, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
@@ -386,6 +401,32 @@ int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed,
     }
     MH_HIP(hipStreamSynchronize(ctx->stream));
     MH_CHECK(herr == 0, MH_ERR_GAP_CHAR, "Gap in genome sequence");
+    return MH_OK;
+}
+
+int radix_sort_kv(mh_context *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, uint64_t n, int bits,
+                  uint64_t **out_k, uint32_t **out_v)
+{
+    *out_k = k0;
+    *out_v = v0;
+    if (n == 0) return MH_OK;
+    const uint32_t ntiles = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
+    int rc;
+    if ((rc = ctx->sort_tmp[2].ensure((uint64_t)RADIX * ntiles * 4 + 64)) != MH_OK) return rc;
+    uint32_t *counts = ctx->sort_tmp[2].as<uint32_t>();
+    const int passes = (bits + 7) / 8;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(hist_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, n, shift, counts, ntiles);
+        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(SCAN_T), 0, ctx->stream, counts, (uint64_t)RADIX * ntiles);
+        hipLaunchKernelGGL(scatter_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, v0, k1, v1, n, shift,
+                           counts, ntiles);
+        std::swap(k0, k1);
+        std::swap(v0, v1);
+    }
+    MH_HIP(hipGetLastError());
+    *out_k = k0;
+    *out_v = v0;
     return MH_OK;
 }
 

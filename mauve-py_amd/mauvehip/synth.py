@@ -196,7 +196,7 @@ def _dp_chunk(args):
     return A, B, blen
 
 
-def dp_jobs(n=100_000, seed=301, min_len=10, max_ratio=2000, div=(0.02, 0.30), chunk=2000, workers=None):
+def dp_jobs(n=100_000, seed=301, min_len=10, max_ratio=2000, div=(0.02, 0.30), chunk=2000, workers=1):
     """Batched DP job set as letter codes (0..3).
 
     len(A) = floor(min_len * max_ratio**u), u ~ U(0,1); B derived from A at
@@ -205,8 +205,9 @@ def dp_jobs(n=100_000, seed=301, min_len=10, max_ratio=2000, div=(0.02, 0.30), c
     Geom(0.3) length.  An empty B gets one random base (the DP rejects empty
     operands, as the reference does).  Lengths and divergences come from
     PCG64(seed); the sequences of each run of `chunk` pairs come from its own
-    spawned PCG64 stream, so chunks can be generated by parallel workers
-    (spawned processes; call this before opening a GPU context).
+    spawned PCG64 stream, so chunks can be generated by `workers` parallel
+    processes (spawned: the calling script needs an `if __name__ == "__main__"`
+    guard; call this before opening a GPU context).
 
     Returns (a_codes, a_off, a_len, b_codes, b_off, b_len); all A sequences
     are concatenated in a_codes (uint8) and likewise for B.
@@ -219,10 +220,7 @@ def dp_jobs(n=100_000, seed=301, min_len=10, max_ratio=2000, div=(0.02, 0.30), c
     bounds = list(range(0, n, chunk)) + [n]
     kids = ss.spawn(len(bounds) - 1)
     tasks = [(kids[i], la[bounds[i]:bounds[i + 1]], d[bounds[i]:bounds[i + 1]]) for i in range(len(bounds) - 1)]
-    if workers is None:
-        import os
-        workers = min(16, os.cpu_count() or 1) if n > 4 * chunk else 1
-    if workers > 1:
+    if workers > 1 and n > 4 * chunk:
         import concurrent.futures as cf
         import multiprocessing as mp
         with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:

@@ -70,6 +70,23 @@ def test_match_recoded_4_6Mb(ctx):
     assert n > 10000
 
 
+@pytest.mark.parametrize("seed", [po.get_seed(17, 3), po.get_seed(15, 3)])
+def test_match_bucket_alias(ctx, seed):
+    """Diagonals 40000 apart share a MemHash bucket: the second half of the
+    derivative sits 40000 bp further along, so both halves hash to bucket 0,
+    next to an inverted block (reverse-strand seeds) and SNP-split runs."""
+    rng = np.random.default_rng(77)
+    ref = synth.BASES[rng.integers(0, 4, size=120000)]
+    ref[30000:30050] = ref[10000:10050]           # a repeat inside genome 0
+    mid = synth.BASES[rng.integers(0, 4, size=40000)]
+    mut = np.concatenate([ref[:60000], mid, ref[60000:]])
+    mut[80000:86000] = synth.revcomp(mut[80000:86000])
+    hit = rng.random(len(mut)) < 0.01
+    mut[hit] = synth.BASES[(np.searchsorted(synth.BASES, mut[hit]) + 1) % 4]
+    n = _compare(ctx, ref.tobytes(), mut.tobytes(), seed)
+    assert n > 100
+
+
 def test_match_repeat_limit_refused(ctx):
     rng = np.random.default_rng(9)
     ref = np.concatenate([synth.BASES[rng.integers(0, 4, 5000)], np.full(1200, ord("A"), np.uint8),
