@@ -81,16 +81,25 @@ def main():
     d_paths = ctx.alloc(2 * (La + Lb) + 16)
     d_res = ctx.alloc(16 * (La + Lb) + 16)
     state = {}
+    stage_s = {"sml": 0.0, "match": 0.0, "chain": 0.0, "dp": 0.0, "free": 0.0}
 
     def step():
+        t = [time.perf_counter()]
         sa = mauvehip.SortedMerList.from_device(ctx, dA, La, seed)
         sb = mauvehip.SortedMerList.from_device(ctx, dB, Lb, seed)
+        t.append(time.perf_counter())
         st, ln, stats = mauvehip.find_matches(ctx, [sa, sb])
+        t.append(time.perf_counter())
         chain, jobs = mauvehip.chain_gap_jobs(st, ln, La, Lb, MAX_GAP)
+        t.append(time.perf_counter())
         mauvehip.global_align_device(ctx, cA, cB, jobs, d_paths, d_res)
         ctx.synchronize()
+        t.append(time.perf_counter())
         sa.free()
         sb.free()
+        t.append(time.perf_counter())
+        for k, name in enumerate(("sml", "match", "chain", "dp", "free")):
+            stage_s[name] += t[k + 1] - t[k]
         state.update(stats=stats, jobs=jobs, chain=chain, starts=st, lengths=ln)
 
     for _ in range(args.warmup):
@@ -98,6 +107,8 @@ def main():
     ctx.set_timing(True)
     ctx.reset_kernel_times()
     ctx.synchronize()
+    for k in stage_s:
+        stage_s[k] = 0.0
     t0 = time.perf_counter()
     for k in range(args.steps):
         step()
@@ -150,6 +161,7 @@ def main():
         "roofline": roofline,
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "launches_per_step": v["launches"] / args.steps}
                     for k, v in kern.items()},
+        "stages_ms_per_step": {k: round(1e3 * v / args.steps, 3) for k, v in stage_s.items()},
     }
 
     if dp is not None:

@@ -29,14 +29,19 @@ int mh_chain_gap_jobs(const mh_match *matches, uint64_t n, uint64_t len_a, uint6
                       uint32_t *chain, uint64_t *n_chain, mh_dp_job *jobs, uint64_t cap, uint64_t *n_jobs)
 {
     MH_CHECK((matches || n == 0) && n_chain && n_jobs && (jobs || cap == 0), MH_ERR_INVALID, "null argument");
-    std::vector<Anc> a;
-    a.reserve(n);
+    // anchors in (s0, s1, match index) order; the sort runs on packed keys
+    std::vector<std::pair<uint64_t, uint32_t>> ord;
+    ord.reserve(n);
     for (uint64_t i = 0; i < n; ++i)
         if (matches[i].start[0] > 0 && matches[i].start[1] > 0)
-            a.push_back({matches[i].start[0] - 1, matches[i].start[1] - 1, (int64_t)matches[i].length, (uint32_t)i});
-    std::sort(a.begin(), a.end(), [](const Anc &x, const Anc &y) {
-        return x.s0 != y.s0 ? x.s0 < y.s0 : (x.s1 != y.s1 ? x.s1 < y.s1 : x.idx < y.idx);
-    });
+            ord.push_back({((uint64_t)(matches[i].start[0] - 1) << 32) | (uint64_t)(matches[i].start[1] - 1),
+                           (uint32_t)i});
+    std::sort(ord.begin(), ord.end());
+    std::vector<Anc> a(ord.size());
+    for (size_t q = 0; q < ord.size(); ++q) {
+        const mh_match &mt = matches[ord[q].second];
+        a[q] = {mt.start[0] - 1, mt.start[1] - 1, (int64_t)mt.length, ord[q].second};
+    }
     const size_t m = a.size();
     // best[i] = len_i + max best[j] over anchors ending (in both genomes) before i starts.
     // Sweep by start; anchors become eligible once their end0 <= current start0;
@@ -57,16 +62,15 @@ int mh_chain_gap_jobs(const mh_match *matches, uint64_t n, uint64_t len_a, uint6
             if (fen[r].first > b.first) b = fen[r];
         return b;
     };
-    std::vector<size_t> by_end(m);
-    for (size_t i = 0; i < m; ++i) by_end[i] = i;
-    std::sort(by_end.begin(), by_end.end(), [&](size_t x, size_t y) {
-        return a[x].s0 + a[x].len != a[y].s0 + a[y].len ? a[x].s0 + a[x].len < a[y].s0 + a[y].len : x < y;
-    });
+    // anchors by end in genome 0 (ties by anchor rank)
+    std::vector<uint64_t> by_end(m);
+    for (size_t i = 0; i < m; ++i) by_end[i] = ((uint64_t)(a[i].s0 + a[i].len) << 32) | (uint64_t)i;
+    std::sort(by_end.begin(), by_end.end());
     std::vector<int64_t> best(m, 0), prev(m, -1);
     size_t e = 0;
     for (size_t i = 0; i < m; ++i) {
-        while (e < m && a[by_end[e]].s0 + a[by_end[e]].len <= a[i].s0) {
-            const size_t j = by_end[e++];
+        while (e < m && (int64_t)(by_end[e] >> 32) <= a[i].s0) {
+            const size_t j = (size_t)(by_end[e++] & 0xFFFFFFFFu);
             if (best[j] > 0) upd(std::lower_bound(keys.begin(), keys.end(), ends1[j]) - keys.begin(), {best[j], (int64_t)j});
         }
         const size_t r = std::upper_bound(keys.begin(), keys.end(), a[i].s1) - keys.begin();

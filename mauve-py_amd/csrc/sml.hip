@@ -132,30 +132,25 @@ __global__ __launch_bounds__(SORT_THREADS) void hist_kernel(const uint64_t *__re
     counts[(uint64_t)t * ntiles + blockIdx.x] = h[t];
 }
 
-// Exclusive scan of the digit-major table (RADIX * ntiles entries) by one
-// workgroup.  The table is streamed through LDS in chunks of 16K entries: a
-// coalesced load, a per-thread scan of 16 consecutive entries, a workgroup
-// scan of the thread totals, and a coalesced write-back.
-constexpr int SCAN_T = 1024, SCAN_K = 16, SCAN_CHUNK = SCAN_T * SCAN_K;
+// Exclusive scan of the digit-major table (RADIX rows of ntiles counts): one
+// workgroup per digit scans its row in place (chunks of 2048 counts, 8 per
+// thread, carried across chunks) and writes the row total; the scatter adds
+// the exclusive scan of the 256 totals itself.
+constexpr int SCAN_T = 256, SCAN_K = 8, SCAN_CHUNK = SCAN_T * SCAN_K;
 
-__global__ __launch_bounds__(SCAN_T) void scan_kernel(uint32_t *__restrict__ counts, uint64_t total)
+__global__ __launch_bounds__(SCAN_T) void digit_scan_kernel(uint32_t *__restrict__ counts, uint32_t ntiles,
+                                                            uint32_t *__restrict__ totals)
 {
-    __shared__ uint32_t buf[SCAN_CHUNK];
     __shared__ uint32_t part[SCAN_T];
-    __shared__ uint32_t carry;
     const int t = threadIdx.x;
-    if (t == 0) carry = 0;
-    for (uint64_t base = 0; base < total; base += SCAN_CHUNK) {
-#pragma unroll
-        for (int r = 0; r < SCAN_K; ++r) {
-            const uint64_t x = base + (uint64_t)r * SCAN_T + t;
-            buf[r * SCAN_T + t] = x < total ? counts[x] : 0u;
-        }
-        __syncthreads();
+    uint32_t *row = counts + (uint64_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += SCAN_CHUNK) {
         uint32_t v[SCAN_K], s = 0;
 #pragma unroll
         for (int r = 0; r < SCAN_K; ++r) {
-            v[r] = buf[t * SCAN_K + r];
+            const uint32_t x = base + (uint32_t)t * SCAN_K + r;
+            v[r] = x < ntiles ? row[x] : 0u;
             s += v[r];
         }
         part[t] = s;
@@ -169,18 +164,14 @@ __global__ __launch_bounds__(SCAN_T) void scan_kernel(uint32_t *__restrict__ cou
         uint32_t run = carry + part[t] - s;
 #pragma unroll
         for (int r = 0; r < SCAN_K; ++r) {
-            buf[t * SCAN_K + r] = run;
+            const uint32_t x = base + (uint32_t)t * SCAN_K + r;
+            if (x < ntiles) row[x] = run;
             run += v[r];
         }
-        __syncthreads();
-        if (t == SCAN_T - 1) carry += part[SCAN_T - 1];
-#pragma unroll
-        for (int r = 0; r < SCAN_K; ++r) {
-            const uint64_t x = base + (uint64_t)r * SCAN_T + t;
-            if (x < total) counts[x] = buf[r * SCAN_T + t];
-        }
+        carry += part[SCAN_T - 1];
         __syncthreads();
     }
+    if (t == 0) totals[blockIdx.x] = carry;
 }
 
 __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *__restrict__ kin,
@@ -188,14 +179,16 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *_
                                                               uint64_t *__restrict__ kout,
                                                               uint32_t *__restrict__ vout, uint64_t n,
                                                               int shift, const uint32_t *__restrict__ offs,
-                                                              uint32_t ntiles)
+                                                              uint32_t ntiles, const uint32_t *__restrict__ totals)
 {
     __shared__ uint64_t sk[SORT_TILE];
     __shared__ uint32_t sv[SORT_TILE];
     __shared__ uint32_t wcnt[SORT_THREADS / 64][RADIX];
     __shared__ uint32_t run[RADIX];
     __shared__ uint32_t start[RADIX];
+    __shared__ uint32_t dbase[RADIX];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    dbase[t] = totals[t];
     const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
     const uint64_t lane_lt = (1ull << lane) - 1ull;
     uint64_t key[SORT_ITEMS];
@@ -237,20 +230,24 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *_
         for (int w = 0; w < SORT_THREADS / 64; ++w) wcnt[w][t] = 0;
         __syncthreads();
     }
-    // exclusive scan of the tile's digit counts -> start of each digit run
+    // exclusive scans of the tile's digit counts (start of each digit run in
+    // the tile) and of the global digit totals (start of each digit's output)
     {
-        const uint32_t c = run[t];
+        const uint32_t c = run[t], g = dbase[t];
         start[t] = c;
         __syncthreads();
         for (int off = 1; off < RADIX; off <<= 1) {
             const uint32_t v = t >= off ? start[t - off] : 0u;
+            const uint32_t w = t >= off ? dbase[t - off] : 0u;
             __syncthreads();
             start[t] += v;
+            dbase[t] += w;
             __syncthreads();
         }
-        const uint32_t incl = start[t];
+        const uint32_t incl = start[t], gincl = dbase[t];
         __syncthreads();
         start[t] = incl - c;
+        dbase[t] = gincl - g;
         __syncthreads();
     }
 #pragma unroll
@@ -267,7 +264,7 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *_
     for (uint32_t p = t; p < tile_n; p += SORT_THREADS) {
         const uint64_t k = sk[p];
         const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
-        const uint64_t g = (uint64_t)offs[(uint64_t)d * ntiles + blockIdx.x] + (p - start[d]);
+        const uint64_t g = (uint64_t)dbase[d] + offs[(uint64_t)d * ntiles + blockIdx.x] + (p - start[d]);
         kout[g] = k;
         vout[g] = sv[p];
     }
@@ -412,15 +409,16 @@ int radix_sort_kv(mh_context *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uin
     if (n == 0) return MH_OK;
     const uint32_t ntiles = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
     int rc;
-    if ((rc = ctx->sort_tmp[2].ensure((uint64_t)RADIX * ntiles * 4 + 64)) != MH_OK) return rc;
+    if ((rc = ctx->sort_tmp[2].ensure((uint64_t)RADIX * ntiles * 4 + RADIX * 4 + 64)) != MH_OK) return rc;
     uint32_t *counts = ctx->sort_tmp[2].as<uint32_t>();
+    uint32_t *totals = counts + (uint64_t)RADIX * ntiles;
     const int passes = (bits + 7) / 8;
     for (int pass = 0; pass < passes; ++pass) {
         const int shift = 8 * pass;
         hipLaunchKernelGGL(hist_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, n, shift, counts, ntiles);
-        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(SCAN_T), 0, ctx->stream, counts, (uint64_t)RADIX * ntiles);
+        hipLaunchKernelGGL(digit_scan_kernel, dim3(RADIX), dim3(SCAN_T), 0, ctx->stream, counts, ntiles, totals);
         hipLaunchKernelGGL(scatter_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, v0, k1, v1, n, shift,
-                           counts, ntiles);
+                           counts, ntiles, totals);
         std::swap(k0, k1);
         std::swap(v0, v1);
     }

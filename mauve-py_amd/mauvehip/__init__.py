@@ -372,9 +372,7 @@ def chain_gap_jobs(starts, lengths, len_a, len_b, max_gap=20000):
         m["length"][:n] = lengths
     chain = np.zeros(max(n, 1), dtype=np.uint32)
     nc, nj = C.c_uint64(), C.c_uint64()
-    _check(L.mh_chain_gap_jobs(m.ctypes.data, n, len_a, len_b, max_gap, chain.ctypes.data, C.byref(nc), None, 0,
-                               C.byref(nj)))
-    jobs = np.zeros(max(nj.value, 1), dtype=DP_JOB_DTYPE)
+    jobs = np.empty(n + 1, dtype=DP_JOB_DTYPE)    # a chain of k anchors has at most k + 1 gaps
     _check(L.mh_chain_gap_jobs(m.ctypes.data, n, len_a, len_b, max_gap, chain.ctypes.data, C.byref(nc),
-                               jobs.ctypes.data, nj.value, C.byref(nj)))
+                               jobs.ctypes.data, n + 1, C.byref(nj)))
     return chain[:nc.value].copy(), jobs[:nj.value].copy()

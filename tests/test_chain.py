@@ -1,0 +1,67 @@
+"""The anchor-chain gap-job producer (host code, no GPU).
+
+mh_chain_gap_jobs has no counterpart in the reference: it stands in for the
+LCB / anchored profile-profile job producer so the path runs end to end.  Its
+contract is checked against an O(n^2) chain DP on small random match sets:
+the chain weight is the maximum over collinear, non-overlapping
+forward-strand chains, consecutive anchors are ordered and disjoint in both
+genomes, and the jobs are exactly the non-empty gaps of at most max_gap on
+both sides.
+"""
+import numpy as np
+
+import mauvehip
+
+
+def _brute_best(s0, s1, ln):
+    order = np.lexsort((np.arange(len(s0)), s1, s0))
+    best = {}
+    for i in order:
+        b = ln[i]
+        for j in order:
+            if j in best and s0[j] + ln[j] <= s0[i] and s1[j] + ln[j] <= s1[i]:
+                b = max(b, best[j] + ln[i])
+        best[i] = b
+    return max(best.values()) if best else 0
+
+
+def _check(s0, s1, ln, La, Lb, max_gap):
+    starts = np.stack([s0 + 1, s1 + 1], 1).astype(np.int64)
+    chain, jobs = mauvehip.chain_gap_jobs(starts, ln.astype(np.uint64), La, Lb, max_gap)
+    w = int(ln[chain].sum()) if len(chain) else 0
+    assert w == _brute_best(s0, s1, ln)
+    pa = pb = 0
+    want = []
+    for q in chain:
+        assert s0[q] >= pa and s1[q] >= pb
+        ga, gb = s0[q] - pa, s1[q] - pb
+        if 0 < ga <= max_gap and 0 < gb <= max_gap:
+            want.append((pa, pb, ga, gb))
+        pa, pb = s0[q] + ln[q], s1[q] + ln[q]
+    ga, gb = La - pa, Lb - pb
+    if 0 < ga <= max_gap and 0 < gb <= max_gap:
+        want.append((pa, pb, ga, gb))
+    got = [(int(j["a_off"]), int(j["b_off"]), int(j["a_len"]), int(j["b_len"])) for j in jobs]
+    assert got == want
+
+
+def test_chain_matches_brute_force():
+    rng = np.random.default_rng(5)
+    for trial in range(60):
+        n = int(rng.integers(0, 40))
+        La, Lb = 2000, 2100
+        ln = rng.integers(5, 60, size=n)
+        s0 = np.array([rng.integers(0, La - l) for l in ln], dtype=np.int64)
+        if trial % 2:   # mostly collinear, like real anchors
+            s1 = np.clip(s0 + rng.integers(-30, 30, size=n), 0, None)
+            s1 = np.minimum(s1, Lb - ln)
+        else:
+            s1 = np.array([rng.integers(0, Lb - l) for l in ln], dtype=np.int64)
+        _check(s0, s1, ln, La, Lb, max_gap=int(rng.choice([50, 500, 5000])))
+
+
+def test_chain_empty_and_reverse_strand_ignored():
+    starts = np.array([[10, -50], [-5, 20]], dtype=np.int64)
+    chain, jobs = mauvehip.chain_gap_jobs(starts, np.array([20, 20], dtype=np.uint64), 100, 100, 5000)
+    assert len(chain) == 0
+    assert [(int(j["a_off"]), int(j["b_off"]), int(j["a_len"]), int(j["b_len"])) for j in jobs] == [(0, 0, 100, 100)]

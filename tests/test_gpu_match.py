@@ -80,11 +80,11 @@ def test_match_bucket_alias(ctx, seed):
     ref[30000:30050] = ref[10000:10050]           # a repeat inside genome 0
     mid = synth.BASES[rng.integers(0, 4, size=40000)]
     mut = np.concatenate([ref[:60000], mid, ref[60000:]])
-    mut[80000:86000] = synth.revcomp(mut[80000:86000])
+    mut[20000:26000] = synth.revcomp(mut[20000:26000])
     hit = rng.random(len(mut)) < 0.01
     mut[hit] = synth.BASES[(np.searchsorted(synth.BASES, mut[hit]) + 1) % 4]
     n = _compare(ctx, ref.tobytes(), mut.tobytes(), seed)
-    assert n > 100
+    assert n > 20
 
 
 def test_match_repeat_limit_refused(ctx):
