@@ -165,11 +165,43 @@ int mh_dp_align(mh_context *ctx, const uint8_t *a_codes, uint64_t a_bytes,
                 uint8_t *paths, mh_dp_result *results);
 
 /* Device-resident variant: d_a, d_b, d_paths, d_results are device pointers;
- * jobs is a host array (the scheduler reads it).  Runs on the context stream
- * and returns without synchronising. */
+ * jobs is a host array (the scheduler reads it).  Runs on the context stream;
+ * results are complete when it returns. */
 int mh_dp_align_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
                        const mh_dp_job *jobs, uint64_t njobs,
                        uint8_t *d_paths, mh_dp_result *d_results);
+
+/* Banded mode (no counterpart in the reference, whose NWSmall always fills
+ * the whole matrix; the full DP stays the default and the judge of it).
+ * bands[j] = W > 0 restricts job j to the stripes' rows that meet the band
+ * |j - i*b_len/a_len| <= W around the corner-to-corner line (a superset of
+ * the band, see mh_dp_stripe_width); W = 0 or bands == NULL is the full DP.
+ * The result equals the full DP's whenever the full DP's path keeps within
+ * distance < W of the line.  Edge fallback: a job whose banded path comes
+ * within distance W of the line's band edge (d_dev[j] >= W) is rerun with 2W,
+ * and if it reaches that edge too, with the full DP, unless flags has
+ * MH_BAND_NO_FALLBACK.  d_dev (device, nullable) receives each final path's
+ * largest distance from the line, ceil(max |i*b_len - j*a_len| / a_len) over
+ * its positions (i, j): for full jobs this is the band label. */
+#define MH_BAND_NO_FALLBACK 1u
+typedef struct {
+    uint64_t banded;          /* jobs run with a band in the first pass        */
+    uint64_t rerun_doubled;   /* jobs rerun with the band doubled              */
+    uint64_t rerun_full;      /* jobs rerun with the full DP                   */
+    uint64_t cells;           /* DP cells evaluated over all passes            */
+} mh_band_stats;
+int mh_dp_align_banded_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
+                              const mh_dp_job *jobs, uint64_t njobs, const uint32_t *bands,
+                              uint32_t flags, uint8_t *d_paths, mh_dp_result *d_results,
+                              uint32_t *d_dev, mh_band_stats *stats);
+
+/* The shape the scheduler gives a job: returns the effective band (0 when the
+ * band's region is the whole matrix) and writes the stripe width w (columns).
+ * A banded job evaluates, in each stripe s (columns c0 = s*w+1 ..
+ * c1 = min((s+1)*w, b_len)), the rows i with
+ * ceil(a_len*(c0-W)/b_len) <= i <= floor(a_len*(c1+W)/b_len), clipped to
+ * [1, a_len]. */
+uint32_t mh_dp_band_shape(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *stripe_w);
 
 /* ASCII genome (device) -> letter codes 0..3 (A,C,G,T, either case) for the DP.
  * Any other byte is refused with MH_ERR_UNSUPPORTED (the DP implements the

@@ -33,6 +33,7 @@
 #include <cstdlib>
 #include <utility>
 #include <type_traits>
+#include <cstring>
 
 namespace mh {
 namespace {
@@ -44,17 +45,62 @@ constexpr int SBIAS = 65;           // smallest NUC_SP entry is -125 + 60 = -65
 struct DevJob {
     uint64_t a_off, b_off;
     uint32_t la, lb;
-    uint64_t tb_off;    // 32-bit words
+    uint64_t tb_off;    // bytes
     uint64_t bnd_off;   // int32 elements
     uint64_t path_off;  // bytes (slot start)
     uint32_t user;      // index in the caller's job array
-    uint32_t cols;      // columns per lane (8 or 16): stripe width 64*cols
+    uint32_t cols;      // columns per lane (4, 8 or 16): stripe width 64*cols
+    uint32_t band;      // 0: the whole matrix; W > 0: the stripes' rows that meet |j - i*lb/la| <= W
+    uint32_t pad_;
 };
 
 struct DpOut {
     int32_t score;
     int32_t edge;       // 'M', 'D' or 'I'
 };
+
+}  // namespace
+
+// Rows [r0, r1] (1-based) that stripe s (columns s*sw+1 .. min((s+1)*sw, lb))
+// computes.  Full DP: every row.  Band W: the rows where some column of the
+// stripe lies within W of the corner-to-corner line j = i*lb/la, i.e.
+// la*(c0-W)/lb <= i <= la*(c1+W)/lb; empty when r1 < r0.
+__host__ __device__ inline void stripe_rows(uint32_t la, uint32_t lb, uint32_t band, uint32_t sw, uint32_t s,
+                                            int &r0, int &r1)
+{
+    if (band == 0) {
+        r0 = 1;
+        r1 = (int)la;
+        return;
+    }
+    const int64_t c0 = (int64_t)s * sw + 1;
+    const int64_t c1e = (int64_t)(s + 1) * sw;
+    const int64_t c1 = c1e < (int64_t)lb ? c1e : (int64_t)lb;
+    const int64_t lo = c0 - (int64_t)band;
+    int64_t a0 = lo <= 0 ? 1 : ((int64_t)la * lo + lb - 1) / lb;
+    if (a0 < 1) a0 = 1;
+    int64_t a1 = ((int64_t)la * (c1 + (int64_t)band)) / lb;
+    if (a1 > (int64_t)la) a1 = la;
+    r0 = (int)a0;
+    r1 = (int)a1;
+}
+
+// Trace bytes of one job: per non-empty stripe (rows + 63) steps of 64 lanes
+// x cols/2 bytes.
+uint64_t dp_trace_bytes(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols)
+{
+    const uint32_t sw = 64 * cols;
+    const uint32_t ns = (lb + sw - 1) / sw;
+    uint64_t steps = 0;
+    for (uint32_t s = 0; s < ns; ++s) {
+        int r0, r1;
+        stripe_rows(la, lb, band, sw, s, r0, r1);
+        if (r1 >= r0) steps += (uint64_t)(r1 - r0 + 1) + 63;
+    }
+    return steps * 64 * (cols / 2);
+}
+
+namespace {
 
 __device__ __forceinline__ int shr1_old(int v, int old)
 {
@@ -83,21 +129,25 @@ __device__ __forceinline__ int max3i(int a, int b, int c)
 // its own and the left column), then I left to right (it chains along the
 // row).  Both update the registers in place.
 //
-// Lanes outside their row range sit out the step (exec mask).  Trace bits are the
-// reference's 4-bit code (MU/types.h:26-46): bits 0-1 the M source (MM=0,
-// DM=1, IM=2), bit 2 MD, bit 3 MI; COLS/8 32-bit words per lane per step, so
-// a wavefront stores 64*COLS/2 contiguous bytes per step.  Cell (i, j) of lane
-// k, column c is in word (stripe*(LA+63) + i-1+k)*64 + k)*COLS/8 + c/8, bits
-// 4*(c%8) .. +3.
+// Stripe s computes rows [r0, r1] (stripe_rows); at step t lane k is on row
+// r0 + t - k.  Cells outside those rows are outside the computed region and
+// read as NEG, so a banded job is the exact DP restricted to that region;
+// with band 0 every stripe runs all rows.  Lanes outside their row range sit
+// out the step (exec mask).  Trace bits are the reference's 4-bit code
+// (MU/types.h:26-46): bits 0-1 the M source (MM=0, DM=1, IM=2), bit 2 MD,
+// bit 3 MI; each lane stores COLS/2 bytes per step, so a wavefront stores
+// 32*COLS contiguous bytes per step.  Cell (i, j) of lane k, column c is at
+// byte  stripe_base + ((i - r0 + k)*64 + k)*COLS/2 + c/2, nibble c%2.
 template <int COLS>
 __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restrict__ A,
-                                       const uint8_t *__restrict__ B, uint32_t *__restrict__ tb,
+                                       const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
                                        int *__restrict__ bnd, DpOut *__restrict__ out, uint32_t sr0,
                                        uint32_t sr1, uint32_t sr2, uint32_t sr3, int goh, int (*lds_in)[64],
                                        int (*lds_out)[64])
 {
     constexpr int STRIPE_W = LANES * COLS;
-    constexpr int NW = COLS / 8;           // trace words per lane per step
+    constexpr int NW = COLS >= 8 ? COLS / 8 : 1;   // 32-bit trace words per lane per step
+    constexpr int LANE_BYTES = COLS / 2;
     const int lane = threadIdx.x;
     const int la = (int)jb.la, lb = (int)jb.lb;
     const uint8_t *a = A + jb.a_off;
@@ -106,14 +156,24 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
     int *bM = bnd + jb.bnd_off;
     int *bD = bM + nr;
     int *bI = bD + nr;
-    uint32_t *tbj = tb + jb.tb_off;
+    uint8_t *tbj = tb + jb.tb_off;
     const int nstripes = (lb + STRIPE_W - 1) / STRIPE_W;
     const int fin_lane = ((lb - 1) % STRIPE_W) / COLS;
     const int fin_col = (lb - 1) % COLS;
     const int go2 = 2 * goh;
     int finM = NEG, finD = NEG, finI = NEG;
+    int prev_r0 = 1, prev_r1 = 0;     // rows of the previous stripe's last column held in bnd
+    uint64_t tbs = 0;                 // byte offset of the stripe's trace block
 
     for (int s = 0; s < nstripes; ++s) {
+        int r0, r1;
+        stripe_rows(jb.la, jb.lb, jb.band, STRIPE_W, s, r0, r1);
+        if (r1 < r0) {
+            prev_r0 = 1;
+            prev_r1 = 0;
+            continue;
+        }
+        const int nrows = r1 - r0 + 1;
         const int j0 = s * STRIPE_W + 1;
         const int jl = j0 + lane * COLS;
         const int rem = lb - (jl - 1);
@@ -121,29 +181,56 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
         const int span = lb - s * STRIPE_W;
         const int nl = span >= STRIPE_W ? LANES : (span + COLS - 1) / COLS;
         const bool last_stripe = (s == nstripes - 1);
+        const bool top0 = (r0 == 1);
 
         uint32_t bsh[COLS];
 #pragma unroll
         for (int c = 0; c < COLS; ++c) bsh[c] = (c < ncols) ? 8u * (uint32_t)b[jl - 1 + c] : 0u;
 
-        // row-0 state: M = NEG, D' = NEG + goh, I' = 0 + goh; left column likewise,
-        // or column 0 (M(0,0) = 0, D' = I' = NEG + goh) for the first lane
-        const int iM0 = jl == 1 ? 0 : NEG, iI0 = jl == 1 ? NEG + goh : goh;
+        // the row above the stripe: virtual row 0 (M = NEG, D' = NEG + goh,
+        // I' = 0 + goh) when the stripe starts at row 1, else outside (NEG)
         int Mp[COLS], Dp[COLS], Ip[COLS];
 #pragma unroll
-        for (int c = 0; c < COLS; ++c) { Mp[c] = NEG; Dp[c] = NEG + goh; Ip[c] = goh; }
-        int lpM = iM0, lpD = NEG + goh, lpI = iI0;
+        for (int c = 0; c < COLS; ++c) {
+            Mp[c] = NEG;
+            Dp[c] = top0 ? NEG + goh : NEG;
+            Ip[c] = top0 ? goh : NEG;
+        }
+        // the cell above-left of the lane's first column: M(0,0) = 0 for column 1,
+        // virtual row 0 for other columns of row 0, the previous stripe's last
+        // column (row r0-1) for lane 0 when it holds that row, else outside
+        int lpM = NEG, lpD = NEG, lpI = NEG;
+        if (top0) {
+            lpM = jl == 1 ? 0 : NEG;
+            lpD = NEG + goh;
+            lpI = jl == 1 ? NEG + goh : goh;
+        } else if (lane == 0 && s > 0 && r0 - 1 >= prev_r0 && r0 - 1 <= prev_r1) {
+            lpM = bM[r0 - 2];
+            lpD = bD[r0 - 2];
+            lpI = bI[r0 - 2];
+        }
         int outM = NEG, outD = NEG, outI = NEG, acode = 0;
         const int ob2_0 = (jl == 1) ? goh : go2;      // open_B[j-1] + goh for the lane's first column
 
         // lane 0's inputs, 64 rows per chunk: column jl-1 of the previous stripe
-        // (or column 0: M = NEG, D' = 0 + goh, I' = NEG + goh) and the A codes;
-        // the next chunk is loaded one chunk ahead
-        int nM = NEG, nD = goh, nI = NEG + goh, nA = 0;
-        if (s > 0 && lane < la) { nM = bM[lane]; nD = bD[lane]; nI = bI[lane]; }
-        if (lane < la) nA = a[lane];
+        // where it computed that row (else NEG), or column 0 (M = NEG,
+        // D' = 0 + goh, I' = NEG + goh); and the A codes.  The next chunk is
+        // loaded one chunk ahead.
+        auto left_in = [&](int x, int &m, int &d, int &ii, int &ac) {
+            const int i = r0 + x;
+            if (s == 0) {
+                m = NEG; d = goh; ii = NEG + goh;
+            } else if (i >= prev_r0 && i <= prev_r1) {
+                m = bM[i - 1]; d = bD[i - 1]; ii = bI[i - 1];
+            } else {
+                m = NEG; d = NEG; ii = NEG;
+            }
+            ac = i <= r1 ? a[i - 1] : 0;
+        };
+        int nM, nD, nI, nA;
+        left_in(lane, nM, nD, nI, nA);
 
-        const int nsteps = la + nl - 1;
+        const int nsteps = nrows + nl - 1;
         for (int t = 0; t < nsteps; ++t) {
             const int q = t & 63;
             if (q == 0) {
@@ -151,10 +238,7 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
                 lds_in[1][lane] = nD;
                 lds_in[2][lane] = nI;
                 lds_in[3][lane] = nA;
-                const int r = t + 64 + lane;
-                nM = NEG; nD = goh; nI = NEG + goh; nA = 0;
-                if (s > 0 && r < la) { nM = bM[r]; nD = bD[r]; nI = bI[r]; }
-                if (r < la) nA = a[r];
+                left_in(t + 64 + lane, nM, nD, nI, nA);
             }
             const int rM = shr1_old(outM, lds_in[0][q]);
             const int rD = shr1_old(outD, lds_in[1][q]);
@@ -162,8 +246,8 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
             const int ra = shr1_old(acode, lds_in[3][q]);
             acode = ra;
 
-            const int i = t - lane + 1;
-            if (i >= 1 && i <= la && lane < nl) {
+            const int i = r0 + t - lane;
+            if (i >= r0 && i <= r1 && lane < nl) {
             const uint32_t srow = (ra & 2) ? ((ra & 1) ? sr3 : sr2) : ((ra & 1) ? sr1 : sr0);
             const int oa2 = (i == 1) ? goh : go2;   // open_A[i-1] + goh
             uint32_t word[NW];
@@ -210,11 +294,13 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
             lpM = rM;
             lpD = rD;
             lpI = rI;
-            uint32_t *dst = tbj + (((size_t)s * (size_t)(la + 63) + (size_t)t) * 64 + lane) * NW;
-            if constexpr (NW == 2) {
+            uint8_t *dst = tbj + tbs + ((size_t)t * 64 + lane) * LANE_BYTES;
+            if constexpr (COLS == 16) {
                 *reinterpret_cast<uint2 *>(dst) = make_uint2(word[0], word[1]);
+            } else if constexpr (COLS == 8) {
+                *reinterpret_cast<uint32_t *>(dst) = word[0];
             } else {
-                dst[0] = word[0];
+                *reinterpret_cast<uint16_t *>(dst) = (uint16_t)word[0];
             }
             if (last_stripe && i == la && lane == fin_lane) {
 #pragma unroll
@@ -223,20 +309,28 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
             }
             }
             if (!last_stripe) {
-                // lane 63 just finished row t-62 of the stripe's last column
-                const int i63 = t - 62;
-                if (lane == 63 && i63 >= 1 && i63 <= la) {
-                    const int slot = (i63 - 1) & 63;
+                // lane 63 just finished relative row t-63 of the stripe's last column
+                const int x63 = t - 63;
+                if (lane == 63 && x63 >= 0 && x63 < nrows) {
+                    const int slot = x63 & 63;
                     lds_out[0][slot] = outM;
                     lds_out[1][slot] = outD;
                     lds_out[2][slot] = outI;
                 }
-                if (i63 >= 1 && i63 <= la && (((i63 - 1) & 63) == 63 || i63 == la)) {
-                    const int r = ((i63 - 1) & ~63) + lane;
-                    if (r < la) { bM[r] = lds_out[0][lane]; bD[r] = lds_out[1][lane]; bI[r] = lds_out[2][lane]; }
+                if (x63 >= 0 && x63 < nrows && ((x63 & 63) == 63 || x63 == nrows - 1)) {
+                    const int x = (x63 & ~63) + lane;
+                    if (x < nrows) {
+                        const int r = r0 - 1 + x;
+                        bM[r] = lds_out[0][lane];
+                        bD[r] = lds_out[1][lane];
+                        bI[r] = lds_out[2][lane];
+                    }
                 }
             }
         }
+        prev_r0 = r0;
+        prev_r1 = r1;
+        tbs += (uint64_t)(nrows + 63) * 64 * LANE_BYTES;
         __threadfence();
     }
     if (lane == fin_lane) {
@@ -252,7 +346,7 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
 }
 
 __global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs, const uint8_t *__restrict__ A,
-                                                 const uint8_t *__restrict__ B, uint32_t *__restrict__ tb,
+                                                 const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
                                                  int *__restrict__ bnd, DpOut *__restrict__ outs, uint32_t sr0,
                                                  uint32_t sr1, uint32_t sr2, uint32_t sr3, int goh)
 {
@@ -261,8 +355,10 @@ __global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs
     const DevJob jb = jobs[blockIdx.x];
     if (jb.cols == 16)
         dp_job<16>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
-    else
+    else if (jb.cols == 8)
         dp_job<8>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+    else
+        dp_job<4>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
 }
 
 __device__ __forceinline__ int xchar(uint32_t bits, int t)
@@ -274,34 +370,55 @@ __device__ __forceinline__ int xchar(uint32_t bits, int t)
 
 // BitTraceBack, MU/bittraceback.cpp:138-209: one thread per job walks the
 // trace bits from (LA, LB) and writes the edges backwards into the job's
-// path slot.
+// path slot.  It also records the path's largest distance from the
+// corner-to-corner line, ceil(max |i*LB - j*LA| / LA) over the visited
+// positions, which is the band label and the edge test of a banded job.
 __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__restrict__ outs,
-                             const uint32_t *__restrict__ tb, uint8_t *__restrict__ paths,
-                             mh_dp_result *__restrict__ res, int njobs)
+                             const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
+                             mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev, int njobs)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= njobs) return;
     const DevJob jb = jobs[x];
     const int la = (int)jb.la, lb = (int)jb.lb;
-    const uint32_t *t = tb + jb.tb_off;
+    const int cols = (int)jb.cols, sw = 64 * cols, lane_bytes = cols / 2;
+    const uint8_t *tj = tb + jb.tb_off;
     uint8_t *slot = paths + jb.path_off;
+    // trace block of the last stripe: the sum over the stripes before it
+    int cur = (lb - 1) / sw, r0 = 1, r1 = 0;
+    uint64_t base = 0;
+    for (int s = 0; s < cur; ++s) {
+        int a0, a1;
+        stripe_rows(jb.la, jb.lb, jb.band, sw, s, a0, a1);
+        if (a1 >= a0) base += (uint64_t)(a1 - a0 + 64) * 64 * lane_bytes;
+    }
+    stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
     int pos = la + lb;
     int i = la, j = lb;
     int type = outs[x].edge;
     bool ok = true;
+    int64_t maxdev = 0;
     for (;;) {
         slot[--pos] = (uint8_t)type;
         uint32_t nib = 0;
         if (i > 0 && j > 0) {
-            const int cols = (int)jb.cols, wpl = cols / 8, sw = 64 * cols;
-            const int s = (j - 1) / sw, cc = (j - 1) % sw, k = cc / cols, c = cc % cols;
-            const uint32_t w = t[(((size_t)s * (size_t)(la + 63) + (size_t)(i - 1 + k)) * 64 + k) * wpl + (c >> 3)];
-            nib = (w >> (4 * (c & 7))) & 15u;
+            const int s = (j - 1) / sw;
+            while (cur > s) {   // moving left one stripe: step back over its trace block
+                --cur;
+                stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
+                if (r1 >= r0) base -= (uint64_t)(r1 - r0 + 64) * 64 * lane_bytes;
+            }
+            if (i < r0 || i > r1) { ok = false; break; }
+            const int cc = (j - 1) % sw, k = cc / cols, c = cc % cols;
+            const uint8_t byte = tj[base + ((uint64_t)(i - r0 + k) * 64 + k) * lane_bytes + (c >> 1)];
+            nib = (byte >> (4 * (c & 1))) & 15u;
         }
         const int nx = xchar(nib, type);
         if (type == 'M') { if (i == 0 || j == 0) { ok = false; break; } --i; --j; }
         else if (type == 'D') { if (i == 0) { ok = false; break; } --i; }
         else { if (j == 0) { ok = false; break; } --j; }
+        const int64_t d = (int64_t)i * lb - (int64_t)j * la;
+        maxdev = max(maxdev, d < 0 ? -d : d);
         if (i == 0 && j == 0) break;
         if (pos == 0 || nx == '?') { ok = false; break; }
         type = nx;
@@ -311,6 +428,7 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
     r.path_len = ok ? (uint32_t)(la + lb - pos) : 0u;
     r.path_start = jb.path_off + (uint64_t)pos;
     res[jb.user] = r;
+    if (dev) dev[jb.user] = ok ? (uint32_t)((maxdev + la - 1) / la) : 0xFFFFFFFFu;
 }
 
 // NUC_SP (MU/nucmx.cpp:8-25) rows packed as biased bytes: row a, column b at
@@ -329,44 +447,73 @@ void packed_rows(uint32_t r[4])
 
 }  // namespace
 
-int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
-           uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results)
+namespace {
+
+struct DpItem {
+    uint32_t job;     // index in the caller's job array
+    uint32_t band;    // 0 = full matrix
+    uint32_t cols;
+    uint64_t tb_bytes;
+};
+
+// Relative cost per computed cell of the three stripe shapes (narrow stripes
+// spend more of each step on the row hand-off); used to pick a banded job's
+// shape, which decides how much of each stripe lies outside the band.
+constexpr double COST_PER_CELL[3] = {1.30, 1.10, 1.00};   // cols 4, 8, 16
+
+int force_cols()
 {
-    if (njobs == 0) return MH_OK;
-    MH_CHECK(njobs < (1ull << 31), MH_ERR_INVALID, "too many DP jobs in one call");
-    std::vector<uint64_t> slot(njobs);
-    uint64_t acc = 0;
-    for (uint64_t j = 0; j < njobs; ++j) {
-        MH_CHECK(jobs[j].a_len > 0 && jobs[j].b_len > 0, MH_ERR_ZERO_LENGTH,
-                 "NWSmall: length=0 (job " + std::to_string(j) + ")");
-        MH_CHECK(jobs[j].a_len < (1u << 24) && jobs[j].b_len < (1u << 24), MH_ERR_UNSUPPORTED,
-                 "DP operand longer than 2^24");
-        slot[j] = acc;
-        acc += (uint64_t)jobs[j].a_len + jobs[j].b_len;
+    if (const char *e = getenv("MH_DP_COLS")) {
+        const int c = atoi(e);
+        if (c == 4 || c == 8 || c == 16) return c;
     }
-    // largest jobs first so the long tails start early
-    std::vector<uint32_t> order(njobs);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        return (uint64_t)jobs[x].a_len * jobs[x].b_len > (uint64_t)jobs[y].a_len * jobs[y].b_len;
-    });
-    // columns per lane: 8 (512-column stripes) for narrow jobs, 16 (1024) for
-    // wide ones; MH_DP_COLS=8|16 forces one shape (used for tuning sweeps)
-    int force_cols = 0;
-    if (const char *e = getenv("MH_DP_COLS")) force_cols = atoi(e);
-    auto cols_of = [&](uint32_t j) -> uint32_t {
-        if (force_cols == 8 || force_cols == 16) return (uint32_t)force_cols;
-        return jobs[j].b_len > 512 ? 16u : 8u;
-    };
-    auto tb_words = [&](uint32_t j) {
-        const uint32_t c = cols_of(j);
-        const uint64_t ns = (jobs[j].b_len + 64 * c - 1) / (64 * c);
-        return ns * (uint64_t)(jobs[j].a_len + 63) * 64 * (c / 8);
-    };
-    auto bnd_ints = [&](uint32_t j) {
-        const uint32_t c = cols_of(j);
-        const uint64_t ns = (jobs[j].b_len + 64 * c - 1) / (64 * c);
-        return ns > 1 ? 3ull * ((jobs[j].a_len + 63) & ~63ull) : 0ull;
+    return 0;
+}
+
+}  // namespace
+
+uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band)
+{
+    const int fc = force_cols();
+    if (fc) return (uint32_t)fc;
+    if (band == 0) return lb > 512 ? 16u : 8u;
+    uint32_t best = 16;
+    double best_cost = 0;
+    for (int q = 0; q < 3; ++q) {
+        const uint32_t c = 4u << q;
+        const double cost = (double)dp_trace_bytes(la, lb, band, c) * COST_PER_CELL[q];
+        if (q == 0 || cost < best_cost) {
+            best = c;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
+// A band whose region is the whole matrix is the full DP.
+uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols)
+{
+    if (band == 0) return 0;
+    const uint32_t sw = 64 * cols;
+    const uint32_t ns = (lb + sw - 1) / sw;
+    int f0, f1, l0, l1;
+    stripe_rows(la, lb, band, sw, 0, f0, f1);
+    stripe_rows(la, lb, band, sw, ns - 1, l0, l1);
+    return (l0 == 1 && f1 == (int)la) ? 0u : band;
+}
+
+static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
+                        const std::vector<uint64_t> &slot, std::vector<DpItem> &items, uint8_t *d_paths,
+                        mh_dp_result *d_results, uint32_t *d_dev, uint64_t *cells)
+{
+    const uint64_t nitems = items.size();
+    if (nitems == 0) return MH_OK;
+    // largest first so the long tails start early
+    std::stable_sort(items.begin(), items.end(),
+                     [](const DpItem &x, const DpItem &y) { return x.tb_bytes > y.tb_bytes; });
+    auto bnd_ints = [&](const DpItem &it) {
+        const uint64_t ns = (jobs[it.job].b_len + 64 * it.cols - 1) / (64 * it.cols);
+        return ns > 1 ? 3ull * ((jobs[it.job].a_len + 63) & ~63ull) : 0ull;
     };
     // trace arena budget: 70% of the memory that is free or already held by
     // this context's arena (at most 192 GB); once grown the arena is kept
@@ -379,7 +526,7 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
     {
         // size the arena once for the whole call: the largest round fits the budget
         uint64_t need = 0;
-        for (uint64_t j = 0; j < njobs; ++j) need += (tb_words(order[j]) + bnd_ints(order[j])) * 4;
+        for (const DpItem &it : items) need += it.tb_bytes + bnd_ints(it) * 4;
         if (need > budget) need = budget;
         int rc0;
         if (ctx->dp_tb.bytes < need) {
@@ -392,13 +539,13 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
 
     uint64_t first = 0;
     std::vector<DevJob> dj;
-    while (first < njobs) {
+    while (first < nitems) {
         // one round: as many jobs (in schedule order) as fit the trace budget
-        uint64_t tbw = 0, bi = 0, last = first;
-        while (last < njobs) {
-            const uint64_t w = tb_words(order[last]), b = bnd_ints(order[last]);
-            if (last > first && (tbw + w) * 4 + (bi + b) * 4 > budget) break;
-            tbw += w;
+        uint64_t tbb = 0, bi = 0, last = first;
+        while (last < nitems) {
+            const uint64_t w = items[last].tb_bytes, b = bnd_ints(items[last]);
+            if (last > first && tbb + w + (bi + b) * 4 > budget) break;
+            tbb += w;
             bi += b;
             ++last;
         }
@@ -406,7 +553,8 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
         dj.resize(n);
         uint64_t tw = 0, bo = 0;
         for (uint64_t x = 0; x < n; ++x) {
-            const uint32_t j = order[first + x];
+            const DpItem &it = items[first + x];
+            const uint32_t j = it.job;
             DevJob &d = dj[x];
             d.a_off = jobs[j].a_off;
             d.b_off = jobs[j].b_off;
@@ -416,12 +564,15 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
             d.bnd_off = bo;
             d.path_off = slot[j];
             d.user = j;
-            d.cols = cols_of(j);
-            tw += tb_words(j);
-            bo += bnd_ints(j);
+            d.cols = it.cols;
+            d.band = it.band;
+            d.pad_ = 0;
+            tw += (it.tb_bytes + 15) & ~15ull;
+            bo += bnd_ints(it);
+            if (cells) *cells += 2 * it.tb_bytes;
         }
         int rc;
-        if ((rc = ctx->dp_tb.ensure(tw * 4 + 256)) != MH_OK) return rc;
+        if ((rc = ctx->dp_tb.ensure(tw + 256)) != MH_OK) return rc;
         if ((rc = ctx->dp_bnd.ensure(bo * 4 + 256)) != MH_OK) return rc;
         if ((rc = ctx->dp_jobs.ensure(n * sizeof(DevJob) + n * sizeof(DpOut) + 256)) != MH_OK) return rc;
         DevJob *d_jobs = ctx->dp_jobs.as<DevJob>();
@@ -430,20 +581,119 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
         {
             TimedLaunch tl(ctx, "dp_forward");
             hipLaunchKernelGGL(dp_forward, dim3((unsigned)n), dim3(64), 0, ctx->stream, d_jobs, d_a, d_b,
-                               ctx->dp_tb.as<uint32_t>(), ctx->dp_bnd.as<int>(), d_outs, sr[0], sr[1], sr[2],
+                               ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs, sr[0], sr[1], sr[2],
                                sr[3], -200);
             MH_HIP(hipGetLastError());
         }
         {
             TimedLaunch tl(ctx, "dp_traceback");
             hipLaunchKernelGGL(dp_traceback, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ctx->stream, d_jobs,
-                               d_outs, ctx->dp_tb.as<uint32_t>(), d_paths, d_results, (int)n);
+                               d_outs, ctx->dp_tb.as<uint8_t>(), d_paths, d_results, d_dev, (int)n);
             MH_HIP(hipGetLastError());
         }
         // the host job table is reused by the next round
         MH_HIP(hipStreamSynchronize(ctx->stream));
         first = last;
     }
+    return MH_OK;
+}
+
+static int dp_check_jobs(const mh_dp_job *jobs, uint64_t njobs, std::vector<uint64_t> &slot)
+{
+    MH_CHECK(njobs < (1ull << 31), MH_ERR_INVALID, "too many DP jobs in one call");
+    slot.resize(njobs);
+    uint64_t acc = 0;
+    for (uint64_t j = 0; j < njobs; ++j) {
+        MH_CHECK(jobs[j].a_len > 0 && jobs[j].b_len > 0, MH_ERR_ZERO_LENGTH,
+                 "NWSmall: length=0 (job " + std::to_string(j) + ")");
+        MH_CHECK(jobs[j].a_len < (1u << 24) && jobs[j].b_len < (1u << 24), MH_ERR_UNSUPPORTED,
+                 "DP operand longer than 2^24");
+        slot[j] = acc;
+        acc += (uint64_t)jobs[j].a_len + jobs[j].b_len;
+    }
+    return MH_OK;
+}
+
+static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band)
+{
+    DpItem it;
+    it.job = job;
+    it.cols = dp_cols_for(jb.a_len, jb.b_len, band);
+    it.band = dp_effective_band(jb.a_len, jb.b_len, band, it.cols);
+    if (it.band == 0 && band != 0) it.cols = dp_cols_for(jb.a_len, jb.b_len, 0);
+    it.tb_bytes = dp_trace_bytes(jb.a_len, jb.b_len, it.band, it.cols);
+    return it;
+}
+
+int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
+           uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results)
+{
+    if (njobs == 0) return MH_OK;
+    std::vector<uint64_t> slot;
+    int rc;
+    if ((rc = dp_check_jobs(jobs, njobs, slot)) != MH_OK) return rc;
+    std::vector<DpItem> items(njobs);
+    for (uint64_t j = 0; j < njobs; ++j) items[j] = make_item(jobs[j], (uint32_t)j, 0);
+    return dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, nullptr, nullptr);
+}
+
+int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs, uint64_t njobs,
+                  const uint32_t *bands, uint32_t flags, uint8_t *d_paths, mh_dp_result *d_results,
+                  uint32_t *d_dev, mh_band_stats *stats)
+{
+    mh_band_stats st;
+    std::memset(&st, 0, sizeof(st));
+    if (stats) *stats = st;
+    if (njobs == 0) return MH_OK;
+    std::vector<uint64_t> slot;
+    int rc;
+    if ((rc = dp_check_jobs(jobs, njobs, slot)) != MH_OK) return rc;
+    if (!d_dev) {
+        if ((rc = ctx->dp_dev.ensure(njobs * 4 + 64)) != MH_OK) return rc;
+        d_dev = ctx->dp_dev.as<uint32_t>();
+    }
+    std::vector<DpItem> items(njobs);
+    std::vector<uint32_t> band1(njobs, 0);
+    for (uint64_t j = 0; j < njobs; ++j) {
+        items[j] = make_item(jobs[j], (uint32_t)j, bands ? bands[j] : 0u);
+        band1[j] = items[j].band;
+        if (band1[j]) ++st.banded;
+    }
+    if ((rc = dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, d_dev, &st.cells)) != MH_OK)
+        return rc;
+    if (!(flags & MH_BAND_NO_FALLBACK) && st.banded) {
+        // edge fallback: a banded path that reaches the band edge is rerun with
+        // the band doubled, and one that reaches that edge too with the full DP
+        std::vector<uint32_t> dev(njobs);
+        MH_HIP(hipMemcpyAsync(dev.data(), d_dev, njobs * 4, hipMemcpyDeviceToHost, ctx->stream));
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<DpItem> again;
+        std::vector<uint32_t> band2(njobs, 0);
+        for (uint64_t j = 0; j < njobs; ++j)
+            if (band1[j] && dev[j] >= band1[j]) {
+                const uint64_t w2 = 2ull * band1[j];
+                again.push_back(make_item(jobs[j], (uint32_t)j, w2 > 0xFFFFFFFFull ? 0u : (uint32_t)w2));
+                band2[j] = again.back().band;
+                if (band2[j]) ++st.rerun_doubled;
+                else ++st.rerun_full;
+            }
+        if (!again.empty()) {
+            if ((rc = dp_run_items(ctx, d_a, d_b, jobs, slot, again, d_paths, d_results, d_dev, &st.cells)) != MH_OK)
+                return rc;
+            MH_HIP(hipMemcpyAsync(dev.data(), d_dev, njobs * 4, hipMemcpyDeviceToHost, ctx->stream));
+            MH_HIP(hipStreamSynchronize(ctx->stream));
+            std::vector<DpItem> full;
+            for (uint64_t j = 0; j < njobs; ++j)
+                if (band2[j] && dev[j] >= band2[j]) {
+                    full.push_back(make_item(jobs[j], (uint32_t)j, 0));
+                    ++st.rerun_full;
+                }
+            if (!full.empty() &&
+                (rc = dp_run_items(ctx, d_a, d_b, jobs, slot, full, d_paths, d_results, d_dev, &st.cells)) != MH_OK)
+                return rc;
+        }
+    }
+    if (stats) *stats = st;
     return MH_OK;
 }
 

@@ -53,7 +53,7 @@ struct mh_context {
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
     // DP scratch
-    mh::DevBuf dp_tb, dp_bnd, dp_jobs, dp_a, dp_b, dp_paths, dp_res;
+    mh::DevBuf dp_tb, dp_bnd, dp_jobs, dp_a, dp_b, dp_paths, dp_res, dp_dev;
     // seed-matching scratch
     mh::DevBuf mf_scratch[16];
     // scratch for sorting
@@ -97,6 +97,13 @@ int pinned(mh_context *ctx, size_t bytes, void **out);
 // Launchers (return MH_OK or an error code); all run on ctx->stream.
 int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
            uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results);
+// Banded DP with edge fallback (bands null: all full); d_dev may be null.
+int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs, uint64_t njobs,
+                  const uint32_t *bands, uint32_t flags, uint8_t *d_paths, mh_dp_result *d_results,
+                  uint32_t *d_dev, mh_band_stats *stats);
+// Stripe shape (columns per lane) and effective band the scheduler uses.
+uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band);
+uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols);
 
 int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed, mh_sml *s);
 int sml_seed_mers(mh_context *ctx, const mh_sml *s, const uint32_t *d_pos, uint64_t n, uint64_t *d_out);

@@ -322,6 +322,29 @@ int mh_dp_align_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, 
     return dp_run(ctx, d_a, d_b, jobs, njobs, d_paths, d_results);
 }
 
+int mh_dp_align_banded_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
+                              uint64_t njobs, const uint32_t *bands, uint32_t flags, uint8_t *d_paths,
+                              mh_dp_result *d_results, uint32_t *d_dev, mh_band_stats *stats)
+{
+    MH_CHECK(ctx, MH_ERR_INVALID, "null context");
+    MH_CHECK(njobs == 0 || (jobs && d_a && d_b && d_paths && d_results), MH_ERR_INVALID, "null argument");
+    MH_HIP(hipSetDevice(ctx->device));
+    return dp_run_banded(ctx, d_a, d_b, jobs, njobs, bands, flags, d_paths, d_results, d_dev, stats);
+}
+
+uint32_t mh_dp_band_shape(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *stripe_w)
+{
+    if (a_len == 0 || b_len == 0) {
+        if (stripe_w) *stripe_w = 0;
+        return 0;
+    }
+    uint32_t c = dp_cols_for(a_len, b_len, band);
+    const uint32_t eff = band ? dp_effective_band(a_len, b_len, band, c) : 0u;
+    if (eff == 0) c = dp_cols_for(a_len, b_len, 0);
+    if (stripe_w) *stripe_w = 64 * c;
+    return eff;
+}
+
 int mh_dp_align(mh_context *ctx, const uint8_t *a_codes, uint64_t a_bytes, const uint8_t *b_codes,
                 uint64_t b_bytes, const mh_dp_job *jobs, uint64_t njobs, uint8_t *paths,
                 mh_dp_result *results)

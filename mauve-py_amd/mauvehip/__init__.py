@@ -42,6 +42,7 @@ EXPORTS = [
     "mh_device_alloc", "mh_device_free", "mh_memcpy_h2d", "mh_memcpy_d2h",
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free",
+    "mh_dp_align_banded_device", "mh_dp_band_shape",
 ]
 
 
@@ -57,6 +58,14 @@ class DpJob(C.Structure):
 
 class DpResult(C.Structure):
     _fields_ = [("score", C.c_int32), ("path_len", C.c_uint32), ("path_start", C.c_uint64)]
+
+
+class BandStats(C.Structure):
+    _fields_ = [("banded", C.c_uint64), ("rerun_doubled", C.c_uint64), ("rerun_full", C.c_uint64),
+                ("cells", C.c_uint64)]
+
+
+BAND_NO_FALLBACK = 1
 
 
 class Bmer(C.Structure):
@@ -119,6 +128,9 @@ def load_library(path=LIB_PATH):
                                                C.POINTER(MatchStats)]),
         "mh_dp_align": (C.c_int, [vp, vp, u64, vp, u64, vp, u64, vp, vp]),
         "mh_dp_align_device": (C.c_int, [vp, vp, vp, vp, u64, vp, vp]),
+        "mh_dp_align_banded_device": (C.c_int, [vp, vp, vp, vp, u64, vp, C.c_uint32, vp, vp, vp,
+                                                C.POINTER(BandStats)]),
+        "mh_dp_band_shape": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
         "mh_device_alloc": (C.c_int, [vp, u64, C.POINTER(vp)]),
         "mh_device_free": (C.c_int, [vp, vp]),
         "mh_memcpy_h2d": (C.c_int, [vp, vp, vp, u64]),
@@ -262,6 +274,55 @@ def path_of(paths, res, j):
 def global_align_device(ctx, d_a, d_b, jobs, d_paths, d_results):
     jobs = np.ascontiguousarray(jobs, dtype=DP_JOB_DTYPE)
     _check(ctx.L.mh_dp_align_device(ctx.h, d_a, d_b, jobs.ctypes.data, len(jobs), d_paths, d_results))
+
+
+def global_align_banded_device(ctx, d_a, d_b, jobs, bands, d_paths, d_results, d_dev=None, flags=0):
+    """Banded DP with edge fallback on device buffers; bands: uint32 per job
+    (0 = full DP) or None.  Returns BandStats."""
+    jobs = np.ascontiguousarray(jobs, dtype=DP_JOB_DTYPE)
+    bp = None
+    if bands is not None:
+        bands = np.ascontiguousarray(bands, dtype=np.uint32)
+        assert len(bands) == len(jobs)
+        bp = bands.ctypes.data
+    st = BandStats()
+    _check(ctx.L.mh_dp_align_banded_device(ctx.h, d_a, d_b, jobs.ctypes.data, len(jobs), bp, flags, d_paths,
+                                           d_results, d_dev, C.byref(st)))
+    return st
+
+
+def global_align_banded(ctx, a_codes, b_codes, jobs, bands, flags=0):
+    """Host-buffer convenience over global_align_banded_device:
+    (paths, results, dev, stats)."""
+    a = np.ascontiguousarray(a_codes, dtype=np.uint8)
+    b = np.ascontiguousarray(b_codes, dtype=np.uint8)
+    jobs = np.ascontiguousarray(jobs, dtype=DP_JOB_DTYPE)
+    n = len(jobs)
+    total = int((jobs["a_len"].astype(np.uint64) + jobs["b_len"]).sum())
+    bufs = [ctx.alloc(a.nbytes + 16), ctx.alloc(b.nbytes + 16), ctx.alloc(total + 16),
+            ctx.alloc(16 * n + 16), ctx.alloc(4 * n + 16)]
+    try:
+        dA, dB, dP, dR, dD = bufs
+        ctx.h2d(dA, a)
+        ctx.h2d(dB, b)
+        st = global_align_banded_device(ctx, dA, dB, jobs, bands, dP, dR, dD, flags)
+        paths = np.zeros(max(total, 1), dtype=np.uint8)
+        res = np.zeros(n, dtype=DP_RESULT_DTYPE)
+        dev = np.zeros(n, dtype=np.uint32)
+        ctx.d2h(paths, dP)
+        ctx.d2h(res, dR)
+        ctx.d2h(dev, dD)
+    finally:
+        for p in bufs:
+            ctx.free(p)
+    return paths, res, dev, st
+
+
+def band_shape(a_len, b_len, band):
+    """(effective band, stripe width) the scheduler uses for a job."""
+    w = C.c_uint32()
+    eff = load_library().mh_dp_band_shape(int(a_len), int(b_len), int(band), C.byref(w))
+    return int(eff), int(w.value)
 
 
 # ---- sorted mer lists (DNAFileSML / DNAMemorySML) -------------------------

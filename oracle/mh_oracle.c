@@ -1095,3 +1095,133 @@ float or_path_score(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB
     free(GA); free(GB);
     return s;
 }
+
+/* ======================================================================
+ * Banded NWSmall over a stripe region (no reference counterpart; the
+ * recurrences, boundaries and tie rules are MU/nwsmall.cpp's, written with
+ * virtual row/column 0 values)
+ * ====================================================================== */
+
+#define RNEG (-(int64_t)1 << 40)
+
+static void region_rows(uint32_t LA, uint32_t LB, uint32_t band, uint32_t sw, uint32_t s, int64_t *r0, int64_t *r1)
+{
+    if (band == 0) { *r0 = 1; *r1 = LA; return; }
+    int64_t c0 = (int64_t)s * sw + 1, c1 = (int64_t)(s + 1) * sw;
+    if (c1 > LB) c1 = LB;
+    int64_t lo = c0 - band;
+    int64_t a0 = lo <= 0 ? 1 : ((int64_t)LA * lo + LB - 1) / LB;
+    if (a0 < 1) a0 = 1;
+    int64_t a1 = ((int64_t)LA * (c1 + band)) / LB;
+    if (a1 > LA) a1 = LA;
+    *r0 = a0;
+    *r1 = a1;
+}
+
+uint32_t or_path_dev(uint32_t LA, uint32_t LB, const char *path, int64_t plen)
+{
+    int64_t i = 0, j = 0, mx = 0;
+    for (int64_t k = 0; k < plen; ++k) {
+        if (path[k] == 'M') { ++i; ++j; }
+        else if (path[k] == 'D') ++i;
+        else ++j;
+        int64_t d = i * (int64_t)LB - j * (int64_t)LA;
+        if (d < 0) d = -d;
+        if (d > mx) mx = d;
+    }
+    return (uint32_t)((mx + LA - 1) / LA);
+}
+
+int64_t or_nwsmall_region(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
+                          uint32_t band, uint32_t sw, char *path_out, int32_t *score_out, uint32_t *dev_out)
+{
+    if (LA == 0 || LB == 0) return -1;
+    if (sw == 0) sw = LB;
+    const int64_t W = LB + 1;
+    const size_t cells = (size_t)(LA + 1) * (size_t)W;
+    int64_t *M = (int64_t *)malloc(cells * sizeof(int64_t));
+    int64_t *D = (int64_t *)malloc(cells * sizeof(int64_t));
+    int64_t *I = (int64_t *)malloc(cells * sizeof(int64_t));
+    uint8_t *T = (uint8_t *)calloc(cells, 1);
+    int64_t *lo = (int64_t *)malloc((LB + 1) * sizeof(int64_t)), *hi = (int64_t *)malloc((LB + 1) * sizeof(int64_t));
+    for (uint32_t j = 1; j <= LB; ++j) region_rows(LA, LB, band, sw, (j - 1) / sw, &lo[j], &hi[j]);
+#define IX(i, j) ((size_t)(i) * W + (j))
+    /* gap scores of a one-sequence profile with half terminal gaps
+     * (MU/profilefrommsa.cpp:285-292, MU/termgaps.cpp:19-33): open[0] = 0,
+     * close[L-1] = 0 when L > 1, otherwise -200; extend 0 */
+#define OPEN_A(x) ((x) == 0 ? 0 : -200)
+#define OPEN_B(x) ((x) == 0 ? 0 : -200)
+#define CLOSE_A(x) (((x) == LA - 1 && LA > 1) ? 0 : -200)
+#define CLOSE_B(x) (((x) == LB - 1 && LB > 1) ? 0 : -200)
+    /* virtual boundary: M(0,0) = 0; row 0: I(0,j) = 0 (free leading gap),
+     * M = D = -inf; column 0: D(i,0) = 0, M = I = -inf */
+    M[IX(0, 0)] = 0; D[IX(0, 0)] = RNEG; I[IX(0, 0)] = RNEG;
+    for (uint32_t j = 1; j <= LB; ++j) { M[IX(0, j)] = RNEG; D[IX(0, j)] = RNEG; I[IX(0, j)] = 0; }
+    for (uint32_t i = 1; i <= LA; ++i) { M[IX(i, 0)] = RNEG; D[IX(i, 0)] = 0; I[IX(i, 0)] = RNEG; }
+    for (uint32_t i = 1; i <= LA; ++i) {
+        for (uint32_t j = 1; j <= LB; ++j) {
+            const size_t x = IX(i, j);
+            if ((int64_t)i < lo[j] || (int64_t)i > hi[j]) { M[x] = D[x] = I[x] = RNEG; continue; }
+            uint8_t tb = 0;
+            /* RECURSE_D (MU/nwsmall.cpp:68-86): tie -> M */
+            {
+                const int64_t dd = D[IX(i - 1, j)], md = M[IX(i - 1, j)] + OPEN_A(i - 1);
+                if (dd > md) D[x] = dd; else { D[x] = md; tb |= BIT_MD; }
+            }
+            /* RECURSE_I (91-106): tie -> M */
+            {
+                const int64_t ii = I[IX(i, j - 1)], mi = M[IX(i, j - 1)] + OPEN_B(j - 1);
+                if (mi >= ii) { I[x] = mi; tb |= BIT_MI; } else I[x] = ii;
+            }
+            /* RECURSE_M (111-142): M(i,j) = s + max(M, D + close_A[i-2], I + close_B[j-2])
+             * at (i-1, j-1), tie order M, D, I; the one-row case (LA == 1) puts
+             * close_A[0] on M(1,1) with a D source (616-626) */
+            {
+                const int64_t s = (int64_t)NUC_SP[A[i - 1]][B[j - 1]];
+                if (LA == 1 && i == 1 && j == 1) {
+                    M[x] = s + CLOSE_A(0);
+                    tb |= BIT_DM;
+                } else {
+                    const int64_t mm = M[IX(i - 1, j - 1)];
+                    const int64_t dm = D[IX(i - 1, j - 1)] + (i >= 2 ? CLOSE_A(i - 2) : -200);
+                    const int64_t im = I[IX(i - 1, j - 1)] + (j >= 2 ? CLOSE_B(j - 2) : -200);
+                    if (mm >= dm && mm >= im) { M[x] = s + mm; tb |= BIT_MM; }
+                    else if (dm >= mm && dm >= im) { M[x] = s + dm; tb |= BIT_DM; }
+                    else { M[x] = s + im; tb |= BIT_IM; }
+                }
+            }
+            T[x] = tb;
+        }
+    }
+    /* final pick (641-656): M, then D if greater, then I if greater */
+    int64_t sc = M[IX(LA, LB)];
+    char t = 'M';
+    if (D[IX(LA, LB)] > sc) { sc = D[IX(LA, LB)]; t = 'D'; }
+    if (I[IX(LA, LB)] > sc) { sc = I[IX(LA, LB)]; t = 'I'; }
+    int64_t n = 0;
+    if (sc < RNEG / 2) {
+        n = -2;
+    } else {
+        int64_t pa = LA, pb = LB;
+        for (;;) {
+            path_out[n++] = t;
+            const char nx = xchar(T[IX(pa, pb)], t);
+            if (t == 'M') { --pa; --pb; } else if (t == 'D') --pa; else --pb;
+            if (pa == 0 && pb == 0) break;
+            t = nx;
+            if (t == '?' || pa < 0 || pb < 0 || (t == 'M' && (pa == 0 || pb == 0)) || (t == 'D' && pa == 0) ||
+                (t == 'I' && pb == 0)) { n = -2; break; }
+        }
+        if (n > 0)
+            for (int64_t a = 0, b = n - 1; a < b; ++a, --b) { char c = path_out[a]; path_out[a] = path_out[b]; path_out[b] = c; }
+    }
+    if (score_out) *score_out = (int32_t)sc;
+    if (dev_out) *dev_out = n > 0 ? or_path_dev(LA, LB, path_out, n) : 0xFFFFFFFFu;
+#undef IX
+#undef OPEN_A
+#undef OPEN_B
+#undef CLOSE_A
+#undef CLOSE_B
+    free(M); free(D); free(I); free(T); free(lo); free(hi);
+    return n;
+}

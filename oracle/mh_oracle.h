@@ -104,6 +104,23 @@ int64_t or_nwsmall_params(const uint8_t *A, uint32_t LA, const uint8_t *B, uint3
                           float gap_open, float gap_extend,
                           char *path_out, float *score_out);
 
+/* Banded NWSmall: the same DP restricted to the region that the library's
+ * banded mode evaluates -- columns cut into stripes of `stripe_w`; stripe s
+ * (columns c0 = s*w+1 .. c1 = min((s+1)*w, LB)) holds rows
+ * ceil(LA*(c0-W)/LB) .. floor(LA*(c1+W)/LB), clipped to [1, LA]; band 0 is
+ * the whole matrix.  Cells outside the region are -infinity; the virtual
+ * row 0 / column 0 boundary and every tie rule are NWSmall's.  Integer
+ * arithmetic (every operand on this path is an integer).  No counterpart in
+ * the reference.  Returns the path length, -1 on an empty operand, -2 when
+ * the region holds no path.  *dev_out = ceil(max |i*LB - j*LA| / LA) over the
+ * path's positions. */
+int64_t or_nwsmall_region(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
+                          uint32_t band, uint32_t stripe_w,
+                          char *path_out, int32_t *score_out, uint32_t *dev_out);
+
+/* ceil(max |i*LB - j*LA| / LA) over the positions of a path. */
+uint32_t or_path_dev(uint32_t LA, uint32_t LB, const char *path, int64_t plen);
+
 /* Score of a given path under the same scheme (used by tests). */
 float or_path_score(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
                     const char *path, int64_t plen);

@@ -66,6 +66,11 @@ def lib():
         L.or_nwsmall.restype = C.c_int64
         L.or_nwsmall.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_char_p,
                                  C.POINTER(C.c_float)]
+        L.or_nwsmall_region.restype = C.c_int64
+        L.or_nwsmall_region.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_char_p, C.c_void_p, C.c_void_p]
+        L.or_path_dev.restype = C.c_uint32
+        L.or_path_dev.argtypes = [C.c_uint32, C.c_uint32, C.c_char_p, C.c_int64]
         L.or_path_score.restype = C.c_float
         L.or_path_score.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_char_p, C.c_int64]
         _LIB = L
@@ -203,6 +208,28 @@ def nwsmall(a, b):
     if n < 0:
         raise ValueError("NWSmall: length=0")
     return buf.raw[:n].decode(), sc.value
+
+
+def nwsmall_region(a, b, band, stripe_w):
+    """Banded NWSmall over the stripe region (band 0: whole matrix):
+    (path, int score, deviation)."""
+    A = codes(a) if isinstance(a, (str, bytes)) else np.ascontiguousarray(a, dtype=np.uint8)
+    B = codes(b) if isinstance(b, (str, bytes)) else np.ascontiguousarray(b, dtype=np.uint8)
+    buf = C.create_string_buffer(len(A) + len(B) + 1)
+    sc = C.c_int32()
+    dv = C.c_uint32()
+    n = lib().or_nwsmall_region(A.ctypes.data, len(A), B.ctypes.data, len(B), int(band), int(stripe_w), buf,
+                                C.byref(sc), C.byref(dv))
+    if n == -1:
+        raise ValueError("NWSmall: length=0")
+    if n < 0:
+        return None, None, None
+    return buf.raw[:n].decode(), int(sc.value), int(dv.value)
+
+
+def path_dev(la, lb, path):
+    """ceil(max |i*LB - j*LA| / LA) over the path's positions."""
+    return int(lib().or_path_dev(int(la), int(lb), _b(path), len(path)))
 
 
 def path_score(a, b, path):

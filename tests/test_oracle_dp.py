@@ -65,3 +65,90 @@ def test_scores_are_integers_and_sentinel_free():
         p, sc = po.nwsmall(rng.integers(0, 4, la, dtype=np.uint8), rng.integers(0, 4, lb, dtype=np.uint8))
         assert sc == int(sc) and abs(sc) < 1e7
         assert p.count("M") + p.count("D") == la and p.count("M") + p.count("I") == lb
+
+
+# ---- banded region DP (no reference counterpart) ----------------------------
+
+def _region_rows(la, lb, band, sw, s):
+    if band == 0:
+        return 1, la
+    c0, c1 = s * sw + 1, min((s + 1) * sw, lb)
+    lo = c0 - band
+    r0 = 1 if lo <= 0 else max(1, -(-la * lo // lb))
+    r1 = min(la, la * (c1 + band) // lb)
+    return r0, r1
+
+
+def _in_region(la, lb, band, sw, path):
+    i = j = 0
+    for e in path:
+        i += e != "I"
+        j += e != "D"
+        if i >= 1 and j >= 1:
+            r0, r1 = _region_rows(la, lb, band, sw, (j - 1) // sw)
+            if not r0 <= i <= r1:
+                return False
+    return True
+
+
+def test_region_whole_matrix_equals_nwsmall():
+    """Band 0 is the whole matrix: the virtual-boundary integer DP reproduces
+    the literal NWSmall restatement (and so libMUSCLE) path for path."""
+    for a, b, ra, rb in _gold():
+        p, sc = po.nwsmall(a, b)
+        q, sq, dv = po.nwsmall_region(a, b, 0, 512)
+        assert q == p and sq == int(sc), (a, b)
+        assert dv == po.path_dev(len(a), len(b), p)
+    rng = np.random.default_rng(11)
+    for _ in range(40):
+        a = rng.integers(0, 4, int(rng.integers(1, 200)), dtype=np.uint8)
+        b = rng.integers(0, 4, int(rng.integers(1, 200)), dtype=np.uint8)
+        p, sc = po.nwsmall(a, b)
+        q, sq, _ = po.nwsmall_region(a, b, 0, 64)
+        assert q == p and sq == int(sc)
+
+
+@pytest.mark.parametrize("la,lb", [(1, 4), (4, 1), (3, 3), (4, 5), (5, 3), (2, 6), (6, 6)])
+def test_region_brute_force_optimum(la, lb):
+    """The banded result is the best path that stays in the region, and no
+    path when the region holds none."""
+    rng = np.random.default_rng(la * 7 + lb)
+    for _ in range(6):
+        a = rng.integers(0, 4, la, dtype=np.uint8)
+        b = rng.integers(0, 4, lb, dtype=np.uint8)
+        for band in (1, 2, 3):
+            for sw in (1, 2, 3):
+                inside = [q for q in all_paths(la, lb) if _in_region(la, lb, band, sw, q)]
+                p, sc, dv = po.nwsmall_region(a, b, band, sw)
+                if not inside:
+                    assert p is None
+                    continue
+                assert p in inside
+                assert po.path_score(a, b, p) == sc
+                assert sc == max(po.path_score(a, b, q) for q in inside)
+                assert dv == po.path_dev(la, lb, p)
+
+
+def test_region_contains_full_path():
+    """Whenever the full DP's path stays within distance < W of the line, the
+    banded result is the full result exactly (score and path); otherwise its
+    score can only be lower."""
+    rng = np.random.default_rng(12)
+    for _ in range(30):
+        la = int(rng.integers(20, 300))
+        a = rng.integers(0, 4, la, dtype=np.uint8)
+        b = a.copy()
+        hit = rng.random(la) < 0.15
+        b[hit] = rng.integers(0, 4, int(hit.sum()), dtype=np.uint8)
+        cut = int(rng.integers(0, la))
+        b = np.concatenate([b[:cut], rng.integers(0, 4, int(rng.integers(0, 12)), dtype=np.uint8), b[cut + 3:]])
+        if len(b) == 0:
+            continue
+        p, sc = po.nwsmall(a, b)
+        d = po.path_dev(la, len(b), p)
+        for sw in (4, 16, 256):
+            q, sq, dq = po.nwsmall_region(a, b, d + 1, sw)
+            assert q == p and sq == int(sc) and dq == d
+            if d > 1:
+                q2, sq2, _ = po.nwsmall_region(a, b, max(1, d // 2), sw)
+                assert q2 is None or sq2 <= int(sc)
