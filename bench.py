@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dp-pairs", type=int, default=100_000, help="pairs in the batched DP leg (0 = skip)")
+    ap.add_argument("--band-pairs", type=int, default=100_000,
+                    help="held-out gaps in the learned-band leg (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -65,6 +67,12 @@ def main():
         t0 = time.time()
         dp = synth.dp_jobs(n=args.dp_pairs, seed=301, workers=min(16, os.cpu_count() or 1))
         log(f"[bench] {args.dp_pairs} DP pairs generated in {time.time() - t0:.1f}s")
+
+    bandset = None
+    if args.band_pairs > 0:
+        t0 = time.time()
+        bandset = synth.dp_jobs(n=args.band_pairs, seed=402, workers=min(16, os.cpu_count() or 1))
+        log(f"[bench] {args.band_pairs} held-out band gaps generated in {time.time() - t0:.1f}s")
 
     import mauvehip
     ctx = mauvehip.Context(0)
@@ -196,6 +204,9 @@ def main():
         for p in (dA3, dB3, dP3, dR3):
             ctx.free(p)
 
+    if bandset is not None:
+        out["learned_band"] = learned_band_leg(ctx, bandset)
+
     if not args.no_cpu_baseline:
         from oracle import pyoracle as po
         t0 = time.perf_counter()
@@ -214,6 +225,70 @@ def main():
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def learned_band_leg(ctx, bandset):
+    """Full DP vs learned band (features + network + banded DP with edge
+    fallback, all timed) vs the best constant band, on held-out gaps."""
+    import mauvehip
+    from mauvehip import band
+    A, ao, al, B, bo, bl = bandset
+    dj = band.DeviceJobs(ctx, A, B, mauvehip.make_jobs(ao, al, bo, bl))
+    mlp, delta, min_band = band.load_default(ctx)
+    info = json.loads(str(np.load(os.path.join(ROOT, "mauve-py_amd", "mauvehip", "band_model.npz"),
+                                  allow_pickle=False)["info_json"]))
+    w_const = int(info["best_constant_band"])
+    # warm-up of every kernel on a small prefix
+    small = band.DeviceJobs(ctx, A, B, mauvehip.make_jobs(ao[:2000], al[:2000], bo[:2000], bl[:2000]))
+    small.features()
+    small.align(mlp.predict_bands_device(small.dF, small.n, delta, min_band))
+    small.align(None)
+    small.free()
+
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    dj.align(None)
+    t_full = time.perf_counter() - t0
+    res_f, lab = dj.results()
+    paths_f = dj.paths()
+
+    ctx.reset_kernel_times()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    dj.features()
+    bands = mlp.predict_bands_device(dj.dF, dj.n, delta, min_band)
+    st_l = dj.align(bands)
+    t_learned = time.perf_counter() - t0
+    fwd_ms, fwd_n = ctx.kernel_time("dp_forward")
+    res_l, dev_l = dj.results()
+    sm_l, pm_l = band.count_mismatches(res_f, paths_f, res_l, dj.paths())
+
+    t0 = time.perf_counter()
+    st_c = dj.align(np.full(dj.n, w_const, dtype=np.uint32))
+    t_const = time.perf_counter() - t0
+    res_c, _ = dj.results()
+    sm_c, pm_c = band.count_mismatches(res_f, paths_f, res_c, dj.paths())
+
+    cells = dj.cells
+    gk = st_l.cells / (fwd_ms * fwd_n * 1e-3) / 1e9 if fwd_n else 0.0
+    out = {"pairs": int(dj.n), "seed": 402, "full_dp_cells": cells,
+           "full_seconds": round(t_full, 4), "full_gcups": round(cells / t_full / 1e9, 1),
+           "learned_seconds": round(t_learned, 4), "learned_effective_gcups": round(cells / t_learned / 1e9, 1),
+           "speedup_vs_full_dp": round(t_full / t_learned, 2),
+           "learned_cells_computed": int(st_l.cells), "learned_reruns_2w": int(st_l.rerun_doubled),
+           "learned_reruns_full": int(st_l.rerun_full),
+           "score_mismatches": sm_l, "path_mismatches": pm_l, "delta": delta,
+           "constant_band": w_const, "constant_seconds": round(t_const, 4),
+           "constant_effective_gcups": round(cells / t_const / 1e9, 1),
+           "constant_cells_computed": int(st_c.cells), "constant_score_mismatches": sm_c,
+           "constant_path_mismatches": pm_c,
+           "band_kernel_computed_gcups": round(gk, 1),
+           "label_quantiles": {str(q): float(np.percentile(lab, q)) for q in (50, 90, 99, 100)}}
+    log(f"[bench] learned band: full {t_full:.3f}s, learned {t_learned:.3f}s ({t_full / t_learned:.1f}x), "
+        f"constant W={w_const} {t_const:.3f}s; mismatches score {sm_l} path {pm_l}")
+    mlp.free()
+    dj.free()
+    return out
 
 
 if __name__ == "__main__":

@@ -203,6 +203,50 @@ int mh_dp_align_banded_device(mh_context *ctx, const uint8_t *d_a, const uint8_t
  * [1, a_len]. */
 uint32_t mh_dp_band_shape(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *stripe_w);
 
+/* ---- learned band (no counterpart in the reference) ----------------------
+ *
+ * A 272-256-128-1 ReLU MLP with bf16 weights and fp32 accumulation predicts
+ * y = log2(1 + label) for a gap, where the label is the full DP path's
+ * largest distance from the corner-to-corner line (mh_dp_align_banded_device
+ * d_dev of a full run).  Parameters are fp32 masters in one flat array:
+ * W1[256][272], b1[256], W2[128][256], b2[128], w3[128], b3. */
+#define MH_MLP_IN 272
+#define MH_MLP_H1 256
+#define MH_MLP_H2 128
+#define MH_MLP_NPARAMS (MH_MLP_H1 * MH_MLP_IN + MH_MLP_H1 + MH_MLP_H2 * MH_MLP_H1 + MH_MLP_H2 + MH_MLP_H2 + 1)
+typedef struct mh_mlp mh_mlp;
+typedef struct {
+    uint32_t epochs;
+    uint32_t batch;          /* rows per step (<= 65536); the tail of an epoch is dropped */
+    float    lr;             /* Adam step size, cosine-decayed to 0 over the run        */
+    float    weight_decay;   /* decoupled, per step                                     */
+    uint64_t seed;           /* shuffling                                               */
+} mh_mlp_train_opts;
+
+/* Per-gap features (float[njobs][272], device) from the DP jobs' operands:
+ * log lengths, length difference, 4-mer L1 distance, the 256 normalised 4-mer
+ * count differences, and 12 flank seed densities (8-mers of A near each end
+ * found in B within 8 of the corner-to-corner line, 6 window sizes). */
+int  mh_gap_features_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
+                            const mh_dp_job *jobs, uint64_t njobs, float *d_feat);
+int  mh_mlp_create(mh_context *ctx, uint64_t seed, mh_mlp **out);      /* He-uniform init */
+void mh_mlp_free(mh_mlp *mlp);
+int  mh_mlp_set_params(mh_context *ctx, mh_mlp *mlp, const float *params);   /* resets Adam */
+int  mh_mlp_get_params(mh_context *ctx, const mh_mlp *mlp, float *params);
+/* y[rows] (device) = network output for feature rows d_x (device). */
+int  mh_mlp_forward(mh_context *ctx, mh_mlp *mlp, const float *d_x, uint64_t rows, float *d_y);
+/* Loss sum (y-t)^2/(2 rows) and its gradient (host float[MH_MLP_NPARAMS])
+ * for one batch of rows <= 65536 (device inputs). */
+int  mh_mlp_grad(mh_context *ctx, mh_mlp *mlp, const float *d_x, const float *d_t, uint64_t rows,
+                 float *grads, float *loss);
+/* Adam training on device features/targets; epoch_loss (host, nullable)
+ * receives the mean batch loss of each epoch. */
+int  mh_mlp_train(mh_context *ctx, mh_mlp *mlp, const float *d_x, const float *d_t, uint64_t rows,
+                  const mh_mlp_train_opts *opts, float *epoch_loss);
+/* Bands (host uint32[rows]) = max(min_band, ceil(2^(y + delta))). */
+int  mh_band_predict(mh_context *ctx, mh_mlp *mlp, const float *d_feat, uint64_t rows, float delta,
+                     uint32_t min_band, uint32_t *bands);
+
 /* ASCII genome (device) -> letter codes 0..3 (A,C,G,T, either case) for the DP.
  * Any other byte is refused with MH_ERR_UNSUPPORTED (the DP implements the
  * four-letter alphabet only). */

@@ -34,6 +34,7 @@
 #include <utility>
 #include <type_traits>
 #include <cstring>
+#include <thread>
 
 namespace mh {
 namespace {
@@ -461,7 +462,9 @@ struct DpItem {
 // shape, which decides how much of each stripe lies outside the band.
 constexpr double COST_PER_CELL[3] = {1.30, 1.10, 1.00};   // cols 4, 8, 16
 
-int force_cols()
+}  // namespace
+
+int dp_force_cols()
 {
     if (const char *e = getenv("MH_DP_COLS")) {
         const int c = atoi(e);
@@ -470,18 +473,22 @@ int force_cols()
     return 0;
 }
 
-}  // namespace
-
-uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band)
+// Stripe shape of a job: the full DP uses 512-column stripes for narrow jobs
+// and 1024 for wide ones; a banded job takes the shape with the lowest
+// estimated cost, ceil(lb/w) stripes of min(la, la*(w+2W)/lb + 1) + 63 steps
+// of w cells.  force (4, 8 or 16) overrides.
+uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force)
 {
-    const int fc = force_cols();
-    if (fc) return (uint32_t)fc;
+    if (force) return (uint32_t)force;
     if (band == 0) return lb > 512 ? 16u : 8u;
     uint32_t best = 16;
     double best_cost = 0;
     for (int q = 0; q < 3; ++q) {
-        const uint32_t c = 4u << q;
-        const double cost = (double)dp_trace_bytes(la, lb, band, c) * COST_PER_CELL[q];
+        const uint32_t c = 4u << q, sw = 64 * c;
+        const double ns = (double)((lb + sw - 1) / sw);
+        double rows = (double)la * (sw + 2.0 * band) / lb + 1.0;
+        if (rows > la) rows = la;
+        const double cost = ns * (rows + 63.0) * sw * COST_PER_CELL[q];
         if (q == 0 || cost < best_cost) {
             best = c;
             best_cost = cost;
@@ -614,15 +621,39 @@ static int dp_check_jobs(const mh_dp_job *jobs, uint64_t njobs, std::vector<uint
     return MH_OK;
 }
 
-static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band)
+static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band, int force)
 {
     DpItem it;
     it.job = job;
-    it.cols = dp_cols_for(jb.a_len, jb.b_len, band);
+    it.cols = dp_cols_for(jb.a_len, jb.b_len, band, force);
     it.band = dp_effective_band(jb.a_len, jb.b_len, band, it.cols);
-    if (it.band == 0 && band != 0) it.cols = dp_cols_for(jb.a_len, jb.b_len, 0);
+    if (it.band == 0 && band != 0) it.cols = dp_cols_for(jb.a_len, jb.b_len, 0, force);
     it.tb_bytes = dp_trace_bytes(jb.a_len, jb.b_len, it.band, it.cols);
     return it;
+}
+
+// items[x] = make_item(jobs[x], x, bands[x]) for every job, in parallel host
+// threads for large batches
+static void make_items(const mh_dp_job *jobs, uint64_t njobs, const uint32_t *bands, std::vector<DpItem> &items)
+{
+    const int force = dp_force_cols();
+    items.resize(njobs);
+    auto fill = [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t j = lo; j < hi; ++j) items[j] = make_item(jobs[j], (uint32_t)j, bands ? bands[j] : 0u, force);
+    };
+    unsigned nt = njobs >= 8192 ? std::thread::hardware_concurrency() : 1;
+    if (nt > 8) nt = 8;
+    if (nt <= 1) {
+        fill(0, njobs);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const uint64_t chunk = (njobs + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t lo = t * chunk, hi = std::min<uint64_t>(njobs, lo + chunk);
+        if (lo < hi) pool.emplace_back(fill, lo, hi);
+    }
+    for (auto &th : pool) th.join();
 }
 
 int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
@@ -632,8 +663,8 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
     std::vector<uint64_t> slot;
     int rc;
     if ((rc = dp_check_jobs(jobs, njobs, slot)) != MH_OK) return rc;
-    std::vector<DpItem> items(njobs);
-    for (uint64_t j = 0; j < njobs; ++j) items[j] = make_item(jobs[j], (uint32_t)j, 0);
+    std::vector<DpItem> items;
+    make_items(jobs, njobs, nullptr, items);
     return dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, nullptr, nullptr);
 }
 
@@ -652,13 +683,16 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
         if ((rc = ctx->dp_dev.ensure(njobs * 4 + 64)) != MH_OK) return rc;
         d_dev = ctx->dp_dev.as<uint32_t>();
     }
-    std::vector<DpItem> items(njobs);
+    const double tp0 = now_ms();
+    std::vector<DpItem> items;
+    make_items(jobs, njobs, bands, items);
     std::vector<uint32_t> band1(njobs, 0);
     for (uint64_t j = 0; j < njobs; ++j) {
-        items[j] = make_item(jobs[j], (uint32_t)j, bands ? bands[j] : 0u);
         band1[j] = items[j].band;
         if (band1[j]) ++st.banded;
     }
+    host_time(ctx, "host_dp_plan", now_ms() - tp0);
+    const int force = dp_force_cols();
     if ((rc = dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, d_dev, &st.cells)) != MH_OK)
         return rc;
     if (!(flags & MH_BAND_NO_FALLBACK) && st.banded) {
@@ -672,7 +706,7 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
         for (uint64_t j = 0; j < njobs; ++j)
             if (band1[j] && dev[j] >= band1[j]) {
                 const uint64_t w2 = 2ull * band1[j];
-                again.push_back(make_item(jobs[j], (uint32_t)j, w2 > 0xFFFFFFFFull ? 0u : (uint32_t)w2));
+                again.push_back(make_item(jobs[j], (uint32_t)j, w2 > 0xFFFFFFFFull ? 0u : (uint32_t)w2, force));
                 band2[j] = again.back().band;
                 if (band2[j]) ++st.rerun_doubled;
                 else ++st.rerun_full;
@@ -685,7 +719,7 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
             std::vector<DpItem> full;
             for (uint64_t j = 0; j < njobs; ++j)
                 if (band2[j] && dev[j] >= band2[j]) {
-                    full.push_back(make_item(jobs[j], (uint32_t)j, 0));
+                    full.push_back(make_item(jobs[j], (uint32_t)j, 0, force));
                     ++st.rerun_full;
                 }
             if (!full.empty() &&

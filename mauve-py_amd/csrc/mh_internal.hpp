@@ -54,6 +54,7 @@ struct mh_context {
     std::vector<hipEvent_t> event_pool;
     // DP scratch
     mh::DevBuf dp_tb, dp_bnd, dp_jobs, dp_a, dp_b, dp_paths, dp_res, dp_dev;
+    mh::DevBuf ml_jobs, ml_y;   // learned band: job table, network outputs + bands
     // seed-matching scratch
     mh::DevBuf mf_scratch[16];
     // scratch for sorting
@@ -102,7 +103,8 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
                   const uint32_t *bands, uint32_t flags, uint8_t *d_paths, mh_dp_result *d_results,
                   uint32_t *d_dev, mh_band_stats *stats);
 // Stripe shape (columns per lane) and effective band the scheduler uses.
-uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band);
+int dp_force_cols();   // MH_DP_COLS = 4 | 8 | 16 forces one stripe shape (tuning sweeps)
+uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force);
 uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols);
 
 int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed, mh_sml *s);

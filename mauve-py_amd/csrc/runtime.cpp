@@ -338,9 +338,10 @@ uint32_t mh_dp_band_shape(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_
         if (stripe_w) *stripe_w = 0;
         return 0;
     }
-    uint32_t c = dp_cols_for(a_len, b_len, band);
+    const int force = dp_force_cols();
+    uint32_t c = dp_cols_for(a_len, b_len, band, force);
     const uint32_t eff = band ? dp_effective_band(a_len, b_len, band, c) : 0u;
-    if (eff == 0) c = dp_cols_for(a_len, b_len, 0);
+    if (eff == 0) c = dp_cols_for(a_len, b_len, 0, force);
     if (stripe_w) *stripe_w = 64 * c;
     return eff;
 }

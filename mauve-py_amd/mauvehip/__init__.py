@@ -43,6 +43,8 @@ EXPORTS = [
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free",
     "mh_dp_align_banded_device", "mh_dp_band_shape",
+    "mh_gap_features_device", "mh_mlp_create", "mh_mlp_free", "mh_mlp_set_params", "mh_mlp_get_params",
+    "mh_mlp_forward", "mh_mlp_grad", "mh_mlp_train", "mh_band_predict",
 ]
 
 
@@ -66,6 +68,13 @@ class BandStats(C.Structure):
 
 
 BAND_NO_FALLBACK = 1
+MLP_IN, MLP_H1, MLP_H2 = 272, 256, 128
+MLP_NPARAMS = MLP_H1 * MLP_IN + MLP_H1 + MLP_H2 * MLP_H1 + MLP_H2 + MLP_H2 + 1
+
+
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("weight_decay", C.c_float),
+                ("seed", C.c_uint64)]
 
 
 class Bmer(C.Structure):
@@ -131,6 +140,15 @@ def load_library(path=LIB_PATH):
         "mh_dp_align_banded_device": (C.c_int, [vp, vp, vp, vp, u64, vp, C.c_uint32, vp, vp, vp,
                                                 C.POINTER(BandStats)]),
         "mh_dp_band_shape": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
+        "mh_gap_features_device": (C.c_int, [vp, vp, vp, vp, u64, vp]),
+        "mh_mlp_create": (C.c_int, [vp, u64, C.POINTER(vp)]),
+        "mh_mlp_free": (None, [vp]),
+        "mh_mlp_set_params": (C.c_int, [vp, vp, vp]),
+        "mh_mlp_get_params": (C.c_int, [vp, vp, vp]),
+        "mh_mlp_forward": (C.c_int, [vp, vp, vp, u64, vp]),
+        "mh_mlp_grad": (C.c_int, [vp, vp, vp, vp, u64, vp, C.POINTER(C.c_float)]),
+        "mh_mlp_train": (C.c_int, [vp, vp, vp, vp, u64, C.POINTER(MlpTrainOpts), vp]),
+        "mh_band_predict": (C.c_int, [vp, vp, vp, u64, C.c_float, C.c_uint32, vp]),
         "mh_device_alloc": (C.c_int, [vp, u64, C.POINTER(vp)]),
         "mh_device_free": (C.c_int, [vp, vp]),
         "mh_memcpy_h2d": (C.c_int, [vp, vp, vp, u64]),
@@ -323,6 +341,68 @@ def band_shape(a_len, b_len, band):
     w = C.c_uint32()
     eff = load_library().mh_dp_band_shape(int(a_len), int(b_len), int(band), C.byref(w))
     return int(eff), int(w.value)
+
+
+# ---- learned band ------------------------------------------------------------
+
+def gap_features_device(ctx, d_a, d_b, jobs, d_feat):
+    """float[len(jobs)][272] features into device buffer d_feat."""
+    jobs = np.ascontiguousarray(jobs, dtype=DP_JOB_DTYPE)
+    _check(ctx.L.mh_gap_features_device(ctx.h, d_a, d_b, jobs.ctypes.data, len(jobs), d_feat))
+
+
+class Mlp:
+    """The band network (mh_mlp): 272-256-128-1, bf16 weights, fp32 masters."""
+
+    def __init__(self, ctx, seed=0, params=None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(ctx.L.mh_mlp_create(ctx.h, seed, C.byref(h)))
+        self.h = h.value
+        if params is not None:
+            self.set_params(params)
+
+    def free(self):
+        if self.h:
+            self.ctx.L.mh_mlp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(MLP_NPARAMS, dtype=np.float32)
+        _check(self.ctx.L.mh_mlp_get_params(self.ctx.h, self.h, p.ctypes.data))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == MLP_NPARAMS
+        _check(self.ctx.L.mh_mlp_set_params(self.ctx.h, self.h, p.ctypes.data))
+
+    def forward_device(self, d_x, rows, d_y):
+        _check(self.ctx.L.mh_mlp_forward(self.ctx.h, self.h, d_x, rows, d_y))
+
+    def grad_device(self, d_x, d_t, rows):
+        g = np.zeros(MLP_NPARAMS, dtype=np.float32)
+        loss = C.c_float()
+        _check(self.ctx.L.mh_mlp_grad(self.ctx.h, self.h, d_x, d_t, rows, g.ctypes.data, C.byref(loss)))
+        return loss.value, g
+
+    def train_device(self, d_x, d_t, rows, epochs=4, batch=1024, lr=1e-3, weight_decay=0.0, seed=0):
+        o = MlpTrainOpts(epochs, batch, lr, weight_decay, seed)
+        curve = np.zeros(epochs, dtype=np.float32)
+        _check(self.ctx.L.mh_mlp_train(self.ctx.h, self.h, d_x, d_t, rows, C.byref(o), curve.ctypes.data))
+        return curve
+
+    def predict_bands_device(self, d_feat, rows, delta, min_band=4):
+        b = np.zeros(rows, dtype=np.uint32)
+        _check(self.ctx.L.mh_band_predict(self.ctx.h, self.h, d_feat, rows, float(delta), int(min_band),
+                                          b.ctypes.data))
+        return b
 
 
 # ---- sorted mer lists (DNAFileSML / DNAMemorySML) -------------------------
