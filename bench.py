@@ -96,9 +96,9 @@ def main():
         sa = mauvehip.SortedMerList.from_device(ctx, dA, La, seed)
         sb = mauvehip.SortedMerList.from_device(ctx, dB, Lb, seed)
         t.append(time.perf_counter())
-        st, ln, stats = mauvehip.find_matches(ctx, [sa, sb])
+        mt, stats = mauvehip.find_matches_raw(ctx, [sa, sb])
         t.append(time.perf_counter())
-        chain, jobs = mauvehip.chain_gap_jobs(st, ln, La, Lb, MAX_GAP)
+        chain, jobs = mauvehip.chain_gap_jobs_raw(mt, La, Lb, MAX_GAP)
         t.append(time.perf_counter())
         mauvehip.global_align_device(ctx, cA, cB, jobs, d_paths, d_res)
         ctx.synchronize()
@@ -108,7 +108,7 @@ def main():
         t.append(time.perf_counter())
         for k, name in enumerate(("sml", "match", "chain", "dp", "free")):
             stage_s[name] += t[k + 1] - t[k]
-        state.update(stats=stats, jobs=jobs, chain=chain, starts=st, lengths=ln)
+        state.update(stats=stats, jobs=jobs, chain=chain, matches=mt)
 
     for _ in range(args.warmup):
         step()
@@ -134,7 +134,7 @@ def main():
     passes = (2 * w + 1 + 7) // 8
     kern = {}
     for name in ("sml_pack", "sml_keys", "sml_sort", "join", "extend", "partition", "dp_forward", "dp_traceback",
-                 "host_wait_seeds", "host_partition", "host_replay", "host_list"):
+                 "host_wait_seeds", "host_partition", "host_replay", "host_list", "host_dp_plan", "host_dp_sort"):
         ms, n = ctx.kernel_time(name)
         kern[name] = {"avg_ms": ms, "launches": n, "ms_per_step": ms * n / args.steps}
     nseeds = int(state["stats"].seed_tuples)
@@ -215,7 +215,9 @@ def main():
         for j in jobs:
             po.nwsmall(codes_a[j["a_off"]:j["a_off"] + j["a_len"]], codes_b[j["b_off"]:j["b_off"] + j["b_len"]])
         cpu_s = time.perf_counter() - t0
-        same = len(ln_o) == len(state["lengths"]) and np.array_equal(ln_o, state["lengths"])
+        mt = state["matches"]
+        same = (len(ln_o) == len(mt) and np.array_equal(ln_o, mt["length"])
+                and np.array_equal(st_o, mt["start"][:, :2]))
         out["cpu_baseline"] = {"value": round((La + Lb) / 1e6 / cpu_s, 4), "unit": "Mb/s", "cores": 1,
                                "kind": "port", "seconds": round(cpu_s, 2),
                                "sample": "one full step of the same workload (oracle sorted mer lists, "

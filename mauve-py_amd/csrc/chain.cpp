@@ -11,6 +11,7 @@
 #include "mh_internal.hpp"
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 using namespace mh;
 
@@ -21,6 +22,37 @@ struct Anc {
     uint32_t idx;
 };
 
+// Stable LSD radix sort of (key, value) pairs, 11-bit digits, as many passes
+// as the largest key needs.
+void radix_sort_pairs(std::vector<uint64_t> &key, std::vector<uint32_t> &val)
+{
+    const size_t n = key.size();
+    uint64_t mx = 0;
+    for (uint64_t k : key) mx |= k;
+    int bits = 0;
+    while (bits < 64 && (mx >> bits)) ++bits;
+    std::vector<uint64_t> k2(n);
+    std::vector<uint32_t> v2(n);
+    std::vector<uint32_t> cnt(2048);
+    for (int sh = 0; sh < bits; sh += 11) {
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (size_t i = 0; i < n; ++i) ++cnt[(key[i] >> sh) & 2047u];
+        uint32_t acc = 0;
+        for (uint32_t &c : cnt) {
+            const uint32_t t = c;
+            c = acc;
+            acc += t;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t d = (uint32_t)((key[i] >> sh) & 2047u);
+            k2[cnt[d]] = key[i];
+            v2[cnt[d]++] = val[i];
+        }
+        key.swap(k2);
+        val.swap(v2);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -29,54 +61,89 @@ int mh_chain_gap_jobs(const mh_match *matches, uint64_t n, uint64_t len_a, uint6
                       uint32_t *chain, uint64_t *n_chain, mh_dp_job *jobs, uint64_t cap, uint64_t *n_jobs)
 {
     MH_CHECK((matches || n == 0) && n_chain && n_jobs && (jobs || cap == 0), MH_ERR_INVALID, "null argument");
-    // anchors in (s0, s1, match index) order; the sort runs on packed keys
-    std::vector<std::pair<uint64_t, uint32_t>> ord;
-    ord.reserve(n);
+    MH_CHECK(len_a < (1ull << 31) && len_b < (1ull << 31) && n < (1ull << 31), MH_ERR_UNSUPPORTED,
+             "chain producer limited to 2^31 positions and matches");
+    // anchors in (s0, s1, match index) order (stable sort of packed starts)
+    std::vector<uint64_t> key;
+    std::vector<uint32_t> val;
+    key.reserve(n);
+    val.reserve(n);
     for (uint64_t i = 0; i < n; ++i)
-        if (matches[i].start[0] > 0 && matches[i].start[1] > 0)
-            ord.push_back({((uint64_t)(matches[i].start[0] - 1) << 32) | (uint64_t)(matches[i].start[1] - 1),
-                           (uint32_t)i});
-    std::sort(ord.begin(), ord.end());
-    std::vector<Anc> a(ord.size());
-    for (size_t q = 0; q < ord.size(); ++q) {
-        const mh_match &mt = matches[ord[q].second];
-        a[q] = {mt.start[0] - 1, mt.start[1] - 1, (int64_t)mt.length, ord[q].second};
+        if (matches[i].start[0] > 0 && matches[i].start[1] > 0) {
+            key.push_back(((uint64_t)(matches[i].start[0] - 1) << 32) | (uint64_t)(matches[i].start[1] - 1));
+            val.push_back((uint32_t)i);
+        }
+    radix_sort_pairs(key, val);
+    const size_t m = key.size();
+    std::vector<Anc> a(m);
+    for (size_t q = 0; q < m; ++q) {
+        const mh_match &mt = matches[val[q]];
+        a[q] = {mt.start[0] - 1, mt.start[1] - 1, (int64_t)mt.length, val[q]};
     }
-    const size_t m = a.size();
     // best[i] = len_i + max best[j] over anchors ending (in both genomes) before i starts.
     // Sweep by start; anchors become eligible once their end0 <= current start0;
-    // a Fenwick tree over end1 ranks answers the prefix maximum.
-    std::vector<int64_t> ends1(m);
-    for (size_t i = 0; i < m; ++i) ends1[i] = a[i].s1 + a[i].len;
-    std::vector<int64_t> keys = ends1;
-    std::sort(keys.begin(), keys.end());
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-    std::vector<std::pair<int64_t, int64_t>> fen(keys.size() + 1, {0, -1});   // (score, anchor)
-    auto upd = [&](size_t r, std::pair<int64_t, int64_t> v) {
-        for (++r; r <= keys.size(); r += r & (~r + 1))
-            if (v.first > fen[r].first) fen[r] = v;
+    // a Fenwick tree over the distinct end1 values answers the prefix maximum.
+    // Ranks come from one sort of every end1 (tag 0) and start1 (tag 1):
+    // an end's rank is its distinct value's index, a start's query bound the
+    // number of distinct ends <= it.
+    std::vector<uint32_t> end_rank(m), qry_rank(m);
+    size_t nkeys = 0;
+    {
+        std::vector<uint64_t> k2(2 * m);
+        std::vector<uint32_t> v2(2 * m);
+        for (size_t i = 0; i < m; ++i) {
+            k2[2 * i] = ((uint64_t)(a[i].s1 + a[i].len) << 1);
+            v2[2 * i] = (uint32_t)(2 * i);
+            k2[2 * i + 1] = ((uint64_t)a[i].s1 << 1) | 1u;
+            v2[2 * i + 1] = (uint32_t)(2 * i + 1);
+        }
+        radix_sort_pairs(k2, v2);
+        uint64_t last = ~0ull;
+        for (size_t x = 0; x < 2 * m; ++x) {
+            const uint32_t i = v2[x] >> 1;
+            if (!(v2[x] & 1u)) {
+                if ((k2[x] >> 1) != last) {
+                    last = k2[x] >> 1;
+                    ++nkeys;
+                }
+                end_rank[i] = (uint32_t)(nkeys - 1);
+            } else {
+                qry_rank[i] = (uint32_t)nkeys;
+            }
+        }
+    }
+    struct Fen {
+        int64_t score, anchor;
+    };
+    std::vector<Fen> fen(nkeys + 1, Fen{0, -1});
+    auto upd = [&](size_t r, Fen v) {
+        for (++r; r <= nkeys; r += r & (~r + 1))
+            if (v.score > fen[r].score) fen[r] = v;
     };
     auto qry = [&](size_t r) {   // max over ranks [0, r)
-        std::pair<int64_t, int64_t> b{0, -1};
+        Fen b{0, -1};
         for (; r > 0; r -= r & (~r + 1))
-            if (fen[r].first > b.first) b = fen[r];
+            if (fen[r].score > b.score) b = fen[r];
         return b;
     };
     // anchors by end in genome 0 (ties by anchor rank)
     std::vector<uint64_t> by_end(m);
-    for (size_t i = 0; i < m; ++i) by_end[i] = ((uint64_t)(a[i].s0 + a[i].len) << 32) | (uint64_t)i;
-    std::sort(by_end.begin(), by_end.end());
+    std::vector<uint32_t> by_end_i(m);
+    for (size_t i = 0; i < m; ++i) {
+        by_end[i] = (uint64_t)(a[i].s0 + a[i].len);
+        by_end_i[i] = (uint32_t)i;
+    }
+    radix_sort_pairs(by_end, by_end_i);
     std::vector<int64_t> best(m, 0), prev(m, -1);
     size_t e = 0;
     for (size_t i = 0; i < m; ++i) {
-        while (e < m && (int64_t)(by_end[e] >> 32) <= a[i].s0) {
-            const size_t j = (size_t)(by_end[e++] & 0xFFFFFFFFu);
-            if (best[j] > 0) upd(std::lower_bound(keys.begin(), keys.end(), ends1[j]) - keys.begin(), {best[j], (int64_t)j});
+        while (e < m && (int64_t)by_end[e] <= a[i].s0) {
+            const size_t j = by_end_i[e++];
+            if (best[j] > 0) upd(end_rank[j], Fen{best[j], (int64_t)j});
         }
-        const size_t r = std::upper_bound(keys.begin(), keys.end(), a[i].s1) - keys.begin();
-        const auto b = qry(r);
-        best[i] = a[i].len + b.first;
-        prev[i] = b.second;
+        const Fen b = qry(qry_rank[i]);
+        best[i] = a[i].len + b.score;
+        prev[i] = b.anchor;
     }
     int64_t top = -1, topv = 0;
     for (size_t i = 0; i < m; ++i)

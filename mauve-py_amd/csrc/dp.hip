@@ -362,6 +362,96 @@ __global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs
         dp_job<4>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
 }
 
+// Narrow jobs (lb <= 16): one lane per job.  The lane keeps a whole row of
+// M, D' and I' in registers and walks the rows of A, with the same two
+// register passes per row as dp_job; columns past lb are computed and
+// ignored (nothing flows leftwards).  The left boundary is column 0 and the
+// top row 0, as for the first stripe of dp_job.  Trace: 8 bytes per row,
+// cell (i, j) at byte (i-1)*8 + (j-1)/2, nibble (j-1)%2.
+constexpr int TINY_LB = 16;
+constexpr uint32_t TINY_COLS_TAG = 1;   // DevJob::cols of a one-lane job
+
+__global__ __launch_bounds__(64) void dp_tiny(const DevJob *__restrict__ jobs, int n, const uint8_t *__restrict__ A,
+                                              const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
+                                              DpOut *__restrict__ outs, uint32_t sr0, uint32_t sr1, uint32_t sr2,
+                                              uint32_t sr3, int goh)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    if (x >= n) return;
+    const DevJob jb = jobs[x];
+    const int la = (int)jb.la, lb = (int)jb.lb;
+    const uint8_t *a = A + jb.a_off;
+    const uint8_t *b = B + jb.b_off;
+    uint8_t *tbj = tb + jb.tb_off;
+    const int go2 = 2 * goh;
+    uint32_t bsh[TINY_LB];
+#pragma unroll
+    for (int c = 0; c < TINY_LB; ++c) bsh[c] = c < lb ? 8u * (uint32_t)b[c] : 0u;
+    int Mp[TINY_LB], Dp[TINY_LB], Ip[TINY_LB];
+#pragma unroll
+    for (int c = 0; c < TINY_LB; ++c) {   // row 0: M = NEG, D' = NEG + goh, I' = 0 + goh
+        Mp[c] = NEG;
+        Dp[c] = NEG + goh;
+        Ip[c] = goh;
+    }
+    int finM = NEG, finD = NEG, finI = NEG;
+    for (int i = 1; i <= la; ++i) {
+        const int ra = a[i - 1];
+        const uint32_t srow = (ra & 2) ? ((ra & 1) ? sr3 : sr2) : ((ra & 1) ? sr1 : sr0);
+        const int oa2 = (i == 1) ? goh : go2;
+        // column 0: (i-1, 0) is M(0,0) = 0 on row 1, else M = NEG, D' = 0 + goh
+        const int lpM = (i == 1) ? 0 : NEG, lpD = (i == 1) ? NEG + goh : goh, lpI = NEG + goh;
+        uint32_t word[2] = {0u, 0u};
+        static_for(std::make_integer_sequence<int, TINY_LB>{}, [&](auto ic) {
+            constexpr int c = TINY_LB - 1 - decltype(ic)::value;
+            constexpr uint32_t K = 1u << (4 * (c & 7));
+            const int md = Mp[c] + oa2;
+            const bool tD = md >= Dp[c];
+            const int Dn = max(md, Dp[c]);
+            const int dM = c ? Mp[c > 0 ? c - 1 : 0] : lpM;
+            const int dD = c ? Dp[c > 0 ? c - 1 : 0] : lpD;
+            const int dI = c ? Ip[c > 0 ? c - 1 : 0] : lpI;
+            const int best = max3i(dM, dD, dI);
+            uint32_t srcK = (dM != best) ? ((dD == best) ? K : 2u * K) : 0u;
+            const int sc = (int)__builtin_amdgcn_ubfe(srow, bsh[c], 8) - SBIAS;
+            int Mn = sc + best;
+            if (c == 0 && la == 1 && i == 1) {
+                // one-row quirk: with LA == 1, M(1,1) carries close_A[0] and a 'D' bit
+                Mn = sc + goh;
+                srcK = K;
+            }
+            Mp[c] = Mn;
+            Dp[c] = Dn;
+            word[c >> 3] |= srcK | (tD ? 4u * K : 0u);
+        });
+        // column 0 on row i: M = NEG, I' = NEG + goh; open_B[0] + goh = goh
+        int iL = NEG + goh, mL = NEG;
+        static_for(std::make_integer_sequence<int, TINY_LB>{}, [&](auto ic) {
+            constexpr int c = decltype(ic)::value;
+            constexpr uint32_t K = 1u << (4 * (c & 7));
+            const int mi = mL + (c == 0 ? goh : go2);
+            const bool tI = mi >= iL;
+            iL = max(mi, iL);
+            Ip[c] = iL;
+            mL = Mp[c];
+            word[c >> 3] |= tI ? 8u * K : 0u;
+        });
+        *reinterpret_cast<uint2 *>(tbj + (size_t)(i - 1) * 8) = make_uint2(word[0], word[1]);
+        if (i == la) {
+#pragma unroll
+            for (int c = 0; c < TINY_LB; ++c)
+                if (c == lb - 1) { finM = Mp[c]; finD = Dp[c]; finI = Ip[c]; }
+        }
+    }
+    finD -= goh;
+    finI -= goh;
+    int score = finM, edge = 'M';
+    if (finD > score) { score = finD; edge = 'D'; }
+    if (finI > score) { score = finI; edge = 'I'; }
+    outs[x].score = score;
+    outs[x].edge = edge;
+}
+
 __device__ __forceinline__ int xchar(uint32_t bits, int t)
 {
     if (t == 'M') { const uint32_t v = bits & 3u; return v == 0 ? 'M' : (v == 1 ? 'D' : (v == 2 ? 'I' : '?')); }
@@ -409,10 +499,15 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
                 stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
                 if (r1 >= r0) base -= (uint64_t)(r1 - r0 + 64) * 64 * lane_bytes;
             }
-            if (i < r0 || i > r1) { ok = false; break; }
-            const int cc = (j - 1) % sw, k = cc / cols, c = cc % cols;
-            const uint8_t byte = tj[base + ((uint64_t)(i - r0 + k) * 64 + k) * lane_bytes + (c >> 1)];
-            nib = (byte >> (4 * (c & 1))) & 15u;
+            if (jb.cols == TINY_COLS_TAG) {
+                const uint8_t byte = tj[(uint64_t)(i - 1) * 8 + ((j - 1) >> 1)];
+                nib = (byte >> (4 * ((j - 1) & 1))) & 15u;
+            } else {
+                if (i < r0 || i > r1) { ok = false; break; }
+                const int cc = (j - 1) % sw, k = cc / cols, c = cc % cols;
+                const uint8_t byte = tj[base + ((uint64_t)(i - r0 + k) * 64 + k) * lane_bytes + (c >> 1)];
+                nib = (byte >> (4 * (c & 1))) & 15u;
+            }
         }
         const int nx = xchar(nib, type);
         if (type == 'M') { if (i == 0 || j == 0) { ok = false; break; } --i; --j; }
@@ -516,8 +611,10 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
     const uint64_t nitems = items.size();
     if (nitems == 0) return MH_OK;
     // largest first so the long tails start early
+    const double ts0 = now_ms();
     std::stable_sort(items.begin(), items.end(),
                      [](const DpItem &x, const DpItem &y) { return x.tb_bytes > y.tb_bytes; });
+    host_time(ctx, "host_dp_sort", now_ms() - ts0);
     auto bnd_ints = [&](const DpItem &it) {
         const uint64_t ns = (jobs[it.job].b_len + 64 * it.cols - 1) / (64 * it.cols);
         return ns > 1 ? 3ull * ((jobs[it.job].a_len + 63) & ~63ull) : 0ull;
@@ -557,6 +654,11 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             ++last;
         }
         const uint64_t n = last - first;
+        // one-lane jobs go after the wavefront jobs, keeping their order
+        std::stable_partition(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)last,
+                              [](const DpItem &it) { return it.cols != TINY_COLS_TAG; });
+        uint64_t nbig = 0;
+        while (nbig < n && items[first + nbig].cols != TINY_COLS_TAG) ++nbig;
         dj.resize(n);
         uint64_t tw = 0, bo = 0;
         for (uint64_t x = 0; x < n; ++x) {
@@ -587,9 +689,14 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         MH_HIP(hipMemcpyAsync(d_jobs, dj.data(), n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
         {
             TimedLaunch tl(ctx, "dp_forward");
-            hipLaunchKernelGGL(dp_forward, dim3((unsigned)n), dim3(64), 0, ctx->stream, d_jobs, d_a, d_b,
-                               ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs, sr[0], sr[1], sr[2],
-                               sr[3], -200);
+            if (nbig)
+                hipLaunchKernelGGL(dp_forward, dim3((unsigned)nbig), dim3(64), 0, ctx->stream, d_jobs, d_a, d_b,
+                                   ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs, sr[0], sr[1], sr[2],
+                                   sr[3], -200);
+            if (n > nbig)
+                hipLaunchKernelGGL(dp_tiny, dim3((unsigned)((n - nbig + 63) / 64)), dim3(64), 0, ctx->stream,
+                                   d_jobs + nbig, (int)(n - nbig), d_a, d_b, ctx->dp_tb.as<uint8_t>(),
+                                   d_outs + nbig, sr[0], sr[1], sr[2], sr[3], -200);
             MH_HIP(hipGetLastError());
         }
         {
@@ -628,6 +735,11 @@ static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band, int fo
     it.cols = dp_cols_for(jb.a_len, jb.b_len, band, force);
     it.band = dp_effective_band(jb.a_len, jb.b_len, band, it.cols);
     if (it.band == 0 && band != 0) it.cols = dp_cols_for(jb.a_len, jb.b_len, 0, force);
+    if (it.band == 0 && !force && jb.b_len <= (uint32_t)TINY_LB) {
+        it.cols = TINY_COLS_TAG;
+        it.tb_bytes = (uint64_t)jb.a_len * 8;
+        return it;
+    }
     it.tb_bytes = dp_trace_bytes(jb.a_len, jb.b_len, it.band, it.cols);
     return it;
 }
@@ -664,7 +776,9 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
     int rc;
     if ((rc = dp_check_jobs(jobs, njobs, slot)) != MH_OK) return rc;
     std::vector<DpItem> items;
+    const double tp0 = now_ms();
     make_items(jobs, njobs, nullptr, items);
+    host_time(ctx, "host_dp_plan", now_ms() - tp0);
     return dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, nullptr, nullptr);
 }
 

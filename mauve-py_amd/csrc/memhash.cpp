@@ -205,6 +205,8 @@ int mh_sml_create_device(mh_context *ctx, const void *d_seq, uint64_t len, uint6
     MH_CHECK(ctx && out && (d_seq || len == 0), MH_ERR_INVALID, "null argument");
     MH_HIP(hipSetDevice(ctx->device));
     mh_sml *s = new mh_sml();
+    s->ctx = ctx;
+    ctx->smls.insert(s);
     int rc = sml_build(ctx, d_seq, len, seed, s);
     if (rc != MH_OK) {
         mh_sml_free(s);
@@ -231,10 +233,24 @@ int mh_sml_create(mh_context *ctx, const char *seq, uint64_t len, uint64_t seed,
 void mh_sml_free(mh_sml *s)
 {
     if (!s) return;
-    if (s->d_words) (void)hipFree(s->d_words);
-    if (s->d_keys_pos) (void)hipFree(s->d_keys_pos);
-    if (s->d_keys) (void)hipFree(s->d_keys);
-    if (s->d_pos) (void)hipFree(s->d_pos);
+    SmlBufs b;
+    b.words = s->d_words;
+    b.keys_pos = s->d_keys_pos;
+    b.keys = s->d_keys;
+    b.pos = s->d_pos;
+    b.cap_words = s->cap_words;
+    b.cap_n = s->cap_n;
+    mh_context *ctx = s->ctx;
+    if (ctx) {
+        // the buffers stay with the context for the next build (a few sets at
+        // most); work on them is ordered on the context stream, so no wait
+        ctx->smls.erase(s);
+        if (b.words && b.keys_pos && b.keys && b.pos && ctx->sml_pool.size() < 4) {
+            ctx->sml_pool.push_back(b);
+            b = SmlBufs();
+        }
+    }
+    b.release();
     delete s;
 }
 

@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <string>
 #include <vector>
+#include <set>
 #include <map>
 #include "mauvehip.h"
 
@@ -33,9 +34,22 @@ void set_error(const std::string &msg);
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
     int ensure(size_t need);   // MH_OK or MH_ERR_NOMEM
     void release();
     template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Device buffers of one sorted mer list, kept by the context for reuse.
+struct SmlBufs {
+    uint32_t *words = nullptr;
+    uint64_t *keys_pos = nullptr, *keys = nullptr;
+    uint32_t *pos = nullptr;
+    uint64_t cap_words = 0, cap_n = 0;
+    void release();
 };
 
 struct KernelTimer {
@@ -62,6 +76,10 @@ struct mh_context {
     // pinned host staging for device-to-host transfers
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
+    // sorted mer lists: live ones (orphaned when the context goes first) and
+    // released buffer sets kept for the next build
+    std::set<mh_sml *> smls;
+    std::vector<mh::SmlBufs> sml_pool;
 };
 
 struct mh_sml {
@@ -75,6 +93,7 @@ struct mh_sml {
     uint64_t *d_keys_pos = nullptr;  // GetDnaSeedMer per position (position order)
     uint64_t *d_keys = nullptr;      // sorted keys
     uint32_t *d_pos = nullptr;       // sorted positions
+    uint64_t cap_words = 0, cap_n = 0;   // capacities of the buffers above
 };
 
 namespace mh {

@@ -39,6 +39,15 @@ void DevBuf::release()
     bytes = 0;
 }
 
+void SmlBufs::release()
+{
+    if (words) (void)hipFree(words);
+    if (keys_pos) (void)hipFree(keys_pos);
+    if (keys) (void)hipFree(keys);
+    if (pos) (void)hipFree(pos);
+    *this = SmlBufs();
+}
+
 static hipEvent_t take_event(mh_context *ctx)
 {
     if (!ctx->event_pool.empty()) {
@@ -212,6 +221,8 @@ void mh_destroy(mh_context *ctx)
     ctx->dp_res.release();
     for (auto &b : ctx->mf_scratch) b.release();
     for (auto &b : ctx->sort_tmp) b.release();
+    for (mh_sml *s : ctx->smls) s->ctx = nullptr;   // lists outliving the context free their own buffers
+    for (auto &b : ctx->sml_pool) b.release();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

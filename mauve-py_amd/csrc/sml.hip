@@ -335,6 +335,33 @@ SeedPattern make_pattern(uint64_t seed)
 static inline uint64_t full_key(uint64_t c, int sw) { return ((c >> 1) << (64 - 2 * sw)) | (c & 1ull); }
 static inline uint64_t compact_key(uint64_t k, int sw) { return ((k >> (64 - 2 * sw)) << 1) | (k & 1ull); }
 
+// Buffers for s (n_words, n set): a released set from the context's pool
+// that is large enough, else fresh allocations.
+static int sml_take_buffers(mh_context *ctx, mh_sml *s)
+{
+    const uint64_t n = s->n ? s->n : 1;
+    for (size_t q = 0; q < ctx->sml_pool.size(); ++q) {
+        SmlBufs &b = ctx->sml_pool[q];
+        if (b.cap_words >= s->n_words && b.cap_n >= n) {
+            s->d_words = b.words;
+            s->d_keys_pos = b.keys_pos;
+            s->d_keys = b.keys;
+            s->d_pos = b.pos;
+            s->cap_words = b.cap_words;
+            s->cap_n = b.cap_n;
+            ctx->sml_pool.erase(ctx->sml_pool.begin() + (ptrdiff_t)q);
+            return MH_OK;
+        }
+    }
+    s->cap_words = s->n_words;
+    s->cap_n = n;
+    MH_HIP(hipMalloc(&s->d_words, s->cap_words * 4));
+    MH_HIP(hipMalloc(&s->d_keys_pos, n * 8));
+    MH_HIP(hipMalloc(&s->d_keys, n * 8));
+    MH_HIP(hipMalloc(&s->d_pos, n * 4));
+    return MH_OK;
+}
+
 int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed, mh_sml *s)
 {
     const int sl = seed_length((int64_t)seed), sw = seed_weight((int64_t)seed);
@@ -349,11 +376,8 @@ int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed,
     s->n = len >= (uint64_t)sl ? len - sl + 1 : 0;
     s->n_words = (len * 2 + 31) / 32 + 2;
     const uint64_t n = s->n;
-    MH_HIP(hipMalloc(&s->d_words, s->n_words * 4));
-    MH_HIP(hipMalloc(&s->d_keys_pos, (n ? n : 1) * 8));
-    MH_HIP(hipMalloc(&s->d_keys, (n ? n : 1) * 8));
-    MH_HIP(hipMalloc(&s->d_pos, (n ? n : 1) * 4));
     int rc;
+    if ((rc = sml_take_buffers(ctx, s)) != MH_OK) return rc;
     if ((rc = ctx->sort_tmp[0].ensure((n ? n : 1) * 8 + 64)) != MH_OK) return rc;
     if ((rc = ctx->sort_tmp[1].ensure((n ? n : 1) * 4 + 64)) != MH_OK) return rc;
     if ((rc = ctx->sort_tmp[3].ensure(64)) != MH_OK) return rc;

@@ -482,8 +482,8 @@ class SortedMerList:
         return out
 
 
-def find_matches(ctx, smls):
-    """PairwiseMatchFinder::FindMatches: (starts int64 (n, k), lengths uint64, MatchStats)."""
+def find_matches_raw(ctx, smls):
+    """PairwiseMatchFinder::FindMatches as one mh_match array (MATCH_DTYPE) and MatchStats."""
     k = len(smls)
     arr = (C.c_void_p * k)(*[s.h.value for s in smls])
     st = MatchStats()
@@ -491,12 +491,18 @@ def find_matches(ctx, smls):
     _check(ctx.L.mh_find_matches_pairwise_list(ctx.h, arr, k, C.byref(h), C.byref(st)))
     try:
         n = ctx.L.mh_match_list_size(h)
-        out = np.zeros(n, dtype=MATCH_DTYPE)
+        out = np.empty(n, dtype=MATCH_DTYPE)
         if n:
             C.memmove(out.ctypes.data, ctx.L.mh_match_list_data(h), n * MATCH_DTYPE.itemsize)
     finally:
         ctx.L.mh_match_list_free(h)
-    return out["start"][:, :k].copy(), out["length"].copy(), st
+    return out, st
+
+
+def find_matches(ctx, smls):
+    """PairwiseMatchFinder::FindMatches: (starts int64 (n, k), lengths uint64, MatchStats)."""
+    out, st = find_matches_raw(ctx, smls)
+    return out["start"][:, :len(smls)].copy(), out["length"].copy(), st
 
 
 def ascii_to_codes_device(ctx, d_ascii, length, d_codes):
@@ -505,15 +511,22 @@ def ascii_to_codes_device(ctx, d_ascii, length, d_codes):
 
 def chain_gap_jobs(starts, lengths, len_a, len_b, max_gap=20000):
     """Stand-in DP job producer (see mh_chain_gap_jobs): (chain indices, jobs)."""
-    L = load_library()
     n = len(lengths)
-    m = np.zeros(max(n, 1), dtype=MATCH_DTYPE)
+    m = np.zeros(n, dtype=MATCH_DTYPE)
     if n:
-        m["start"][:n, :starts.shape[1]] = starts
-        m["length"][:n] = lengths
-    chain = np.zeros(max(n, 1), dtype=np.uint32)
+        m["start"][:, :starts.shape[1]] = starts
+        m["length"] = lengths
+    return chain_gap_jobs_raw(m, len_a, len_b, max_gap)
+
+
+def chain_gap_jobs_raw(matches, len_a, len_b, max_gap=20000):
+    """chain_gap_jobs on an mh_match array (as find_matches_raw returns it)."""
+    L = load_library()
+    m = np.ascontiguousarray(matches, dtype=MATCH_DTYPE)
+    n = len(m)
+    chain = np.empty(max(n, 1), dtype=np.uint32)
     nc, nj = C.c_uint64(), C.c_uint64()
     jobs = np.empty(n + 1, dtype=DP_JOB_DTYPE)    # a chain of k anchors has at most k + 1 gaps
-    _check(L.mh_chain_gap_jobs(m.ctypes.data, n, len_a, len_b, max_gap, chain.ctypes.data, C.byref(nc),
-                               jobs.ctypes.data, n + 1, C.byref(nj)))
-    return chain[:nc.value].copy(), jobs[:nj.value].copy()
+    _check(L.mh_chain_gap_jobs(m.ctypes.data if n else None, n, len_a, len_b, max_gap, chain.ctypes.data,
+                               C.byref(nc), jobs.ctypes.data, n + 1, C.byref(nj)))
+    return chain[:nc.value], jobs[:nj.value]
