@@ -610,10 +610,28 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
 {
     const uint64_t nitems = items.size();
     if (nitems == 0) return MH_OK;
-    // largest first so the long tails start early
+    // largest first so the long tails start early: a stable counting sort on
+    // the trace size quantised to 1/16 octave (the order only balances load)
     const double ts0 = now_ms();
-    std::stable_sort(items.begin(), items.end(),
-                     [](const DpItem &x, const DpItem &y) { return x.tb_bytes > y.tb_bytes; });
+    {
+        auto q = [](uint64_t v) -> uint32_t {
+            if (v < 16) return (uint32_t)v;
+            const int e = 63 - __builtin_clzll(v);
+            return (uint32_t)(e * 16 + ((v >> (e - 4)) & 15u));
+        };
+        constexpr uint32_t NQ = 64 * 16 + 16;
+        std::vector<uint32_t> cnt(NQ + 1, 0);
+        for (const DpItem &it : items) ++cnt[NQ - 1 - q(it.tb_bytes)];
+        uint32_t acc = 0;
+        for (uint32_t &c : cnt) {
+            const uint32_t t = c;
+            c = acc;
+            acc += t;
+        }
+        std::vector<DpItem> sorted(nitems);
+        for (const DpItem &it : items) sorted[cnt[NQ - 1 - q(it.tb_bytes)]++] = it;
+        items.swap(sorted);
+    }
     host_time(ctx, "host_dp_sort", now_ms() - ts0);
     auto bnd_ints = [&](const DpItem &it) {
         const uint64_t ns = (jobs[it.job].b_len + 64 * it.cols - 1) / (64 * it.cols);

@@ -410,17 +410,23 @@ __global__ void seed_kernel_solid(const uint32_t *__restrict__ pi, const uint32_
 
 // ---- bucket partition of the seeds ----------------------------------------
 // AddHashEntry puts a seed in bucket ((offset mod 40000) + 40000) mod 40000
-// (LM/MemHash.cpp:213).  In a bucket whose seeds are all on the forward
-// strand, share one offset and come from an odd solid seed, every extension is
-// the maximal exact match through its seed on that one diagonal, so two
-// extensions are disjoint or equal, and MheCompare (LM/MatchHashEntry.h:121-143)
-// restricted to the bucket orders entries by start and calls a seed equivalent
-// exactly to the extension containing it.  lower_bound then finds that
-// extension whenever it is stored, whatever its probe sequence: the first seed
-// of each extension inserts it, every later one collides, and the bucket ends
-// up holding the distinct extensions by start.  Such "clean" buckets are
-// resolved here by sorting on (bucket, extended start) and keeping the first of
-// each run; every other bucket's seeds go to the host for the literal replay.
+// (LM/MemHash.cpp:213).  Call a bucket clean when all its seeds come from an
+// odd solid seed, lie on one strand and share one offset, i.e. one diagonal
+// (forward) or one anti-diagonal (reverse: s0 + |s1| + len is constant).
+// Then every extension is the maximal exact match through its seed on that
+// line, so two extensions are disjoint or equal in genome 0.  On one line,
+// Contains (LM/MatchHashEntry.cpp:164-200) reduces to interval containment in
+// genome 0 (for the reverse strand its diff_rc == di test holds exactly on
+// the anti-diagonal), and strict_start_lessthan_ptr (48-67) decides disjoint
+// entries by their genome-0 start, which is positive.  So MheCompare
+// (LM/MatchHashEntry.h:121-143) restricted to the bucket orders entries by
+// start and calls a seed equivalent exactly to the extension containing it,
+// and lower_bound finds that extension whenever it is stored, whatever its
+// probe sequence: the first seed of each extension inserts it, every later
+// one collides, and the bucket ends up holding the distinct extensions by
+// start.  Clean buckets are resolved here by sorting on (bucket, extended
+// start) and keeping the first of each run; every other bucket's seeds go to
+// the host for the literal replay.
 
 constexpr uint32_t TABLE_SIZE = 40000;   // DEFAULT_MEM_TABLE_SIZE, LM/MemHash.h:30
 
@@ -443,29 +449,34 @@ __global__ void bucket_key_kernel(const SeedRec *__restrict__ seeds, uint64_t ns
     val[x] = (uint32_t)x;
 }
 
+// the seed's line: its offset and strand
+__device__ __forceinline__ int64_t seed_line(const SeedRec &r, int k)
+{
+    return seed_offset(r, k) * 2 + (r.s1 < 0 ? 1 : 0);
+}
+
 __global__ void bucket_first_kernel(const uint64_t *__restrict__ skey, const uint32_t *__restrict__ sval,
                                     const SeedRec *__restrict__ seeds, uint64_t ns, int k,
-                                    int64_t *__restrict__ first_off)
+                                    int64_t *__restrict__ first_line)
 {
     const uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= ns) return;
     const uint64_t b = skey[y];
-    if (y == 0 || skey[y - 1] != b) first_off[b] = seed_offset(seeds[sval[y]], k);
+    if (y == 0 || skey[y - 1] != b) first_line[b] = seed_line(seeds[sval[y]], k);
 }
 
 __global__ void bucket_dirty_kernel(const uint64_t *__restrict__ skey, const uint32_t *__restrict__ sval,
                                     const SeedRec *__restrict__ seeds, uint64_t ns, int k, int solid_odd,
-                                    const int64_t *__restrict__ first_off, uint32_t *__restrict__ dirty)
+                                    const int64_t *__restrict__ first_line, uint32_t *__restrict__ dirty)
 {
     const uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= ns) return;
     const uint64_t b = skey[y];
-    const SeedRec r = seeds[sval[y]];
-    if (!solid_odd || r.s1 < 0 || seed_offset(r, k) != first_off[b]) dirty[b] = 1u;
+    if (!solid_odd || seed_line(seeds[sval[y]], k) != first_line[b]) dirty[b] = 1u;
 }
 
 __global__ void key2_kernel(const uint64_t *__restrict__ skey, const uint32_t *__restrict__ sval,
-                            const SeedRec *__restrict__ seeds, uint64_t ns, const uint32_t *__restrict__ dirty,
+                            const SeedRec *__restrict__ seeds, uint64_t ns, int eb, const uint32_t *__restrict__ dirty,
                             uint64_t *__restrict__ key2, uint32_t *__restrict__ val2, uint32_t *__restrict__ flag_dirty)
 {
     const uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -473,18 +484,18 @@ __global__ void key2_kernel(const uint64_t *__restrict__ skey, const uint32_t *_
     const uint64_t b = skey[y];
     const uint32_t v = sval[y];
     const bool d = dirty[b] != 0;
-    key2[y] = (b << 32) | (d ? 0xFFFFFFFFull : (uint64_t)(uint32_t)seeds[v].e0);
+    key2[y] = (b << eb) | (d ? ((1ull << eb) - 1) : (uint64_t)(uint32_t)seeds[v].e0);
     val2[y] = v;
     flag_dirty[y] = d ? 1u : 0u;
 }
 
-__global__ void distinct_kernel(const uint64_t *__restrict__ key2, uint64_t ns, const uint32_t *__restrict__ dirty,
+__global__ void distinct_kernel(const uint64_t *__restrict__ key2, uint64_t ns, int eb, const uint32_t *__restrict__ dirty,
                                 uint32_t *__restrict__ flag)
 {
     const uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (z >= ns) return;
     const uint64_t kz = key2[z];
-    flag[z] = (!dirty[kz >> 32] && (z == 0 || key2[z - 1] != kz)) ? 1u : 0u;
+    flag[z] = (!dirty[kz >> eb] && (z == 0 || key2[z - 1] != kz)) ? 1u : 0u;
 }
 
 __global__ __launch_bounds__(JT) void compact_idx_kernel(const uint32_t *__restrict__ flag, uint64_t n,
@@ -516,7 +527,7 @@ __global__ __launch_bounds__(JT) void compact_idx_kernel(const uint32_t *__restr
         if (f[r]) out_idx[o++] = (uint32_t)(base + r);
 }
 
-__global__ void gather_clean_kernel(const uint32_t *__restrict__ idx, uint64_t m, const uint64_t *__restrict__ key2,
+__global__ void gather_clean_kernel(const uint32_t *__restrict__ idx, uint64_t m, int eb, const uint64_t *__restrict__ key2,
                                     const uint32_t *__restrict__ val2, const SeedRec *__restrict__ seeds,
                                     CleanEnt *__restrict__ out)
 {
@@ -525,7 +536,7 @@ __global__ void gather_clean_kernel(const uint32_t *__restrict__ idx, uint64_t m
     const uint32_t z = idx[x];
     const SeedRec r = seeds[val2[z]];
     CleanEnt e;
-    e.bucket = (uint32_t)(key2[z] >> 32);
+    e.bucket = (uint32_t)(key2[z] >> eb);
     e.e0 = r.e0;
     e.e1 = r.e1;
     e.elen = r.elen;
@@ -647,9 +658,13 @@ static int compact_flags(mh_context *ctx, const uint32_t *flag, uint64_t n, uint
     return MH_OK;
 }
 
-int partition_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t ns, int k, bool solid_odd,
+int partition_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t ns, int k, bool solid_odd, uint64_t len_a,
                     std::vector<CleanEnt> &clean, std::vector<DirtySeed> &dirty, uint64_t *n_clean_seeds)
 {
+    // the second sort key is (bucket << eb) | extended start, eb bits holding
+    // every start (1-based, <= len_a) and the all-ones mark of other buckets
+    int eb = 1;
+    while (eb < 32 && (1ull << eb) - 1 <= len_a) ++eb;
     clean.clear();
     dirty.clear();
     *n_clean_seeds = 0;
@@ -689,8 +704,8 @@ int partition_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t ns, int k,
         // (sk, sv) now hold the bucket order; the other pair is free for the second sort
         uint64_t *k2a = (sk == kA) ? kB : kA;
         uint32_t *v2a = (sv == vA) ? vB : vA;
-        hipLaunchKernelGGL(key2_kernel, dim3(grid), dim3(256), 0, ctx->stream, sk, sv, d_seeds, ns, dirty_b, k2a, v2a,
-                           flag);
+        hipLaunchKernelGGL(key2_kernel, dim3(grid), dim3(256), 0, ctx->stream, sk, sv, d_seeds, ns, eb, dirty_b, k2a,
+                           v2a, flag);
         MH_HIP(hipGetLastError());
         // dirty seeds in (bucket, call) order
         uint64_t nd = 0;
@@ -708,14 +723,14 @@ int partition_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t ns, int k,
             // second sort: (bucket, extended start); the bucket-order pair is scratch now
             uint64_t *k2b = (k2a == kA) ? kB : kA;
             uint32_t *v2b = (v2a == vA) ? vB : vA;
-            if ((rc = radix_sort_kv(ctx, k2a, v2a, k2b, v2b, ns, 48, &s2, &v2)) != MH_OK) return rc;
-            hipLaunchKernelGGL(distinct_kernel, dim3(grid), dim3(256), 0, ctx->stream, s2, ns, dirty_b, flag);
+            if ((rc = radix_sort_kv(ctx, k2a, v2a, k2b, v2b, ns, 16 + eb, &s2, &v2)) != MH_OK) return rc;
+            hipLaunchKernelGGL(distinct_kernel, dim3(grid), dim3(256), 0, ctx->stream, s2, ns, eb, dirty_b, flag);
             uint64_t nc = 0;
             if ((rc = compact_flags(ctx, flag, ns, idx, tiles, d_count, &nc)) != MH_OK) return rc;
             CleanEnt *d_clean = static_cast<CleanEnt *>(gather);
             if (nc) {
                 hipLaunchKernelGGL(gather_clean_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, ctx->stream,
-                                   idx, nc, s2, v2, d_seeds, d_clean);
+                                   idx, nc, eb, s2, v2, d_seeds, d_clean);
                 clean.resize(nc);
                 MH_HIP(hipMemcpyAsync(clean.data(), d_clean, nc * sizeof(CleanEnt), hipMemcpyDeviceToHost,
                                       ctx->stream));

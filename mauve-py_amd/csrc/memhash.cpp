@@ -7,8 +7,8 @@
 // (LM/MemHash.cpp:209-251).  MheCompare is not a strict weak order, so the
 // result depends on the exact probe sequence of lower_bound and on the
 // insertion history of the bucket; both are reproduced here literally.
-// Buckets whose outcome is fixed in closed form (one diagonal, forward strand,
-// odd solid seed) were resolved on the GPU; the others never interact, so
+// Buckets whose outcome is fixed in closed form (one diagonal or
+// anti-diagonal, odd solid seed) were resolved on the GPU; the others never interact, so
 // they are replayed in parallel host threads, each in the seeds' HashMatch
 // order.  Extensions were computed on the GPU.
 #include "mh_internal.hpp"
@@ -108,7 +108,7 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
     std::vector<CleanEnt> clean;
     std::vector<DirtySeed> dirty;
     uint64_t n_clean_seeds = 0;
-    if ((rc = partition_seeds(ctx, d_seeds, nseeds, (int)sl, solid_odd, clean, dirty, &n_clean_seeds)) != MH_OK)
+    if ((rc = partition_seeds(ctx, d_seeds, nseeds, (int)sl, solid_odd, a->len, clean, dirty, &n_clean_seeds)) != MH_OK)
         return rc;
     const double tg1 = now_ms();
     // the remaining buckets: runs of equal bucket in the dirty list, each in HashMatch order

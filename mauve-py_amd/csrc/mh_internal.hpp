@@ -155,7 +155,7 @@ struct DirtySeed {
 // Splits the seeds by MemHash bucket: clean buckets resolved on the device
 // (distinct extensions, bucket then start order), the rest returned in bucket
 // then HashMatch order for the literal replay.
-int partition_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t ns, int k, bool solid_odd,
+int partition_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t ns, int k, bool solid_odd, uint64_t len_a,
                     std::vector<CleanEnt> &clean, std::vector<DirtySeed> &dirty, uint64_t *n_clean_seeds);
 int sml_find_mer_dev(mh_context *ctx, const mh_sml *s, uint64_t mer, uint64_t *index);
 uint64_t sml_full_key(uint64_t c, int sw);

@@ -174,6 +174,14 @@ __global__ __launch_bounds__(SCAN_T) void digit_scan_kernel(uint32_t *__restrict
     if (t == 0) totals[blockIdx.x] = carry;
 }
 
+// Stable scatter of one tile.  Wave w ranks its own contiguous quarter of the
+// tile (16 rounds of 64 keys): per round, lanes with the same digit find each
+// other with 8 ballots, read the wave's running count of that digit from LDS
+// and the round's leader advances it.  LDS operations of one wave complete in
+// order, so no workgroup barrier is needed between rounds.  At the end the
+// per-wave counts become wave offsets, the tile is reordered by digit in LDS,
+// and each digit run is written to its global offset (digit start from the
+// scanned totals + this tile's offset within the digit).
 __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *__restrict__ kin,
                                                               const uint32_t *__restrict__ vin,
                                                               uint64_t *__restrict__ kout,
@@ -181,30 +189,30 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *_
                                                               int shift, const uint32_t *__restrict__ offs,
                                                               uint32_t ntiles, const uint32_t *__restrict__ totals)
 {
+    constexpr int NWAVES = SORT_THREADS / 64;
     __shared__ uint64_t sk[SORT_TILE];
     __shared__ uint32_t sv[SORT_TILE];
-    __shared__ uint32_t wcnt[SORT_THREADS / 64][RADIX];
-    __shared__ uint32_t run[RADIX];
+    __shared__ uint32_t wcnt[NWAVES][RADIX];
     __shared__ uint32_t start[RADIX];
     __shared__ uint32_t dbase[RADIX];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     dbase[t] = totals[t];
+    for (int w = 0; w < NWAVES; ++w) wcnt[w][t] = 0;
     const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
+    const uint64_t wbase = base + (uint64_t)wv * (SORT_ITEMS * 64);
     const uint64_t lane_lt = (1ull << lane) - 1ull;
     uint64_t key[SORT_ITEMS];
     uint32_t val[SORT_ITEMS], rank[SORT_ITEMS];
-    run[t] = 0;
-    for (int w = 0; w < SORT_THREADS / 64; ++w) wcnt[w][t] = 0;
 #pragma unroll
     for (int r = 0; r < SORT_ITEMS; ++r) {
-        const uint64_t x = base + (uint64_t)r * SORT_THREADS + t;
+        const uint64_t x = wbase + (uint64_t)r * 64 + lane;
         key[r] = x < n ? kin[x] : 0ull;
         val[r] = x < n ? vin[x] : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < SORT_ITEMS; ++r) {
-        const bool valid = base + (uint64_t)r * SORT_THREADS + t < n;
+        const bool valid = wbase + (uint64_t)r * 64 + lane < n;
         const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -213,28 +221,25 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *_
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
         const uint32_t below = (uint32_t)__popcll(peers & lane_lt);
-        if (valid && below == 0) wcnt[wv][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {   // thread t owns digit t: wave offsets for this round
-            uint32_t acc = run[t];
-            for (int w = 0; w < SORT_THREADS / 64; ++w) {
-                const uint32_t c = wcnt[w][t];
-                wcnt[w][t] = acc;
-                acc += c;
-            }
-            run[t] = acc;
-        }
-        __syncthreads();
-        rank[r] = valid ? wcnt[wv][d] + below : 0u;
-        __syncthreads();
-        for (int w = 0; w < SORT_THREADS / 64; ++w) wcnt[w][t] = 0;
-        __syncthreads();
+        const uint32_t cur = wcnt[wv][d];
+        rank[r] = cur + below;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) wcnt[wv][d] = cur + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // thread t owns digit t: wave offsets within the tile and the tile count
+    uint32_t cnt = 0;
+    for (int w = 0; w < NWAVES; ++w) {
+        const uint32_t c = wcnt[w][t];
+        wcnt[w][t] = cnt;
+        cnt += c;
     }
     // exclusive scans of the tile's digit counts (start of each digit run in
     // the tile) and of the global digit totals (start of each digit's output)
     {
-        const uint32_t c = run[t], g = dbase[t];
-        start[t] = c;
+        const uint32_t g = dbase[t];
+        start[t] = cnt;
         __syncthreads();
         for (int off = 1; off < RADIX; off <<= 1) {
             const uint32_t v = t >= off ? start[t - off] : 0u;
@@ -246,15 +251,15 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *_
         }
         const uint32_t incl = start[t], gincl = dbase[t];
         __syncthreads();
-        start[t] = incl - c;
+        start[t] = incl - cnt;
         dbase[t] = gincl - g;
         __syncthreads();
     }
 #pragma unroll
     for (int r = 0; r < SORT_ITEMS; ++r) {
-        if (base + (uint64_t)r * SORT_THREADS + t < n) {
+        if (wbase + (uint64_t)r * 64 + lane < n) {
             const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
-            const uint32_t p = start[d] + rank[r];
+            const uint32_t p = start[d] + wcnt[wv][d] + rank[r];
             sk[p] = key[r];
             sv[p] = val[r];
         }

@@ -127,3 +127,36 @@ def test_spaced_seed_invariants():
     for (s0, s1), L in zip(starts, lengths):
         if s1 > 0:
             assert any(s0 <= t0 and t0 + sl <= s0 + L and t1 - t0 == s1 - s0 for t0, t1 in st if t1 > 0)
+
+
+@pytest.mark.parametrize("k", [11, 13, 17])
+def test_clean_bucket_closed_form(k):
+    """A bucket whose seeds all lie on one diagonal or one anti-diagonal
+    (same offset, same strand) ends as its seeds' distinct extensions in
+    genome-0 start order -- the closed form the GPU resolves such buckets
+    with -- for both strands, checked against the literal replay."""
+    rng = np.random.default_rng(300 + k)
+    a, b = _pair(rng, 6000, snp=0.02)
+    seed = (1 << k) - 1
+    starts, lengths, _ = po.find_matches([a, b], seed)
+    st, off = po.seed_tuples([a, b], seed)
+
+    def bucket(o):
+        return ((int(o) % 40000) + 40000) % 40000
+    lines = {}
+    for (s0, s1), o in zip(st, off):
+        lines.setdefault(bucket(o), set()).add((int(o), s1 < 0))
+    got = {}
+    for s, L in zip(starts, lengths):
+        s0, s1 = int(s[0]), int(s[1])
+        o = s1 - s0 - (int(L) if s1 < 0 else 0)
+        got.setdefault(bucket(o), []).append((s0, s1, int(L)))
+    clean_rc = 0
+    for bk, ls in lines.items():
+        if len(ls) != 1:
+            continue
+        members = [(int(s0), int(s1)) for (s0, s1), o in zip(st, off) if bucket(o) == bk]
+        ext = sorted({brute_extend(a, b, s0, s1, k) for s0, s1 in members})
+        assert got[bk] == ext, bk
+        clean_rc += next(iter(ls))[1]
+    assert clean_rc > 0   # reverse-strand clean buckets were exercised
