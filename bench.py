@@ -247,6 +247,7 @@ def learned_band_leg(ctx, bandset):
     small.align(None)
     small.free()
 
+    dj.align(None)      # grows the trace arena to its working size
     ctx.synchronize()
     t0 = time.perf_counter()
     dj.align(None)

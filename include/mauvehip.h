@@ -173,16 +173,17 @@ int mh_dp_align_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
 
 /* Banded mode (no counterpart in the reference, whose NWSmall always fills
  * the whole matrix; the full DP stays the default and the judge of it).
- * bands[j] = W > 0 restricts job j to the stripes' rows that meet the band
- * |j - i*b_len/a_len| <= W around the corner-to-corner line (a superset of
- * the band, see mh_dp_stripe_width); W = 0 or bands == NULL is the full DP.
- * The result equals the full DP's whenever the full DP's path keeps within
- * distance < W of the line.  Edge fallback: a job whose banded path comes
- * within distance W of the line's band edge (d_dev[j] >= W) is rerun with 2W,
- * and if it reaches that edge too, with the full DP, unless flags has
- * MH_BAND_NO_FALLBACK.  d_dev (device, nullable) receives each final path's
- * largest distance from the line, ceil(max |i*b_len - j*a_len| / a_len) over
- * its positions (i, j): for full jobs this is the band label. */
+ * bands[j] = W > 0 restricts job j to the region of band 2W around the
+ * corner-to-corner line (mh_dp_band_region(a_len, b_len, 2W), a superset of
+ * the cells within 2W of the line); W = 0 or bands == NULL is the full DP.
+ * The result equals the full DP's whenever the full DP's path stays inside
+ * that region.  Edge fallback: a job whose path comes within distance W of
+ * the line's band edge (d_dev[j] >= W) is rerun in the region of band 4W and
+ * accepted within 2W, else rerun with the full DP, unless flags has
+ * MH_BAND_NO_FALLBACK (which returns the first region's result).  d_dev
+ * (device, nullable) receives each final path's largest distance from the
+ * line, ceil(max |i*b_len - j*a_len| / a_len) over its positions (i, j): for
+ * full jobs this is the band label. */
 #define MH_BAND_NO_FALLBACK 1u
 typedef struct {
     uint64_t banded;          /* jobs run with a band in the first pass        */
@@ -195,13 +196,28 @@ int mh_dp_align_banded_device(mh_context *ctx, const uint8_t *d_a, const uint8_t
                               uint32_t flags, uint8_t *d_paths, mh_dp_result *d_results,
                               uint32_t *d_dev, mh_band_stats *stats);
 
-/* The shape the scheduler gives a job: returns the effective band (0 when the
- * band's region is the whole matrix) and writes the stripe width w (columns).
- * A banded job evaluates, in each stripe s (columns c0 = s*w+1 ..
- * c1 = min((s+1)*w, b_len)), the rows i with
- * ceil(a_len*(c0-W)/b_len) <= i <= floor(a_len*(c1+W)/b_len), clipped to
- * [1, a_len]. */
-uint32_t mh_dp_band_shape(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *stripe_w);
+/* The region the scheduler gives a job: returns the effective band (0 when
+ * the band's region is the whole matrix) and writes the region kind and width.
+ *   MH_REGION_FULL     every cell.
+ *   MH_REGION_STRIPES  columns cut into stripes of `width`; stripe s (columns
+ *                      c0 = s*w+1 .. c1 = min((s+1)*w, b_len)) evaluates rows
+ *                      ceil(a_len*(c0-W)/b_len) .. floor(a_len*(c1+W)/b_len),
+ *                      clipped to [1, a_len].
+ *   MH_REGION_WINDOW   row i evaluates the `width` columns base(i)+1 ..
+ *                      base(i)+width, base(i) = ceil((i*b_len - W*a_len)/a_len)
+ *                      - 1 clamped to [0, b_len - width].
+ * Both banded regions contain every cell within W of the corner-to-corner
+ * line; a window reaches at least 2W on each side.  Stripes are the default;
+ * MH_DP_BAND_KIND=window in the environment selects the window where it
+ * fits (slope <= 4, band <= 127). */
+#define MH_REGION_FULL 0u
+#define MH_REGION_STRIPES 1u
+#define MH_REGION_WINDOW 2u
+uint32_t mh_dp_band_region(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *kind, uint32_t *width);
+/* DP cells the scheduler evaluates for each (a_len, b_len, band) -- the
+ * region's cells, 64C per row for a window -- for cost models (host only). */
+int mh_dp_band_cells(const uint32_t *a_len, const uint32_t *b_len, const uint32_t *band, uint64_t n,
+                     uint64_t *cells);
 
 /* ---- learned band (no counterpart in the reference) ----------------------
  *

@@ -362,6 +362,259 @@ __global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs
         dp_job<4>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
 }
 
+// ---- sliding-window banded DP ---------------------------------------------
+//
+// A banded job whose band fits 64*C columns (C = 1, 2, 4, 8) runs as a window
+// that slides along the corner-to-corner line: all 64 lanes work on one row,
+// lane k holding window columns base(i)+1+kC .. +C, with
+//   base(i) = clamp(ceil((i*lb - h*la)/la) - 1, 0, lb - 64C),  h = (64C-1)/2
+// so the window is centred on the line and holds every cell within h of it.
+// Cells outside the window are -infinity: the result is the exact DP over the
+// window region.  Per row: the previous row's M, D', I' shift left by
+// s = base(i) - base(i-1) columns (DPP wave_shl:1 per column step), M and D'
+// follow from the previous row (right-to-left register pass, as dp_job), and
+// I' -- a running maximum along the row since the gap extension is 0 -- is a
+// lane-local prefix maximum joined by a cross-lane exclusive prefix maximum
+// (DPP row_shr within 16-lane rows, readlane across rows).  Trace: C*4 bits
+// per lane per row, 8/C rows per 32-bit word; the word of row i, lane k is
+// at ((i-1) / (8/C)) * 64 + k.
+constexpr uint32_t WINDOW_TAG = 64;   // DevJob::cols = WINDOW_TAG + C
+
+__host__ __device__ inline int window_base(int la, int lb, int width, int i)
+{
+    const int64_t num = (int64_t)i * lb - (int64_t)((width - 1) / 2) * la;
+    const int64_t q = num >= 0 ? (num + la - 1) / la : -((-num) / la);
+    int64_t b = q - 1;
+    const int64_t top = lb > width ? (int64_t)lb - width : 0;
+    if (b > top) b = top;
+    if (b < 0) b = 0;
+    return (int)b;
+}
+
+__device__ __forceinline__ int shl1_old(int v, int old)
+{
+    // wave_shl:1 -- lane k receives lane k+1's value, lane 63 receives `old`
+    return __builtin_amdgcn_update_dpp(old, v, 0x130, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ int row_shr_old(int v, int old, int n)
+{
+    switch (n) {
+    case 1: return __builtin_amdgcn_update_dpp(old, v, 0x111, 0xF, 0xF, false);
+    case 2: return __builtin_amdgcn_update_dpp(old, v, 0x112, 0xF, 0xF, false);
+    case 4: return __builtin_amdgcn_update_dpp(old, v, 0x114, 0xF, 0xF, false);
+    default: return __builtin_amdgcn_update_dpp(old, v, 0x118, 0xF, 0xF, false);
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void dp_window_job(const DevJob &jb, const uint8_t *__restrict__ A,
+                                              const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
+                                              DpOut *__restrict__ out, uint32_t sr0, uint32_t sr1, uint32_t sr2,
+                                              uint32_t sr3, int goh)
+{
+    constexpr int WD = 64 * C;
+    constexpr int RPD = 8 / C;   // rows per trace word
+    const int lane = threadIdx.x;
+    const int la = (int)jb.la, lb = (int)jb.lb;
+    const uint8_t *a = A + jb.a_off;
+    const uint8_t *b = B + jb.b_off;
+    uint32_t *tbj = reinterpret_cast<uint32_t *>(tb + jb.tb_off);
+    const int go2 = 2 * goh;
+    // base(i) without a division per row: q = ceil((i*lb - h*la)/la) kept with
+    // its remainder rem = q*la - (i*lb - h*la) in [0, la)
+    const int top = lb > WD ? lb - WD : 0;
+    const int dq = lb / la, dr = lb % la;
+    int q, rem;
+    {
+        const int64_t num = (int64_t)lb - (int64_t)((WD - 1) / 2) * la;
+        const int64_t qq = num >= 0 ? (num + la - 1) / la : -((-num) / la);
+        q = (int)qq;
+        rem = (int)(qq * la - num);
+    }
+    auto clampb = [&](int v) { return v - 1 > top ? top : (v - 1 < 0 ? 0 : v - 1); };
+    int bprev = clampb(q);
+    int Mp[C], Dp[C], Ip[C];
+    uint32_t bsh[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {   // row 0 in the frame of base(1)
+        Mp[c] = NEG;
+        Dp[c] = NEG + goh;
+        Ip[c] = goh;
+        const int j0 = bprev + lane * C + c;
+        bsh[c] = j0 < lb ? 8u * (uint32_t)b[j0] : 0u;
+    }
+    // A codes 64 rows at a time, the next chunk loaded one chunk ahead
+    int achunk = lane < la ? a[lane] : 0;
+    int anext = 64 + lane < la ? a[64 + lane] : 0;
+    uint32_t bnx[C];   // B codes of the next row's window, loaded a row ahead
+#pragma unroll
+    for (int c = 0; c < C; ++c) bnx[c] = bsh[c];
+    uint32_t tbw = 0;
+    int finM = NEG, finD = NEG, finI = NEG;
+    for (int i = 1; i <= la; ++i) {
+        if (i > 1) {
+            q += dq;
+            rem -= dr;
+            if (rem < 0) {
+                ++q;
+                rem += la;
+            }
+        }
+        const int bi = clampb(q);
+        const int s = bi - bprev;
+        if (s) {
+            // this row's window codes, loaded during the previous row
+#pragma unroll
+            for (int c = 0; c < C; ++c) bsh[c] = bnx[c];
+        }
+        {   // issue the next row's B codes if its window moves
+            int qn = q + dq;
+            if (rem - dr < 0) ++qn;
+            const int bn = clampb(qn);
+            if (bn != bi && i < la) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const int j0 = bn + lane * C + c;
+                    bnx[c] = j0 < lb ? 8u * (uint32_t)b[j0] : 0u;
+                }
+            }
+        }
+        // (i-1, bi): the cell left of the window on the previous row
+        int ldM, ldD, ldI;
+        if (i == 1) {
+            ldM = bi == 0 ? 0 : NEG;
+            ldD = NEG + goh;
+            ldI = bi == 0 ? NEG + goh : goh;
+        } else if (bi == 0) {
+            ldM = NEG;
+            ldD = goh;
+            ldI = NEG + goh;
+        } else {
+            ldM = NEG;
+            ldD = NEG;
+            ldI = NEG;
+            for (int st = 0; st < s; ++st) {
+                if (st == s - 1) {
+                    ldM = __builtin_amdgcn_readlane(Mp[0], 0);
+                    ldD = __builtin_amdgcn_readlane(Dp[0], 0);
+                    ldI = __builtin_amdgcn_readlane(Ip[0], 0);
+                }
+                const int eM = shl1_old(Mp[0], NEG), eD = shl1_old(Dp[0], NEG), eI = shl1_old(Ip[0], NEG);
+#pragma unroll
+                for (int c = 0; c + 1 < C; ++c) {
+                    Mp[c] = Mp[c + 1];
+                    Dp[c] = Dp[c + 1];
+                    Ip[c] = Ip[c + 1];
+                }
+                Mp[C - 1] = eM;
+                Dp[C - 1] = eD;
+                Ip[C - 1] = eI;
+            }
+        }
+        if (i > 1 && ((i - 1) & 63) == 0) {
+            achunk = anext;
+            anext = (i + 63 + lane < la) ? a[i + 63 + lane] : 0;
+        }
+        const int ra = __builtin_amdgcn_readlane(achunk, (i - 1) & 63);
+        const uint32_t srow = (ra & 2) ? ((ra & 1) ? sr3 : sr2) : ((ra & 1) ? sr1 : sr0);
+        const int oa2 = (i == 1) ? goh : go2;   // open_A[i-1] + goh
+        // diagonal inputs of the lane's first column: lane k-1's last column
+        // of the previous row, or the cell left of the window for lane 0
+        const int dM0 = shr1_old(Mp[C - 1], ldM), dD0 = shr1_old(Dp[C - 1], ldD), dI0 = shr1_old(Ip[C - 1], ldI);
+        uint32_t bits = 0;
+        // pass 1, right to left: D'(i,j) and M(i,j) from row i-1
+        static_for(std::make_integer_sequence<int, C>{}, [&](auto ic) {
+            constexpr int c = C - 1 - decltype(ic)::value;
+            constexpr uint32_t K = 1u << (4 * c);
+            const int md = Mp[c] + oa2;
+            const bool tD = md >= Dp[c];
+            const int Dn = max(md, Dp[c]);
+            const int dM = c ? Mp[c > 0 ? c - 1 : 0] : dM0;
+            const int dD = c ? Dp[c > 0 ? c - 1 : 0] : dD0;
+            const int dI = c ? Ip[c > 0 ? c - 1 : 0] : dI0;
+            const int best = max3i(dM, dD, dI);
+            uint32_t srcK = (dM != best) ? ((dD == best) ? K : 2u * K) : 0u;
+            const int sc = (int)__builtin_amdgcn_ubfe(srow, bsh[c], 8) - SBIAS;
+            int Mn = sc + best;
+            if (c == 0 && la == 1 && i == 1 && bi == 0 && lane == 0) {
+                // one-row quirk: with LA == 1, M(1,1) carries close_A[0] and a 'D' bit
+                Mn = sc + goh;
+                srcK = K;
+            }
+            Mp[c] = Mn;
+            Dp[c] = Dn;
+            bits |= srcK | (tD ? 4u * K : 0u);
+        });
+        // pass 2: I'(i,j) = max(I'(i,j-1), M(i,j-1) + open_B[j-1] + goh) along the row
+        const int mleft = shr1_old(Mp[C - 1], NEG);   // M(i, first column - 1); left of the window: NEG
+        const int j1 = bi + lane * C + 1;                // the lane's first column
+        int cand[C], pre[C];
+        int run = NEG;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int mprev = c ? Mp[c > 0 ? c - 1 : 0] : mleft;
+            cand[c] = mprev + ((j1 + c == 1) ? goh : go2);
+            run = max(run, cand[c]);
+            pre[c] = run;
+        }
+        // exclusive prefix maximum of the lane totals across the wave
+        int v = run;
+        v = max(v, row_shr_old(v, NEG, 1));
+        v = max(v, row_shr_old(v, NEG, 2));
+        v = max(v, row_shr_old(v, NEG, 4));
+        v = max(v, row_shr_old(v, NEG, 8));
+        const int t15 = __builtin_amdgcn_readlane(v, 15), t31 = __builtin_amdgcn_readlane(v, 31);
+        const int t47 = __builtin_amdgcn_readlane(v, 47);
+        const int r1 = t15, r2 = max(t15, t31), r3 = max(r2, t47);
+        v = max(v, lane >= 48 ? r3 : (lane >= 32 ? r2 : (lane >= 16 ? r1 : NEG)));
+        const int leftI = bi == 0 ? NEG + goh : NEG;    // I'(i, window start - 1)
+        const int excl = shr1_old(v, leftI);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const int iprev = c ? max(excl, pre[c > 0 ? c - 1 : 0]) : excl;
+            const bool tI = cand[c] >= iprev;
+            Ip[c] = max(iprev, cand[c]);
+            bits |= tI ? (8u << (4 * c)) : 0u;
+        }
+        const int sub = (i - 1) % RPD;
+        tbw |= bits << (4 * C * sub);
+        if (sub == RPD - 1 || i == la) {
+            tbj[(size_t)((i - 1) / RPD) * 64 + lane] = tbw;
+            tbw = 0;
+        }
+        if (i == la && lane == 63) {
+            finM = Mp[C - 1];
+            finD = Dp[C - 1];
+            finI = Ip[C - 1];
+        }
+        bprev = bi;
+    }
+    if (lane == 63) {
+        finD -= goh;
+        finI -= goh;
+        int score = finM, edge = 'M';
+        if (finD > score) { score = finD; edge = 'D'; }
+        if (finI > score) { score = finI; edge = 'I'; }
+        out->score = score;
+        out->edge = edge;
+    }
+}
+
+__global__ __launch_bounds__(64) void dp_window(const DevJob *__restrict__ jobs, const uint8_t *__restrict__ A,
+                                                const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
+                                                DpOut *__restrict__ outs, uint32_t sr0, uint32_t sr1, uint32_t sr2,
+                                                uint32_t sr3, int goh)
+{
+    const DevJob jb = jobs[blockIdx.x];
+    switch (jb.cols - WINDOW_TAG) {
+    case 1: dp_window_job<1>(jb, A, B, tb, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh); break;
+    case 2: dp_window_job<2>(jb, A, B, tb, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh); break;
+    case 4: dp_window_job<4>(jb, A, B, tb, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh); break;
+    default: dp_window_job<8>(jb, A, B, tb, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh); break;
+    }
+}
+
 // Narrow jobs (lb <= 16): one lane per job.  The lane keeps a whole row of
 // M, D' and I' in registers and walks the rows of A, with the same two
 // register passes per row as dp_job; columns past lb are computed and
@@ -502,6 +755,12 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
             if (jb.cols == TINY_COLS_TAG) {
                 const uint8_t byte = tj[(uint64_t)(i - 1) * 8 + ((j - 1) >> 1)];
                 nib = (byte >> (4 * ((j - 1) & 1))) & 15u;
+            } else if (jb.cols > WINDOW_TAG) {
+                const int C = (int)jb.cols - (int)WINDOW_TAG, rpd = 8 / C;
+                const int x = j - window_base(la, lb, 64 * C, i) - 1;
+                if (x < 0 || x >= 64 * C) { ok = false; break; }
+                const uint32_t w = reinterpret_cast<const uint32_t *>(tj)[(uint64_t)((i - 1) / rpd) * 64 + x / C];
+                nib = (w >> (4 * (C * ((i - 1) % rpd) + x % C))) & 15u;
             } else {
                 if (i < r0 || i > r1) { ok = false; break; }
                 const int cc = (j - 1) % sw, k = cc / cols, c = cc % cols;
@@ -592,6 +851,34 @@ uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force)
     return best;
 }
 
+// The sliding window spends much of each row on the shift and the cross-lane
+// scan: measured on MI355X over 100k learned bands it evaluates its cells
+// about 1.45x slower than the stripes evaluate theirs and saves fewer cells
+// than that, so the scheduler uses it only when MH_DP_BAND_KIND=window asks.
+int dp_force_kind()
+{
+    if (const char *e = getenv("MH_DP_BAND_KIND")) {
+        if (!strcmp(e, "stripes")) return 1;
+        if (!strcmp(e, "window")) return 2;
+    }
+    return 0;
+}
+
+// Columns per lane of the sliding window for a banded region (0: use
+// stripes): when the window is asked for (kind 2), the narrowest window
+// reaching the region band on each side of the line (64C >= 2R+2), if it is
+// narrower than B and the slope lb/la is at most 4.
+uint32_t dp_window_cols(uint32_t la, uint32_t lb, uint32_t band, int kind)
+{
+    if (band == 0 || kind != 2 || (uint64_t)lb > 4ull * la) return 0;
+    int q = 0;
+    while (q < 4 && 64u * (1u << q) < 2u * band + 2u) ++q;
+    if (q == 4) return 0;
+    const uint32_t c = 1u << q;
+    if (64u * c >= lb) return 0;
+    return c;
+}
+
 // A band whose region is the whole matrix is the full DP.
 uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols)
 {
@@ -634,6 +921,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
     }
     host_time(ctx, "host_dp_sort", now_ms() - ts0);
     auto bnd_ints = [&](const DpItem &it) {
+        if (it.cols == TINY_COLS_TAG || it.cols > WINDOW_TAG) return 0ull;
         const uint64_t ns = (jobs[it.job].b_len + 64 * it.cols - 1) / (64 * it.cols);
         return ns > 1 ? 3ull * ((jobs[it.job].a_len + 63) & ~63ull) : 0ull;
     };
@@ -672,11 +960,15 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             ++last;
         }
         const uint64_t n = last - first;
-        // one-lane jobs go after the wavefront jobs, keeping their order
+        // stripe jobs, then sliding-window jobs, then one-lane jobs, each in order
         std::stable_partition(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)last,
                               [](const DpItem &it) { return it.cols != TINY_COLS_TAG; });
         uint64_t nbig = 0;
         while (nbig < n && items[first + nbig].cols != TINY_COLS_TAG) ++nbig;
+        std::stable_partition(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)(first + nbig),
+                              [](const DpItem &it) { return it.cols < WINDOW_TAG; });
+        uint64_t nstripe = 0;
+        while (nstripe < nbig && items[first + nstripe].cols < WINDOW_TAG) ++nstripe;
         dj.resize(n);
         uint64_t tw = 0, bo = 0;
         for (uint64_t x = 0; x < n; ++x) {
@@ -707,10 +999,14 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         MH_HIP(hipMemcpyAsync(d_jobs, dj.data(), n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
         {
             TimedLaunch tl(ctx, "dp_forward");
-            if (nbig)
-                hipLaunchKernelGGL(dp_forward, dim3((unsigned)nbig), dim3(64), 0, ctx->stream, d_jobs, d_a, d_b,
+            if (nstripe)
+                hipLaunchKernelGGL(dp_forward, dim3((unsigned)nstripe), dim3(64), 0, ctx->stream, d_jobs, d_a, d_b,
                                    ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs, sr[0], sr[1], sr[2],
                                    sr[3], -200);
+            if (nbig > nstripe)
+                hipLaunchKernelGGL(dp_window, dim3((unsigned)(nbig - nstripe)), dim3(64), 0, ctx->stream,
+                                   d_jobs + nstripe, d_a, d_b, ctx->dp_tb.as<uint8_t>(), d_outs + nstripe, sr[0],
+                                   sr[1], sr[2], sr[3], -200);
             if (n > nbig)
                 hipLaunchKernelGGL(dp_tiny, dim3((unsigned)((n - nbig + 63) / 64)), dim3(64), 0, ctx->stream,
                                    d_jobs + nbig, (int)(n - nbig), d_a, d_b, ctx->dp_tb.as<uint8_t>(),
@@ -759,6 +1055,13 @@ static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band, int fo
         return it;
     }
     it.tb_bytes = dp_trace_bytes(jb.a_len, jb.b_len, it.band, it.cols);
+    if (it.band) {
+        const uint32_t wc = dp_window_cols(jb.a_len, jb.b_len, it.band, dp_force_kind());
+        if (wc) {
+            it.cols = WINDOW_TAG + wc;
+            it.tb_bytes = (uint64_t)((jb.a_len + 8 / wc - 1) / (8 / wc)) * 256;
+        }
+    }
     return it;
 }
 
@@ -784,6 +1087,15 @@ static void make_items(const mh_dp_job *jobs, uint64_t njobs, const uint32_t *ba
         if (lo < hi) pool.emplace_back(fill, lo, hi);
     }
     for (auto &th : pool) th.join();
+}
+
+uint64_t dp_item_cells(uint32_t la, uint32_t lb, uint32_t band)
+{
+    if (la == 0 || lb == 0) return 0;
+    const DpItem it = make_item(mh_dp_job{0, 0, la, lb}, 0, band, dp_force_cols());
+    if (it.cols > WINDOW_TAG) return (uint64_t)la * 64 * (it.cols - WINDOW_TAG);
+    if (it.cols == TINY_COLS_TAG) return (uint64_t)la * lb;
+    return 2 * it.tb_bytes;
 }
 
 int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
@@ -816,32 +1128,46 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
         d_dev = ctx->dp_dev.as<uint32_t>();
     }
     const double tp0 = now_ms();
-    std::vector<DpItem> items;
-    make_items(jobs, njobs, bands, items);
-    std::vector<uint32_t> band1(njobs, 0);
+    // a job with band W evaluates the region of band 2W and accepts its path
+    // when it stays within W of the line (the region reaches at least W past
+    // the accepted band, where a path that wants to leave shows itself)
+    std::vector<uint32_t> region(njobs, 0);
     for (uint64_t j = 0; j < njobs; ++j) {
-        band1[j] = items[j].band;
-        if (band1[j]) ++st.banded;
+        const uint64_t w = bands ? bands[j] : 0u;
+        region[j] = 2 * w > 0xFFFFFFFFull ? 0u : (uint32_t)(2 * w);
+    }
+    std::vector<DpItem> items;
+    make_items(jobs, njobs, region.data(), items);
+    std::vector<uint32_t> accept1(njobs, 0);
+    for (uint64_t j = 0; j < njobs; ++j) {
+        if (items[j].band) {
+            accept1[j] = bands[j];
+            ++st.banded;
+        }
     }
     host_time(ctx, "host_dp_plan", now_ms() - tp0);
     const int force = dp_force_cols();
     if ((rc = dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, d_dev, &st.cells)) != MH_OK)
         return rc;
     if (!(flags & MH_BAND_NO_FALLBACK) && st.banded) {
-        // edge fallback: a banded path that reaches the band edge is rerun with
-        // the band doubled, and one that reaches that edge too with the full DP
+        // edge fallback: a banded path that reaches distance W is rerun with
+        // the band doubled (region 4W, accepted within 2W), and one that
+        // reaches 2W too with the full DP
         std::vector<uint32_t> dev(njobs);
         MH_HIP(hipMemcpyAsync(dev.data(), d_dev, njobs * 4, hipMemcpyDeviceToHost, ctx->stream));
         MH_HIP(hipStreamSynchronize(ctx->stream));
         std::vector<DpItem> again;
-        std::vector<uint32_t> band2(njobs, 0);
+        std::vector<uint32_t> accept2(njobs, 0);
         for (uint64_t j = 0; j < njobs; ++j)
-            if (band1[j] && dev[j] >= band1[j]) {
-                const uint64_t w2 = 2ull * band1[j];
-                again.push_back(make_item(jobs[j], (uint32_t)j, w2 > 0xFFFFFFFFull ? 0u : (uint32_t)w2, force));
-                band2[j] = again.back().band;
-                if (band2[j]) ++st.rerun_doubled;
-                else ++st.rerun_full;
+            if (accept1[j] && dev[j] >= accept1[j]) {
+                const uint64_t r2 = 4ull * accept1[j];
+                again.push_back(make_item(jobs[j], (uint32_t)j, r2 > 0xFFFFFFFFull ? 0u : (uint32_t)r2, force));
+                if (again.back().band) {
+                    accept2[j] = 2 * accept1[j];
+                    ++st.rerun_doubled;
+                } else {
+                    ++st.rerun_full;
+                }
             }
         if (!again.empty()) {
             if ((rc = dp_run_items(ctx, d_a, d_b, jobs, slot, again, d_paths, d_results, d_dev, &st.cells)) != MH_OK)
@@ -850,7 +1176,7 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
             MH_HIP(hipStreamSynchronize(ctx->stream));
             std::vector<DpItem> full;
             for (uint64_t j = 0; j < njobs; ++j)
-                if (band2[j] && dev[j] >= band2[j]) {
+                if (accept2[j] && dev[j] >= accept2[j]) {
                     full.push_back(make_item(jobs[j], (uint32_t)j, 0, force));
                     ++st.rerun_full;
                 }

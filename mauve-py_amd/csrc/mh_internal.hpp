@@ -125,6 +125,9 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
 int dp_force_cols();   // MH_DP_COLS = 4 | 8 | 16 forces one stripe shape (tuning sweeps)
 uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force);
 uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols);
+int dp_force_kind();   // MH_DP_BAND_KIND = stripes | window forces one banded region (tests, sweeps)
+uint32_t dp_window_cols(uint32_t la, uint32_t lb, uint32_t band, int kind);
+uint64_t dp_item_cells(uint32_t la, uint32_t lb, uint32_t band);   // cells of the scheduled region
 
 int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed, mh_sml *s);
 int sml_seed_mers(mh_context *ctx, const mh_sml *s, const uint32_t *d_pos, uint64_t n, uint64_t *d_out);

@@ -1,6 +1,8 @@
 // runtime.cpp -- context, errors, device buffers, kernel timers and the
 // C-ABI entry points that are not tied to one kernel family.
 #include "mh_internal.hpp"
+#include <algorithm>
+#include <thread>
 #include <cstring>
 #include <cmath>
 #include <chrono>
@@ -343,18 +345,44 @@ int mh_dp_align_banded_device(mh_context *ctx, const uint8_t *d_a, const uint8_t
     return dp_run_banded(ctx, d_a, d_b, jobs, njobs, bands, flags, d_paths, d_results, d_dev, stats);
 }
 
-uint32_t mh_dp_band_shape(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *stripe_w)
+uint32_t mh_dp_band_region(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *kind, uint32_t *width)
 {
-    if (a_len == 0 || b_len == 0) {
-        if (stripe_w) *stripe_w = 0;
-        return 0;
+    uint32_t k = MH_REGION_FULL, w = 0, eff = 0;
+    if (a_len && b_len) {
+        const int force = dp_force_cols();
+        uint32_t c = dp_cols_for(a_len, b_len, band, force);
+        eff = band ? dp_effective_band(a_len, b_len, band, c) : 0u;
+        if (eff) {
+            const uint32_t wc = dp_window_cols(a_len, b_len, eff, dp_force_kind());
+            k = wc ? MH_REGION_WINDOW : MH_REGION_STRIPES;
+            w = wc ? 64 * wc : 64 * c;
+        }
     }
-    const int force = dp_force_cols();
-    uint32_t c = dp_cols_for(a_len, b_len, band, force);
-    const uint32_t eff = band ? dp_effective_band(a_len, b_len, band, c) : 0u;
-    if (eff == 0) c = dp_cols_for(a_len, b_len, 0, force);
-    if (stripe_w) *stripe_w = 64 * c;
+    if (kind) *kind = k;
+    if (width) *width = w;
     return eff;
+}
+
+int mh_dp_band_cells(const uint32_t *a_len, const uint32_t *b_len, const uint32_t *band, uint64_t n, uint64_t *cells)
+{
+    MH_CHECK((a_len && b_len && cells) || n == 0, MH_ERR_INVALID, "null argument");
+    auto fill = [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t x = lo; x < hi; ++x) cells[x] = dp_item_cells(a_len[x], b_len[x], band ? band[x] : 0u);
+    };
+    unsigned nt = n >= 65536 ? std::thread::hardware_concurrency() : 1;
+    if (nt > 16) nt = 16;
+    if (nt <= 1) {
+        fill(0, n);
+        return MH_OK;
+    }
+    std::vector<std::thread> pool;
+    const uint64_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t lo = t * chunk, hi = std::min<uint64_t>(n, lo + chunk);
+        if (lo < hi) pool.emplace_back(fill, lo, hi);
+    }
+    for (auto &th : pool) th.join();
+    return MH_OK;
 }
 
 int mh_dp_align(mh_context *ctx, const uint8_t *a_codes, uint64_t a_bytes, const uint8_t *b_codes,

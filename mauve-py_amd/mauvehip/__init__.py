@@ -42,7 +42,7 @@ EXPORTS = [
     "mh_device_alloc", "mh_device_free", "mh_memcpy_h2d", "mh_memcpy_d2h",
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free",
-    "mh_dp_align_banded_device", "mh_dp_band_shape",
+    "mh_dp_align_banded_device", "mh_dp_band_region", "mh_dp_band_cells",
     "mh_gap_features_device", "mh_mlp_create", "mh_mlp_free", "mh_mlp_set_params", "mh_mlp_get_params",
     "mh_mlp_forward", "mh_mlp_grad", "mh_mlp_train", "mh_band_predict",
 ]
@@ -139,7 +139,9 @@ def load_library(path=LIB_PATH):
         "mh_dp_align_device": (C.c_int, [vp, vp, vp, vp, u64, vp, vp]),
         "mh_dp_align_banded_device": (C.c_int, [vp, vp, vp, vp, u64, vp, C.c_uint32, vp, vp, vp,
                                                 C.POINTER(BandStats)]),
-        "mh_dp_band_shape": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
+        "mh_dp_band_region": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
+                                           C.POINTER(C.c_uint32)]),
+        "mh_dp_band_cells": (C.c_int, [vp, vp, vp, u64, vp]),
         "mh_gap_features_device": (C.c_int, [vp, vp, vp, vp, u64, vp]),
         "mh_mlp_create": (C.c_int, [vp, u64, C.POINTER(vp)]),
         "mh_mlp_free": (None, [vp]),
@@ -336,11 +338,25 @@ def global_align_banded(ctx, a_codes, b_codes, jobs, bands, flags=0):
     return paths, res, dev, st
 
 
-def band_shape(a_len, b_len, band):
-    """(effective band, stripe width) the scheduler uses for a job."""
-    w = C.c_uint32()
-    eff = load_library().mh_dp_band_shape(int(a_len), int(b_len), int(band), C.byref(w))
-    return int(eff), int(w.value)
+REGION_FULL, REGION_STRIPES, REGION_WINDOW = 0, 1, 2
+
+
+def band_cells(a_len, b_len, bands):
+    """DP cells the scheduler evaluates per job at these bands (0 = full)."""
+    la = np.ascontiguousarray(a_len, dtype=np.uint32)
+    lb = np.ascontiguousarray(b_len, dtype=np.uint32)
+    bd = np.ascontiguousarray(bands, dtype=np.uint32)
+    out = np.zeros(len(la), dtype=np.uint64)
+    _check(load_library().mh_dp_band_cells(la.ctypes.data, lb.ctypes.data, bd.ctypes.data, len(la),
+                                           out.ctypes.data))
+    return out
+
+
+def band_region(a_len, b_len, band):
+    """(effective band, region kind, width) the scheduler uses for a job."""
+    k, w = C.c_uint32(), C.c_uint32()
+    eff = load_library().mh_dp_band_region(int(a_len), int(b_len), int(band), C.byref(k), C.byref(w))
+    return int(eff), int(k.value), int(w.value)
 
 
 # ---- learned band ------------------------------------------------------------

@@ -4,8 +4,10 @@ The network (mh_mlp, 272-256-128-1, bf16 weights) predicts
 y = log2(1 + label) for a gap, where the label is the full DP path's largest
 distance from the corner-to-corner line, ceil(max |i*LB - j*LA| / LA).  A job
 gets the band W = max(min_band, ceil(2^(y + delta))); delta is calibrated on
-the training labels to minimise the expected DP work including the edge
-fallback (W, then 2W, then the full matrix).  The full DP stays the default
+the training labels to minimise the modelled time of the banded run with its
+edge fallback (W, then 2W, then the full matrix), from the cells of the
+regions the library schedules (mh_dp_band_cells) and the single-wavefront
+latency of the largest rerun.  The full DP stays the default
 and the judge: every banded result whose path reaches the band edge is rerun.
 
 Everything numeric runs on the GPU through the library: features
@@ -15,27 +17,57 @@ training (mh_mlp_train: forward, backward and Adam kernels) and inference
 """
 import numpy as np
 
-from . import (MLP_IN, Mlp, gap_features_device, global_align_banded_device,
+from . import (MLP_IN, Mlp, band_cells, gap_features_device, global_align_banded_device,
                DP_JOB_DTYPE, DP_RESULT_DTYPE)
 
 MIN_BAND = 4
 DELTA_GRID = np.round(np.arange(0.0, 3.01, 0.05), 3)
+# grids measured by train_band.py on a calibration chunk
+CALIBRATION_DELTAS = (0.6, 0.75, 0.9, 1.05, 1.2, 1.4)
+CALIBRATION_CONSTANTS = (48, 64, 90, 117, 150, 200, 256)
 
 
 def est_cells(la, lb, w):
-    """Rough DP work of a band: the stripe region is about 2W + 256 wide."""
-    la = np.asarray(la, dtype=np.float64)
-    lb = np.asarray(lb, dtype=np.float64)
-    return la * np.minimum(lb, 2.0 * np.asarray(w, dtype=np.float64) + 256.0)
+    """DP cells of the region the library schedules for region band w."""
+    w = np.broadcast_to(np.asarray(w, dtype=np.uint32), np.shape(la))
+    return band_cells(la, lb, w).astype(np.float64)
+
+
+# Time model of a banded run on one MI355X (seconds), from measurements of the
+# DP kernels: each pass of the fallback costs its cells at the batch rate
+# (~950 GCUPS of region cells) or its largest job at the rate of one
+# wavefront (a job runs on one wavefront: ~1.7 GCUPS), whichever is longer;
+# then the trace-back, whose longest walk takes ~0.6 us per step; plus a
+# fixed cost for planning and synchronisation.
+BATCH_CELLS_PER_S = 0.95e12
+WAVE_CELLS_PER_S = 1.7e9
+TRACE_STEP_SECONDS = 0.6e-6
+PASS_SECONDS = 3e-3
+
+
+def _pass_time(cells, la, lb):
+    if len(cells) == 0:
+        return 0.0
+    walk = float((np.asarray(la, np.float64) + np.asarray(lb, np.float64)).max())
+    return (max(float(cells.sum()) / BATCH_CELLS_PER_S, float(cells.max()) / WAVE_CELLS_PER_S)
+            + walk * TRACE_STEP_SECONDS + PASS_SECONDS)
 
 
 def expected_cost(la, lb, label, w):
-    """Work of banding at w with the edge fallback, given the true labels."""
-    c = est_cells(la, lb, w)
+    """Modelled seconds of banding at w with the edge fallback (region 2W
+    accepted within W, region 4W accepted within 2W, full), given the true
+    labels."""
+    la = np.asarray(la)
+    lb = np.asarray(lb)
+    w = np.broadcast_to(np.asarray(w, dtype=np.uint32), np.shape(la)).astype(np.uint64)
+    t = _pass_time(est_cells(la, lb, 2 * w), la, lb)
     miss1 = label >= w
-    c = c + np.where(miss1, est_cells(la, lb, 2 * w), 0.0)
-    miss2 = label >= 2 * w
-    return float((c + np.where(miss2, np.asarray(la, np.float64) * lb, 0.0)).sum())
+    if miss1.any():
+        t += _pass_time(est_cells(la[miss1], lb[miss1], 4 * w[miss1]), la[miss1], lb[miss1])
+        miss2 = label >= 2 * w
+        if miss2.any():
+            t += _pass_time(la[miss2].astype(np.float64) * lb[miss2].astype(np.float64), la[miss2], lb[miss2])
+    return t
 
 
 def bands_from_output(y, delta, min_band=MIN_BAND):
@@ -43,18 +75,46 @@ def bands_from_output(y, delta, min_band=MIN_BAND):
     return np.maximum(np.ceil(np.exp2(e)).astype(np.uint32), np.uint32(min_band))
 
 
+CALIBRATION_BATCH = 100_000
+
+
+def _batches(n, k=CALIBRATION_BATCH, seed=0):
+    """Disjoint random batches of the deployment size covering the rows (the
+    time model's fixed and latency terms do not scale with the batch, and
+    averaging over batches weighs rare large reruns by their frequency)."""
+    perm = np.random.default_rng(seed).permutation(n)
+    nb = max(1, n // k)
+    return [np.sort(perm[b * k:(b + 1) * k]) if n >= k else perm for b in range(nb)]
+
+
+def _mean_cost(la, lb, label, w):
+    la, lb, label, w = (np.asarray(v) for v in (la, lb, label, w))
+    bs = _batches(len(la))
+    return float(np.mean([expected_cost(la[r], lb[r], label[r], w[r]) for r in bs]))
+
+
 def calibrate_delta(y, la, lb, label, min_band=MIN_BAND):
-    costs = [expected_cost(la, lb, label, bands_from_output(y, d, min_band)) for d in DELTA_GRID]
-    k = int(np.argmin(costs))
-    return float(DELTA_GRID[k]), costs[k]
+    """delta on the 0.05 grid of [0, 3] minimising expected_cost averaged over
+    100k-gap batches of the training rows (coarse 0.2 search, then the 0.05
+    steps around the best)."""
+    def cost(d):
+        return _mean_cost(la, lb, label, bands_from_output(y, d, min_band))
+    coarse = {float(d): cost(d) for d in np.round(np.arange(0.0, 3.01, 0.2), 3)}
+    c0 = min(coarse, key=coarse.get)
+    fine = dict(coarse)
+    for d in np.round(np.arange(max(0.0, c0 - 0.2), min(3.0, c0 + 0.2) + 1e-9, 0.05), 3):
+        if float(d) not in fine:
+            fine[float(d)] = cost(d)
+    best = min(fine, key=fine.get)
+    return best, fine[best]
 
 
 def best_constant_band(la, lb, label, grid=None):
     """The method without the network: one band for every gap, chosen on the
     same labels by the same cost."""
     if grid is None:
-        grid = np.unique(np.round(np.exp2(np.arange(2, 14, 0.125))).astype(np.uint32))
-    costs = [expected_cost(la, lb, label, np.full(len(la), w, dtype=np.uint32)) for w in grid]
+        grid = np.unique(np.round(np.exp2(np.arange(2, 12, 0.125))).astype(np.uint32))
+    costs = [_mean_cost(la, lb, label, np.full(len(la), w, dtype=np.uint32)) for w in grid]
     k = int(np.argmin(costs))
     return int(grid[k]), costs[k]
 

@@ -9,8 +9,13 @@ length Geom(0.3)), from PCG64(seed), generated and labelled in chunks of
 100k: each chunk's features (mh_gap_features_device) and full-DP labels
 (largest distance of the optimal path from the corner-to-corner line, from
 mh_dp_align_banded_device) stay on the device.  Then mh_mlp_train runs Adam
-over all rows, delta is calibrated on the training labels, and the
-parameters, delta and training record go to an .npz file.
+over all rows.  delta is calibrated by measurement: on the first 100k-gap
+chunk of the training set the banded DP with its edge fallback is timed for
+a grid of deltas, and among those whose results equal the full DP's (scores
+and paths) and run within 5% of the fastest, the widest is kept (fewest
+reruns); the best constant band (the method without the network) is chosen
+the same way.  Parameters, delta and the training
+record go to an .npz file.
 """
 import argparse
 import json
@@ -95,17 +100,56 @@ def main(argv=None):
     mlp.forward_device(X, args.rows, Y)
     y = np.zeros(args.rows, dtype=np.float32)
     ctx.d2h(y, Y)
-    delta, cost = band.calibrate_delta(y, la_all, lb_all, labels)
-    wc, cost_c = band.best_constant_band(la_all, lb_all, labels)
     resid = t - y
+    # calibration by measurement on the first chunk
+    A, ao, al, B, bo, bl = chunks[0]
+    cal = band.DeviceJobs(ctx, A, B, mauvehip.make_jobs(ao, al, bo, bl))
+    cal.features()
+    cal.align(None)
+    ref_res, _ = cal.results()
+    ref_paths = cal.paths()
+    ycal = band.predict_output(ctx, mlp, cal)
+
+    def measure(bands):
+        cal.align(bands)                        # warm (arena, caches)
+        t0 = time.perf_counter()
+        st = cal.align(bands)
+        el = time.perf_counter() - t0
+        r, _ = cal.results()
+        sm, pm = band.count_mismatches(ref_res, ref_paths, r, cal.paths())
+        return el, sm + pm, st
+    trials = {}
+    for d in band.CALIBRATION_DELTAS:
+        el, mm, st = measure(band.bands_from_output(ycal, d))
+        trials[float(d)] = {"seconds": el, "mismatches": mm, "cells": int(st.cells),
+                            "reruns": [int(st.rerun_doubled), int(st.rerun_full)]}
+        log(f"[train] delta {d}: {el:.3f}s, mismatches {mm}, reruns {st.rerun_doubled}/{st.rerun_full}")
+    # the rarer a rerun, the smaller the chance that a large gap's full-DP
+    # rerun (one wavefront, a few hundred ms) lands in a batch: among settings
+    # within 5% of the fastest, take the widest
+    ok = [d for d, v in trials.items() if v["mismatches"] == 0] or list(trials)
+    tbest = min(trials[d]["seconds"] for d in ok)
+    delta = max(d for d in ok if trials[d]["seconds"] <= 1.05 * tbest)
+    const = {}
+    for w in band.CALIBRATION_CONSTANTS:
+        el, mm, st = measure(np.full(cal.n, w, dtype=np.uint32))
+        const[int(w)] = {"seconds": el, "mismatches": mm, "cells": int(st.cells)}
+        log(f"[train] constant band {w}: {el:.3f}s, mismatches {mm}")
+    okc = [w for w, v in const.items() if v["mismatches"] == 0] or list(const)
+    tbc = min(const[w]["seconds"] for w in okc)
+    wc = max(w for w in okc if const[w]["seconds"] <= 1.05 * tbc)
+    cal.free()
     info = {"rows": args.rows, "seed": args.seed, "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
             "epoch_loss": [float(v) for v in curve], "train_seconds": round(train_s, 2),
-            "label_dp_cells": cells, "delta": delta, "expected_cells_learned": cost,
-            "best_constant_band": wc, "expected_cells_constant": cost_c,
+            "label_dp_cells": cells, "delta": delta, "calibration_seconds_learned": trials[delta]["seconds"],
+            "best_constant_band": wc, "calibration_seconds_constant": const[wc]["seconds"],
+            "calibration_trials": {str(k): v for k, v in trials.items()},
+            "calibration_constants": {str(k): v for k, v in const.items()},
             "residual_std": float(resid.std()), "label_quantiles": {
                 str(q): float(np.percentile(labels, q)) for q in (50, 90, 99, 100)}}
     band.save_model(args.out, mlp.params(), delta, {"info_json": json.dumps(info)})
-    log(f"[train] delta {delta}, expected cells learned {cost:.3e} vs constant W={wc} {cost_c:.3e}")
+    log(f"[train] delta {delta}: {trials[delta]['seconds']:.3f}s; best constant band {wc}: "
+        f"{const[wc]['seconds']:.3f}s (calibration chunk)")
     print(json.dumps(info), flush=True)
     for p in (X, T, Y):
         ctx.free(p)

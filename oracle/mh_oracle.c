@@ -1132,19 +1132,17 @@ uint32_t or_path_dev(uint32_t LA, uint32_t LB, const char *path, int64_t plen)
     return (uint32_t)((mx + LA - 1) / LA);
 }
 
-int64_t or_nwsmall_region(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
-                          uint32_t band, uint32_t sw, char *path_out, int32_t *score_out, uint32_t *dev_out)
+/* The DP over the cells (i, j) with jlo[i] <= j <= jhi[i] (i = 1..LA); every
+ * other cell of rows/columns >= 1 is -infinity. */
+static int64_t region_dp(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB, const int64_t *jlo,
+                         const int64_t *jhi, char *path_out, int32_t *score_out, uint32_t *dev_out)
 {
-    if (LA == 0 || LB == 0) return -1;
-    if (sw == 0) sw = LB;
     const int64_t W = LB + 1;
     const size_t cells = (size_t)(LA + 1) * (size_t)W;
     int64_t *M = (int64_t *)malloc(cells * sizeof(int64_t));
     int64_t *D = (int64_t *)malloc(cells * sizeof(int64_t));
     int64_t *I = (int64_t *)malloc(cells * sizeof(int64_t));
     uint8_t *T = (uint8_t *)calloc(cells, 1);
-    int64_t *lo = (int64_t *)malloc((LB + 1) * sizeof(int64_t)), *hi = (int64_t *)malloc((LB + 1) * sizeof(int64_t));
-    for (uint32_t j = 1; j <= LB; ++j) region_rows(LA, LB, band, sw, (j - 1) / sw, &lo[j], &hi[j]);
 #define IX(i, j) ((size_t)(i) * W + (j))
     /* gap scores of a one-sequence profile with half terminal gaps
      * (MU/profilefrommsa.cpp:285-292, MU/termgaps.cpp:19-33): open[0] = 0,
@@ -1161,7 +1159,7 @@ int64_t or_nwsmall_region(const uint8_t *A, uint32_t LA, const uint8_t *B, uint3
     for (uint32_t i = 1; i <= LA; ++i) {
         for (uint32_t j = 1; j <= LB; ++j) {
             const size_t x = IX(i, j);
-            if ((int64_t)i < lo[j] || (int64_t)i > hi[j]) { M[x] = D[x] = I[x] = RNEG; continue; }
+            if ((int64_t)j < jlo[i] || (int64_t)j > jhi[i]) { M[x] = D[x] = I[x] = RNEG; continue; }
             uint8_t tb = 0;
             /* RECURSE_D (MU/nwsmall.cpp:68-86): tie -> M */
             {
@@ -1222,6 +1220,62 @@ int64_t or_nwsmall_region(const uint8_t *A, uint32_t LA, const uint8_t *B, uint3
 #undef OPEN_B
 #undef CLOSE_A
 #undef CLOSE_B
-    free(M); free(D); free(I); free(T); free(lo); free(hi);
+    free(M); free(D); free(I); free(T);
+    return n;
+}
+
+int64_t or_nwsmall_region(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
+                          uint32_t band, uint32_t sw, char *path_out, int32_t *score_out, uint32_t *dev_out)
+{
+    if (LA == 0 || LB == 0) return -1;
+    if (sw == 0) sw = LB;
+    /* the stripes holding a row are consecutive (their row ranges grow with
+     * the stripe index), so each row's cells are one column range */
+    int64_t *jlo = (int64_t *)malloc((LA + 1) * sizeof(int64_t)), *jhi = (int64_t *)malloc((LA + 1) * sizeof(int64_t));
+    for (uint32_t i = 0; i <= LA; ++i) { jlo[i] = (int64_t)LB + 1; jhi[i] = 0; }
+    const uint32_t ns = (LB + sw - 1) / sw;
+    for (uint32_t s = 0; s < ns; ++s) {
+        int64_t r0, r1;
+        region_rows(LA, LB, band, sw, s, &r0, &r1);
+        const int64_t c0 = (int64_t)s * sw + 1, c1 = (int64_t)(s + 1) * sw < LB ? (int64_t)(s + 1) * sw : LB;
+        for (int64_t i = r0; i <= r1; ++i) {
+            if (c0 < jlo[i]) jlo[i] = c0;
+            if (c1 > jhi[i]) jhi[i] = c1;
+        }
+    }
+    const int64_t n = region_dp(A, LA, B, LB, jlo, jhi, path_out, score_out, dev_out);
+    free(jlo); free(jhi);
+    return n;
+}
+
+int64_t or_window_base(uint32_t LA, uint32_t LB, uint32_t band, uint32_t width, uint32_t i)
+{
+    /* ceil((i*LB - h*LA) / LA) - 1 with h = (width-1)/2, clamped to
+     * [0, max(0, LB - width)]: the window is centred on the line */
+    (void)band;
+    const int64_t h = ((int64_t)width - 1) / 2;
+    const int64_t num = (int64_t)i * LB - h * LA;
+    const int64_t q = num >= 0 ? (num + LA - 1) / LA : -((-num) / LA);
+    int64_t b = q - 1;
+    const int64_t top = (int64_t)LB > (int64_t)width ? (int64_t)LB - width : 0;
+    if (b > top) b = top;
+    if (b < 0) b = 0;
+    return b;
+}
+
+int64_t or_nwsmall_window(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
+                          uint32_t band, uint32_t width, char *path_out, int32_t *score_out, uint32_t *dev_out)
+{
+    if (LA == 0 || LB == 0) return -1;
+    int64_t *jlo = (int64_t *)malloc((LA + 1) * sizeof(int64_t)), *jhi = (int64_t *)malloc((LA + 1) * sizeof(int64_t));
+    jlo[0] = 1;
+    jhi[0] = 0;
+    for (uint32_t i = 1; i <= LA; ++i) {
+        const int64_t b = or_window_base(LA, LB, band, width, i);
+        jlo[i] = b + 1;
+        jhi[i] = b + width;
+    }
+    const int64_t n = region_dp(A, LA, B, LB, jlo, jhi, path_out, score_out, dev_out);
+    free(jlo); free(jhi);
     return n;
 }

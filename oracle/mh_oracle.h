@@ -118,6 +118,18 @@ int64_t or_nwsmall_region(const uint8_t *A, uint32_t LA, const uint8_t *B, uint3
                           uint32_t band, uint32_t stripe_w,
                           char *path_out, int32_t *score_out, uint32_t *dev_out);
 
+/* Banded NWSmall over a sliding window: row i holds the `width` columns
+ * base(i)+1 .. base(i)+width, base(i) = ceil((i*LB - h*LA)/LA) - 1 with
+ * h = (width-1)/2, clamped to [0, max(0, LB - width)] (or_window_base): the
+ * window is centred on the corner-to-corner line and contains every cell
+ * within h of it.  `band` does not move the window (the library sizes the
+ * width from the band).  Same DP, boundaries, ties and returns as
+ * or_nwsmall_region.  No counterpart in the reference. */
+int64_t or_window_base(uint32_t LA, uint32_t LB, uint32_t band, uint32_t width, uint32_t i);
+int64_t or_nwsmall_window(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
+                          uint32_t band, uint32_t width,
+                          char *path_out, int32_t *score_out, uint32_t *dev_out);
+
 /* ceil(max |i*LB - j*LA| / LA) over the positions of a path. */
 uint32_t or_path_dev(uint32_t LA, uint32_t LB, const char *path, int64_t plen);
 

@@ -69,6 +69,11 @@ def lib():
         L.or_nwsmall_region.restype = C.c_int64
         L.or_nwsmall_region.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                         C.c_char_p, C.c_void_p, C.c_void_p]
+        L.or_nwsmall_window.restype = C.c_int64
+        L.or_nwsmall_window.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_char_p, C.c_void_p, C.c_void_p]
+        L.or_window_base.restype = C.c_int64
+        L.or_window_base.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
         L.or_path_dev.restype = C.c_uint32
         L.or_path_dev.argtypes = [C.c_uint32, C.c_uint32, C.c_char_p, C.c_int64]
         L.or_path_score.restype = C.c_float
@@ -225,6 +230,27 @@ def nwsmall_region(a, b, band, stripe_w):
     if n < 0:
         return None, None, None
     return buf.raw[:n].decode(), int(sc.value), int(dv.value)
+
+
+def nwsmall_window(a, b, band, width):
+    """Banded NWSmall over the sliding window of `width` columns:
+    (path, int score, deviation)."""
+    A = codes(a) if isinstance(a, (str, bytes)) else np.ascontiguousarray(a, dtype=np.uint8)
+    B = codes(b) if isinstance(b, (str, bytes)) else np.ascontiguousarray(b, dtype=np.uint8)
+    buf = C.create_string_buffer(len(A) + len(B) + 1)
+    sc = C.c_int32()
+    dv = C.c_uint32()
+    n = lib().or_nwsmall_window(A.ctypes.data, len(A), B.ctypes.data, len(B), int(band), int(width), buf,
+                                C.byref(sc), C.byref(dv))
+    if n == -1:
+        raise ValueError("NWSmall: length=0")
+    if n < 0:
+        return None, None, None
+    return buf.raw[:n].decode(), int(sc.value), int(dv.value)
+
+
+def window_base(la, lb, band, width, i):
+    return int(lib().or_window_base(int(la), int(lb), int(band), int(width), int(i)))
 
 
 def path_dev(la, lb, path):

@@ -5,7 +5,8 @@ job equals the oracle's banded region DP (or_nwsmall_region) at the stripe
 width the scheduler reports; the region oracle is pinned by
 tests/test_oracle_dp.py (whole matrix = NWSmall = libMUSCLE, brute force over
 the region's paths, containment of the full path).  With fallback, every job
-equals the oracle model of the fallback chain (W, then 2W, then full).
+equals the oracle model of the fallback chain (region of band 2W accepted
+within W, region 4W accepted within 2W, then full).
 """
 import os
 
@@ -45,6 +46,14 @@ def _pairs(rng, n, la_range, div=0.1):
     return np.concatenate(A), np.concatenate(B), mauvehip.make_jobs(j[:, 0], j[:, 1], j[:, 2], j[:, 3])
 
 
+def _oracle(a, b, band):
+    """The oracle DP over the region the library gives this job and band."""
+    eff, kind, width = mauvehip.band_region(len(a), len(b), band)
+    if kind == mauvehip.REGION_WINDOW:
+        return po.nwsmall_window(a, b, eff, width), eff
+    return po.nwsmall_region(a, b, eff, width if kind == mauvehip.REGION_STRIPES else 0), eff
+
+
 def _ops(A, B, jobs, k):
     a = A[jobs[k]["a_off"]:jobs[k]["a_off"] + jobs[k]["a_len"]]
     b = B[jobs[k]["b_off"]:jobs[k]["b_off"] + jobs[k]["b_len"]]
@@ -61,13 +70,26 @@ def cols_env():
         os.environ["MH_DP_COLS"] = old
 
 
-@pytest.mark.parametrize("cols", ["4", "8", "16", "auto"])
-def test_banded_equals_region_oracle(ctx, cols, cols_env):
-    if cols == "auto":
-        os.environ.pop("MH_DP_COLS", None)
+@pytest.fixture
+def kind_env():
+    old = os.environ.get("MH_DP_BAND_KIND")
+    yield
+    if old is None:
+        os.environ.pop("MH_DP_BAND_KIND", None)
     else:
+        os.environ["MH_DP_BAND_KIND"] = old
+
+
+@pytest.mark.parametrize("cols", ["4", "8", "16", "auto", "window"])
+def test_banded_equals_region_oracle(ctx, cols, cols_env, kind_env):
+    os.environ.pop("MH_DP_COLS", None)
+    os.environ.pop("MH_DP_BAND_KIND", None)
+    if cols == "window":
+        os.environ["MH_DP_BAND_KIND"] = "window"
+    elif cols != "auto":
         os.environ["MH_DP_COLS"] = cols
-    rng = np.random.default_rng(int(cols) if cols != "auto" else 99)
+        os.environ["MH_DP_BAND_KIND"] = "stripes"
+    rng = np.random.default_rng(int(cols) if cols.isdigit() else (99 if cols == "auto" else 98))
     A, B, jobs = _pairs(rng, 60, (1, 900))
     # skewed shapes: empty stripes on the long side, one-row and one-column jobs
     extra = [(1, 700), (700, 1), (40, 1500), (1500, 40), (1, 1), (2, 3000)]
@@ -84,12 +106,12 @@ def test_banded_equals_region_oracle(ctx, cols, cols_env):
     assert st.rerun_doubled == 0 and st.rerun_full == 0
     for k in range(len(jobs)):
         a, b = _ops(A, B, jobs, k)
-        eff, sw = mauvehip.band_shape(len(a), len(b), int(bands[k]))
-        p, sc, dv = po.nwsmall_region(a, b, eff, sw)
+        (p, sc, dv), _ = _oracle(a, b, 2 * int(bands[k]))   # band W evaluates the region of band 2W
         if p is None:     # the region holds no path: the library flags a failed trace-back
             assert res[k]["path_len"] == 0
             continue
-        assert mauvehip.path_of(paths, res, k) == p, (k, len(a), len(b), int(bands[k]), sw)
+        assert mauvehip.path_of(paths, res, k) == p, (k, len(a), len(b), int(bands[k]),
+                                                     mauvehip.band_region(len(a), len(b), 2 * int(bands[k])))
         assert int(res[k]["score"]) == sc
         assert int(dev[k]) == dv
 
@@ -115,13 +137,12 @@ def test_fallback_chain(ctx):
     for k in range(len(jobs)):
         a, b = _ops(A, B, jobs, k)
         la, lb, W = len(a), len(b), int(bands[k])
-        e1, w1 = mauvehip.band_shape(la, lb, W)
-        want = po.nwsmall_region(a, b, e1, w1)
-        if e1 and (want[0] is None or want[2] >= e1):
-            e2, w2 = mauvehip.band_shape(la, lb, 2 * e1)
-            want = po.nwsmall_region(a, b, e2, w2)
+        # region 2W accepted within W, region 4W accepted within 2W, then full
+        want, e1 = _oracle(a, b, 2 * W)
+        if e1 and (want[0] is None or want[2] >= W):
+            want, e2 = _oracle(a, b, 4 * W)
             n2 += 1
-            if e2 and (want[0] is None or want[2] >= e2):
+            if e2 and (want[0] is None or want[2] >= 2 * W):
                 want = po.nwsmall_region(a, b, 0, 0)
         assert mauvehip.path_of(paths, res, k) == want[0], k
         assert int(res[k]["score"]) == want[1] and int(dev[k]) == want[2]

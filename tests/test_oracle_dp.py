@@ -152,3 +152,64 @@ def test_region_contains_full_path():
             if d > 1:
                 q2, sq2, _ = po.nwsmall_region(a, b, max(1, d // 2), sw)
                 assert q2 is None or sq2 <= int(sc)
+
+
+def _in_window(la, lb, band, width, path):
+    i = j = 0
+    for e in path:
+        i += e != "I"
+        j += e != "D"
+        if i >= 1 and j >= 1:
+            b = po.window_base(la, lb, band, width, i)
+            if not b + 1 <= j <= b + width:
+                return False
+    return True
+
+
+@pytest.mark.parametrize("la,lb", [(1, 4), (4, 1), (3, 3), (4, 5), (5, 3), (2, 6), (6, 6), (6, 2)])
+def test_window_brute_force_optimum(la, lb):
+    """The sliding-window result is the best path inside the window."""
+    rng = np.random.default_rng(la * 13 + lb)
+    for _ in range(6):
+        a = rng.integers(0, 4, la, dtype=np.uint8)
+        b = rng.integers(0, 4, lb, dtype=np.uint8)
+        for band in (1, 2):
+            for width in (2, 3, 4):
+                inside = [q for q in all_paths(la, lb) if _in_window(la, lb, band, width, q)]
+                p, sc, dv = po.nwsmall_window(a, b, band, width)
+                if not inside:
+                    assert p is None
+                    continue
+                assert p in inside
+                assert po.path_score(a, b, p) == sc
+                assert sc == max(po.path_score(a, b, q) for q in inside)
+                assert dv == po.path_dev(la, lb, p)
+
+
+def test_window_covers_band_and_contains_full_path():
+    rng = np.random.default_rng(14)
+    # coverage: every cell within h = (width-1)/2 of the line is in the window
+    for la, lb, band in [(50, 80, 3), (80, 50, 5), (200, 230, 10), (30, 120, 4), (7, 9, 2)]:
+        width = 4 * band + 2
+        for i in range(1, la + 1):
+            b = po.window_base(la, lb, band, width, i)
+            for j in range(1, lb + 1):
+                if abs(i * lb - j * la) <= ((width - 1) // 2) * la:
+                    assert b + 1 <= j <= b + width, (la, lb, band, i, j)
+    for _ in range(30):
+        la = int(rng.integers(20, 300))
+        a = rng.integers(0, 4, la, dtype=np.uint8)
+        bb = a.copy()
+        hit = rng.random(la) < 0.15
+        bb[hit] = rng.integers(0, 4, int(hit.sum()), dtype=np.uint8)
+        cut = int(rng.integers(0, la))
+        bb = np.concatenate([bb[:cut], rng.integers(0, 4, int(rng.integers(0, 12)), dtype=np.uint8), bb[cut + 3:]])
+        if len(bb) == 0:
+            continue
+        p, sc = po.nwsmall(a, bb)
+        d = po.path_dev(la, len(bb), p)
+        for width in (2 * d + 3, 2 * d + 40, 64):
+            if (width - 1) // 2 < d + 1:
+                continue
+            q, sq, dq = po.nwsmall_window(a, bb, d + 1, width)
+            assert q == p and sq == int(sc) and dq == d
