@@ -737,6 +737,11 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
         if (a1 >= a0) base += (uint64_t)(a1 - a0 + 64) * 64 * lane_bytes;
     }
     stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
+    // stripe layout: column j-1 = cur*sw + k*cols + c, tracked without
+    // divisions; dnum = i*lb - j*la, tracked as the walk moves
+    const bool stripes = jb.cols != TINY_COLS_TAG && jb.cols <= WINDOW_TAG;
+    int k = stripes ? ((lb - 1) % sw) / cols : 0, c = stripes ? (lb - 1) % cols : 0;
+    int64_t dnum = 0;
     int pos = la + lb;
     int i = la, j = lb;
     int type = outs[x].edge;
@@ -746,34 +751,36 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
         slot[--pos] = (uint8_t)type;
         uint32_t nib = 0;
         if (i > 0 && j > 0) {
-            const int s = (j - 1) / sw;
-            while (cur > s) {   // moving left one stripe: step back over its trace block
-                --cur;
-                stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
-                if (r1 >= r0) base -= (uint64_t)(r1 - r0 + 64) * 64 * lane_bytes;
-            }
             if (jb.cols == TINY_COLS_TAG) {
                 const uint8_t byte = tj[(uint64_t)(i - 1) * 8 + ((j - 1) >> 1)];
                 nib = (byte >> (4 * ((j - 1) & 1))) & 15u;
             } else if (jb.cols > WINDOW_TAG) {
                 const int C = (int)jb.cols - (int)WINDOW_TAG, rpd = 8 / C;
-                const int x = j - window_base(la, lb, 64 * C, i) - 1;
-                if (x < 0 || x >= 64 * C) { ok = false; break; }
-                const uint32_t w = reinterpret_cast<const uint32_t *>(tj)[(uint64_t)((i - 1) / rpd) * 64 + x / C];
-                nib = (w >> (4 * (C * ((i - 1) % rpd) + x % C))) & 15u;
+                const int xw = j - window_base(la, lb, 64 * C, i) - 1;
+                if (xw < 0 || xw >= 64 * C) { ok = false; break; }
+                const uint32_t w = reinterpret_cast<const uint32_t *>(tj)[(uint64_t)((i - 1) / rpd) * 64 + xw / C];
+                nib = (w >> (4 * (C * ((i - 1) % rpd) + xw % C))) & 15u;
             } else {
                 if (i < r0 || i > r1) { ok = false; break; }
-                const int cc = (j - 1) % sw, k = cc / cols, c = cc % cols;
                 const uint8_t byte = tj[base + ((uint64_t)(i - r0 + k) * 64 + k) * lane_bytes + (c >> 1)];
                 nib = (byte >> (4 * (c & 1))) & 15u;
             }
         }
         const int nx = xchar(nib, type);
-        if (type == 'M') { if (i == 0 || j == 0) { ok = false; break; } --i; --j; }
-        else if (type == 'D') { if (i == 0) { ok = false; break; } --i; }
-        else { if (j == 0) { ok = false; break; } --j; }
-        const int64_t d = (int64_t)i * lb - (int64_t)j * la;
-        maxdev = max(maxdev, d < 0 ? -d : d);
+        bool left = false;
+        if (type == 'M') { if (i == 0 || j == 0) { ok = false; break; } --i; --j; dnum += (int64_t)la - lb; left = true; }
+        else if (type == 'D') { if (i == 0) { ok = false; break; } --i; dnum -= lb; }
+        else { if (j == 0) { ok = false; break; } --j; dnum += la; left = true; }
+        if (stripes && left && j > 0 && --c < 0) {
+            c = cols - 1;
+            if (--k < 0) {   // into the previous stripe: step back over its trace block
+                k = 63;
+                --cur;
+                stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
+                if (r1 >= r0) base -= (uint64_t)(r1 - r0 + 64) * 64 * lane_bytes;
+            }
+        }
+        maxdev = max(maxdev, dnum < 0 ? -dnum : dnum);
         if (i == 0 && j == 0) break;
         if (pos == 0 || nx == '?') { ok = false; break; }
         type = nx;
@@ -784,6 +791,105 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
     r.path_start = jb.path_off + (uint64_t)pos;
     res[jb.user] = r;
     if (dev) dev[jb.user] = ok ? (uint32_t)((maxdev + la - 1) / la) : 0xFFFFFFFFu;
+}
+
+// The same trace-back for stripe-layout jobs by one wavefront per job.  The
+// walk itself is sequential (all lanes run it in step), but its trace bytes
+// come from LDS: the wave loads a tile of 128 steps x 128 columns of the
+// current stripe (128 segments of 64 contiguous bytes, 16-byte aligned) in
+// one round of loads and refills it only when the walk leaves it, which
+// replaces a dependent global load per step with one per ~100 steps.
+constexpr int TBC_STEPS = 128, TBC_SEG = 64;
+constexpr uint32_t WAVE_WALK_MIN = 1024;   // a + b above which a job's walk gets a wavefront
+
+__global__ __launch_bounds__(64) void dp_traceback_wave(const DevJob *__restrict__ jobs,
+                                                        const DpOut *__restrict__ outs,
+                                                        const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
+                                                        mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev)
+{
+    __shared__ __align__(16) uint8_t cache[TBC_STEPS * TBC_SEG];
+    const int lane = threadIdx.x;
+    const DevJob jb = jobs[blockIdx.x];
+    const int la = (int)jb.la, lb = (int)jb.lb;
+    const int cols = (int)jb.cols, sw = 64 * cols, lane_bytes = cols / 2;
+    const int K = TBC_SEG / lane_bytes;      // lanes per segment: 128 columns
+    const int kalign = 16 / lane_bytes;      // segment starts on a 16-byte boundary
+    const uint8_t *tj = tb + jb.tb_off;
+    uint8_t *slot = paths + jb.path_off;
+    int cur = (lb - 1) / sw, r0 = 1, r1 = 0;
+    uint64_t base = 0;
+    for (int s = 0; s < cur; ++s) {
+        int a0, a1;
+        stripe_rows(jb.la, jb.lb, jb.band, sw, s, a0, a1);
+        if (a1 >= a0) base += (uint64_t)(a1 - a0 + 64) * 64 * lane_bytes;
+    }
+    stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
+    int c_s = -1, c_tlo = 0, c_thi = -1, c_klo = 0;   // cached stripe, steps and first lane
+    const int kshift = kalign == 2 ? 1 : (kalign == 4 ? 2 : 3);
+    // column j-1 = cur*sw + k*cols + c, tracked without divisions; dnum =
+    // i*lb - j*la, tracked as the walk moves
+    int k = ((lb - 1) % sw) / cols, c = (lb - 1) % cols;
+    int64_t dnum = 0;
+    int pos = la + lb;
+    int i = la, j = lb;
+    int type = outs[blockIdx.x].edge;
+    bool ok = true;
+    int64_t maxdev = 0;
+    for (;;) {
+        --pos;
+        if (lane == 0) slot[pos] = (uint8_t)type;
+        uint32_t nib = 0;
+        if (i > 0 && j > 0) {
+            if (i < r0 || i > r1) { ok = false; break; }
+            const int t = i - r0 + k;
+            if (cur != c_s || t < c_tlo || t > c_thi || k < c_klo || k >= c_klo + K) {
+                c_s = cur;
+                c_thi = t;
+                c_tlo = t - TBC_STEPS + 1 < 0 ? 0 : t - TBC_STEPS + 1;
+                const int kl = (((k >> kshift) + 1) << kshift) - K;
+                c_klo = kl < 0 ? 0 : kl;
+                const int nseg = c_thi - c_tlo + 1;
+                __syncthreads();
+                for (int e = lane; e < nseg * 4; e += 64) {
+                    const int seg = e >> 2, part = e & 3;
+                    const uint8_t *src = tj + base + ((uint64_t)(c_tlo + seg) * 64 + c_klo) * lane_bytes + part * 16;
+                    *reinterpret_cast<uint4 *>(cache + seg * TBC_SEG + part * 16) = *reinterpret_cast<const uint4 *>(src);
+                }
+                __syncthreads();
+            }
+            // one LDS byte read by every lane; made wave-uniform so the walk's
+            // bookkeeping runs on the scalar unit
+            const uint32_t byte = (uint32_t)__builtin_amdgcn_readfirstlane(
+                (int)cache[(t - c_tlo) * TBC_SEG + (k - c_klo) * lane_bytes + (c >> 1)]);
+            nib = (byte >> (4 * (c & 1))) & 15u;
+        }
+        const int nx = xchar(nib, type);
+        bool left = false;
+        if (type == 'M') { if (i == 0 || j == 0) { ok = false; break; } --i; --j; dnum += (int64_t)la - lb; left = true; }
+        else if (type == 'D') { if (i == 0) { ok = false; break; } --i; dnum -= lb; }
+        else { if (j == 0) { ok = false; break; } --j; dnum += la; left = true; }
+        if (left && j > 0 && --c < 0) {
+            c = cols - 1;
+            if (--k < 0) {   // into the previous stripe: step back over its trace block
+                k = 63;
+                --cur;
+                stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
+                if (r1 >= r0) base -= (uint64_t)(r1 - r0 + 64) * 64 * lane_bytes;
+            }
+        }
+        maxdev = max(maxdev, dnum < 0 ? -dnum : dnum);
+        if (i == 0 && j == 0) break;
+        if (pos == 0 || nx == '?') { ok = false; break; }
+        type = nx;
+    }
+    if (lane == 0) {
+        mh_dp_result r;
+        r.score = outs[blockIdx.x].score;
+        r.path_len = ok ? (uint32_t)(la + lb - pos) : 0u;
+        r.path_start = jb.path_off + (uint64_t)pos;
+        res[jb.user] = r;
+        if (dev) dev[jb.user] = ok ? (uint32_t)((maxdev + la - 1) / la) : 0xFFFFFFFFu;
+    }
 }
 
 // NUC_SP (MU/nucmx.cpp:8-25) rows packed as biased bytes: row a, column b at
@@ -969,6 +1075,15 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
                               [](const DpItem &it) { return it.cols < WINDOW_TAG; });
         uint64_t nstripe = 0;
         while (nstripe < nbig && items[first + nstripe].cols < WINDOW_TAG) ++nstripe;
+        // stripe jobs with long trace-back walks first: they get a wavefront each
+        std::stable_partition(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)(first + nstripe),
+                              [&](const DpItem &it) {
+                                  return jobs[it.job].a_len + jobs[it.job].b_len >= WAVE_WALK_MIN;
+                              });
+        uint64_t nwave = 0;
+        while (nwave < nstripe &&
+               jobs[items[first + nwave].job].a_len + jobs[items[first + nwave].job].b_len >= WAVE_WALK_MIN)
+            ++nwave;
         dj.resize(n);
         uint64_t tw = 0, bo = 0;
         for (uint64_t x = 0; x < n; ++x) {
@@ -1014,9 +1129,16 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             MH_HIP(hipGetLastError());
         }
         {
+            // stripe-layout jobs with long walks: one wavefront each; the rest:
+            // one thread each
             TimedLaunch tl(ctx, "dp_traceback");
-            hipLaunchKernelGGL(dp_traceback, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ctx->stream, d_jobs,
-                               d_outs, ctx->dp_tb.as<uint8_t>(), d_paths, d_results, d_dev, (int)n);
+            if (nwave)
+                hipLaunchKernelGGL(dp_traceback_wave, dim3((unsigned)nwave), dim3(64), 0, ctx->stream, d_jobs,
+                                   d_outs, ctx->dp_tb.as<uint8_t>(), d_paths, d_results, d_dev);
+            if (n > nwave)
+                hipLaunchKernelGGL(dp_traceback, dim3((unsigned)((n - nwave + 63) / 64)), dim3(64), 0, ctx->stream,
+                                   d_jobs + nwave, d_outs + nwave, ctx->dp_tb.as<uint8_t>(), d_paths, d_results,
+                                   d_dev, (int)(n - nwave));
             MH_HIP(hipGetLastError());
         }
         // the host job table is reused by the next round
