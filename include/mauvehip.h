@@ -28,6 +28,7 @@ extern "C" {
 #define MH_ERR_GAP_CHAR      -4   /* '-' in a genome (LM/SortedMerList.cpp:433-437) */
 #define MH_ERR_ZERO_LENGTH   -5   /* empty DP operand (MU/nwsmall.cpp:503-504 Quit) */
 #define MH_ERR_UNSUPPORTED   -6   /* input outside what this build implements       */
+#define MH_ERR_IO            -7   /* file missing, short or of another format       */
 
 #define MH_MAX_SEQ 4              /* genomes per pairwise match search (the CLI uses
                                      PairwiseMatchFinder for <= 4 genomes,
@@ -100,6 +101,23 @@ int mh_sml_find_mer(mh_context *ctx, const mh_sml *sml, uint64_t mer, uint64_t *
 /* The packed sequence words (translate32, LM/SortedMerList.cpp:425-460);
  * n_words = ceil(2L/32)+2, trailing words zero. */
 int mh_sml_get_sequence(mh_context *ctx, const mh_sml *sml, uint32_t *words, uint64_t n_words);
+
+/* .sslist files, format version 5 (the SMLHeader of LM/SortedMerList.h:48-63,
+ * 2352 bytes, then the packed sequence words, then SMLLength uint32 sorted
+ * positions).  mh_sml_save replaces the file writing of FileSML::Create
+ * (LM/FileSML.cpp:427-446); a file it writes loads in the reference and the
+ * other way round.  MH_ERR_IO if the file cannot be written. */
+int mh_sml_save(mh_context *ctx, const mh_sml *sml, const char *path);
+/* Replaces FileSML::LoadFile2 (LM/FileSML.cpp:120-192).  *status (may be
+ * NULL) gets LoadFile2's code: 0 loaded; 1 cannot open; 2 short header;
+ * 3 version is not 5; 4 short sequence data; 5 fewer than SMLLength
+ * positions (MH_ERR_IO for 1-5).  Two codes of this build: 6 a header this
+ * build does not represent (alphabet_bits != 2, circular, seed fields that do
+ * not match the seed, length >= 2^32; MH_ERR_UNSUPPORTED) and 7 positions
+ * that are not a permutation of [0, SMLLength) or keys out of order
+ * (MH_ERR_INVALID).  The keys are recomputed from the sequence on the
+ * device; the loaded list behaves as one built by mh_sml_create. */
+int mh_sml_load(mh_context *ctx, const char *path, mh_sml **out, int *status);
 
 /* ---- seed matching ---------------------------------------------------- */
 

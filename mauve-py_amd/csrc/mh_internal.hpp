@@ -130,6 +130,13 @@ uint32_t dp_window_cols(uint32_t la, uint32_t lb, uint32_t band, int kind);
 uint64_t dp_item_cells(uint32_t la, uint32_t lb, uint32_t band);   // cells of the scheduled region
 
 int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed, mh_sml *s);
+// Device buffers for s (s->n_words and s->n set), from the context's pool or fresh.
+int sml_alloc(mh_context *ctx, mh_sml *s);
+// For a list whose packed words and sorted positions are on the device (a
+// list read from a file, positions already checked to be a permutation of
+// [0, n)): computes the per-position keys and the sorted keys; *unsorted is
+// set when the keys at the given positions are not in ascending order.
+int sml_rebuild_keys(mh_context *ctx, mh_sml *s, int *unsorted);
 int sml_seed_mers(mh_context *ctx, const mh_sml *s, const uint32_t *d_pos, uint64_t n, uint64_t *d_out);
 
 int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k,

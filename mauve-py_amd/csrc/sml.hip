@@ -303,6 +303,14 @@ __global__ void gather_keys_kernel(const uint64_t *__restrict__ keys_pos, uint64
     out[x] = p < n ? keys_pos[p] : ~0ull;
 }
 
+// flags a descending neighbour pair of sorted keys
+__global__ void check_sorted_kernel(const uint64_t *__restrict__ keys, uint64_t n, int *__restrict__ bad)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x + 1 >= n) return;
+    if (keys[x] > keys[x + 1]) atomicOr(bad, 1);
+}
+
 // ASCII -> DP letter codes (A,C,G,T either case -> 0..3; anything else flags)
 __global__ void codes_kernel(const uint8_t *__restrict__ ascii, uint64_t len, uint8_t *__restrict__ codes,
                              int *__restrict__ bad)
@@ -440,6 +448,34 @@ int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed,
     }
     MH_HIP(hipStreamSynchronize(ctx->stream));
     MH_CHECK(herr == 0, MH_ERR_GAP_CHAR, "Gap in genome sequence");
+    return MH_OK;
+}
+
+int sml_alloc(mh_context *ctx, mh_sml *s) { return sml_take_buffers(ctx, s); }
+
+int sml_rebuild_keys(mh_context *ctx, mh_sml *s, int *unsorted)
+{
+    *unsorted = 0;
+    const uint64_t n = s->n;
+    if (n == 0) return MH_OK;
+    int rc;
+    if ((rc = ctx->sort_tmp[0].ensure(n * 8 + 64)) != MH_OK) return rc;
+    if ((rc = ctx->sort_tmp[1].ensure(n * 4 + 64)) != MH_OK) return rc;
+    if ((rc = ctx->sort_tmp[3].ensure(64)) != MH_OK) return rc;
+    uint64_t *ck = ctx->sort_tmp[0].as<uint64_t>();
+    int *d_bad = ctx->sort_tmp[3].as<int>();
+    const SeedPattern sp = make_pattern(s->seed);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    // keys per position (and the compact keys in position order), then the
+    // list's keys gathered at its sorted positions
+    hipLaunchKernelGGL(keys_kernel, grid, dim3(256), 0, ctx->stream, s->d_words, n, sp, s->d_keys_pos, ck,
+                       ctx->sort_tmp[1].as<uint32_t>());
+    hipLaunchKernelGGL(gather_keys_kernel, grid, dim3(256), 0, ctx->stream, ck, n, s->d_pos, n, s->d_keys);
+    MH_HIP(hipMemsetAsync(d_bad, 0, 4, ctx->stream));
+    hipLaunchKernelGGL(check_sorted_kernel, grid, dim3(256), 0, ctx->stream, s->d_keys, n, d_bad);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipMemcpyAsync(unsorted, d_bad, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
     return MH_OK;
 }
 

@@ -30,6 +30,7 @@ ERRORS = {
     -4: "MH_ERR_GAP_CHAR",
     -5: "MH_ERR_ZERO_LENGTH",
     -6: "MH_ERR_UNSUPPORTED",
+    -7: "MH_ERR_IO",
 }
 
 # Exported symbols of the C ABI (checked by the CPU test suite).
@@ -38,7 +39,7 @@ EXPORTS = [
     "mh_kernel_time_reset", "mh_set_timing", "mh_get_seed", "mh_get_seed_length",
     "mh_get_seed_weight", "mh_default_seed_weight", "mh_sml_create", "mh_sml_create_device",
     "mh_sml_free", "mh_sml_get_header", "mh_sml_read", "mh_sml_get_seed_mers", "mh_sml_find_mer",
-    "mh_sml_get_sequence", "mh_find_matches_pairwise", "mh_dp_align", "mh_dp_align_device",
+    "mh_sml_get_sequence", "mh_sml_save", "mh_sml_load", "mh_find_matches_pairwise", "mh_dp_align", "mh_dp_align_device",
     "mh_device_alloc", "mh_device_free", "mh_memcpy_h2d", "mh_memcpy_d2h",
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free",
@@ -133,6 +134,8 @@ def load_library(path=LIB_PATH):
         "mh_sml_get_seed_mers": (C.c_int, [vp, vp, vp, u64, vp]),
         "mh_sml_find_mer": (C.c_int, [vp, vp, u64, C.POINTER(u64), C.POINTER(C.c_int)]),
         "mh_sml_get_sequence": (C.c_int, [vp, vp, vp, u64]),
+        "mh_sml_save": (C.c_int, [vp, vp, C.c_char_p]),
+        "mh_sml_load": (C.c_int, [vp, C.c_char_p, C.POINTER(vp), C.POINTER(C.c_int)]),
         "mh_find_matches_pairwise": (C.c_int, [vp, C.POINTER(vp), C.c_int, vp, u64, C.POINTER(u64),
                                                C.POINTER(MatchStats)]),
         "mh_dp_align": (C.c_int, [vp, vp, u64, vp, u64, vp, u64, vp, vp]),
@@ -496,6 +499,81 @@ class SortedMerList:
         out = np.zeros(nw, dtype=np.uint32)
         _check(self.ctx.L.mh_sml_get_sequence(self.ctx.h, self.h, out.ctypes.data, nw))
         return out
+
+    def save(self, path):
+        """Write the list as an .sslist file (the file FileSML::Create leaves,
+        LM/FileSML.cpp:427-446)."""
+        _check(self.ctx.L.mh_sml_save(self.ctx.h, self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, ctx, path):
+        """Read an .sslist file (FileSML::LoadFile, LM/FileSML.cpp:48-118).
+        Raises SmlLoadError carrying LoadFile2's status code (1 cannot open,
+        2 short header, 3 other format version, 4 short sequence, 5 short
+        position list; 6 header this build does not represent, 7 not a sorted
+        list)."""
+        h = C.c_void_p()
+        st = C.c_int()
+        rc = ctx.L.mh_sml_load(ctx.h, os.fsencode(path), C.byref(h), C.byref(st))
+        if rc != MH_OK:
+            msg = ctx.L.mh_last_error()
+            raise SmlLoadError(rc, st.value, msg.decode() if msg else "")
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.h = h
+        return self
+
+    def seed(self):
+        return self.header().seed
+
+
+class SmlLoadError(MhError):
+    def __init__(self, code, status, msg):
+        super().__init__(code, f"status {status}: {msg}")
+        self.status = status
+
+
+def pattern_text(seed):
+    """The seed pattern as text, '1'/'0' from its highest set bit down
+    (getPatternText, MA/progressiveMauve.cpp:198-213)."""
+    return format(int(seed) & 0xFFFFFFFFFFFFFFFF, "b")
+
+
+def default_sml_file_names(seq_files, seed_weight, seed_rank=0):
+    """<sequence file>.<seed pattern>.sslist for each sequence file
+    (getDefaultSmlFileNames, MA/progressiveMauve.cpp:215-225)."""
+    pat = pattern_text(get_seed(seed_weight, seed_rank))
+    return [f"{f}.{pat}.sslist" for f in seq_files]
+
+
+def load_smls(ctx, seqs, sml_files, seed, force_create=False, log=None):
+    """GenericMatchList::LoadSMLs (LM/MatchList.h:264-378) for sequences
+    given as ASCII: each list is loaded from its file; a list whose file is
+    missing or unreadable, whose seed differs from ``seed``, or every list
+    when ``force_create``, is built on the device and written to its file.
+    Returns the lists (device-resident) in sequence order."""
+    out = []
+    for seq, path in zip(seqs, sml_files):
+        sml = None
+        if not force_create:
+            try:
+                sml = SortedMerList.load(ctx, path)
+            except SmlLoadError:
+                sml = None
+            if sml is not None and sml.seed() != seed:
+                if log:
+                    log("Default seed mismatch.  A new sorted mer list will be created.")
+                sml.free()
+                sml = None
+            elif sml is not None and log:
+                log("Sorted mer list loaded successfully")
+        if sml is None:
+            if log:
+                log("Creating sorted mer list")
+            sml = SortedMerList(ctx, seq, seed)
+            sml.save(path)
+        out.append(sml)
+    return out
 
 
 def find_matches_raw(ctx, smls):
