@@ -152,6 +152,26 @@ int mh_find_matches_pairwise_list(mh_context *ctx, const mh_sml *const *smls, in
 uint64_t        mh_match_list_size(const mh_match_list *list);
 const mh_match *mh_match_list_data(const mh_match_list *list);
 void            mh_match_list_free(mh_match_list *list);
+/* Sequence count, file names and lengths a list carries (the genomes it was
+ * found in, or the header of a file it was read from; names "" when unknown). */
+int             mh_match_list_seq_count(const mh_match_list *list);
+const char     *mh_match_list_seq_file(const mh_match_list *list, int i);
+uint64_t        mh_match_list_seq_length(const mh_match_list *list, int i);
+
+/* Match-list files, format version 3 (`progressiveMauve --mums` output).
+ * Replaces WriteList, LM/MatchList.h:617-662: header lines, then one line per
+ * match "<length>\t<starts...>\t<id>\t0\t0".  seq_files[i] NULL (or
+ * seq_files NULL) writes "null", seq_lengths NULL writes 0, as the reference
+ * does for sequences it has no name or length for.  The id column is
+ * 0x<index+1> where the reference prints the match's address.  n == 0 leaves
+ * an empty file.  MH_ERR_IO if the file cannot be written. */
+int mh_match_list_write(const char *path, const mh_match *matches, uint64_t n, int seq_count,
+                        const char *const *seq_files, const uint64_t *seq_lengths);
+/* Replaces ReadList, LM/MatchList.h:524-613 (`--match-input`): MH_ERR_IO for
+ * a file that is missing or not format 3 with >= 2 sequences, a match line
+ * with subset or superset entries, or a match count that differs from the
+ * lines read; MH_ERR_UNSUPPORTED above MH_MAX_SEQ sequences. */
+int mh_match_list_read(const char *path, mh_match_list **out);
 
 /* ---- inter-anchor gapped DP ----------------------------------------------- */
 

@@ -357,10 +357,6 @@ int mh_find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, 
 
 }  // extern "C"
 
-struct mh_match_list {
-    std::vector<mh_match> v;
-};
-
 extern "C" {
 
 int mh_find_matches_pairwise_list(mh_context *ctx, const mh_sml *const *smls, int k, mh_match_list **out,
@@ -375,6 +371,9 @@ int mh_find_matches_pairwise_list(mh_context *ctx, const mh_sml *const *smls, in
         delete l;
         return rc;
     }
+    l->seq_count = k;
+    l->files.assign(k, std::string());
+    for (int q = 0; q < k; ++q) l->lengths.push_back(smls[q]->len);
     *out = l;
     return MH_OK;
 }

@@ -9,6 +9,15 @@
 #include <map>
 #include "mauvehip.h"
 
+// A match list owned by the caller: matches in GetMatchList order, and the
+// sequence names and lengths a match-list file carries.
+struct mh_match_list {
+    std::vector<mh_match> v;
+    int seq_count = 0;
+    std::vector<std::string> files;
+    std::vector<uint64_t> lengths;
+};
+
 namespace mh {
 
 void set_error(const std::string &msg);

@@ -42,7 +42,8 @@ EXPORTS = [
     "mh_sml_get_sequence", "mh_sml_save", "mh_sml_load", "mh_find_matches_pairwise", "mh_dp_align", "mh_dp_align_device",
     "mh_device_alloc", "mh_device_free", "mh_memcpy_h2d", "mh_memcpy_d2h",
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
-    "mh_match_list_size", "mh_match_list_data", "mh_match_list_free",
+    "mh_match_list_size", "mh_match_list_data", "mh_match_list_free", "mh_match_list_seq_count",
+    "mh_match_list_seq_file", "mh_match_list_seq_length", "mh_match_list_write", "mh_match_list_read",
     "mh_dp_align_banded_device", "mh_dp_band_region", "mh_dp_band_cells",
     "mh_gap_features_device", "mh_mlp_create", "mh_mlp_free", "mh_mlp_set_params", "mh_mlp_get_params",
     "mh_mlp_forward", "mh_mlp_grad", "mh_mlp_train", "mh_band_predict",
@@ -164,6 +165,11 @@ def load_library(path=LIB_PATH):
         "mh_match_list_size": (u64, [vp]),
         "mh_match_list_data": (vp, [vp]),
         "mh_match_list_free": (None, [vp]),
+        "mh_match_list_seq_count": (C.c_int, [vp]),
+        "mh_match_list_seq_file": (C.c_char_p, [vp, C.c_int]),
+        "mh_match_list_seq_length": (u64, [vp, C.c_int]),
+        "mh_match_list_write": (C.c_int, [C.c_char_p, vp, u64, C.c_int, C.POINTER(C.c_char_p), C.POINTER(u64)]),
+        "mh_match_list_read": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
         "mh_chain_gap_jobs": (C.c_int, [vp, u64, u64, u64, C.c_uint32, vp, C.POINTER(u64), vp, u64,
                                         C.POINTER(u64)]),
     }
@@ -597,6 +603,45 @@ def find_matches(ctx, smls):
     """PairwiseMatchFinder::FindMatches: (starts int64 (n, k), lengths uint64, MatchStats)."""
     out, st = find_matches_raw(ctx, smls)
     return out["start"][:, :len(smls)].copy(), out["length"].copy(), st
+
+
+def write_match_list(path, starts, lengths, seq_files=None, seq_lengths=None):
+    """WriteList (LM/MatchList.h:617-662): the `--mums` match-list file for
+    matches given as starts (n, k) int64 and lengths (n,)."""
+    L = load_library()
+    starts = np.asarray(starts, dtype=np.int64)
+    k = starts.shape[1] if starts.ndim == 2 else len(seq_files or ())
+    n = len(lengths)
+    m = np.zeros(n, dtype=MATCH_DTYPE)
+    if n:
+        m["start"][:, :k] = starts
+        m["length"] = np.asarray(lengths, dtype=np.uint64)
+    files = None
+    if seq_files is not None:
+        files = (C.c_char_p * k)(*[None if f is None else os.fsencode(f) for f in seq_files])
+    lens = None
+    if seq_lengths is not None:
+        lens = (C.c_uint64 * k)(*[int(v) for v in seq_lengths])
+    _check(L.mh_match_list_write(os.fsencode(path), m.ctypes.data, n, k, files, lens))
+
+
+def read_match_list(path):
+    """ReadList (LM/MatchList.h:524-613): (starts (n, k) int64, lengths
+    uint64, sequence file names, sequence lengths)."""
+    L = load_library()
+    h = C.c_void_p()
+    _check(L.mh_match_list_read(os.fsencode(path), C.byref(h)))
+    try:
+        k = L.mh_match_list_seq_count(h)
+        n = L.mh_match_list_size(h)
+        out = np.empty(n, dtype=MATCH_DTYPE)
+        if n:
+            C.memmove(out.ctypes.data, L.mh_match_list_data(h), n * MATCH_DTYPE.itemsize)
+        files = [L.mh_match_list_seq_file(h, i).decode() for i in range(k)]
+        lens = [L.mh_match_list_seq_length(h, i) for i in range(k)]
+    finally:
+        L.mh_match_list_free(h)
+    return out["start"][:, :k].copy(), out["length"].copy(), files, lens
 
 
 def ascii_to_codes_device(ctx, d_ascii, length, d_codes):

@@ -125,6 +125,11 @@ def test_loaded_lists_find_the_same_matches(ctx, tmp_path):
     assert len(l1) > 100
     assert np.array_equal(s1, s2) and np.array_equal(l1, l2)
     assert (st1.seed_tuples, st1.collisions) == (st2.seed_tuples, st2.collisions)
+    # the --mums file of the match list reads back to the same list
+    mums = str(tmp_path / "ab.mums")
+    mauvehip.write_match_list(mums, s1, l1, paths, [len(a), len(b)])
+    s3, l3, f3, n3 = mauvehip.read_match_list(mums)
+    assert np.array_equal(s3, s1) and np.array_equal(l3, l1) and f3 == paths and n3 == [len(a), len(b)]
     for s in built + loaded:
         s.free()
 
