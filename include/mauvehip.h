@@ -322,6 +322,30 @@ int mh_device_free(mh_context *ctx, void *d_ptr);
 int mh_memcpy_h2d(mh_context *ctx, void *d_dst, const void *h_src, uint64_t bytes);
 int mh_memcpy_d2h(mh_context *ctx, void *h_dst, const void *d_src, uint64_t bytes);
 
+/* ---- genome index LUT (mauve.buildIndex) ------------------------------ */
+
+/* The column walk of buildIndex, PY/buildindex.py:110-129: lut[genome_length]
+ * set to -1, then for each XMFA group (PY/xmfa.py:15-68 grammar) holding
+ * sequences 1 and 2, every column whose two characters are equal maps the
+ * current 0-based position of sequence 1 to that of sequence 2 (each advanced
+ * on a non-gap character from start-1; strand ignored).  *n_groups (may be
+ * NULL) gets the number of groups.  MH_ERR_INVALID if a sequence-2 row is
+ * shorter than its sequence-1 row (the reference's IndexError). */
+int mh_index_from_xmfa(const char *xmfa, uint64_t len, uint64_t genome_length, int32_t *lut,
+                       uint64_t *n_groups);
+/* fixZeroIdx, PY/indexutils.pyx:27-43: an unmapped lut[0] takes
+ * lut[k] - k from the first mapped k >= 10 when the first bases agree.
+ * MH_ERR_INVALID when no such k exists (the reference's IndexError). */
+int mh_index_fix_zero(int32_t *lut, uint64_t n, const char *genome, uint64_t genome_len, const char *ref,
+                      uint64_t ref_len);
+/* fillGaps, PY/indexutils.pyx:46-74: left to right, a run of -1 after a
+ * mapped entry is filled with consecutive values when an entry within
+ * max_gap_width of the run's start continues the left neighbour's diagonal. */
+int mh_index_fill_gaps(int32_t *lut, uint64_t n, int max_gap_width);
+/* smoothEdges, PY/indexutils.pyx:77-108: the same repair at every break of
+ * consecutive mapping after a mapped entry, within smoothing_radius. */
+int mh_index_smooth_edges(int32_t *lut, uint64_t n, int smoothing_radius);
+
 #ifdef __cplusplus
 }
 #endif
