@@ -322,6 +322,26 @@ int mh_device_free(mh_context *ctx, void *d_ptr);
 int mh_memcpy_h2d(mh_context *ctx, void *d_dst, const void *h_src, uint64_t bytes);
 int mh_memcpy_d2h(mh_context *ctx, void *h_dst, const void *d_src, uint64_t bytes);
 
+/* ---- XMFA output --------------------------------------------------------- */
+
+/* One genome's row of an aligned interval. */
+typedef struct {
+    int64_t     start;     /* 1-based left end, negative on the reverse strand, 0 absent */
+    uint64_t    length;    /* bases of the genome in the interval                       */
+    const char *row;       /* aligned row ('-' for gaps), row_len characters            */
+    uint64_t    row_len;   /* the interval's alignment length (equal for all its rows)  */
+} mh_xmfa_row;
+
+/* Replaces GenericIntervalList::WriteStandardAlignment (LM/IntervalList.h:353-442):
+ * rows[iv * seq_count + s] is genome s of interval iv.  Header lines name each
+ * genome's file and format (by extension, as libGenome's gnSourceFactory picks
+ * it); intervals of alignment length 0 are skipped; a genome absent from an
+ * interval (start 0) is written as 0-0 in the first interval only; rows are
+ * wrapped at 80 columns; "=" ends each interval.  n_intervals == 0 leaves an
+ * empty file.  backbone_file may be NULL. */
+int mh_xmfa_write(const char *path, int seq_count, const char *const *seq_files, const char *backbone_file,
+                  const mh_xmfa_row *rows, uint64_t n_intervals);
+
 /* ---- genome index LUT (mauve.buildIndex) ------------------------------ */
 
 /* The column walk of buildIndex, PY/buildindex.py:110-129: lut[genome_length]

@@ -44,7 +44,7 @@ EXPORTS = [
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free", "mh_match_list_seq_count",
     "mh_match_list_seq_file", "mh_match_list_seq_length", "mh_match_list_write", "mh_match_list_read",
-    "mh_index_from_xmfa", "mh_index_fix_zero", "mh_index_fill_gaps", "mh_index_smooth_edges",
+    "mh_index_from_xmfa", "mh_index_fix_zero", "mh_index_fill_gaps", "mh_index_smooth_edges", "mh_xmfa_write",
     "mh_dp_align_banded_device", "mh_dp_band_region", "mh_dp_band_cells",
     "mh_gap_features_device", "mh_mlp_create", "mh_mlp_free", "mh_mlp_set_params", "mh_mlp_get_params",
     "mh_mlp_forward", "mh_mlp_grad", "mh_mlp_train", "mh_band_predict",
@@ -175,6 +175,7 @@ def load_library(path=LIB_PATH):
         "mh_index_fix_zero": (C.c_int, [vp, u64, C.c_char_p, u64, C.c_char_p, u64]),
         "mh_index_fill_gaps": (C.c_int, [vp, u64, C.c_int]),
         "mh_index_smooth_edges": (C.c_int, [vp, u64, C.c_int]),
+        "mh_xmfa_write": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.c_char_p, vp, u64]),
         "mh_chain_gap_jobs": (C.c_int, [vp, u64, u64, u64, C.c_uint32, vp, C.POINTER(u64), vp, u64,
                                         C.POINTER(u64)]),
     }
@@ -608,6 +609,31 @@ def find_matches(ctx, smls):
     """PairwiseMatchFinder::FindMatches: (starts int64 (n, k), lengths uint64, MatchStats)."""
     out, st = find_matches_raw(ctx, smls)
     return out["start"][:, :len(smls)].copy(), out["length"].copy(), st
+
+
+class XmfaRow(C.Structure):
+    _fields_ = [("start", C.c_int64), ("length", C.c_uint64), ("row", C.c_char_p), ("row_len", C.c_uint64)]
+
+
+def write_xmfa(path, intervals, seq_files, backbone_file=None):
+    """WriteStandardAlignment (LM/IntervalList.h:353-442).  ``intervals`` is a
+    list of intervals, each a list (one per genome) of (start, length, row):
+    start 1-based, negative on the reverse strand, 0 if absent; row the
+    aligned string."""
+    L = load_library()
+    k = len(seq_files)
+    rows = (XmfaRow * max(1, k * len(intervals)))()
+    keep = []
+    for i, iv in enumerate(intervals):
+        if len(iv) != k:
+            raise ValueError("every interval needs one row per genome")
+        for s, (start, length, row) in enumerate(iv):
+            b = row.encode() if isinstance(row, str) else bytes(row)
+            keep.append(b)
+            rows[i * k + s] = XmfaRow(int(start), int(length), b, len(b))
+    files = (C.c_char_p * k)(*[os.fsencode(f) for f in seq_files])
+    _check(L.mh_xmfa_write(os.fsencode(path), k, files, os.fsencode(backbone_file) if backbone_file else None,
+                           C.cast(rows, C.c_void_p), len(intervals)))
 
 
 def write_match_list(path, starts, lengths, seq_files=None, seq_lengths=None):
