@@ -322,6 +322,40 @@ int mh_device_free(mh_context *ctx, void *d_ptr);
 int mh_memcpy_h2d(mh_context *ctx, void *d_dst, const void *h_src, uint64_t bytes);
 int mh_memcpy_d2h(mh_context *ctx, void *h_dst, const void *d_src, uint64_t bytes);
 
+/* ---- refinement of pairwise gapped alignments ---------------------------- */
+
+#define MH_REFINE_PROFILE  1u   /* refineAlignment(profile_aln = true)  */
+#define MH_REFINE_WINDOWS  2u   /* refineAlignment(profile_aln = false) */
+
+/* Replaces the gapped-alignment passes progressiveMauve runs over each aligned
+ * interval of two genomes (LM/ProgressiveAligner.cpp:1119-1290; for two genomes
+ * the profile pass, then the final refinement pass, 2561-2565 and 3751-3755).
+ * The profile pass cuts each interval into pieces (removeLargeGapsPP and the
+ * density halving, 1015-1117, 1142-1166) and re-aligns each piece that holds
+ * both genomes as ProfileAlignFast(anchored) does (LM/MuscleInterface.cpp:
+ * 1053-1162, MU/anchoredpp.cpp:353-552): anchor columns from the smoothed
+ * column score, then every column range between them re-aligned from its
+ * residues.  The refinement pass re-aligns each piece in windows of 500 or 200
+ * columns by its density (RefineFast/RefineW, LM/MuscleInterface.cpp:823-894,
+ * MU/refinew.cpp:75-200).  Each re-alignment is one NWSmall of genome 1's
+ * residues against genome 2's; all of a pass's jobs, over all n intervals,
+ * run as one mh_dp_align batch.  Rows are ASCII with '-' gaps, residues
+ * A,C,G,T in either case (others: MH_ERR_UNSUPPORTED, as for the DP); an
+ * anchored range with residues of one genome only is MH_ERR_ZERO_LENGTH (the
+ * reference's NWSmall quits).  out1[i]/out2[i] receive out_cols[i] columns;
+ * caps[i] >= 2 * cols[i] always suffices (MH_ERR_INVALID with out_cols set
+ * when a capacity is short). */
+int mh_refine_pairwise(mh_context *ctx, uint64_t n, const char *const *rows1, const char *const *rows2,
+                       const uint64_t *cols, uint32_t passes, char *const *out1, char *const *out2,
+                       const uint64_t *caps, uint64_t *out_cols);
+/* Host-only parts of the above, for checking: FindAnchorColsPP
+ * (MU/anchoredpp.cpp:353-404) over two rows, and the pieces of refineAlignment
+ * (lo, hi, unaligned flag; three values per piece). */
+int mh_refine_anchor_cols(const char *row1, const char *row2, uint64_t cols, uint32_t *anchors, uint64_t cap,
+                          uint64_t *n_anchors);
+int mh_refine_pieces(const char *row1, const char *row2, uint64_t cols, uint64_t *pieces, uint64_t cap,
+                     uint64_t *n_pieces);
+
 /* ---- XMFA output --------------------------------------------------------- */
 
 /* One genome's row of an aligned interval. */

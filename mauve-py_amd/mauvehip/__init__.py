@@ -45,6 +45,7 @@ EXPORTS = [
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free", "mh_match_list_seq_count",
     "mh_match_list_seq_file", "mh_match_list_seq_length", "mh_match_list_write", "mh_match_list_read",
     "mh_index_from_xmfa", "mh_index_fix_zero", "mh_index_fill_gaps", "mh_index_smooth_edges", "mh_xmfa_write",
+    "mh_refine_pairwise", "mh_refine_anchor_cols", "mh_refine_pieces",
     "mh_dp_align_banded_device", "mh_dp_band_region", "mh_dp_band_cells",
     "mh_gap_features_device", "mh_mlp_create", "mh_mlp_free", "mh_mlp_set_params", "mh_mlp_get_params",
     "mh_mlp_forward", "mh_mlp_grad", "mh_mlp_train", "mh_band_predict",
@@ -176,6 +177,10 @@ def load_library(path=LIB_PATH):
         "mh_index_fill_gaps": (C.c_int, [vp, u64, C.c_int]),
         "mh_index_smooth_edges": (C.c_int, [vp, u64, C.c_int]),
         "mh_xmfa_write": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.c_char_p, vp, u64]),
+        "mh_refine_pairwise": (C.c_int, [vp, u64, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.POINTER(u64),
+                                         C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.POINTER(u64), C.POINTER(u64)]),
+        "mh_refine_anchor_cols": (C.c_int, [C.c_char_p, C.c_char_p, u64, vp, u64, C.POINTER(u64)]),
+        "mh_refine_pieces": (C.c_int, [C.c_char_p, C.c_char_p, u64, vp, u64, C.POINTER(u64)]),
         "mh_chain_gap_jobs": (C.c_int, [vp, u64, u64, u64, C.c_uint32, vp, C.POINTER(u64), vp, u64,
                                         C.POINTER(u64)]),
     }
@@ -609,6 +614,59 @@ def find_matches(ctx, smls):
     """PairwiseMatchFinder::FindMatches: (starts int64 (n, k), lengths uint64, MatchStats)."""
     out, st = find_matches_raw(ctx, smls)
     return out["start"][:, :len(smls)].copy(), out["length"].copy(), st
+
+
+REFINE_PROFILE = 1
+REFINE_WINDOWS = 2
+
+
+def _row_bytes(r):
+    return r.encode() if isinstance(r, str) else bytes(r)
+
+
+def refine_anchor_cols(row1, row2):
+    """FindAnchorColsPP over two aligned rows (host part of mh_refine_pairwise)."""
+    L = load_library()
+    a, b = _row_bytes(row1), _row_bytes(row2)
+    if len(a) != len(b):
+        raise ValueError("rows differ in length")
+    out = np.zeros(max(1, len(a)), dtype=np.uint32)
+    n = C.c_uint64()
+    _check(L.mh_refine_anchor_cols(a, b, len(a), out.ctypes.data, len(out), C.byref(n)))
+    return out[:n.value].tolist()
+
+
+def refine_pieces(row1, row2):
+    """The pieces refineAlignment cuts two rows into: [(lo, hi, unaligned)]."""
+    L = load_library()
+    a, b = _row_bytes(row1), _row_bytes(row2)
+    if len(a) != len(b):
+        raise ValueError("rows differ in length")
+    cap = len(a) + 2
+    out = np.zeros(3 * cap, dtype=np.uint64)
+    n = C.c_uint64()
+    _check(L.mh_refine_pieces(a, b, len(a), out.ctypes.data, cap, C.byref(n)))
+    return [(int(out[3 * k]), int(out[3 * k + 1]), bool(out[3 * k + 2])) for k in range(n.value)]
+
+
+def refine_pairwise(ctx, intervals, passes=REFINE_PROFILE | REFINE_WINDOWS):
+    """mh_refine_pairwise over [(row1, row2)]: the refined row pairs."""
+    L = load_library()
+    n = len(intervals)
+    r1 = [_row_bytes(a) for a, _ in intervals]
+    r2 = [_row_bytes(b) for _, b in intervals]
+    for a, b in zip(r1, r2):
+        if len(a) != len(b):
+            raise ValueError("rows of an interval differ in length")
+    cols = (C.c_uint64 * max(1, n))(*[len(a) for a in r1])
+    caps = (C.c_uint64 * max(1, n))(*[2 * len(a) + 1 for a in r1])
+    o1 = [C.create_string_buffer(2 * len(a) + 1) for a in r1]
+    o2 = [C.create_string_buffer(2 * len(a) + 1) for a in r1]
+    outc = (C.c_uint64 * max(1, n))()
+    _check(L.mh_refine_pairwise(ctx.h, n, (C.c_char_p * max(1, n))(*r1), (C.c_char_p * max(1, n))(*r2), cols,
+                                passes, (C.c_void_p * max(1, n))(*[C.addressof(b) for b in o1]),
+                                (C.c_void_p * max(1, n))(*[C.addressof(b) for b in o2]), caps, outc))
+    return [(o1[i].raw[:outc[i]].decode(), o2[i].raw[:outc[i]].decode()) for i in range(n)]
 
 
 class XmfaRow(C.Structure):

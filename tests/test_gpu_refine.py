@@ -1,0 +1,53 @@
+"""GPU parity of mh_refine_pairwise (the pairwise gapped-alignment passes) with
+the refinement oracle: identical rows for every interval of a batch, for the
+profile pass alone, the refinement pass alone and both."""
+import numpy as np
+import pytest
+
+import mauvehip
+from oracle import refine_oracle as ro
+from refine_cases import alignment
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases(seed, count):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.choice([60, 400, 2500, 9000, 24000]))
+        out.append(alignment(rng, n, div=float(rng.choice([0.02, 0.1, 0.25])),
+                             marble=float(rng.choice([0.0, 0.2, 0.6])),
+                             big_gap=int(rng.choice([0, 0, 5600])) if n > 5000 else 0, lower=bool(k % 4 == 3)))
+    return out
+
+
+@pytest.mark.parametrize("passes", [mauvehip.REFINE_PROFILE, mauvehip.REFINE_WINDOWS,
+                                    mauvehip.REFINE_PROFILE | mauvehip.REFINE_WINDOWS])
+def test_refine_batch_matches_oracle(ctx, passes):
+    ivs = _cases(7 + passes, 8)
+    got = mauvehip.refine_pairwise(ctx, ivs, passes)
+    for (r1, r2), (g1, g2) in zip(ivs, got):
+        if passes == mauvehip.REFINE_PROFILE:
+            want = ro.refine_alignment(r1, r2, True)
+        elif passes == mauvehip.REFINE_WINDOWS:
+            want = ro.refine_alignment(r1, r2, False)
+        else:
+            want = ro.gapped_alignment(r1, r2)
+        assert (g1, g2) == want
+        assert len(g1) == len(g2)
+        assert g1.replace("-", "") == r1.replace("-", "") and g2.replace("-", "") == r2.replace("-", "")
+
+
+def test_refine_single_interval_and_corners(ctx):
+    cases = [("ACGTACGTAC", "ACGTACGTAC"), ("A" * 30, "-" * 30), ("ACGT" * 30, "AC-T" * 30)]
+    got = mauvehip.refine_pairwise(ctx, cases)
+    for (r1, r2), g in zip(cases, got):
+        assert g == ro.gapped_alignment(r1, r2)
+    assert got[1] == cases[1]                       # one genome only: left as it is
+
+
+def test_refine_rejects_other_residues(ctx):
+    with pytest.raises(mauvehip.MhError) as e:
+        mauvehip.refine_pairwise(ctx, [("ACGTNACGT" * 20, "ACGTAACGT" * 20)])
+    assert e.value.code == -6

@@ -1,0 +1,32 @@
+"""The host parts of mh_refine_pairwise (anchor columns, pieces) against the
+refinement oracle, bit-exact, on the CPU."""
+import numpy as np
+import pytest
+
+import mauvehip
+from oracle import refine_oracle as ro
+from refine_cases import alignment
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_anchor_cols_match_oracle(seed):
+    rng = np.random.default_rng(seed)
+    r1, r2 = alignment(rng, int(rng.integers(30, 3000)), div=float(rng.choice([0.02, 0.1, 0.3])),
+                       marble=float(rng.choice([0.0, 0.1, 0.5])), lower=bool(seed % 3 == 0))
+    assert mauvehip.refine_anchor_cols(r1, r2) == ro.anchor_cols(r1, r2)
+
+
+def test_anchor_cols_corners():
+    for r1, r2 in [("A", "A"), ("A" * 21, "A" * 21), ("A" * 22, "A" * 22), ("-A-", "AA-"),
+                   ("AC--GT" * 50, "ACGTGT" * 50), ("N" * 40, "A" * 40), ("A.C" * 30, "AGC" * 30)]:
+        assert mauvehip.refine_anchor_cols(r1, r2) == ro.anchor_cols(r1, r2), (r1[:10], r2[:10])
+
+
+@pytest.mark.parametrize("seed,big", [(0, 0), (1, 6000), (2, 5001), (3, 12000), (4, 0)])
+def test_pieces_match_oracle(seed, big):
+    rng = np.random.default_rng(40 + seed)
+    r1, r2 = alignment(rng, int(rng.integers(2000, 30000)), div=0.2, marble=0.5, big_gap=big)
+    got = mauvehip.refine_pieces(r1, r2)
+    assert got == ro.pieces_of(r1, r2)
+    assert got[0][0] == 0 and got[-1][1] == len(r1)
+    assert all(p[1] == q[0] for p, q in zip(got, got[1:]))
