@@ -322,6 +322,28 @@ int mh_device_free(mh_context *ctx, void *d_ptr);
 int mh_memcpy_h2d(mh_context *ctx, void *d_dst, const void *h_src, uint64_t bytes);
 int mh_memcpy_d2h(mh_context *ctx, void *h_dst, const void *d_src, uint64_t bytes);
 
+/* ---- recursive anchoring in the gaps between anchors --------------------- */
+
+/* A gap between two anchors as getInterveningCoordinates gives it
+ * (LM/ProgressiveAligner.cpp:566-587), per genome: lend = first position after
+ * the left anchor, rend = first position of the right anchor (1-based). */
+typedef struct {
+    int64_t lend[2];
+    int64_t rend[2];
+} mh_anchor_gap;
+
+/* Replaces ProgressiveAligner::pairwiseAnchorSearch
+ * (LM/ProgressiveAligner.cpp:590-679) for a list of gaps of the two genomes
+ * d_seq_a / d_seq_b (device ASCII).  Per gap: the pieces lend .. lend +
+ * (rend - lend - 1) - 1 of each genome, the seed getSeed(getDefaultSeedWeight
+ * (average piece length), 0) (no anchors below weight 5), sorted mer lists and
+ * the pairwise match search on the GPU, then EliminateOverlaps_v2
+ * (LM/ProgressiveAligner.h:301-394) and LengthFilter(12), shifted to genome
+ * coordinates.  The anchors of gap i are entries [first[i], first[i+1]) of the
+ * returned list (first has n_gaps + 1 entries), in the reference's order. */
+int mh_anchor_gaps(mh_context *ctx, const void *d_seq_a, uint64_t len_a, const void *d_seq_b, uint64_t len_b,
+                   const mh_anchor_gap *gaps, uint64_t n_gaps, mh_match_list **out, uint64_t *first);
+
 /* ---- refinement of pairwise gapped alignments ---------------------------- */
 
 #define MH_REFINE_PROFILE  1u   /* refineAlignment(profile_aln = true)  */

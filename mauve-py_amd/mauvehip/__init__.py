@@ -45,7 +45,7 @@ EXPORTS = [
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free", "mh_match_list_seq_count",
     "mh_match_list_seq_file", "mh_match_list_seq_length", "mh_match_list_write", "mh_match_list_read",
     "mh_index_from_xmfa", "mh_index_fix_zero", "mh_index_fill_gaps", "mh_index_smooth_edges", "mh_xmfa_write",
-    "mh_refine_pairwise", "mh_refine_anchor_cols", "mh_refine_pieces",
+    "mh_refine_pairwise", "mh_refine_anchor_cols", "mh_refine_pieces", "mh_anchor_gaps",
     "mh_dp_align_banded_device", "mh_dp_band_region", "mh_dp_band_cells",
     "mh_gap_features_device", "mh_mlp_create", "mh_mlp_free", "mh_mlp_set_params", "mh_mlp_get_params",
     "mh_mlp_forward", "mh_mlp_grad", "mh_mlp_train", "mh_band_predict",
@@ -181,6 +181,7 @@ def load_library(path=LIB_PATH):
                                          C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.POINTER(u64), C.POINTER(u64)]),
         "mh_refine_anchor_cols": (C.c_int, [C.c_char_p, C.c_char_p, u64, vp, u64, C.POINTER(u64)]),
         "mh_refine_pieces": (C.c_int, [C.c_char_p, C.c_char_p, u64, vp, u64, C.POINTER(u64)]),
+        "mh_anchor_gaps": (C.c_int, [vp, vp, u64, vp, u64, vp, u64, C.POINTER(vp), vp]),
         "mh_chain_gap_jobs": (C.c_int, [vp, u64, u64, u64, C.c_uint32, vp, C.POINTER(u64), vp, u64,
                                         C.POINTER(u64)]),
     }
@@ -614,6 +615,41 @@ def find_matches(ctx, smls):
     """PairwiseMatchFinder::FindMatches: (starts int64 (n, k), lengths uint64, MatchStats)."""
     out, st = find_matches_raw(ctx, smls)
     return out["start"][:, :len(smls)].copy(), out["length"].copy(), st
+
+
+ANCHOR_GAP_DTYPE = np.dtype([("lend", "<i8", (2,)), ("rend", "<i8", (2,))])
+
+
+def anchor_gaps_device(ctx, d_a, len_a, d_b, len_b, gaps):
+    """pairwiseAnchorSearch over gaps (ANCHOR_GAP_DTYPE records) of two device
+    ASCII genomes: (starts (n, 2), lengths, first (n_gaps + 1))."""
+    g = np.ascontiguousarray(gaps, dtype=ANCHOR_GAP_DTYPE)
+    first = np.zeros(len(g) + 1, dtype=np.uint64)
+    h = C.c_void_p()
+    _check(ctx.L.mh_anchor_gaps(ctx.h, d_a, len_a, d_b, len_b, g.ctypes.data if len(g) else None, len(g),
+                                C.byref(h), first.ctypes.data))
+    try:
+        n = ctx.L.mh_match_list_size(h)
+        out = np.empty(n, dtype=MATCH_DTYPE)
+        if n:
+            C.memmove(out.ctypes.data, ctx.L.mh_match_list_data(h), n * MATCH_DTYPE.itemsize)
+    finally:
+        ctx.L.mh_match_list_free(h)
+    return out["start"][:, :2].copy(), out["length"].copy(), first
+
+
+def anchor_gaps(ctx, seq_a, seq_b, gaps):
+    """anchor_gaps_device with host ASCII genomes."""
+    a = seq_a.encode() if isinstance(seq_a, str) else bytes(seq_a)
+    b = seq_b.encode() if isinstance(seq_b, str) else bytes(seq_b)
+    da, db = ctx.alloc(len(a) + 16), ctx.alloc(len(b) + 16)
+    try:
+        ctx.h2d(da, np.frombuffer(a, dtype=np.uint8))
+        ctx.h2d(db, np.frombuffer(b, dtype=np.uint8))
+        return anchor_gaps_device(ctx, da, len(a), db, len(b), gaps)
+    finally:
+        ctx.free(da)
+        ctx.free(db)
 
 
 REFINE_PROFILE = 1

@@ -72,6 +72,11 @@ typedef struct {
     int64_t  matches;         /* entries in the final match list          */
 } or_match_stats;
 
+/* EliminateOverlaps_v2 (LM/ProgressiveAligner.h:301-394) for matches of two
+ * genomes, then LengthFilter(min_length) (LM/MatchList.h:680-693); ml is
+ * rewritten in the resulting order.  Returns the number kept.  (overlap.cpp) */
+int64_t or_eliminate_overlaps_pairwise(or_match *ml, int64_t n, uint64_t min_length);
+
 /* Full pairwise match finding over k genomes (k <= OR_MAX_SEQ):
  * SML build + SearchRange + PairwiseMatchFinder::EnumerateMatches +
  * HashMatch + AddHashEntry/ExtendMatch + GetMatchList order.

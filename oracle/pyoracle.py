@@ -63,6 +63,8 @@ def lib():
                                               C.c_uint64, C.POINTER(OrMatch), C.c_void_p, C.c_int64]
         L.or_extend_seed.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
                                      C.c_uint64, C.POINTER(OrMatch)]
+        L.or_eliminate_overlaps_pairwise.restype = C.c_int64
+        L.or_eliminate_overlaps_pairwise.argtypes = [C.POINTER(OrMatch), C.c_int64, C.c_uint64]
         L.or_nwsmall.restype = C.c_int64
         L.or_nwsmall.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_char_p,
                                  C.POINTER(C.c_float)]
@@ -174,6 +176,57 @@ def find_matches(seqs, seed, max_out=None):
     starts = np.array([[out[i].start[s] for s in range(k)] for i in range(n)], dtype=np.int64).reshape(n, k)
     lengths = np.array([out[i].length for i in range(n)], dtype=np.uint64)
     return starts, lengths, st
+
+
+def eliminate_overlaps(starts, lengths, min_length=0):
+    """EliminateOverlaps_v2 then LengthFilter for two-genome matches:
+    (starts (n, 2), lengths) in the resulting order."""
+    n = len(lengths)
+    arr = (OrMatch * max(n, 1))()
+    for i in range(n):
+        arr[i].start[0] = int(starts[i][0])
+        arr[i].start[1] = int(starts[i][1])
+        arr[i].length = int(lengths[i])
+    m = lib().or_eliminate_overlaps_pairwise(arr, n, min_length)
+    st = np.array([[arr[i].start[0], arr[i].start[1]] for i in range(m)], dtype=np.int64).reshape(m, 2)
+    return st, np.array([arr[i].length for i in range(m)], dtype=np.uint64)
+
+
+MIN_ANCHOR_LENGTH = 9                                 # LM/Aligner.h:266
+MIN_DNA_SEED_WEIGHT = 5                               # LM/SeedMasks.h:375
+
+
+def anchor_gap(seq_a, seq_b, lend, rend):
+    """ProgressiveAligner::pairwiseAnchorSearch (LM/ProgressiveAligner.cpp:
+    590-679) for one gap: lend/rend are getInterveningCoordinates' gap_lend /
+    gap_rend in each genome (1-based); the searched sequence starts at lend and
+    is rend - lend - 1 long (when positive).  Returns (starts (n, 2) in genome
+    coordinates, lengths)."""
+    seqs = []
+    for g, s in enumerate((_b(seq_a), _b(seq_b))):
+        diff = rend[g] - lend[g]
+        diff = diff - 1 if diff > 0 else 0
+        if diff > 0 and lend[g] + diff - 1 <= len(s):
+            seqs.append(s[lend[g] - 1:lend[g] - 1 + diff])
+        else:
+            seqs.append(b"")
+    avg = (len(seqs[0]) + len(seqs[1])) // 2
+    w = default_seed_weight(avg)
+    empty = (np.zeros((0, 2), dtype=np.int64), np.zeros(0, dtype=np.uint64))
+    if w < MIN_DNA_SEED_WEIGHT:
+        return empty
+    seed = get_seed(w, 0)
+    if min(len(s) for s in seqs) < seed_length(seed):
+        return empty                                  # a list without seeds pairs with nothing
+    st, ln, _ = find_matches(seqs, seed)
+    if len(ln) == 0:
+        return empty
+    st, ln = eliminate_overlaps(st, ln, MIN_ANCHOR_LENGTH + 3)
+    for g in range(2):
+        # SetLeftEnd(LeftEnd + gap_lend - 1), keeping the strand
+        sign = np.where(st[:, g] < 0, -1, 1)
+        st[:, g] = sign * (np.abs(st[:, g]) + lend[g] - 1)
+    return st, ln
 
 
 def seed_tuples(seqs, seed):
