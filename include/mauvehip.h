@@ -344,6 +344,25 @@ typedef struct {
 int mh_anchor_gaps(mh_context *ctx, const void *d_seq_a, uint64_t len_a, const void *d_seq_b, uint64_t len_b,
                    const mh_anchor_gap *gaps, uint64_t n_gaps, mh_match_list **out, uint64_t *first);
 
+/* The rows of a chain from mh_chain_gap_jobs as one interval over both whole
+ * genomes: the chain's anchors aligned column for column, each stretch before,
+ * between and after them unaligned (genome 1's residues against gaps, then
+ * gaps against genome 2's).  No counterpart in the reference: a stand-in for
+ * the alignment the LCB pass hands to the gapped aligner.  *cols = len_a +
+ * len_b - anchored bases; MH_ERR_INVALID if cap is smaller (cols set) or the
+ * chain is not forward, ordered and non-overlapping. */
+int mh_chain_rows(const mh_match *matches, const uint32_t *chain, uint64_t n_chain, const char *seq_a,
+                  uint64_t len_a, const char *seq_b, uint64_t len_b, char *row_a, char *row_b, uint64_t cap,
+                  uint64_t *cols);
+/* The gaps recurseOnPairs hands to pairwiseAnchorSearch
+ * (LM/ProgressiveAligner.cpp:838-905) for an interval on the forward strand of
+ * both genomes starting at left_a / left_b (1-based): between consecutive
+ * columns aligned in both rows (and the interval's ends), every stretch with
+ * more than min_gap residues in each genome (the reference's 200,
+ * LM/Aligner.cpp:179), as getInterveningCoordinates gives it. */
+int mh_recursion_gaps(const char *row_a, const char *row_b, uint64_t cols, int64_t left_a, int64_t left_b,
+                      uint64_t min_gap, mh_anchor_gap *gaps, uint64_t cap, uint64_t *n_gaps);
+
 /* ---- refinement of pairwise gapped alignments ---------------------------- */
 
 #define MH_REFINE_PROFILE  1u   /* refineAlignment(profile_aln = true)  */

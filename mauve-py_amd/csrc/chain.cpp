@@ -1,13 +1,17 @@
-// chain.cpp -- anchor chain and inter-anchor gap jobs.
+// chain.cpp -- anchor chain, inter-anchor gap jobs and the alignment rows of
+// a chain.
 //
-// In the reference the DP jobs come from the LCB pass and MUSCLE's anchored
-// profile-profile ranges (LM/ProgressiveAligner.cpp:1119-1290,
-// MU/anchoredpp.cpp:443-552).  That host logic is not part of this library;
-// this file gives a simple deterministic producer of the same kind of work so
-// the seed-anchoring and DP stages can run end to end: the heaviest chain of
-// non-overlapping, collinear forward-strand matches (weighted by length), and
-// one DP job per gap between consecutive chain anchors where both sides are
-// non-empty and at most max_gap long.
+// In the reference the anchors come from the LCB pass (greedy breakpoint
+// elimination over the match list, LM/ProgressiveAligner.cpp), which this
+// library does not implement.  This file gives a simple deterministic stand-in
+// for it so the path can run end to end: the heaviest chain of non-overlapping,
+// collinear forward-strand matches (weighted by length); the DP jobs of the
+// gaps between consecutive chain anchors (both sides non-empty and at most
+// max_gap long); and the two rows of the chain as one interval, anchors
+// aligned and each stretch between them unaligned, which the refinement passes
+// (refine.cpp) take as input.  mh_recursion_gaps restates the reference's
+// choice of gaps for the recursive anchor search on such rows
+// (LM/ProgressiveAligner.cpp:838-905).
 #include "mh_internal.hpp"
 #include <algorithm>
 #include <cstring>
@@ -177,6 +181,98 @@ int mh_chain_gap_jobs(const mh_match *matches, uint64_t n, uint64_t len_a, uint6
     }
     gap((int64_t)len_a, (int64_t)len_b);
     *n_jobs = nj;
+    return MH_OK;
+}
+
+int mh_chain_rows(const mh_match *matches, const uint32_t *chain, uint64_t n_chain, const char *seq_a,
+                  uint64_t len_a, const char *seq_b, uint64_t len_b, char *row_a, char *row_b, uint64_t cap,
+                  uint64_t *cols)
+{
+    MH_CHECK(cols && (matches || n_chain == 0) && (chain || n_chain == 0) && (seq_a || len_a == 0) &&
+                 (seq_b || len_b == 0),
+             MH_ERR_INVALID, "null argument");
+    // check the chain: forward anchors, in order, not overlapping in either genome
+    uint64_t pa = 0, pb = 0, anchored = 0;
+    for (uint64_t q = 0; q < n_chain; ++q) {
+        const mh_match &m = matches[chain[q]];
+        MH_CHECK(m.start[0] > 0 && m.start[1] > 0, MH_ERR_INVALID, "chain anchor not on the forward strand");
+        const uint64_t s0 = (uint64_t)m.start[0] - 1, s1 = (uint64_t)m.start[1] - 1;
+        MH_CHECK(s0 >= pa && s1 >= pb && s0 + m.length <= len_a && s1 + m.length <= len_b, MH_ERR_INVALID,
+                 "chain anchors out of order, overlapping or outside the genomes");
+        pa = s0 + m.length;
+        pb = s1 + m.length;
+        anchored += m.length;
+    }
+    const uint64_t n = len_a + len_b - anchored;
+    *cols = n;
+    MH_CHECK(cap >= n, MH_ERR_INVALID, "row capacity too small (cols holds the size)");
+    MH_CHECK(row_a && row_b, MH_ERR_INVALID, "null argument");
+    uint64_t c = 0;
+    pa = pb = 0;
+    auto stretch = [&](uint64_t ea, uint64_t eb) {
+        // genome 1's residues against gaps, then gaps against genome 2's
+        const uint64_t ga = ea - pa, gb = eb - pb;
+        std::memcpy(row_a + c, seq_a + pa, ga);
+        std::memset(row_b + c, '-', ga);
+        c += ga;
+        std::memset(row_a + c, '-', gb);
+        std::memcpy(row_b + c, seq_b + pb, gb);
+        c += gb;
+    };
+    for (uint64_t q = 0; q < n_chain; ++q) {
+        const mh_match &m = matches[chain[q]];
+        const uint64_t s0 = (uint64_t)m.start[0] - 1, s1 = (uint64_t)m.start[1] - 1;
+        stretch(s0, s1);
+        std::memcpy(row_a + c, seq_a + s0, m.length);
+        std::memcpy(row_b + c, seq_b + s1, m.length);
+        c += m.length;
+        pa = s0 + m.length;
+        pb = s1 + m.length;
+    }
+    stretch(len_a, len_b);
+    return MH_OK;
+}
+
+int mh_recursion_gaps(const char *row_a, const char *row_b, uint64_t cols, int64_t left_a, int64_t left_b,
+                      uint64_t min_gap, mh_anchor_gap *gaps, uint64_t cap, uint64_t *n_gaps)
+{
+    MH_CHECK(n_gaps && (gaps || cap == 0) && ((row_a && row_b) || cols == 0), MH_ERR_INVALID, "null argument");
+    // the interval's right ends (RightEnd = LeftEnd + residues - 1)
+    uint64_t ra = 0, rb = 0;
+    for (uint64_t c = 0; c < cols; ++c) {
+        ra += row_a[c] != '-';
+        rb += row_b[c] != '-';
+    }
+    const int64_t right_a = left_a + (int64_t)ra - 1, right_b = left_b + (int64_t)rb - 1;
+    uint64_t ci = 0, cj = 0, pi = 0, pj = 0, ng = 0;
+    bool in_gap = false;
+    for (uint64_t col = 0; col <= cols; ++col) {
+        if (col == cols || (row_a[col] != '-' && row_b[col] != '-')) {
+            if (in_gap && ci - pi > min_gap && cj - pj > min_gap) {
+                // zero-length anchors at the previous aligned column and (unless
+                // a count equals the right end, as the reference compares) at
+                // this one; getInterveningCoordinates turns them into the gap
+                mh_anchor_gap g;
+                g.lend[0] = left_a + (int64_t)pi;
+                g.lend[1] = left_b + (int64_t)pj;
+                const bool right = (int64_t)cj != right_b && (int64_t)ci != right_a;
+                g.rend[0] = right ? left_a + (int64_t)ci : right_a + 1;
+                g.rend[1] = right ? left_b + (int64_t)cj : right_b + 1;
+                if (ng < cap) gaps[ng] = g;
+                ++ng;
+            }
+            pi = ci;
+            pj = cj;
+            in_gap = false;
+        } else {
+            in_gap = true;
+        }
+        if (col < cols) {
+            ci += row_a[col] != '-';
+            cj += row_b[col] != '-';
+        }
+    }
+    *n_gaps = ng;
     return MH_OK;
 }
 

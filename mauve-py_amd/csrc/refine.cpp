@@ -27,6 +27,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <thread>
 
 #pragma clang fp contract(off)
@@ -191,25 +192,36 @@ struct RefinePiece {
     bool gap;
 };
 
-static void lengths(const char *r1, const char *r2, uint64_t lo, uint64_t hi, uint64_t &l1, uint64_t &l2)
-{
-    l1 = l2 = 0;
-    for (uint64_t c = lo; c < hi; ++c) {
-        l1 += r1[c] != '-';
-        l2 += r2[c] != '-';
-    }
-}
+}  // namespace mh
 
-// IsDenseEnough, LM/ProgressiveAligner.cpp:1015-1038.
-int refine_density(const char *r1, const char *r2, uint64_t lo, uint64_t hi)
+namespace {
+
+// Counts of residues ('-' is the only gap in Mauve's GappedAlignment) in
+// each row before every column, so a piece's lengths are two subtractions.
+struct Prefix {
+    std::vector<uint32_t> c1, c2;
+    void build(const char *r1, const char *r2, uint64_t n)
+    {
+        c1.resize(n + 1);
+        c2.resize(n + 1);
+        c1[0] = c2[0] = 0;
+        for (uint64_t c = 0; c < n; ++c) {
+            c1[c + 1] = c1[c] + (r1[c] != '-');
+            c2[c + 1] = c2[c] + (r2[c] != '-');
+        }
+    }
+    uint64_t l1(uint64_t lo, uint64_t hi) const { return c1[hi] - c1[lo]; }
+    uint64_t l2(uint64_t lo, uint64_t hi) const { return c2[hi] - c2[lo]; }
+};
+
+// IsDenseEnough (LM/ProgressiveAligner.cpp:1015-1038) from a piece's residue
+// counts and alignment length.
+int density_of(uint64_t l1, uint64_t l2, uint64_t aln)
 {
-    uint64_t l1, l2;
-    lengths(r1, r2, lo, hi, l1, l2);
     double total = 0;
     if (l1) total += (double)l1;
     if (l2) total += (double)l2;
     const unsigned mult = (l1 > 0) + (l2 > 0);
-    const uint64_t aln = hi - lo;
     const double density = total / ((double)aln * (double)mult);
     const double threshold = ((MAX_DENSITY - MIN_DENSITY) / (MIN_WINDOW - MAX_WINDOW)) * ((double)aln - MAX_WINDOW) +
                              MIN_DENSITY;
@@ -218,14 +230,14 @@ int refine_density(const char *r1, const char *r2, uint64_t lo, uint64_t hi)
     return 0;
 }
 
-// removeLargeGapsPP (LM/ProgressiveAligner.cpp:1051-1117), then the halving of
-// pieces too long for their density (1142-1166).
-void refine_pieces(const char *r1, const char *r2, uint64_t n, std::vector<RefinePiece> &pieces)
+// The pieces of one interval: removeLargeGapsPP (LM/ProgressiveAligner.cpp:
+// 1051-1117), then the halving of pieces too long for their density
+// (1142-1166), with the densities from the prefix counts.
+void pieces_fast(const char *r1, const char *r2, uint64_t n, const Prefix &pf, std::vector<RefinePiece> &pieces)
 {
     pieces.clear();
-    uint64_t gap_cols = 0, base = 0;
+    uint64_t gap_cols = 0, base = 0, last = 0;
     bool have_last = false;
-    uint64_t last = 0;
     for (uint64_t col = 0; col < n; ++col) {
         if (r1[col] != '-' && r2[col] != '-') {
             if (gap_cols > MAX_GAP_LENGTH) {
@@ -255,52 +267,46 @@ void refine_pieces(const char *r1, const char *r2, uint64_t n, std::vector<Refin
     } else {
         pieces.push_back({base, n, false});
     }
-    for (size_t i = 0; i < pieces.size();) {
-        const RefinePiece p = pieces[i];
-        const int d = refine_density(r1, r2, p.lo, p.hi);
-        const uint64_t len = p.hi - p.lo;
-        if ((d == 0 && (double)len > MAX_WINDOW / 3) || (d == 1 && (double)len > MAX_WINDOW) ||
-            (d == 2 && (double)len > MAX_WINDOW * 3)) {
-            const uint64_t sp = len / 2;
-            pieces[i].hi = p.hi - sp;
-            pieces.insert(pieces.begin() + (ptrdiff_t)i + 1, RefinePiece{p.hi - sp, p.hi, p.gap});
-            continue;
+    // the halving loop, depth first as the list insertion makes it
+    std::vector<RefinePiece> out;
+    std::vector<RefinePiece> stack;
+    for (const RefinePiece &p0 : pieces) {
+        stack.push_back(p0);
+        while (!stack.empty()) {
+            const RefinePiece p = stack.back();
+            stack.pop_back();
+            const uint64_t len = p.hi - p.lo;
+            const int d = density_of(pf.l1(p.lo, p.hi), pf.l2(p.lo, p.hi), len);
+            if ((d == 0 && (double)len > MAX_WINDOW / 3) || (d == 1 && (double)len > MAX_WINDOW) ||
+                (d == 2 && (double)len > MAX_WINDOW * 3)) {
+                const uint64_t sp = len / 2;
+                stack.push_back(RefinePiece{p.hi - sp, p.hi, p.gap});
+                stack.push_back(RefinePiece{p.lo, p.hi - sp, p.gap});
+                continue;
+            }
+            out.push_back(p);
         }
-        ++i;
     }
+    pieces.swap(out);
 }
 
-}  // namespace mh
-
-namespace {
-
-// One interval's work in a pass: the output of each piece, as a list of
-// segments per row.  A segment is literal text or the A/B row of a DP job.
+// The output of one piece in one pass, as a list of segments per row: a
+// segment is literal text or the A/B row of one of the piece's DP jobs.
 struct Seg {
     int64_t job;               // -1: literal
     std::string text;
 };
-struct PieceOut {
-    int rows;                  // 2, or 1 for a piece re-aligned from one row
+struct Task {
+    size_t iv;
+    RefinePiece p;
+    bool whole;                // the interval is copied as it is
+    int rows = 2;              // 2, or 1 for a piece re-aligned from one row
     std::vector<Seg> row[2];
-};
-struct IntervalPlan {
-    std::vector<PieceOut> pieces;
+    std::vector<std::string> a, b;   // residues of the piece's DP jobs
+    std::string out[2];
     int err = MH_OK;
     std::string msg;
 };
-
-struct JobSet {
-    std::vector<std::string> a, b;   // residues of each job (original characters)
-};
-
-// Queue a DP of residue strings a (A) and b (B); both non-empty.
-int64_t add_job(JobSet &js, std::string a, std::string b)
-{
-    js.a.push_back(std::move(a));
-    js.b.push_back(std::move(b));
-    return (int64_t)js.a.size() - 1;
-}
 
 std::string residues(const char *r, uint64_t lo, uint64_t hi)
 {
@@ -311,94 +317,84 @@ std::string residues(const char *r, uint64_t lo, uint64_t hi)
     return s;
 }
 
-void literal(PieceOut &po, int k, std::string s) { po.row[k].push_back(Seg{-1, std::move(s)}); }
-void job_rows(PieceOut &po, int64_t j)
+void literal(Task &t, int k, std::string s) { t.row[k].push_back(Seg{-1, std::move(s)}); }
+void add_job(Task &t, std::string a, std::string b)
 {
-    po.row[0].push_back(Seg{j, std::string()});
-    po.row[1].push_back(Seg{j, std::string()});
+    const int64_t j = (int64_t)t.a.size();
+    t.a.push_back(std::move(a));
+    t.b.push_back(std::move(b));
+    t.row[0].push_back(Seg{j, std::string()});
+    t.row[1].push_back(Seg{j, std::string()});
 }
 
-// The plan of one interval for one pass; DP jobs go to js (the caller
-// serialises access).
-void plan_interval(const char *r1, const char *r2, uint64_t n, bool profile, IntervalPlan &ip, JobSet &js)
+// Plan one piece for one pass.
+void plan_piece(const char *r1, const char *r2, const Prefix &pf, bool profile, Task &t)
 {
-    // an interval that holds one genome only is not refined (the
-    // Multiplicity() > 1 test of doGappedAlignment, LM/ProgressiveAligner.cpp:1268)
-    uint64_t t1, t2;
-    lengths(r1, r2, 0, n, t1, t2);
-    if (t1 == 0 || t2 == 0) {
-        PieceOut po;
-        po.rows = 2;
-        literal(po, 0, std::string(r1, n));
-        literal(po, 1, std::string(r2, n));
-        ip.pieces.push_back(std::move(po));
+    const RefinePiece &p = t.p;
+    const uint64_t len = p.hi - p.lo;
+    if (t.whole) {
+        literal(t, 0, std::string(r1 + p.lo, len));
+        literal(t, 1, std::string(r2 + p.lo, len));
         return;
     }
-    std::vector<RefinePiece> pieces;
-    refine_pieces(r1, r2, n, pieces);
-    std::vector<unsigned> anchors;
-    for (const RefinePiece &p : pieces) {
-        PieceOut po;
-        po.rows = 2;
-        uint64_t l1, l2;
-        lengths(r1, r2, p.lo, p.hi, l1, l2);
-        const uint64_t len = p.hi - p.lo;
-        if (p.gap || (profile && !(l1 > 0 && l2 > 0))) {
-            literal(po, 0, std::string(r1 + p.lo, len));
-            literal(po, 1, std::string(r2 + p.lo, len));
-        } else if (profile) {
-            // AnchoredProfileProfile: ranges between anchor columns
-            if (len >= (1ull << 32)) {
-                ip.err = MH_ERR_UNSUPPORTED;
-                ip.msg = "piece longer than 2^32 columns";
+    const uint64_t l1 = pf.l1(p.lo, p.hi), l2 = pf.l2(p.lo, p.hi);
+    if (p.gap || (profile && !(l1 > 0 && l2 > 0))) {
+        literal(t, 0, std::string(r1 + p.lo, len));
+        literal(t, 1, std::string(r2 + p.lo, len));
+        return;
+    }
+    if (profile) {
+        // AnchoredProfileProfile: the ranges between anchor columns
+        if (len >= (1ull << 32)) {
+            t.err = MH_ERR_UNSUPPORTED;
+            t.msg = "piece longer than 2^32 columns";
+            return;
+        }
+        std::vector<unsigned> anchors;
+        refine_anchor_cols(r1 + p.lo, r2 + p.lo, (unsigned)len, anchors);
+        for (size_t k = 0; k <= anchors.size(); ++k) {
+            const uint64_t lo = p.lo + (k > 0 ? anchors[k - 1] : 0);
+            const uint64_t hi = p.lo + (k < anchors.size() ? anchors[k] : len);
+            if (hi == lo) continue;
+            std::string a = residues(r1, lo, hi), b = residues(r2, lo, hi);
+            if (a.empty() || b.empty()) {
+                t.err = MH_ERR_ZERO_LENGTH;
+                t.msg = "NWSmall: length=0 (an anchored range has residues of one genome only)";
                 return;
             }
-            refine_anchor_cols(r1 + p.lo, r2 + p.lo, (unsigned)len, anchors);
-            for (size_t k = 0; k <= anchors.size(); ++k) {
-                const uint64_t lo = p.lo + (k > 0 ? anchors[k - 1] : 0);
-                const uint64_t hi = p.lo + (k < anchors.size() ? anchors[k] : len);
-                if (hi == lo) continue;
-                std::string a = residues(r1, lo, hi), b = residues(r2, lo, hi);
-                if (a.empty() || b.empty()) {
-                    ip.err = MH_ERR_ZERO_LENGTH;
-                    ip.msg = "NWSmall: length=0 (an anchored range has residues of one genome only)";
-                    return;
-                }
-                job_rows(po, add_job(js, std::move(a), std::move(b)));
-            }
-        } else {
-            const int d = refine_density(r1, r2, p.lo, p.hi);
-            const char *rows[2] = {r1, r2};
-            int present[2], np = 0;
-            if (l1) present[np++] = 0;
-            if (l2) present[np++] = 1;
-            po.rows = np;
-            if (d == 0) {
-                for (int q = 0; q < np; ++q) literal(po, q, std::string(rows[present[q]] + p.lo, len));
-            } else {
-                const uint64_t w = d == 1 ? 500 : 200;
-                for (uint64_t f = p.lo; f < p.hi; f += w) {
-                    const uint64_t t = std::min(f + w, p.hi);
-                    std::string s[2];
-                    for (int q = 0; q < np; ++q) s[q] = residues(rows[present[q]], f, t);
-                    if (np == 2 && !s[0].empty() && !s[1].empty()) {
-                        job_rows(po, add_job(js, std::move(s[0]), std::move(s[1])));
-                    } else {
-                        int full = -1;
-                        for (int q = 0; q < np; ++q)
-                            if (!s[q].empty()) full = q;
-                        if (full < 0) {
-                            ip.err = MH_ERR_INVALID;
-                            ip.msg = "a refinement window holds no residues";
-                            return;
-                        }
-                        const size_t m = s[full].size();
-                        for (int q = 0; q < np; ++q) literal(po, q, q == full ? s[full] : std::string(m, '-'));
-                    }
-                }
-            }
+            add_job(t, std::move(a), std::move(b));
         }
-        ip.pieces.push_back(std::move(po));
+        return;
+    }
+    const int d = density_of(l1, l2, len);
+    const char *rows[2] = {r1, r2};
+    int present[2], np = 0;
+    if (l1) present[np++] = 0;
+    if (l2) present[np++] = 1;
+    t.rows = np;
+    if (d == 0) {
+        for (int q = 0; q < np; ++q) literal(t, q, std::string(rows[present[q]] + p.lo, len));
+        return;
+    }
+    const uint64_t w = d == 1 ? 500 : 200;
+    for (uint64_t f = p.lo; f < p.hi; f += w) {
+        const uint64_t e = std::min(f + w, p.hi);
+        std::string sq[2];
+        for (int q = 0; q < np; ++q) sq[q] = residues(rows[present[q]], f, e);
+        if (np == 2 && !sq[0].empty() && !sq[1].empty()) {
+            add_job(t, std::move(sq[0]), std::move(sq[1]));
+            continue;
+        }
+        int full = -1;
+        for (int q = 0; q < np; ++q)
+            if (!sq[q].empty()) full = q;
+        if (full < 0) {
+            t.err = MH_ERR_INVALID;
+            t.msg = "a refinement window holds no residues";
+            return;
+        }
+        const size_t m = sq[full].size();
+        for (int q = 0; q < np; ++q) literal(t, q, q == full ? sq[full] : std::string(m, '-'));
     }
 }
 
@@ -424,61 +420,107 @@ void job_text(const std::string &a, const std::string &b, const uint8_t *path, u
     }
 }
 
-// One pass over all intervals: plan, one batched DP, splice.
+template <class F> void parallel_for(size_t n, F f)
+{
+    if (n == 0) return;
+    const size_t nt = std::max<size_t>(1, std::min<size_t>(n, std::max(1u, std::thread::hardware_concurrency())));
+    if (nt == 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::atomic<size_t> next(0);
+    for (size_t t = 0; t < nt; ++t)
+        th.emplace_back([&]() {
+            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto &x : th) x.join();
+}
+
+// One pass over all intervals: plan (in parallel over pieces), one batched
+// DP, splice (in parallel over pieces), reassemble.
 int refine_pass(mh_context *ctx, std::vector<std::string> &r1, std::vector<std::string> &r2, bool profile)
 {
     const size_t n = r1.size();
-    std::vector<IntervalPlan> plans(n);
-    std::vector<JobSet> sets(n);
-    {
-        const unsigned nt = (unsigned)std::max<size_t>(1, std::min<size_t>(n, std::thread::hardware_concurrency()));
-        std::vector<std::thread> th;
-        for (unsigned t = 0; t < nt; ++t)
-            th.emplace_back([&, t]() {
-                for (size_t i = t; i < n; i += nt)
-                    plan_interval(r1[i].data(), r2[i].data(), r1[i].size(), profile, plans[i], sets[i]);
-            });
-        for (auto &x : th) x.join();
+    std::vector<Prefix> pf(n);
+    std::vector<std::vector<RefinePiece>> pcs(n);
+    std::vector<char> whole(n, 0);
+    parallel_for(n, [&](size_t i) {
+        const uint64_t len = r1[i].size();
+        pf[i].build(r1[i].data(), r2[i].data(), len);
+        // an interval that holds one genome only is not refined (the
+        // Multiplicity() > 1 test of doGappedAlignment, LM/ProgressiveAligner.cpp:1268)
+        if (pf[i].l1(0, len) == 0 || pf[i].l2(0, len) == 0) {
+            whole[i] = 1;
+            pcs[i] = {RefinePiece{0, len, false}};
+            return;
+        }
+        pieces_fast(r1[i].data(), r2[i].data(), len, pf[i], pcs[i]);
+    });
+    std::vector<Task> tasks;
+    std::vector<size_t> first(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        first[i] = tasks.size();
+        for (const RefinePiece &p : pcs[i]) {
+            Task t;
+            t.iv = i;
+            t.p = p;
+            t.whole = whole[i] != 0;
+            tasks.push_back(std::move(t));
+        }
     }
-    for (size_t i = 0; i < n; ++i)
-        if (plans[i].err != MH_OK) {
-            set_error(plans[i].msg + " (interval " + std::to_string(i) + ")");
-            return plans[i].err;
+    first[n] = tasks.size();
+    parallel_for(tasks.size(), [&](size_t k) {
+        Task &t = tasks[k];
+        plan_piece(r1[t.iv].data(), r2[t.iv].data(), pf[t.iv], profile, t);
+    });
+    for (const Task &t : tasks)
+        if (t.err != MH_OK) {
+            set_error(t.msg + " (interval " + std::to_string(t.iv) + ")");
+            return t.err;
         }
-    // every job of the pass in one batch
-    std::vector<uint64_t> base(n + 1, 0);
-    for (size_t i = 0; i < n; ++i) base[i + 1] = base[i] + sets[i].a.size();
-    const uint64_t njobs = base[n];
-    std::vector<mh_dp_job> jobs(njobs);
-    uint64_t abytes = 0, bbytes = 0;
-    for (size_t i = 0; i < n; ++i)
-        for (size_t j = 0; j < sets[i].a.size(); ++j) {
-            mh_dp_job &jb = jobs[base[i] + j];
-            MH_CHECK(sets[i].a[j].size() < (1ull << 31) && sets[i].b[j].size() < (1ull << 31), MH_ERR_UNSUPPORTED,
+    // every job of the pass in one batch, in task order
+    std::vector<uint64_t> jbase(tasks.size() + 1, 0), abase(tasks.size() + 1, 0), bbase(tasks.size() + 1, 0);
+    for (size_t k = 0; k < tasks.size(); ++k) {
+        jbase[k + 1] = jbase[k] + tasks[k].a.size();
+        uint64_t sa = 0, sb = 0;
+        for (size_t j = 0; j < tasks[k].a.size(); ++j) {
+            MH_CHECK(tasks[k].a[j].size() < (1ull << 31) && tasks[k].b[j].size() < (1ull << 31), MH_ERR_UNSUPPORTED,
                      "DP operand longer than 2^31");
-            jb.a_off = abytes;
-            jb.b_off = bbytes;
-            jb.a_len = (uint32_t)sets[i].a[j].size();
-            jb.b_len = (uint32_t)sets[i].b[j].size();
-            abytes += jb.a_len;
-            bbytes += jb.b_len;
+            sa += tasks[k].a[j].size();
+            sb += tasks[k].b[j].size();
         }
+        abase[k + 1] = abase[k] + sa;
+        bbase[k + 1] = bbase[k] + sb;
+    }
+    const uint64_t njobs = jbase[tasks.size()], abytes = abase[tasks.size()], bbytes = bbase[tasks.size()];
+    std::vector<mh_dp_job> jobs(njobs);
     std::vector<uint8_t> ac(abytes + 1), bc(bbytes + 1);
-    uint64_t pa = 0, pb = 0, total = 0;
-    for (size_t i = 0; i < n; ++i)
-        for (size_t j = 0; j < sets[i].a.size(); ++j) {
-            for (char c : sets[i].a[j]) {
+    std::atomic<int> bad(0);
+    parallel_for(tasks.size(), [&](size_t k) {
+        const Task &t = tasks[k];
+        uint64_t pa = abase[k], pb = bbase[k];
+        for (size_t j = 0; j < t.a.size(); ++j) {
+            mh_dp_job &jb = jobs[jbase[k] + j];
+            jb.a_off = pa;
+            jb.b_off = pb;
+            jb.a_len = (uint32_t)t.a[j].size();
+            jb.b_len = (uint32_t)t.b[j].size();
+            for (char c : t.a[j]) {
                 const unsigned l = letter(c);
-                MH_CHECK(l < 4, MH_ERR_UNSUPPORTED, "residue other than A,C,G,T in a DP operand");
-                ac[pa++] = (uint8_t)l;
+                if (l >= 4) bad = 1;
+                ac[pa++] = (uint8_t)(l & 3);
             }
-            for (char c : sets[i].b[j]) {
+            for (char c : t.b[j]) {
                 const unsigned l = letter(c);
-                MH_CHECK(l < 4, MH_ERR_UNSUPPORTED, "residue other than A,C,G,T in a DP operand");
-                bc[pb++] = (uint8_t)l;
+                if (l >= 4) bad = 1;
+                bc[pb++] = (uint8_t)(l & 3);
             }
-            total += sets[i].a[j].size() + sets[i].b[j].size();
         }
+    });
+    MH_CHECK(!bad, MH_ERR_UNSUPPORTED, "residue other than A,C,G,T in a DP operand");
+    uint64_t total = 0;
+    for (const mh_dp_job &jb : jobs) total += (uint64_t)jb.a_len + jb.b_len;
     std::vector<uint8_t> paths(total + 1);
     std::vector<mh_dp_result> res(njobs);
     if (njobs) {
@@ -486,42 +528,37 @@ int refine_pass(mh_context *ctx, std::vector<std::string> &r1, std::vector<std::
                                    res.data());
         if (rc != MH_OK) return rc;
     }
-    // splice and reassemble (LM/ProgressiveAligner.cpp:1222-1237)
-    std::string ra, rb;
-    for (size_t i = 0; i < n; ++i) {
-        const JobSet &js = sets[i];
-        std::vector<std::string> out[2];
-        uint64_t new_len = 0;
-        for (const PieceOut &po : plans[i].pieces) {
-            std::string rows[2];
-            for (int k = 0; k < po.rows; ++k)
-                for (const Seg &sg : po.row[k]) {
-                    if (sg.job < 0) {
-                        rows[k] += sg.text;
-                        continue;
-                    }
-                    const uint64_t g = base[i] + (uint64_t)sg.job;
-                    job_text(js.a[sg.job], js.b[sg.job], paths.data() + res[g].path_start, res[g].path_len, ra, rb);
-                    rows[k] += k == 0 ? ra : rb;
+    // splice each piece
+    parallel_for(tasks.size(), [&](size_t k) {
+        Task &t = tasks[k];
+        std::string ra, rb;
+        for (int q = 0; q < t.rows; ++q)
+            for (const Seg &sg : t.row[q]) {
+                if (sg.job < 0) {
+                    t.out[q] += sg.text;
+                    continue;
                 }
-            new_len += rows[0].size();
-            out[0].push_back(std::move(rows[0]));
-            out[1].push_back(po.rows == 2 ? std::move(rows[1]) : std::string());
-        }
-        std::string res_rows[2] = {std::string(new_len, '-'), std::string(new_len, '-')};
-        uint64_t pos[2] = {0, 0};
-        for (size_t q = 0; q < plans[i].pieces.size(); ++q) {
-            const int nr = plans[i].pieces[q].rows;
-            for (int k = 0; k < nr; ++k) {
-                res_rows[k].replace(pos[k], out[k][q].size(), out[k][q]);
-                pos[k] += out[k][q].size();
+                const uint64_t g = jbase[k] + (uint64_t)sg.job;
+                job_text(t.a[sg.job], t.b[sg.job], paths.data() + res[g].path_start, res[g].path_len, ra, rb);
+                t.out[q] += q == 0 ? ra : rb;
             }
-        }
-        res_rows[0].resize(new_len, '-');
-        res_rows[1].resize(new_len, '-');
-        r1[i].swap(res_rows[0]);
-        r2[i].swap(res_rows[1]);
-    }
+    });
+    // reassemble each interval (LM/ProgressiveAligner.cpp:1222-1237)
+    parallel_for(n, [&](size_t i) {
+        uint64_t new_len = 0;
+        for (size_t k = first[i]; k < first[i + 1]; ++k) new_len += tasks[k].out[0].size();
+        std::string rows[2] = {std::string(new_len, '-'), std::string(new_len, '-')};
+        uint64_t pos[2] = {0, 0};
+        for (size_t k = first[i]; k < first[i + 1]; ++k)
+            for (int q = 0; q < tasks[k].rows; ++q) {
+                rows[q].replace(pos[q], tasks[k].out[q].size(), tasks[k].out[q]);
+                pos[q] += tasks[k].out[q].size();
+            }
+        rows[0].resize(new_len, '-');
+        rows[1].resize(new_len, '-');
+        r1[i].swap(rows[0]);
+        r2[i].swap(rows[1]);
+    });
     return MH_OK;
 }
 
@@ -573,7 +610,9 @@ int mh_refine_pieces(const char *row1, const char *row2, uint64_t cols, uint64_t
 {
     MH_CHECK(row1 && row2 && n_pieces && (pieces || cap == 0), MH_ERR_INVALID, "null argument");
     std::vector<RefinePiece> p;
-    refine_pieces(row1, row2, cols, p);
+    Prefix pf;
+    pf.build(row1, row2, cols);
+    pieces_fast(row1, row2, cols, pf, p);
     *n_pieces = p.size();
     for (size_t k = 0; k < p.size() && k < cap; ++k) {
         pieces[3 * k] = p[k].lo;

@@ -45,7 +45,8 @@ EXPORTS = [
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free", "mh_match_list_seq_count",
     "mh_match_list_seq_file", "mh_match_list_seq_length", "mh_match_list_write", "mh_match_list_read",
     "mh_index_from_xmfa", "mh_index_fix_zero", "mh_index_fill_gaps", "mh_index_smooth_edges", "mh_xmfa_write",
-    "mh_refine_pairwise", "mh_refine_anchor_cols", "mh_refine_pieces", "mh_anchor_gaps",
+    "mh_refine_pairwise", "mh_refine_anchor_cols", "mh_refine_pieces", "mh_anchor_gaps", "mh_chain_rows",
+    "mh_recursion_gaps",
     "mh_dp_align_banded_device", "mh_dp_band_region", "mh_dp_band_cells",
     "mh_gap_features_device", "mh_mlp_create", "mh_mlp_free", "mh_mlp_set_params", "mh_mlp_get_params",
     "mh_mlp_forward", "mh_mlp_grad", "mh_mlp_train", "mh_band_predict",
@@ -182,6 +183,8 @@ def load_library(path=LIB_PATH):
         "mh_refine_anchor_cols": (C.c_int, [C.c_char_p, C.c_char_p, u64, vp, u64, C.POINTER(u64)]),
         "mh_refine_pieces": (C.c_int, [C.c_char_p, C.c_char_p, u64, vp, u64, C.POINTER(u64)]),
         "mh_anchor_gaps": (C.c_int, [vp, vp, u64, vp, u64, vp, u64, C.POINTER(vp), vp]),
+        "mh_chain_rows": (C.c_int, [vp, vp, u64, C.c_char_p, u64, C.c_char_p, u64, vp, vp, u64, C.POINTER(u64)]),
+        "mh_recursion_gaps": (C.c_int, [C.c_char_p, C.c_char_p, u64, i64, i64, u64, vp, u64, C.POINTER(u64)]),
         "mh_chain_gap_jobs": (C.c_int, [vp, u64, u64, u64, C.c_uint32, vp, C.POINTER(u64), vp, u64,
                                         C.POINTER(u64)]),
     }
@@ -636,6 +639,37 @@ def anchor_gaps_device(ctx, d_a, len_a, d_b, len_b, gaps):
     finally:
         ctx.L.mh_match_list_free(h)
     return out["start"][:, :2].copy(), out["length"].copy(), first
+
+
+def chain_rows(matches, chain, seq_a, seq_b):
+    """mh_chain_rows: (row_a, row_b) as bytes for a chain of MATCH_DTYPE
+    matches (chain = indices, as chain_gap_jobs_raw returns them)."""
+    L = load_library()
+    m = np.ascontiguousarray(matches, dtype=MATCH_DTYPE)
+    ch = np.ascontiguousarray(chain, dtype=np.uint32)
+    a = seq_a if isinstance(seq_a, bytes) else np.ascontiguousarray(seq_a, dtype=np.uint8).tobytes()
+    b = seq_b if isinstance(seq_b, bytes) else np.ascontiguousarray(seq_b, dtype=np.uint8).tobytes()
+    cols = C.c_uint64()
+    cap = len(a) + len(b)
+    ra = np.empty(cap + 1, dtype=np.uint8)
+    rb = np.empty(cap + 1, dtype=np.uint8)
+    _check(L.mh_chain_rows(m.ctypes.data, ch.ctypes.data, len(ch), a, len(a), b, len(b), ra.ctypes.data,
+                           rb.ctypes.data, cap, C.byref(cols)))
+    n = cols.value
+    return ra[:n].tobytes(), rb[:n].tobytes()
+
+
+def recursion_gaps(row_a, row_b, left_a=1, left_b=1, min_gap=200):
+    """mh_recursion_gaps: ANCHOR_GAP_DTYPE records of the gaps searched recursively."""
+    L = load_library()
+    a, b = _row_bytes(row_a), _row_bytes(row_b)
+    if len(a) != len(b):
+        raise ValueError("rows differ in length")
+    n = C.c_uint64()
+    _check(L.mh_recursion_gaps(a, b, len(a), left_a, left_b, min_gap, None, 0, C.byref(n)))
+    out = np.zeros(max(1, n.value), dtype=ANCHOR_GAP_DTYPE)
+    _check(L.mh_recursion_gaps(a, b, len(a), left_a, left_b, min_gap, out.ctypes.data, len(out), C.byref(n)))
+    return out[:n.value]
 
 
 def anchor_gaps(ctx, seq_a, seq_b, gaps):

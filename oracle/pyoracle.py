@@ -229,6 +229,36 @@ def anchor_gap(seq_a, seq_b, lend, rend):
     return st, ln
 
 
+def recursion_gaps(row_a, row_b, left_a=1, left_b=1, min_gap=200):
+    """The gaps recurseOnPairs searches (LM/ProgressiveAligner.cpp:838-905)
+    for an interval forward in both genomes: [((lend_a, lend_b), (rend_a,
+    rend_b))] as getInterveningCoordinates gives them for the zero-length
+    anchors placed at the aligned columns around each gap."""
+    right_a = left_a + sum(c != "-" for c in row_a) - 1
+    right_b = left_b + sum(c != "-" for c in row_b) - 1
+    out = []
+    ci = cj = pi = pj = 0
+    in_gap = False
+    n = len(row_a)
+    for col in range(n + 1):
+        if col == n or (row_a[col] != "-" and row_b[col] != "-"):
+            if in_gap and ci - pi > min_gap and cj - pj > min_gap:
+                lend = (left_a + pi, left_b + pj)        # End()+1 of a zero-length match = its left end
+                if cj != right_b and ci != right_a:
+                    rend = (left_a + ci, left_b + cj)
+                else:
+                    rend = (right_a + 1, right_b + 1)
+                out.append((lend, rend))
+            pi, pj = ci, cj
+            in_gap = False
+        else:
+            in_gap = True
+        if col < n:
+            ci += row_a[col] != "-"
+            cj += row_b[col] != "-"
+    return out
+
+
 def seed_tuples(seqs, seed):
     k = len(seqs)
     bs, arr, lens = _seq_args(seqs)
