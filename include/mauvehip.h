@@ -385,10 +385,15 @@ int mh_recursion_gaps(const char *row_a, const char *row_b, uint64_t cols, int64
  * anchored range with residues of one genome only is MH_ERR_ZERO_LENGTH (the
  * reference's NWSmall quits).  out1[i]/out2[i] receive out_cols[i] columns;
  * caps[i] >= 2 * cols[i] always suffices (MH_ERR_INVALID with out_cols set
- * when a capacity is short). */
+ * when a capacity is short).  '-' is the gap character.  stats may be NULL. */
+typedef struct {
+    uint64_t profile_cells;   /* DP cells of the profile pass (anchored ranges) */
+    uint64_t window_cells;    /* DP cells of the refinement pass (windows)      */
+} mh_refine_stats;
+
 int mh_refine_pairwise(mh_context *ctx, uint64_t n, const char *const *rows1, const char *const *rows2,
                        const uint64_t *cols, uint32_t passes, char *const *out1, char *const *out2,
-                       const uint64_t *caps, uint64_t *out_cols);
+                       const uint64_t *caps, uint64_t *out_cols, mh_refine_stats *stats);
 /* Host-only parts of the above, for checking: FindAnchorColsPP
  * (MU/anchoredpp.cpp:353-404) over two rows, and the pieces of refineAlignment
  * (lo, hi, unaligned flag; three values per piece). */

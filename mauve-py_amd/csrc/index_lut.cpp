@@ -19,6 +19,7 @@
 #include "mh_internal.hpp"
 
 #include <cstring>
+#include <thread>
 
 using namespace mh;
 
@@ -78,10 +79,83 @@ const char *parse_entry(const char *p, const char *end, Entry &e)
 // The sequence of an entry without whitespace, into out.
 void squeeze(const Entry &e, std::string &out)
 {
-    out.clear();
-    out.reserve((size_t)(e.seq_end - e.seq));
-    for (const char *c = e.seq; c < e.seq_end; ++c)
-        if (!is_ws(*c)) out.push_back(*c);
+    out.resize((size_t)(e.seq_end - e.seq));
+    char *w = &out[0];
+    for (const char *c = e.seq; c < e.seq_end; ++c) {
+        *w = *c;
+        w += !is_ws(*c);
+    }
+    out.resize((size_t)(w - out.data()));
+}
+
+template <class F> void parallel_chunks(uint64_t nc, F f)
+{
+    const unsigned nt = std::max(1u, std::min<unsigned>((unsigned)nc, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t]() {
+            for (uint64_t k = t; k < nc; k += nt) f(k);
+        });
+    for (auto &x : th) x.join();
+}
+
+// The column walk of one group (PY/buildindex.py:119-129): each index
+// advances on a non-gap character, and equal characters map sequence 1's
+// index to sequence 2's.  Long rows are walked in parallel chunks from the
+// indices each chunk starts at; a chunk's writes before its first residue of
+// sequence 1 (all-gap columns) reuse the previous chunk's last index and are
+// applied afterwards in column order, so later columns still win.
+void walk(const std::string &gs, const std::string &rs, int64_t g0, int64_t r0, uint64_t genome_length, int32_t *lut)
+{
+    const uint64_t n = gs.size(), chunk = 1 << 18, nc = (n + chunk - 1) / chunk;
+    auto put = [&](int64_t gi, int64_t ri) {
+        // index -1 is the last entry (numpy's negative indexing)
+        const int64_t at = gi >= 0 ? gi : (int64_t)genome_length + gi;
+        if (at >= 0) lut[at] = (int32_t)ri;
+    };
+    if (nc <= 1 || g0 < 0) {
+        int64_t gi = g0, ri = r0;
+        for (uint64_t c = 0; c < n; ++c) {
+            const char a = gs[c], b = rs[c];
+            gi += a != '-';
+            ri += b != '-';
+            if (a == b && gi < (int64_t)genome_length) put(gi, ri);
+        }
+        return;
+    }
+    std::vector<int64_t> cg(nc + 1, 0), cr(nc + 1, 0);
+    parallel_chunks(nc, [&](uint64_t k) {
+        const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
+        int64_t a = 0, b = 0;
+        for (uint64_t c = lo; c < hi; ++c) {
+            a += gs[c] != '-';
+            b += rs[c] != '-';
+        }
+        cg[k + 1] = a;
+        cr[k + 1] = b;
+    });
+    cg[0] = g0;
+    cr[0] = r0;
+    for (uint64_t k = 0; k < nc; ++k) {
+        cg[k + 1] += cg[k];
+        cr[k + 1] += cr[k];
+    }
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> early(nc);
+    parallel_chunks(nc, [&](uint64_t k) {
+        const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
+        int64_t gi = cg[k], ri = cr[k];
+        for (uint64_t c = lo; c < hi; ++c) {
+            const char a = gs[c], b = rs[c];
+            gi += a != '-';
+            ri += b != '-';
+            if (a == b && gi < (int64_t)genome_length) {
+                if (gi == cg[k]) early[k].push_back({gi, ri});
+                else put(gi, ri);
+            }
+        }
+    });
+    for (uint64_t k = 0; k < nc; ++k)
+        for (const auto &e : early[k]) put(e.first, e.second);
 }
 
 // Starting at a gap start idx whose left neighbour lower = idx-1 is mapped:
@@ -155,17 +229,7 @@ int mh_index_from_xmfa(const char *xmfa, uint64_t len, uint64_t genome_length, i
         squeeze(g, gs);
         squeeze(r, rs);
         MH_CHECK(rs.size() >= gs.size(), MH_ERR_INVALID, "string index out of range (genome 2 row shorter than genome 1 row)");
-        int64_t gi = g.start - 1, ri = r.start - 1;
-        for (size_t c = 0; c < gs.size(); ++c) {
-            const char a = gs[c], b = rs[c];
-            if (a != '-') ++gi;
-            if (b != '-') ++ri;
-            if (a == b && gi < (int64_t)genome_length) {
-                // index -1 is the last entry (numpy's negative indexing)
-                const int64_t at = gi >= 0 ? gi : (int64_t)genome_length + gi;
-                if (at >= 0) lut[at] = (int32_t)ri;
-            }
-        }
+        walk(gs, rs, g.start - 1, r.start - 1, genome_length, lut);
     }
     if (n_groups) *n_groups = groups;
     return MH_OK;

@@ -196,19 +196,60 @@ struct RefinePiece {
 
 namespace {
 
+template <class F> void parallel_for(size_t n, F f)
+{
+    if (n == 0) return;
+    const size_t nt = std::max<size_t>(1, std::min<size_t>(n, std::max(1u, std::thread::hardware_concurrency())));
+    if (nt == 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::atomic<size_t> next(0);
+    for (size_t t = 0; t < nt; ++t)
+        th.emplace_back([&]() {
+            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto &x : th) x.join();
+}
+
 // Counts of residues ('-' is the only gap in Mauve's GappedAlignment) in
 // each row before every column, so a piece's lengths are two subtractions.
+// Built in parallel chunks.
 struct Prefix {
     std::vector<uint32_t> c1, c2;
     void build(const char *r1, const char *r2, uint64_t n)
     {
         c1.resize(n + 1);
         c2.resize(n + 1);
-        c1[0] = c2[0] = 0;
-        for (uint64_t c = 0; c < n; ++c) {
-            c1[c + 1] = c1[c] + (r1[c] != '-');
-            c2[c + 1] = c2[c] + (r2[c] != '-');
+        const uint64_t chunk = 1 << 18, nc = (n + chunk - 1) / chunk;
+        std::vector<uint32_t> t1(nc + 1, 0), t2(nc + 1, 0);
+        parallel_for(nc, [&](size_t k) {
+            const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
+            uint32_t a = 0, b = 0;
+            for (uint64_t c = lo; c < hi; ++c) {
+                a += r1[c] != '-';
+                b += r2[c] != '-';
+            }
+            t1[k + 1] = a;
+            t2[k + 1] = b;
+        });
+        for (uint64_t k = 0; k < nc; ++k) {
+            t1[k + 1] += t1[k];
+            t2[k + 1] += t2[k];
         }
+        parallel_for(nc, [&](size_t k) {
+            const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
+            uint32_t a = t1[k], b = t2[k];
+            for (uint64_t c = lo; c < hi; ++c) {
+                c1[c] = a;
+                c2[c] = b;
+                a += r1[c] != '-';
+                b += r2[c] != '-';
+            }
+        });
+        c1[n] = t1[nc];
+        c2[n] = t2[nc];
     }
     uint64_t l1(uint64_t lo, uint64_t hi) const { return c1[hi] - c1[lo]; }
     uint64_t l2(uint64_t lo, uint64_t hi) const { return c2[hi] - c2[lo]; }
@@ -236,30 +277,52 @@ int density_of(uint64_t l1, uint64_t l2, uint64_t aln)
 void pieces_fast(const char *r1, const char *r2, uint64_t n, const Prefix &pf, std::vector<RefinePiece> &pieces)
 {
     pieces.clear();
-    uint64_t gap_cols = 0, base = 0, last = 0;
-    bool have_last = false;
-    for (uint64_t col = 0; col < n; ++col) {
-        if (r1[col] != '-' && r2[col] != '-') {
-            if (gap_cols > MAX_GAP_LENGTH) {
-                if (have_last) {
-                    const uint64_t sp = last + LCB_HANGOVER - base;
-                    pieces.push_back({base, base + sp, false});
-                    base += sp;
-                }
-                const uint64_t sp = col - LCB_HANGOVER - base;
-                pieces.push_back({base, base + sp, true});
-                base += sp;
+    // The cuts depend only on the runs of more than MAX_GAP_LENGTH columns not
+    // aligned in both rows and the aligned columns around them: find those runs
+    // in parallel chunks (runs inside a chunk, plus each chunk's first and last
+    // aligned column for the runs across chunks), then replay the scan's
+    // decisions over the runs in order.
+    struct Run {
+        int64_t prev, next;    // aligned columns around the run (-1: none before)
+    };
+    const uint64_t chunk = 1 << 18, nc = (n + chunk - 1) / chunk;
+    std::vector<std::vector<Run>> inner(nc);
+    std::vector<int64_t> fst(nc, -1), lst(nc, -1);
+    parallel_for(nc, [&](size_t k) {
+        const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
+        int64_t prev = -1;
+        for (uint64_t c = lo; c < hi; ++c) {
+            if (r1[c] != '-' && r2[c] != '-') {
+                if (prev < 0) fst[k] = (int64_t)c;
+                else if ((int64_t)c - prev - 1 > (int64_t)MAX_GAP_LENGTH) inner[k].push_back(Run{prev, (int64_t)c});
+                prev = (int64_t)c;
             }
-            last = col;
-            have_last = true;
-            gap_cols = 0;
-        } else {
-            ++gap_cols;
         }
+        lst[k] = prev;
+    });
+    std::vector<Run> runs;
+    int64_t last = -1;
+    for (uint64_t k = 0; k < nc; ++k) {
+        if (fst[k] < 0) continue;
+        if (fst[k] - last - 1 > (int64_t)MAX_GAP_LENGTH) runs.push_back(Run{last, fst[k]});
+        runs.insert(runs.end(), inner[k].begin(), inner[k].end());
+        last = lst[k];
     }
-    if (gap_cols > MAX_GAP_LENGTH) {
-        if (have_last) {
-            const uint64_t sp = last + LCB_HANGOVER - base;
+    uint64_t base = 0;
+    for (const Run &r : runs) {
+        if (r.prev >= 0) {
+            const uint64_t sp = (uint64_t)r.prev + LCB_HANGOVER - base;
+            pieces.push_back({base, base + sp, false});
+            base += sp;
+        }
+        const uint64_t sp = (uint64_t)r.next - LCB_HANGOVER - base;
+        pieces.push_back({base, base + sp, true});
+        base += sp;
+    }
+    const uint64_t tail = last < 0 ? n : n - 1 - (uint64_t)last;
+    if (tail > MAX_GAP_LENGTH) {
+        if (last >= 0) {
+            const uint64_t sp = (uint64_t)last + LCB_HANGOVER - base;
             pieces.push_back({base, base + sp, false});
             base += sp;
         }
@@ -290,11 +353,21 @@ void pieces_fast(const char *r1, const char *r2, uint64_t n, const Prefix &pf, s
     pieces.swap(out);
 }
 
-// The output of one piece in one pass, as a list of segments per row: a
-// segment is literal text or the A/B row of one of the piece's DP jobs.
+// The output of one piece in one pass, as a list of segments per output row.
+enum SegKind : uint8_t {
+    S_COPY,   // the columns [lo, hi) of input row src
+    S_RES,    // the residues of input row src in [lo, hi)
+    S_GAPS,   // lo gap characters
+    S_JOB     // row 0: the A row of DP job lo; row 1: its B row
+};
 struct Seg {
-    int64_t job;               // -1: literal
-    std::string text;
+    SegKind kind;
+    uint8_t src;
+    uint64_t lo, hi;
+};
+struct Job {
+    uint64_t lo, hi;           // columns: A = residues of row 0, B = residues of row 1
+    uint32_t la, lb;
 };
 struct Task {
     size_t iv;
@@ -302,152 +375,123 @@ struct Task {
     bool whole;                // the interval is copied as it is
     int rows = 2;              // 2, or 1 for a piece re-aligned from one row
     std::vector<Seg> row[2];
-    std::vector<std::string> a, b;   // residues of the piece's DP jobs
-    std::string out[2];
+    std::vector<Job> jobs;
+    uint64_t out_len[2] = {0, 0};
     int err = MH_OK;
     std::string msg;
 };
-
-std::string residues(const char *r, uint64_t lo, uint64_t hi)
-{
-    std::string s;
-    s.reserve(hi - lo);
-    for (uint64_t c = lo; c < hi; ++c)
-        if (!mu_gap(r[c])) s.push_back(r[c]);
-    return s;
-}
-
-void literal(Task &t, int k, std::string s) { t.row[k].push_back(Seg{-1, std::move(s)}); }
-void add_job(Task &t, std::string a, std::string b)
-{
-    const int64_t j = (int64_t)t.a.size();
-    t.a.push_back(std::move(a));
-    t.b.push_back(std::move(b));
-    t.row[0].push_back(Seg{j, std::string()});
-    t.row[1].push_back(Seg{j, std::string()});
-}
 
 // Plan one piece for one pass.
 void plan_piece(const char *r1, const char *r2, const Prefix &pf, bool profile, Task &t)
 {
     const RefinePiece &p = t.p;
     const uint64_t len = p.hi - p.lo;
+    auto copy_both = [&]() {
+        t.row[0].push_back(Seg{S_COPY, 0, p.lo, p.hi});
+        t.row[1].push_back(Seg{S_COPY, 1, p.lo, p.hi});
+    };
+    auto add_job = [&](uint64_t lo, uint64_t hi, uint64_t la, uint64_t lb) {
+        t.row[0].push_back(Seg{S_JOB, 0, t.jobs.size(), 0});
+        t.row[1].push_back(Seg{S_JOB, 1, t.jobs.size(), 0});
+        t.jobs.push_back(Job{lo, hi, (uint32_t)la, (uint32_t)lb});
+    };
     if (t.whole) {
-        literal(t, 0, std::string(r1 + p.lo, len));
-        literal(t, 1, std::string(r2 + p.lo, len));
+        copy_both();
         return;
     }
     const uint64_t l1 = pf.l1(p.lo, p.hi), l2 = pf.l2(p.lo, p.hi);
     if (p.gap || (profile && !(l1 > 0 && l2 > 0))) {
-        literal(t, 0, std::string(r1 + p.lo, len));
-        literal(t, 1, std::string(r2 + p.lo, len));
+        copy_both();
+        return;
+    }
+    if (len >= (1ull << 31)) {
+        t.err = MH_ERR_UNSUPPORTED;
+        t.msg = "piece longer than 2^31 columns";
         return;
     }
     if (profile) {
         // AnchoredProfileProfile: the ranges between anchor columns
-        if (len >= (1ull << 32)) {
-            t.err = MH_ERR_UNSUPPORTED;
-            t.msg = "piece longer than 2^32 columns";
-            return;
-        }
         std::vector<unsigned> anchors;
         refine_anchor_cols(r1 + p.lo, r2 + p.lo, (unsigned)len, anchors);
         for (size_t k = 0; k <= anchors.size(); ++k) {
             const uint64_t lo = p.lo + (k > 0 ? anchors[k - 1] : 0);
             const uint64_t hi = p.lo + (k < anchors.size() ? anchors[k] : len);
             if (hi == lo) continue;
-            std::string a = residues(r1, lo, hi), b = residues(r2, lo, hi);
-            if (a.empty() || b.empty()) {
+            const uint64_t la = pf.l1(lo, hi), lb = pf.l2(lo, hi);
+            if (la == 0 || lb == 0) {
                 t.err = MH_ERR_ZERO_LENGTH;
                 t.msg = "NWSmall: length=0 (an anchored range has residues of one genome only)";
                 return;
             }
-            add_job(t, std::move(a), std::move(b));
+            add_job(lo, hi, la, lb);
         }
         return;
     }
     const int d = density_of(l1, l2, len);
-    const char *rows[2] = {r1, r2};
     int present[2], np = 0;
     if (l1) present[np++] = 0;
     if (l2) present[np++] = 1;
     t.rows = np;
     if (d == 0) {
-        for (int q = 0; q < np; ++q) literal(t, q, std::string(rows[present[q]] + p.lo, len));
+        for (int q = 0; q < np; ++q) t.row[q].push_back(Seg{S_COPY, (uint8_t)present[q], p.lo, p.hi});
         return;
     }
     const uint64_t w = d == 1 ? 500 : 200;
     for (uint64_t f = p.lo; f < p.hi; f += w) {
         const uint64_t e = std::min(f + w, p.hi);
-        std::string sq[2];
-        for (int q = 0; q < np; ++q) sq[q] = residues(rows[present[q]], f, e);
-        if (np == 2 && !sq[0].empty() && !sq[1].empty()) {
-            add_job(t, std::move(sq[0]), std::move(sq[1]));
+        const uint64_t cnt[2] = {pf.l1(f, e), pf.l2(f, e)};
+        if (np == 2 && cnt[0] && cnt[1]) {
+            add_job(f, e, cnt[0], cnt[1]);
             continue;
         }
         int full = -1;
         for (int q = 0; q < np; ++q)
-            if (!sq[q].empty()) full = q;
+            if (cnt[present[q]]) full = q;
         if (full < 0) {
             t.err = MH_ERR_INVALID;
             t.msg = "a refinement window holds no residues";
             return;
         }
-        const size_t m = sq[full].size();
-        for (int q = 0; q < np; ++q) literal(t, q, q == full ? sq[full] : std::string(m, '-'));
-    }
-}
-
-// Aligned A and B rows of a job from its path.
-void job_text(const std::string &a, const std::string &b, const uint8_t *path, uint32_t plen, std::string &ra,
-              std::string &rb)
-{
-    ra.clear();
-    rb.clear();
-    size_t i = 0, j = 0;
-    for (uint32_t e = 0; e < plen; ++e) {
-        const uint8_t c = path[e];
-        if (c == 'M') {
-            ra.push_back(a[i++]);
-            rb.push_back(b[j++]);
-        } else if (c == 'D') {
-            ra.push_back(a[i++]);
-            rb.push_back('-');
-        } else {
-            ra.push_back('-');
-            rb.push_back(b[j++]);
+        for (int q = 0; q < np; ++q) {
+            if (q == full) t.row[q].push_back(Seg{S_RES, (uint8_t)present[q], f, e});
+            else t.row[q].push_back(Seg{S_GAPS, 0, cnt[present[full]], 0});
         }
     }
 }
 
-template <class F> void parallel_for(size_t n, F f)
+// Residue codes of row r in [lo, hi) into out; false on a letter outside ACGT.
+bool put_codes(const char *r, uint64_t lo, uint64_t hi, uint8_t *out)
 {
-    if (n == 0) return;
-    const size_t nt = std::max<size_t>(1, std::min<size_t>(n, std::max(1u, std::thread::hardware_concurrency())));
-    if (nt == 1) {
-        for (size_t i = 0; i < n; ++i) f(i);
-        return;
+    bool ok = true;
+    for (uint64_t c = lo; c < hi; ++c) {
+        if (r[c] == '-') continue;
+        const unsigned l = letter(r[c]);
+        ok &= l < 4;
+        *out++ = (uint8_t)(l & 3);
     }
-    std::vector<std::thread> th;
-    std::atomic<size_t> next(0);
-    for (size_t t = 0; t < nt; ++t)
-        th.emplace_back([&]() {
-            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-        });
-    for (auto &x : th) x.join();
+    return ok;
 }
 
-// One pass over all intervals: plan (in parallel over pieces), one batched
-// DP, splice (in parallel over pieces), reassemble.
-int refine_pass(mh_context *ctx, std::vector<std::string> &r1, std::vector<std::string> &r2, bool profile)
+// One pass over all intervals: plan (in parallel over pieces), one batched DP
+// through pinned staging, then every piece written straight into the new rows
+// at its place (the reassembly of LM/ProgressiveAligner.cpp:1222-1237).
+int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std::vector<const char *> &in2,
+                const std::vector<uint64_t> &lens, bool profile, std::vector<std::string> &o1,
+                std::vector<std::string> &o2, uint64_t *cells_out)
 {
-    const size_t n = r1.size();
+    const size_t n = lens.size();
+    double tt = now_ms();
+    auto lap = [&](const char *name) {
+        const double t2 = now_ms();
+        host_time(ctx, name, t2 - tt);
+        tt = t2;
+    };
     std::vector<Prefix> pf(n);
     std::vector<std::vector<RefinePiece>> pcs(n);
     std::vector<char> whole(n, 0);
+    for (size_t i = 0; i < n; ++i) pf[i].build(in1[i], in2[i], lens[i]);
     parallel_for(n, [&](size_t i) {
-        const uint64_t len = r1[i].size();
-        pf[i].build(r1[i].data(), r2[i].data(), len);
+        const uint64_t len = lens[i];
         // an interval that holds one genome only is not refined (the
         // Multiplicity() > 1 test of doGappedAlignment, LM/ProgressiveAligner.cpp:1268)
         if (pf[i].l1(0, len) == 0 || pf[i].l2(0, len) == 0) {
@@ -455,7 +499,7 @@ int refine_pass(mh_context *ctx, std::vector<std::string> &r1, std::vector<std::
             pcs[i] = {RefinePiece{0, len, false}};
             return;
         }
-        pieces_fast(r1[i].data(), r2[i].data(), len, pf[i], pcs[i]);
+        pieces_fast(in1[i], in2[i], len, pf[i], pcs[i]);
     });
     std::vector<Task> tasks;
     std::vector<size_t> first(n + 1, 0);
@@ -470,9 +514,10 @@ int refine_pass(mh_context *ctx, std::vector<std::string> &r1, std::vector<std::
         }
     }
     first[n] = tasks.size();
+    lap("host_refine_pieces");
     parallel_for(tasks.size(), [&](size_t k) {
         Task &t = tasks[k];
-        plan_piece(r1[t.iv].data(), r2[t.iv].data(), pf[t.iv], profile, t);
+        plan_piece(in1[t.iv], in2[t.iv], pf[t.iv], profile, t);
     });
     for (const Task &t : tasks)
         if (t.err != MH_OK) {
@@ -480,85 +525,146 @@ int refine_pass(mh_context *ctx, std::vector<std::string> &r1, std::vector<std::
             return t.err;
         }
     // every job of the pass in one batch, in task order
-    std::vector<uint64_t> jbase(tasks.size() + 1, 0), abase(tasks.size() + 1, 0), bbase(tasks.size() + 1, 0);
-    for (size_t k = 0; k < tasks.size(); ++k) {
-        jbase[k + 1] = jbase[k] + tasks[k].a.size();
+    const size_t nt = tasks.size();
+    std::vector<uint64_t> jbase(nt + 1, 0), abase(nt + 1, 0), bbase(nt + 1, 0);
+    uint64_t cells = 0;
+    for (size_t k = 0; k < nt; ++k) {
         uint64_t sa = 0, sb = 0;
-        for (size_t j = 0; j < tasks[k].a.size(); ++j) {
-            MH_CHECK(tasks[k].a[j].size() < (1ull << 31) && tasks[k].b[j].size() < (1ull << 31), MH_ERR_UNSUPPORTED,
-                     "DP operand longer than 2^31");
-            sa += tasks[k].a[j].size();
-            sb += tasks[k].b[j].size();
+        for (const Job &j : tasks[k].jobs) {
+            sa += j.la;
+            sb += j.lb;
+            cells += (uint64_t)j.la * j.lb;
         }
+        jbase[k + 1] = jbase[k] + tasks[k].jobs.size();
         abase[k + 1] = abase[k] + sa;
         bbase[k + 1] = bbase[k] + sb;
     }
-    const uint64_t njobs = jbase[tasks.size()], abytes = abase[tasks.size()], bbytes = bbase[tasks.size()];
+    if (cells_out) *cells_out += cells;
+    const uint64_t njobs = jbase[nt], abytes = abase[nt], bbytes = bbase[nt], pbytes = abytes + bbytes;
     std::vector<mh_dp_job> jobs(njobs);
-    std::vector<uint8_t> ac(abytes + 1), bc(bbytes + 1);
+    // pinned staging: [A codes | B codes | paths | results]
+    const uint64_t res_off = (abytes + bbytes + pbytes + 15) & ~15ull;
+    uint8_t *stage = nullptr;
+    if (njobs) {
+        int rc = pinned(ctx, res_off + njobs * sizeof(mh_dp_result) + 64, (void **)&stage);
+        if (rc != MH_OK) return rc;
+    }
     std::atomic<int> bad(0);
-    parallel_for(tasks.size(), [&](size_t k) {
+    parallel_for(nt, [&](size_t k) {
         const Task &t = tasks[k];
         uint64_t pa = abase[k], pb = bbase[k];
-        for (size_t j = 0; j < t.a.size(); ++j) {
-            mh_dp_job &jb = jobs[jbase[k] + j];
-            jb.a_off = pa;
-            jb.b_off = pb;
-            jb.a_len = (uint32_t)t.a[j].size();
-            jb.b_len = (uint32_t)t.b[j].size();
-            for (char c : t.a[j]) {
-                const unsigned l = letter(c);
-                if (l >= 4) bad = 1;
-                ac[pa++] = (uint8_t)(l & 3);
+        bool ok = true;
+        for (size_t j = 0; j < t.jobs.size(); ++j) {
+            const Job &jb = t.jobs[j];
+            mh_dp_job &dj = jobs[jbase[k] + j];
+            dj.a_off = pa;
+            dj.b_off = pb;
+            dj.a_len = jb.la;
+            dj.b_len = jb.lb;
+            ok &= put_codes(in1[t.iv], jb.lo, jb.hi, stage + pa);
+            ok &= put_codes(in2[t.iv], jb.lo, jb.hi, stage + abytes + pb);
+            pa += jb.la;
+            pb += jb.lb;
+        }
+        if (!ok) bad = 1;
+    });
+    MH_CHECK(!bad, MH_ERR_UNSUPPORTED, "residue other than A,C,G,T in a DP operand");
+    lap("host_refine_plan");
+    const uint8_t *paths = stage + abytes + bbytes;
+    const mh_dp_result *res = reinterpret_cast<const mh_dp_result *>(stage + res_off);
+    if (njobs) {
+        int rc;
+        if ((rc = ctx->dp_a.ensure(abytes + 64)) != MH_OK) return rc;
+        if ((rc = ctx->dp_b.ensure(bbytes + 64)) != MH_OK) return rc;
+        if ((rc = ctx->dp_paths.ensure(pbytes + 64)) != MH_OK) return rc;
+        if ((rc = ctx->dp_res.ensure(njobs * sizeof(mh_dp_result) + 64)) != MH_OK) return rc;
+        MH_HIP(hipMemcpyAsync(ctx->dp_a.ptr, stage, abytes, hipMemcpyHostToDevice, ctx->stream));
+        MH_HIP(hipMemcpyAsync(ctx->dp_b.ptr, stage + abytes, bbytes, hipMemcpyHostToDevice, ctx->stream));
+        rc = dp_run(ctx, ctx->dp_a.as<uint8_t>(), ctx->dp_b.as<uint8_t>(), jobs.data(), njobs,
+                    ctx->dp_paths.as<uint8_t>(), ctx->dp_res.as<mh_dp_result>());
+        if (rc != MH_OK) return rc;
+        MH_HIP(hipMemcpyAsync(stage + abytes + bbytes, ctx->dp_paths.ptr, pbytes, hipMemcpyDeviceToHost, ctx->stream));
+        MH_HIP(hipMemcpyAsync(stage + res_off, ctx->dp_res.ptr, njobs * sizeof(mh_dp_result), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+        for (uint64_t j = 0; j < njobs; ++j)
+            MH_CHECK(res[j].path_len > 0, MH_ERR_DEVICE, "trace-back failed for job " + std::to_string(j));
+    }
+    lap("host_refine_dp");
+    // output lengths, then each piece's place in its interval's rows
+    parallel_for(nt, [&](size_t k) {
+        Task &t = tasks[k];
+        for (int q = 0; q < t.rows; ++q) {
+            uint64_t m = 0;
+            for (const Seg &sg : t.row[q]) {
+                switch (sg.kind) {
+                case S_COPY: m += sg.hi - sg.lo; break;
+                case S_RES: m += sg.src == 0 ? pf[t.iv].l1(sg.lo, sg.hi) : pf[t.iv].l2(sg.lo, sg.hi); break;
+                case S_GAPS: m += sg.lo; break;
+                case S_JOB: m += res[jbase[k] + sg.lo].path_len; break;
+                }
             }
-            for (char c : t.b[j]) {
-                const unsigned l = letter(c);
-                if (l >= 4) bad = 1;
-                bc[pb++] = (uint8_t)(l & 3);
+            t.out_len[q] = m;
+        }
+    });
+    std::vector<uint64_t> pos0(nt), pos1(nt);
+    o1.assign(n, std::string());
+    o2.assign(n, std::string());
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t p0 = 0, p1 = 0;
+        for (size_t k = first[i]; k < first[i + 1]; ++k) {
+            pos0[k] = p0;
+            pos1[k] = p1;
+            p0 += tasks[k].out_len[0];
+            if (tasks[k].rows == 2) p1 += tasks[k].out_len[1];
+        }
+        o1[i].assign(p0, '-');
+        o2[i].assign(p0, '-');
+    }
+    parallel_for(nt, [&](size_t k) {
+        const Task &t = tasks[k];
+        const char *in[2] = {in1[t.iv], in2[t.iv]};
+        // a one-row piece writes its row into genome 1's row (see the file comment)
+        char *dst[2] = {&o1[t.iv][pos0[k]], &o2[t.iv][pos1[k]]};
+        for (int q = 0; q < t.rows; ++q) {
+            char *w = dst[q];
+            for (const Seg &sg : t.row[q]) {
+                switch (sg.kind) {
+                case S_COPY:
+                    std::memcpy(w, in[sg.src] + sg.lo, sg.hi - sg.lo);
+                    w += sg.hi - sg.lo;
+                    break;
+                case S_RES:
+                    for (uint64_t c = sg.lo; c < sg.hi; ++c)
+                        if (in[sg.src][c] != '-') *w++ = in[sg.src][c];
+                    break;
+                case S_GAPS:
+                    std::memset(w, '-', sg.lo);
+                    w += sg.lo;
+                    break;
+                case S_JOB: {
+                    const Job &jb = t.jobs[sg.lo];
+                    const mh_dp_result &r = res[jbase[k] + sg.lo];
+                    const uint8_t *e = paths + r.path_start;
+                    const char *src = in[q];
+                    uint64_t c = jb.lo;
+                    // row 0 takes a residue on M and D, row 1 on M and I
+                    const uint8_t skip = q == 0 ? 'I' : 'D';
+                    for (uint32_t x = 0; x < r.path_len; ++x) {
+                        if (e[x] == skip) {
+                            *w++ = '-';
+                            continue;
+                        }
+                        while (src[c] == '-') ++c;
+                        *w++ = src[c++];
+                    }
+                    break;
+                }
+                }
             }
         }
     });
-    MH_CHECK(!bad, MH_ERR_UNSUPPORTED, "residue other than A,C,G,T in a DP operand");
-    uint64_t total = 0;
-    for (const mh_dp_job &jb : jobs) total += (uint64_t)jb.a_len + jb.b_len;
-    std::vector<uint8_t> paths(total + 1);
-    std::vector<mh_dp_result> res(njobs);
-    if (njobs) {
-        const int rc = mh_dp_align(ctx, ac.data(), abytes, bc.data(), bbytes, jobs.data(), njobs, paths.data(),
-                                   res.data());
-        if (rc != MH_OK) return rc;
-    }
-    // splice each piece
-    parallel_for(tasks.size(), [&](size_t k) {
-        Task &t = tasks[k];
-        std::string ra, rb;
-        for (int q = 0; q < t.rows; ++q)
-            for (const Seg &sg : t.row[q]) {
-                if (sg.job < 0) {
-                    t.out[q] += sg.text;
-                    continue;
-                }
-                const uint64_t g = jbase[k] + (uint64_t)sg.job;
-                job_text(t.a[sg.job], t.b[sg.job], paths.data() + res[g].path_start, res[g].path_len, ra, rb);
-                t.out[q] += q == 0 ? ra : rb;
-            }
-    });
-    // reassemble each interval (LM/ProgressiveAligner.cpp:1222-1237)
-    parallel_for(n, [&](size_t i) {
-        uint64_t new_len = 0;
-        for (size_t k = first[i]; k < first[i + 1]; ++k) new_len += tasks[k].out[0].size();
-        std::string rows[2] = {std::string(new_len, '-'), std::string(new_len, '-')};
-        uint64_t pos[2] = {0, 0};
-        for (size_t k = first[i]; k < first[i + 1]; ++k)
-            for (int q = 0; q < tasks[k].rows; ++q) {
-                rows[q].replace(pos[q], tasks[k].out[q].size(), tasks[k].out[q]);
-                pos[q] += tasks[k].out[q].size();
-            }
-        rows[0].resize(new_len, '-');
-        rows[1].resize(new_len, '-');
-        r1[i].swap(rows[0]);
-        r2[i].swap(rows[1]);
-    });
+    lap("host_refine_splice");
     return MH_OK;
 }
 
@@ -568,28 +674,53 @@ extern "C" {
 
 int mh_refine_pairwise(mh_context *ctx, uint64_t n, const char *const *rows1, const char *const *rows2,
                        const uint64_t *cols, uint32_t passes, char *const *out1, char *const *out2,
-                       const uint64_t *caps, uint64_t *out_cols)
+                       const uint64_t *caps, uint64_t *out_cols, mh_refine_stats *stats)
 {
     MH_CHECK(ctx && (n == 0 || (rows1 && rows2 && cols && out1 && out2 && caps && out_cols)), MH_ERR_INVALID,
              "null argument");
     MH_CHECK((passes & ~3u) == 0, MH_ERR_INVALID, "unknown refinement pass");
-    std::vector<std::string> r1(n), r2(n);
+    MH_HIP(hipSetDevice(ctx->device));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    std::vector<const char *> in1(n), in2(n);
+    std::vector<uint64_t> lens(n);
     for (uint64_t i = 0; i < n; ++i) {
         MH_CHECK(rows1[i] && rows2[i], MH_ERR_INVALID, "null row");
-        r1[i].assign(rows1[i], cols[i]);
-        r2[i].assign(rows2[i], cols[i]);
+        MH_CHECK(!std::memchr(rows1[i], '.', cols[i]) && !std::memchr(rows2[i], '.', cols[i]), MH_ERR_INVALID,
+                 "'.' as a gap character is not supported (use '-')");
+        in1[i] = rows1[i];
+        in2[i] = rows2[i];
+        lens[i] = cols[i];
     }
+    std::vector<std::string> a1, a2, b1, b2;
     int rc;
-    if ((passes & MH_REFINE_PROFILE) && (rc = refine_pass(ctx, r1, r2, true)) != MH_OK) return rc;
-    if ((passes & MH_REFINE_WINDOWS) && (rc = refine_pass(ctx, r1, r2, false)) != MH_OK) return rc;
-    for (uint64_t i = 0; i < n; ++i) out_cols[i] = r1[i].size();
-    for (uint64_t i = 0; i < n; ++i)
-        MH_CHECK(r1[i].size() <= caps[i], MH_ERR_INVALID,
-                 "output capacity too small for interval " + std::to_string(i) + " (out_cols holds the sizes)");
-    for (uint64_t i = 0; i < n; ++i) {
-        std::memcpy(out1[i], r1[i].data(), r1[i].size());
-        std::memcpy(out2[i], r2[i].data(), r2[i].size());
+    if (passes & MH_REFINE_PROFILE) {
+        uint64_t cells = 0;
+        if ((rc = refine_pass(ctx, in1, in2, lens, true, a1, a2, &cells)) != MH_OK) return rc;
+        if (stats) stats->profile_cells = cells;
+        for (uint64_t i = 0; i < n; ++i) {
+            in1[i] = a1[i].data();
+            in2[i] = a2[i].data();
+            lens[i] = a1[i].size();
+        }
     }
+    if (passes & MH_REFINE_WINDOWS) {
+        uint64_t cells = 0;
+        if ((rc = refine_pass(ctx, in1, in2, lens, false, b1, b2, &cells)) != MH_OK) return rc;
+        if (stats) stats->window_cells = cells;
+        for (uint64_t i = 0; i < n; ++i) {
+            in1[i] = b1[i].data();
+            in2[i] = b2[i].data();
+            lens[i] = b1[i].size();
+        }
+    }
+    for (uint64_t i = 0; i < n; ++i) out_cols[i] = lens[i];
+    for (uint64_t i = 0; i < n; ++i)
+        MH_CHECK(lens[i] <= caps[i], MH_ERR_INVALID,
+                 "output capacity too small for interval " + std::to_string(i) + " (out_cols holds the sizes)");
+    parallel_for(n, [&](size_t i) {
+        std::memcpy(out1[i], in1[i], lens[i]);
+        std::memcpy(out2[i], in2[i], lens[i]);
+    });
     return MH_OK;
 }
 

@@ -179,7 +179,8 @@ def load_library(path=LIB_PATH):
         "mh_index_smooth_edges": (C.c_int, [vp, u64, C.c_int]),
         "mh_xmfa_write": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.c_char_p, vp, u64]),
         "mh_refine_pairwise": (C.c_int, [vp, u64, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.POINTER(u64),
-                                         C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.POINTER(u64), C.POINTER(u64)]),
+                                         C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.POINTER(u64), C.POINTER(u64),
+                                         C.POINTER(RefineStats)]),
         "mh_refine_anchor_cols": (C.c_int, [C.c_char_p, C.c_char_p, u64, vp, u64, C.POINTER(u64)]),
         "mh_refine_pieces": (C.c_int, [C.c_char_p, C.c_char_p, u64, vp, u64, C.POINTER(u64)]),
         "mh_anchor_gaps": (C.c_int, [vp, vp, u64, vp, u64, vp, u64, C.POINTER(vp), vp]),
@@ -666,8 +667,8 @@ def recursion_gaps(row_a, row_b, left_a=1, left_b=1, min_gap=200):
     if len(a) != len(b):
         raise ValueError("rows differ in length")
     n = C.c_uint64()
-    _check(L.mh_recursion_gaps(a, b, len(a), left_a, left_b, min_gap, None, 0, C.byref(n)))
-    out = np.zeros(max(1, n.value), dtype=ANCHOR_GAP_DTYPE)
+    # each gap spans more than min_gap columns: this many always suffice
+    out = np.zeros(len(a) // (min_gap + 1) + 2, dtype=ANCHOR_GAP_DTYPE)
     _check(L.mh_recursion_gaps(a, b, len(a), left_a, left_b, min_gap, out.ctypes.data, len(out), C.byref(n)))
     return out[:n.value]
 
@@ -684,6 +685,10 @@ def anchor_gaps(ctx, seq_a, seq_b, gaps):
     finally:
         ctx.free(da)
         ctx.free(db)
+
+
+class RefineStats(C.Structure):
+    _fields_ = [("profile_cells", C.c_uint64), ("window_cells", C.c_uint64)]
 
 
 REFINE_PROFILE = 1
@@ -719,8 +724,9 @@ def refine_pieces(row1, row2):
     return [(int(out[3 * k]), int(out[3 * k + 1]), bool(out[3 * k + 2])) for k in range(n.value)]
 
 
-def refine_pairwise(ctx, intervals, passes=REFINE_PROFILE | REFINE_WINDOWS):
-    """mh_refine_pairwise over [(row1, row2)]: the refined row pairs."""
+def refine_pairwise(ctx, intervals, passes=REFINE_PROFILE | REFINE_WINDOWS, stats=None):
+    """mh_refine_pairwise over [(row1, row2)]: the refined row pairs (bytes).
+    A RefineStats passed as stats receives the DP cells of each pass."""
     L = load_library()
     n = len(intervals)
     r1 = [_row_bytes(a) for a, _ in intervals]
@@ -730,13 +736,14 @@ def refine_pairwise(ctx, intervals, passes=REFINE_PROFILE | REFINE_WINDOWS):
             raise ValueError("rows of an interval differ in length")
     cols = (C.c_uint64 * max(1, n))(*[len(a) for a in r1])
     caps = (C.c_uint64 * max(1, n))(*[2 * len(a) + 1 for a in r1])
-    o1 = [C.create_string_buffer(2 * len(a) + 1) for a in r1]
-    o2 = [C.create_string_buffer(2 * len(a) + 1) for a in r1]
+    o1 = [np.empty(2 * len(a) + 1, dtype=np.uint8) for a in r1]
+    o2 = [np.empty(2 * len(a) + 1, dtype=np.uint8) for a in r1]
     outc = (C.c_uint64 * max(1, n))()
     _check(L.mh_refine_pairwise(ctx.h, n, (C.c_char_p * max(1, n))(*r1), (C.c_char_p * max(1, n))(*r2), cols,
-                                passes, (C.c_void_p * max(1, n))(*[C.addressof(b) for b in o1]),
-                                (C.c_void_p * max(1, n))(*[C.addressof(b) for b in o2]), caps, outc))
-    return [(o1[i].raw[:outc[i]].decode(), o2[i].raw[:outc[i]].decode()) for i in range(n)]
+                                passes, (C.c_void_p * max(1, n))(*[b.ctypes.data for b in o1]),
+                                (C.c_void_p * max(1, n))(*[b.ctypes.data for b in o2]), caps, outc,
+                                C.byref(stats) if stats is not None else None))
+    return [(o1[i][:outc[i]].tobytes(), o2[i][:outc[i]].tobytes()) for i in range(n)]
 
 
 class XmfaRow(C.Structure):

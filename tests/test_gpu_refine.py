@@ -26,7 +26,7 @@ def _cases(seed, count):
                                     mauvehip.REFINE_PROFILE | mauvehip.REFINE_WINDOWS])
 def test_refine_batch_matches_oracle(ctx, passes):
     ivs = _cases(7 + passes, 8)
-    got = mauvehip.refine_pairwise(ctx, ivs, passes)
+    got = [(a.decode(), b.decode()) for a, b in mauvehip.refine_pairwise(ctx, ivs, passes)]
     for (r1, r2), (g1, g2) in zip(ivs, got):
         if passes == mauvehip.REFINE_PROFILE:
             want = ro.refine_alignment(r1, r2, True)
@@ -41,7 +41,7 @@ def test_refine_batch_matches_oracle(ctx, passes):
 
 def test_refine_single_interval_and_corners(ctx):
     cases = [("ACGTACGTAC", "ACGTACGTAC"), ("A" * 30, "-" * 30), ("ACGT" * 30, "AC-T" * 30)]
-    got = mauvehip.refine_pairwise(ctx, cases)
+    got = [(a.decode(), b.decode()) for a, b in mauvehip.refine_pairwise(ctx, cases)]
     for (r1, r2), g in zip(cases, got):
         assert g == ro.gapped_alignment(r1, r2)
     assert got[1] == cases[1]                       # one genome only: left as it is

@@ -168,3 +168,22 @@ def test_helpers():
     assert mi.findEdges(lut).tolist() == [1, 1, 0, 0, 1, 0, 1, 1]
     assert mi.findMismatches(lut, "AAAAAAAA", "CCCCCAACCCCCCCCCCCCCCC").tolist() == [0, 0, 0, 1, 1, 1, 0, 1]
     assert mi.findDuplicateMappings(np.array([3, 3, 4, -1, 0, 5, 5], dtype=np.int32)) == 2
+
+
+def test_long_rows_walked_in_chunks():
+    """Rows longer than the library's 2^18-column walk chunks, with all-gap
+    columns placed at the chunk borders (their writes must keep column order)."""
+    rng = np.random.default_rng(23)
+    n = 600_000
+    alpha = np.array(list("ACGT-"))
+    g = alpha[rng.choice(5, n, p=[.24, .24, .24, .24, .04])]
+    r = g.copy()
+    flip = rng.random(n) < 0.05
+    r[flip] = alpha[rng.integers(0, 5, int(flip.sum()))]
+    for b in (262144, 524288):
+        g[b - 3:b + 3] = "-"
+        r[b - 3:b + 3] = "-"
+    gs, rs = "".join(g), "".join(r)
+    body = lambda s: "\n".join(s[k:k + 80] for k in range(0, len(s), 80))
+    text = f"> 1:5-{n} + g\n{body(gs)}\n> 2:9-{n} + r\n{body(rs)}\n=\n"
+    _both_walks(text, n + 10)

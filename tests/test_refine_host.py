@@ -30,3 +30,17 @@ def test_pieces_match_oracle(seed, big):
     assert got == ro.pieces_of(r1, r2)
     assert got[0][0] == 0 and got[-1][1] == len(r1)
     assert all(p[1] == q[0] for p, q in zip(got, got[1:]))
+
+
+def test_pieces_across_scan_chunks():
+    # unaligned runs placed across the library's 2^18-column scan chunks
+    rng = np.random.default_rng(91)
+    n = 700_000
+    r1 = np.full(n, ord("A"), dtype=np.uint8)
+    r2 = np.full(n, ord("A"), dtype=np.uint8)
+    for lo, ln in [(3000, 6000), (262144 - 2500, 5200), (524288 - 5001, 5001), (524288 + 9000, 4999),
+                   (n - 5100, 5100)]:
+        side = r2 if rng.random() < 0.5 else r1
+        side[lo:lo + ln] = ord("-")
+    a, b = r1.tobytes().decode(), r2.tobytes().decode()
+    assert mauvehip.refine_pieces(a, b) == ro.pieces_of(a, b)
