@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--band-pairs", type=int, default=100_000,
                     help="held-out gaps in the learned-band leg (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--whole-only", action="store_true",
+                    help="run only the whole-path leg (for profiling its kernels on their own)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -96,6 +98,10 @@ def main():
     state = {}
     stage_s = {"sml": 0.0, "match": 0.0, "chain": 0.0, "dp": 0.0, "free": 0.0}
 
+    if args.whole_only:
+        args.dp_pairs = args.band_pairs = 0
+        dp = bandset = None
+
     def step():
         t = [time.perf_counter()]
         sa = mauvehip.SortedMerList.from_device(ctx, dA, La, seed)
@@ -115,68 +121,73 @@ def main():
             stage_s[name] += t[k + 1] - t[k]
         state.update(stats=stats, jobs=jobs, chain=chain, matches=mt)
 
-    for _ in range(args.warmup):
-        step()
-    ctx.set_timing(True)
-    ctx.reset_kernel_times()
-    ctx.synchronize()
-    for k in stage_s:
-        stage_s[k] = 0.0
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step()
-        log(f"[bench] step {k + 1}/{args.steps} done")
-    ctx.synchronize()
-    el = time.perf_counter() - t0
-    ms_step = 1e3 * el / args.steps
-    value = (La + Lb) / 1e6 / (el / args.steps)
-    jobs = state["jobs"]
-    cells = int((jobs["a_len"].astype(np.int64) * jobs["b_len"]).sum())
+    if args.whole_only:
+        out = {"metric": "aligned Mb/s end-to-end", "unit": "Mb/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+               "dtype": "int32", "data": "synthetic", "config": {}}
+    else:
+        for _ in range(args.warmup):
+            step()
+        ctx.set_timing(True)
+        ctx.reset_kernel_times()
+        ctx.synchronize()
+        for k in stage_s:
+            stage_s[k] = 0.0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step()
+            log(f"[bench] step {k + 1}/{args.steps} done")
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        ms_step = 1e3 * el / args.steps
+        value = (La + Lb) / 1e6 / (el / args.steps)
+        jobs = state["jobs"]
+        cells = int((jobs["a_len"].astype(np.int64) * jobs["b_len"]).sum())
 
-    # per-kernel device time over the timed region (HIP events on the library stream)
-    N = La - mauvehip.get_seed_length(seed) + 1
-    w = mauvehip.get_seed_weight(seed)
-    passes = (2 * w + 1 + 7) // 8
-    kern = {}
-    for name in ("sml_pack", "sml_keys", "sml_sort", "join", "extend", "partition", "dp_forward", "dp_traceback",
-                 "host_wait_seeds", "host_partition", "host_replay", "host_list", "host_dp_plan", "host_dp_sort"):
-        ms, n = ctx.kernel_time(name)
-        kern[name] = {"avg_ms": ms, "launches": n, "ms_per_step": ms * n / args.steps}
-    nseeds = int(state["stats"].seed_tuples)
-    # algorithmic work per launch
-    sort_bytes = 24 * N * passes + 4 * N                      # per genome
-    k3_bytes = 2 * 12 * N + 8 * nseeds + 2 * La // 4          # join + extension, per pair
-    lines = {
-        "sml_sort": ("hbm", sort_bytes / (kern["sml_sort"]["avg_ms"] * 1e-3) / 1e9 if kern["sml_sort"]["avg_ms"] else 0,
-                     HBM_PEAK_GBS, "GB/s", sort_bytes),
-        "extend": ("hbm", k3_bytes / ((kern["join"]["avg_ms"] + kern["extend"]["avg_ms"]) * 1e-3) / 1e9
-                   if kern["extend"]["avg_ms"] else 0, HBM_PEAK_GBS, "GB/s", k3_bytes),
-        "dp_forward": ("valu", cells / (kern["dp_forward"]["ms_per_step"] * 1e-3) / 1e9
-                       if kern["dp_forward"]["ms_per_step"] else 0, DP_PEAK_GCUPS, "GCUPS", cells),
-    }
-    dom = max(("sml_sort", "extend", "dp_forward"), key=lambda k: kern[k]["ms_per_step"])
-    b, ach, peak, unit, work = lines[dom]
-    roofline = {"kernel": dom, "bound": b, "achieved": round(ach, 2), "peak": round(peak, 1), "unit": unit,
-                "frac": round(ach / peak, 4) if peak else None,
-                "traffic": MEASURED_TRAFFIC.get(dom) if (La, Lb) == (4600000, 4600339) else None,
-                "work_per_launch": work}
-    log(f"[bench] headline {value:.2f} Mb/s, {ms_step:.1f} ms/step; kernels {json.dumps(kern)}")
+        # per-kernel device time over the timed region (HIP events on the library stream)
+        N = La - mauvehip.get_seed_length(seed) + 1
+        w = mauvehip.get_seed_weight(seed)
+        passes = (2 * w + 1 + 7) // 8
+        kern = {}
+        for name in ("sml_pack", "sml_keys", "sml_sort", "join", "extend", "partition", "dp_forward", "dp_traceback",
+                     "host_wait_seeds", "host_partition", "host_replay", "host_list", "host_dp_plan", "host_dp_sort"):
+            ms, n = ctx.kernel_time(name)
+            kern[name] = {"avg_ms": ms, "launches": n, "ms_per_step": ms * n / args.steps}
+        nseeds = int(state["stats"].seed_tuples)
+        # algorithmic work per launch
+        sort_bytes = 24 * N * passes + 4 * N                      # per genome
+        k3_bytes = 2 * 12 * N + 8 * nseeds + 2 * La // 4          # join + extension, per pair
+        lines = {
+            "sml_sort": ("hbm", sort_bytes / (kern["sml_sort"]["avg_ms"] * 1e-3) / 1e9 if kern["sml_sort"]["avg_ms"] else 0,
+                         HBM_PEAK_GBS, "GB/s", sort_bytes),
+            "extend": ("hbm", k3_bytes / ((kern["join"]["avg_ms"] + kern["extend"]["avg_ms"]) * 1e-3) / 1e9
+                       if kern["extend"]["avg_ms"] else 0, HBM_PEAK_GBS, "GB/s", k3_bytes),
+            "dp_forward": ("valu", cells / (kern["dp_forward"]["ms_per_step"] * 1e-3) / 1e9
+                           if kern["dp_forward"]["ms_per_step"] else 0, DP_PEAK_GCUPS, "GCUPS", cells),
+        }
+        dom = max(("sml_sort", "extend", "dp_forward"), key=lambda k: kern[k]["ms_per_step"])
+        b, ach, peak, unit, work = lines[dom]
+        roofline = {"kernel": dom, "bound": b, "achieved": round(ach, 2), "peak": round(peak, 1), "unit": unit,
+                    "frac": round(ach / peak, 4) if peak else None,
+                    "traffic": MEASURED_TRAFFIC.get(dom) if (La, Lb) == (4600000, 4600339) else None,
+                    "work_per_launch": work}
+        log(f"[bench] headline {value:.2f} Mb/s, {ms_step:.1f} ms/step; kernels {json.dumps(kern)}")
 
-    out = {
-        "metric": "aligned Mb/s end-to-end", "value": round(value, 3), "unit": "Mb/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
-        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": "recoded genome pair, 4.6 Mb, seed weight 17: sorted mer lists x2 + pairwise "
-                               "matching + anchor-chain gap DP",
-                   "genome_bp": [La, Lb], "seed": hex(seed), "seed_tuples": nseeds,
-                   "matches": int(state["stats"].matches), "chain_anchors": int(len(state["chain"])),
-                   "dp_jobs": int(len(jobs)), "dp_cells": cells, "max_gap": MAX_GAP},
-        "roofline": roofline,
-        "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "launches_per_step": v["launches"] / args.steps}
-                    for k, v in kern.items()},
-        "stages_ms_per_step": {k: round(1e3 * v / args.steps, 3) for k, v in stage_s.items()},
-    }
+        out = {
+            "metric": "aligned Mb/s end-to-end", "value": round(value, 3), "unit": "Mb/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic",
+            "config": {"workload": "recoded genome pair, 4.6 Mb, seed weight 17: sorted mer lists x2 + pairwise "
+                                   "matching + anchor-chain gap DP",
+                       "genome_bp": [La, Lb], "seed": hex(seed), "seed_tuples": nseeds,
+                       "matches": int(state["stats"].matches), "chain_anchors": int(len(state["chain"])),
+                       "dp_jobs": int(len(jobs)), "dp_cells": cells, "max_gap": MAX_GAP},
+            "roofline": roofline,
+            "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "launches_per_step": v["launches"] / args.steps}
+                        for k, v in kern.items()},
+            "stages_ms_per_step": {k: round(1e3 * v / args.steps, 3) for k, v in stage_s.items()},
+        }
 
     if dp is not None:
         A, ao, al, B, bo, bl = dp
@@ -216,10 +227,11 @@ def main():
     # the whole path (stand-in LCB chain, recursive anchoring, refinement
     # passes, XMFA and index LUT) is the headline
     full = full_leg(ctx, mut, ref, seed, args.steps, args.warmup)
-    core = {"value": out["value"], "ms_per_step": out["ms_per_step"], "workload": out["config"]["workload"],
-            "roofline": out["roofline"], "stages_ms_per_step": out.pop("stages_ms_per_step"),
-            "dp_jobs": out["config"]["dp_jobs"], "dp_cells": out["config"]["dp_cells"]}
-    out["core"] = core
+    if not args.whole_only:
+        out["core"] = {"value": out["value"], "ms_per_step": out["ms_per_step"],
+                       "workload": out["config"]["workload"], "roofline": out["roofline"],
+                       "stages_ms_per_step": out.pop("stages_ms_per_step"), "dp_jobs": out["config"]["dp_jobs"],
+                       "dp_cells": out["config"]["dp_cells"]}
     out["value"] = full["value"]
     out["ms_per_step"] = full["ms_per_step"]
     out["roofline"] = full["roofline"]
@@ -234,6 +246,7 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_full(mut, ref, seed, full)
         log(f"[bench] CPU oracle (whole path, sampled) {out['cpu_baseline']['seconds']:.2f}s per step")
+    if not args.no_cpu_baseline and not args.whole_only:
         from oracle import pyoracle as po
         t0 = time.perf_counter()
         st_o, ln_o, _ = po.find_matches([ref.tobytes(), mut.tobytes()], seed)

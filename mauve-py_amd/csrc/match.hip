@@ -224,6 +224,46 @@ __device__ bool extend2(const ExtEnv &e, int64_t st[2], int64_t &len)
         int j = 0, i = 2;
         extend_limit = 0;
         extend_attempts = 0;
+        if (dir < 2 && maxlen >= jump) {
+            // The seed-length jumps of phases 0 and 1: the addresses of the
+            // K = maxlen / jump lookups do not depend on their outcomes, so
+            // they are issued in groups of PREFETCH and checked in order; the
+            // state after the first failing step (or after the K-th) is what
+            // the step-by-step loop leaves (len and the forward starts moved by
+            // the successful steps only).
+            constexpr int PREFETCH = 8;
+            const int64_t K = maxlen / jump;
+            int64_t ok_steps = K;
+            bool done = false;
+            for (int64_t k0 = 1; k0 <= K && !done; k0 += PREFETCH) {
+                uint64_t m0[PREFETCH], m1[PREFETCH];
+                int q0[PREFETCH], q1[PREFETCH];
+                bool in[PREFETCH];
+                const int cnt = (int)((K - k0 + 1) < PREFETCH ? (K - k0 + 1) : PREFETCH);
+#pragma unroll
+                for (int u = 0; u < PREFETCH; ++u) {
+                    if (u >= cnt) break;
+                    const int64_t k = k0 + u;
+                    const int64_t ln = len + k * jump;
+                    const int64_t p0 = st[0] > 0 ? st[0] - k * jump : st[0];
+                    const int64_t p1 = st[1] > 0 ? st[1] - k * jump : st[1];
+                    in[u] = lookup(e, 0, p0, ln, st[0], m0[u], q0[u]) && lookup(e, 1, p1, ln, st[1], m1[u], q1[u]);
+                }
+                for (int u = 0; u < cnt; ++u) {
+                    if (++guard > guard_max) return false;
+                    if (!in[u]) return false;
+                    if (m0[u] != m1[u] || q0[u] != q1[u]) {
+                        ok_steps = k0 + u - 1;
+                        done = true;
+                        break;
+                    }
+                }
+            }
+            len += ok_steps * jump;
+            for (int x = 0; x < 2; ++x)
+                if (st[x] > 0) st[x] -= ok_steps * jump;
+            maxlen = 0;   // the loop below does not run
+        }
         while (maxlen - jump >= 0) {
             if (++guard > guard_max) return false;
             len += jump;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cstdlib>
 #include <cstddef>
 #include <string>
 #include <vector>
@@ -61,6 +62,28 @@ struct SmlBufs {
     void release();
 };
 
+// Host memory kept by the context between calls (uninitialised; grows only),
+// so large per-call host arrays do not pay for zero-filling and first-touch
+// page faults every time.
+struct HostBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf() { std::free(ptr); }
+    template <class T> T *get(size_t count)
+    {
+        const size_t need = count * sizeof(T) + 64;
+        if (need > bytes) {
+            std::free(ptr);
+            ptr = std::malloc(need + need / 4);
+            bytes = ptr ? need + need / 4 : 0;
+        }
+        return static_cast<T *>(ptr);
+    }
+};
+
 struct KernelTimer {
     double total_ms = 0;
     int64_t launches = 0;
@@ -89,6 +112,8 @@ struct mh_context {
     // released buffer sets kept for the next build
     std::set<mh_sml *> smls;
     std::vector<mh::SmlBufs> sml_pool;
+    // refinement passes: prefix counts and the rows each pass writes
+    mh::HostBuf refine_prefix[2], refine_rows[2][2];
 };
 
 struct mh_sml {

@@ -217,11 +217,11 @@ template <class F> void parallel_for(size_t n, F f)
 // each row before every column, so a piece's lengths are two subtractions.
 // Built in parallel chunks.
 struct Prefix {
-    std::vector<uint32_t> c1, c2;
-    void build(const char *r1, const char *r2, uint64_t n)
+    uint32_t *c1 = nullptr, *c2 = nullptr;    // n + 1 entries each, caller's memory
+    void build(const char *r1, const char *r2, uint64_t n, uint32_t *m1, uint32_t *m2)
     {
-        c1.resize(n + 1);
-        c2.resize(n + 1);
+        c1 = m1;
+        c2 = m2;
         const uint64_t chunk = 1 << 18, nc = (n + chunk - 1) / chunk;
         std::vector<uint32_t> t1(nc + 1, 0), t2(nc + 1, 0);
         parallel_for(nc, [&](size_t k) {
@@ -476,8 +476,8 @@ bool put_codes(const char *r, uint64_t lo, uint64_t hi, uint8_t *out)
 // through pinned staging, then every piece written straight into the new rows
 // at its place (the reassembly of LM/ProgressiveAligner.cpp:1222-1237).
 int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std::vector<const char *> &in2,
-                const std::vector<uint64_t> &lens, bool profile, std::vector<std::string> &o1,
-                std::vector<std::string> &o2, uint64_t *cells_out)
+                const std::vector<uint64_t> &lens, bool profile, int slot, std::vector<char *> &o1,
+                std::vector<char *> &o2, std::vector<uint64_t> &olen, uint64_t *cells_out)
 {
     const size_t n = lens.size();
     double tt = now_ms();
@@ -489,7 +489,17 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
     std::vector<Prefix> pf(n);
     std::vector<std::vector<RefinePiece>> pcs(n);
     std::vector<char> whole(n, 0);
-    for (size_t i = 0; i < n; ++i) pf[i].build(in1[i], in2[i], lens[i]);
+    {
+        uint64_t tot = 0;
+        for (size_t i = 0; i < n; ++i) tot += lens[i] + 1;
+        uint32_t *m1 = ctx->refine_prefix[0].get<uint32_t>(tot), *m2 = ctx->refine_prefix[1].get<uint32_t>(tot);
+        MH_CHECK(m1 && m2, MH_ERR_NOMEM, "host allocation failed");
+        for (size_t i = 0; i < n; ++i) {
+            pf[i].build(in1[i], in2[i], lens[i], m1, m2);
+            m1 += lens[i] + 1;
+            m2 += lens[i] + 1;
+        }
+    }
     parallel_for(n, [&](size_t i) {
         const uint64_t len = lens[i];
         // an interval that holds one genome only is not refined (the
@@ -607,9 +617,11 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
             t.out_len[q] = m;
         }
     });
-    std::vector<uint64_t> pos0(nt), pos1(nt);
-    o1.assign(n, std::string());
-    o2.assign(n, std::string());
+    std::vector<uint64_t> pos0(nt), pos1(nt), tail(n);
+    o1.assign(n, nullptr);
+    o2.assign(n, nullptr);
+    olen.assign(n, 0);
+    uint64_t total_out = 0;
     for (size_t i = 0; i < n; ++i) {
         uint64_t p0 = 0, p1 = 0;
         for (size_t k = first[i]; k < first[i + 1]; ++k) {
@@ -618,14 +630,28 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
             p0 += tasks[k].out_len[0];
             if (tasks[k].rows == 2) p1 += tasks[k].out_len[1];
         }
-        o1[i].assign(p0, '-');
-        o2[i].assign(p0, '-');
+        olen[i] = p0;
+        tail[i] = p1;
+        total_out += p0;
+    }
+    {
+        char *b1 = ctx->refine_rows[slot][0].get<char>(total_out), *b2 = ctx->refine_rows[slot][1].get<char>(total_out);
+        MH_CHECK(b1 && b2, MH_ERR_NOMEM, "host allocation failed");
+        for (size_t i = 0; i < n; ++i) {
+            o1[i] = b1;
+            o2[i] = b2;
+            b1 += olen[i];
+            b2 += olen[i];
+            // genome 2's row is written up to tail; the rest stays a gap
+            // (a piece re-aligned from one row does not advance it)
+            std::memset(o2[i] + tail[i], '-', olen[i] - tail[i]);
+        }
     }
     parallel_for(nt, [&](size_t k) {
         const Task &t = tasks[k];
         const char *in[2] = {in1[t.iv], in2[t.iv]};
         // a one-row piece writes its row into genome 1's row (see the file comment)
-        char *dst[2] = {&o1[t.iv][pos0[k]], &o2[t.iv][pos1[k]]};
+        char *dst[2] = {o1[t.iv] + pos0[k], o2[t.iv] + pos1[k]};
         for (int q = 0; q < t.rows; ++q) {
             char *w = dst[q];
             for (const Seg &sg : t.row[q]) {
@@ -691,26 +717,27 @@ int mh_refine_pairwise(mh_context *ctx, uint64_t n, const char *const *rows1, co
         in2[i] = rows2[i];
         lens[i] = cols[i];
     }
-    std::vector<std::string> a1, a2, b1, b2;
+    std::vector<char *> o1, o2;
+    std::vector<uint64_t> ol;
     int rc;
     if (passes & MH_REFINE_PROFILE) {
         uint64_t cells = 0;
-        if ((rc = refine_pass(ctx, in1, in2, lens, true, a1, a2, &cells)) != MH_OK) return rc;
+        if ((rc = refine_pass(ctx, in1, in2, lens, true, 0, o1, o2, ol, &cells)) != MH_OK) return rc;
         if (stats) stats->profile_cells = cells;
         for (uint64_t i = 0; i < n; ++i) {
-            in1[i] = a1[i].data();
-            in2[i] = a2[i].data();
-            lens[i] = a1[i].size();
+            in1[i] = o1[i];
+            in2[i] = o2[i];
+            lens[i] = ol[i];
         }
     }
     if (passes & MH_REFINE_WINDOWS) {
         uint64_t cells = 0;
-        if ((rc = refine_pass(ctx, in1, in2, lens, false, b1, b2, &cells)) != MH_OK) return rc;
+        if ((rc = refine_pass(ctx, in1, in2, lens, false, 1, o1, o2, ol, &cells)) != MH_OK) return rc;
         if (stats) stats->window_cells = cells;
         for (uint64_t i = 0; i < n; ++i) {
-            in1[i] = b1[i].data();
-            in2[i] = b2[i].data();
-            lens[i] = b1[i].size();
+            in1[i] = o1[i];
+            in2[i] = o2[i];
+            lens[i] = ol[i];
         }
     }
     for (uint64_t i = 0; i < n; ++i) out_cols[i] = lens[i];
@@ -741,8 +768,9 @@ int mh_refine_pieces(const char *row1, const char *row2, uint64_t cols, uint64_t
 {
     MH_CHECK(row1 && row2 && n_pieces && (pieces || cap == 0), MH_ERR_INVALID, "null argument");
     std::vector<RefinePiece> p;
+    std::vector<uint32_t> m1(cols + 1), m2(cols + 1);
     Prefix pf;
-    pf.build(row1, row2, cols);
+    pf.build(row1, row2, cols, m1.data(), m2.data());
     pieces_fast(row1, row2, cols, pf, p);
     *n_pieces = p.size();
     for (size_t k = 0; k < p.size() && k < cap; ++k) {
