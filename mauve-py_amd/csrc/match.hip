@@ -19,7 +19,10 @@
 // Runs longer than MER_REPEAT_LIMIT (LM/MatchFinder.cpp:166) make the
 // reference restart its merge from recomputed start points (253-276); such
 // runs are counted so the host can refuse inputs that contain them.
+#include "extend.hpp"
 #include "mh_internal.hpp"
+
+#include <algorithm>
 #include <cstring>
 
 namespace mh {
@@ -30,6 +33,7 @@ constexpr int JT = 256;              // threads per join/compaction tile
 constexpr int JITEMS = 16;
 constexpr int JTILE = JT * JITEMS;
 constexpr uint32_t MER_REPEAT_LIMIT = 1000;
+constexpr int64_t LONG_WALK = 256;   // steps of a head's walk before it is resolved per diagonal
 
 __device__ __forceinline__ uint64_t lower_bound_m(const uint64_t *__restrict__ ck, uint64_t n, uint64_t m)
 {
@@ -178,147 +182,11 @@ __global__ __launch_bounds__(JT) void compact_kernel(const uint32_t *__restrict_
     }
 }
 
-struct ExtEnv {
-    const uint64_t *kp0, *kp1;   // GetDnaSeedMer by position
-    uint64_t n0, n1;              // SMLLength
-    int64_t L0, L1;               // sequence lengths
-    int seed_len;
-    uint64_t mer_mask;            // GetSeedMask(): top 2*weight bits
-};
-
-__device__ __forceinline__ bool lookup(const ExtEnv &e, int g, int64_t pos1, int64_t len, int64_t start,
-                                       uint64_t &mer, int &parity)
-{
-    int64_t p = pos1;
-    if (p < 0) p = -p + len - e.seed_len;
-    const int64_t idx = p - 1;
-    const uint64_t n = g ? e.n1 : e.n0;
-    if (idx < 0 || (uint64_t)idx >= n) return false;
-    const uint64_t k = (g ? e.kp1 : e.kp0)[idx];
-    parity = start < 0 ? (int)(k & 1ull) : (int)!(k & 1ull);
-    mer = k & e.mer_mask;
-    return true;
-}
-
-// ExtendMatch, LM/MatchFinder.h:218-374, for a two-genome seed (both starts
-// present, linear sequences).  Returns false if it would read outside the
-// lists (the reference reads out of bounds there; never seen in practice).
-__device__ bool extend2(const ExtEnv &e, int64_t st[2], int64_t &len)
-{
-    const int64_t Lg[2] = {e.L0, e.L1};
-    int jump = e.seed_len;
-    uint32_t extend_limit = 0, extend_attempts = 0;
-    bool extend_again = false;
-    int64_t guard = 0;
-    const int64_t guard_max = 8 * (e.L0 + e.L1) + 64;
-    for (int dir = 0; dir < 4; ++dir) {
-        int64_t maxlen = dir >= 2 ? (int64_t)e.seed_len : (int64_t)4294967295ll;
-        for (int x = 0; x < 2; ++x) {
-            if (st[x] < 0) {
-                const int64_t rc_len = Lg[x] - len + st[x] + 1;
-                if (rc_len < maxlen) maxlen = rc_len;
-            } else if (st[x] - 1 < maxlen) {
-                maxlen = st[x] - 1;
-            }
-        }
-        int j = 0, i = 2;
-        extend_limit = 0;
-        extend_attempts = 0;
-        if (dir < 2 && maxlen >= jump) {
-            // The seed-length jumps of phases 0 and 1: the addresses of the
-            // K = maxlen / jump lookups do not depend on their outcomes, so
-            // they are issued in groups of PREFETCH and checked in order; the
-            // state after the first failing step (or after the K-th) is what
-            // the step-by-step loop leaves (len and the forward starts moved by
-            // the successful steps only).
-            constexpr int PREFETCH = 8;
-            const int64_t K = maxlen / jump;
-            int64_t ok_steps = K;
-            bool done = false;
-            for (int64_t k0 = 1; k0 <= K && !done; k0 += PREFETCH) {
-                uint64_t m0[PREFETCH], m1[PREFETCH];
-                int q0[PREFETCH], q1[PREFETCH];
-                bool in[PREFETCH];
-                const int cnt = (int)((K - k0 + 1) < PREFETCH ? (K - k0 + 1) : PREFETCH);
-#pragma unroll
-                for (int u = 0; u < PREFETCH; ++u) {
-                    if (u >= cnt) break;
-                    const int64_t k = k0 + u;
-                    const int64_t ln = len + k * jump;
-                    const int64_t p0 = st[0] > 0 ? st[0] - k * jump : st[0];
-                    const int64_t p1 = st[1] > 0 ? st[1] - k * jump : st[1];
-                    in[u] = lookup(e, 0, p0, ln, st[0], m0[u], q0[u]) && lookup(e, 1, p1, ln, st[1], m1[u], q1[u]);
-                }
-                for (int u = 0; u < cnt; ++u) {
-                    if (++guard > guard_max) return false;
-                    if (!in[u]) return false;
-                    if (m0[u] != m1[u] || q0[u] != q1[u]) {
-                        ok_steps = k0 + u - 1;
-                        done = true;
-                        break;
-                    }
-                }
-            }
-            len += ok_steps * jump;
-            for (int x = 0; x < 2; ++x)
-                if (st[x] > 0) st[x] -= ok_steps * jump;
-            maxlen = 0;   // the loop below does not run
-        }
-        while (maxlen - jump >= 0) {
-            if (++guard > guard_max) return false;
-            len += jump;
-            maxlen -= jump;
-            for (j = 0; j < 2; ++j)
-                if (st[j] > 0) {
-                    st[j] -= jump;
-                    if (st[j] <= 0) st[j] += Lg[j];
-                }
-            uint64_t cm, om;
-            int cp, op;
-            if (!lookup(e, 0, st[0], len, st[0], cm, cp)) return false;
-            if (!lookup(e, 1, st[1], len, st[1], om, op)) return false;
-            i = 1;
-            if (cm != om || cp != op) {
-                if (dir < 2) maxlen = 0;
-            } else {
-                i = 2;
-            }
-            extend_attempts += jump;
-            if (i == 2) extend_limit = extend_attempts;
-            if (dir > 1 && extend_attempts == (uint32_t)e.seed_len) break;
-        }
-        if (i < 2) {
-            len -= jump;
-            for (; j > 0; j--)
-                if (st[j - 1] >= 0) st[j - 1] += jump;
-        }
-        if (dir > 1 && extend_attempts > 0) {
-            if (extend_limit > 0) extend_again = true;
-            int unmatched = (int)(extend_attempts - extend_limit);
-            if (i < 2) unmatched -= jump;
-            len -= unmatched;
-            for (j = 0; j < 2; ++j)
-                if (st[j] > 0) {
-                    st[j] += unmatched;
-                    if (st[j] > Lg[j]) st[j] -= Lg[j];
-                }
-        }
-        st[0] = -st[0];
-        st[1] = -st[1];
-        if (dir >= 1) jump = 1;
-        if (dir == 3 && extend_again) {
-            dir = -1;
-            jump = e.seed_len;
-            extend_again = false;
-        }
-    }
-    return true;
-}
-
 __global__ void seed_kernel(const uint32_t *__restrict__ pi, const uint32_t *__restrict__ pj, uint64_t ns,
                             const uint64_t *__restrict__ ck0, const uint32_t *__restrict__ pos0,
                             const uint64_t *__restrict__ ck1, const uint32_t *__restrict__ pos1, ExtEnv env,
-                            SeedRec *__restrict__ out, uint32_t *__restrict__ err)
+                            int64_t budget, SeedRec *__restrict__ out, uint32_t *__restrict__ err,
+                            uint32_t *__restrict__ long_flag)
 {
     const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= ns) return;
@@ -329,8 +197,9 @@ __global__ void seed_kernel(const uint32_t *__restrict__ pi, const uint32_t *__r
     if ((ck0[i] & 1ull) != (ck1[j] & 1ull)) s1 = -s1;
     int64_t st[2] = {s0, s1};
     int64_t len = env.seed_len;
-    const bool ok = extend2(env, st, len);
-    if (!ok) atomicAdd(err, 1u);
+    const int rc = extend2(env, st, len, budget);
+    if (long_flag) long_flag[x] = rc == EXT_LONG;
+    if (rc == EXT_ERR) atomicAdd(err, 1u);
     SeedRec r;
     r.s0 = (int32_t)s0;
     r.s1 = (int32_t)s1;
@@ -338,6 +207,297 @@ __global__ void seed_kernel(const uint32_t *__restrict__ pi, const uint32_t *__r
     r.e1 = (int32_t)st[1];
     r.elen = (uint32_t)len;
     out[x] = r;
+}
+
+// ---- spaced seeds: one extension per grid chain -----------------------------
+// Two seeds of the same strand relation whose starts differ by the seed length
+// along their diagonal (forward: s0 + L, s1 + L; reverse: s0 + L, |s1| - L)
+// extend to the same match: the later one's seed-length walk to the left steps
+// onto the earlier one's window (a seed, so it matches) and from there repeats
+// the earlier one's walk, with one step less of room; to the right the earlier
+// one steps onto the later one's window the same way; the single-step phases
+// then start from the same match.  So each chain of such seeds is extended once,
+// from its first seed, and the result is copied to the others.
+
+__global__ void grid_key_kernel(const uint32_t *__restrict__ oi, const uint32_t *__restrict__ oj, uint64_t ns,
+                                const uint64_t *__restrict__ ck0, const uint32_t *__restrict__ pos0,
+                                const uint64_t *__restrict__ ck1, const uint32_t *__restrict__ pos1,
+                                uint64_t *__restrict__ key, uint32_t *__restrict__ val, int32_t *__restrict__ t1)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ns) return;
+    const uint32_t i = oi[x], j = oj[x];
+    const int32_t s1 = (int32_t)pos1[j] + 1;
+    key[x] = (uint64_t)pos0[i] + 1;
+    val[x] = (uint32_t)x;
+    t1[x] = (ck0[i] & 1ull) != (ck1[j] & 1ull) ? -s1 : s1;   // the seed's signed start in genome 2
+}
+
+// link[r] = sorted index of the seed one seed length before r on its chain, or
+// r.  Seeds are sorted by their start in genome 1; the predecessor of (s0, t)
+// is (s0 - L, t - L) with t's sign kept (forward: s1 - L; reverse: the start
+// in genome 2 moves right, -(s1 + L)).  At most `scan` seeds sharing a start
+// are looked at: a seed left without its link only extends by itself.
+__global__ void grid_pred_kernel(const uint64_t *__restrict__ sk, const uint32_t *__restrict__ sv,
+                                 const int32_t *__restrict__ t1, uint64_t ns, uint32_t seed_len, int scan,
+                                 uint32_t *__restrict__ link)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ns) return;
+    const uint64_t k = sk[r];
+    const int32_t t = t1[sv[r]];
+    const int32_t tw = t - (int32_t)seed_len;
+    uint32_t out = (uint32_t)r;
+    if (k > seed_len && (t < 0 || tw > 0)) {
+        const uint64_t want = k - seed_len;
+        uint64_t lo = 0, hi = r;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) / 2;
+            if (sk[mid] < want) lo = mid + 1;
+            else hi = mid;
+        }
+        for (int c = 0; c < scan && lo < r && sk[lo] == want; ++c, ++lo)
+            if (t1[sv[lo]] == tw) {
+                out = (uint32_t)lo;
+                break;
+            }
+    }
+    link[r] = out;
+}
+
+__global__ void grid_jump_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t ns,
+                                 uint32_t *__restrict__ changed)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ns) return;
+    const uint32_t a = in[r], b = in[a];
+    out[r] = b;
+    if (b != a) atomicOr(changed, 1u);
+}
+
+// per seed (original order): is it a chain head, and which seed heads its chain
+__global__ void grid_head_kernel(const uint32_t *__restrict__ sv, const uint32_t *__restrict__ root, uint64_t ns,
+                                 uint32_t *__restrict__ is_head, uint32_t *__restrict__ head_of)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ns) return;
+    const uint32_t x = sv[r];
+    is_head[x] = root[r] == (uint32_t)r;
+    head_of[x] = sv[root[r]];
+}
+
+__global__ void grid_gather_kernel(const uint32_t *__restrict__ heads, uint64_t nh, const uint32_t *__restrict__ oi,
+                                   const uint32_t *__restrict__ oj, uint32_t *__restrict__ hoi,
+                                   uint32_t *__restrict__ hoj, uint32_t *__restrict__ headpos)
+{
+    const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    const uint32_t x = heads[h];
+    hoi[h] = oi[x];
+    hoj[h] = oj[x];
+    headpos[x] = (uint32_t)h;
+}
+
+__global__ void grid_fill_kernel(const uint32_t *__restrict__ oi, const uint32_t *__restrict__ oj, uint64_t ns,
+                                 const uint64_t *__restrict__ ck0, const uint32_t *__restrict__ pos0,
+                                 const uint64_t *__restrict__ ck1, const uint32_t *__restrict__ pos1,
+                                 const uint32_t *__restrict__ head_of, const uint32_t *__restrict__ headpos,
+                                 const SeedRec *__restrict__ hres, SeedRec *__restrict__ out)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ns) return;
+    const uint32_t i = oi[x], j = oj[x];
+    const int64_t s0 = (int64_t)pos0[i] + 1;
+    int64_t s1 = (int64_t)pos1[j] + 1;
+    if ((ck0[i] & 1ull) != (ck1[j] & 1ull)) s1 = -s1;
+    const SeedRec h = hres[headpos[head_of[x]]];
+    SeedRec r;
+    r.s0 = (int32_t)s0;
+    r.s1 = (int32_t)s1;
+    r.e0 = h.e0;
+    r.e1 = h.e1;
+    r.elen = h.elen;
+    out[x] = r;
+}
+
+// ---- long extensions: both ends resolved per diagonal ----------------------
+// ExtendMatch's two ends move independently.  Within one round the left end
+// takes seed-length jumps while the next window to the left matches, then a
+// single-step phase moves it to the furthest matching window within one seed
+// length (or leaves it); the right end does the same to the right; rounds
+// repeat while either single-step phase moved.  Each end's moves depend only
+// on its own position (the room to the sequence ends and the windows read are
+// functions of it, the diagonal fixed), and an end whose single-step phase
+// found nothing finds nothing again.  So the final left end of a seed is the
+// fixed point of F(p) = J(p) - T(J(p)) from its start, J(p) the last window of
+// p's run of matching windows spaced one seed length apart and T(h) the
+// largest t <= seed length with window h - t matching (0 if none); the right
+// end mirrors it.  For the seeds whose step-by-step walk is long, F is built
+// for every window of a range around them on their diagonal and the fixed
+// points found by pointer jumping.  A window whose value depends on windows
+// beyond a range that was cut short (not by the sequence ends) is flagged, and
+// its diagonal is redone with a wider range.
+
+struct DiagGroup {
+    int64_t lo;        // first window start in genome 1 (1-based) of the range
+    int64_t n;         // windows in the range
+    int64_t base;      // first element of the range in the dense arrays
+    int64_t dconst;    // forward: s1 - s0; reverse: s0 + s1 (window starts pair as p + q)
+    int32_t rev, clip_lo, clip_hi, pad;
+};
+
+struct DiagHead {
+    uint32_t head;     // index into the head results
+    uint32_t group;
+    int32_t s0, t1;    // start in genome 1, signed start in genome 2
+};
+
+constexpr uint32_t DG_ESC = 0x80000000u, DG_IDX = 0x7fffffffu;
+
+__device__ __forceinline__ uint32_t dg_find(const DiagGroup *__restrict__ g, uint32_t ng, uint64_t e)
+{
+    uint32_t lo = 0, hi = ng;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if ((uint64_t)g[mid].base <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// window e of its range matches (both windows inside the sequences by construction)
+__global__ void dg_match_kernel(const DiagGroup *__restrict__ g, uint32_t ng, uint64_t N, ExtEnv env,
+                                uint8_t *__restrict__ mg)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N) return;
+    const DiagGroup d = g[dg_find(g, ng, e)];
+    const int64_t p = d.lo + (int64_t)(e - (uint64_t)d.base);
+    const int64_t q = d.rev ? d.dconst - p : p + d.dconst;
+    const uint64_t k0 = env.kp0[p - 1], k1 = env.kp1[q - 1];
+    mg[e] = ((k0 ^ k1) & env.mer_mask) == 0 && ((k0 ^ k1) & 1ull) == (uint64_t)d.rev;
+}
+
+// one seed-length step of the jump runs, left and right
+__global__ void dg_link_kernel(const DiagGroup *__restrict__ g, uint32_t ng, uint64_t N, int L,
+                               const uint8_t *__restrict__ mg, uint32_t *__restrict__ cl, uint32_t *__restrict__ cr)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N) return;
+    const DiagGroup d = g[dg_find(g, ng, e)];
+    const int64_t off = (int64_t)(e - (uint64_t)d.base);
+    uint32_t l = (uint32_t)e, r = (uint32_t)e;
+    if (off >= L) {
+        if (mg[e - L]) l = (uint32_t)(e - L);
+    } else if (d.clip_lo) {
+        l |= DG_ESC;
+    }
+    if (off + L < d.n) {
+        if (mg[e + L]) r = (uint32_t)(e + L);
+    } else if (d.clip_hi) {
+        r |= DG_ESC;
+    }
+    cl[e] = l;
+    cr[e] = r;
+}
+
+// pointer jumping; a root keeps its own flag, which the others take with it
+__global__ void dg_jump_kernel(uint64_t N, const uint32_t *__restrict__ il, const uint32_t *__restrict__ ir,
+                               uint32_t *__restrict__ ol, uint32_t *__restrict__ orr)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N) return;
+    ol[e] = il[il[e] & DG_IDX];
+    orr[e] = ir[ir[e] & DG_IDX];
+}
+
+// F (left) and its mirror (right) from the jump-run ends hl, hr
+__global__ void dg_step_kernel(const DiagGroup *__restrict__ g, uint32_t ng, uint64_t N, int L,
+                               const uint8_t *__restrict__ mg, const uint32_t *__restrict__ hl,
+                               const uint32_t *__restrict__ hr, uint32_t *__restrict__ fl, uint32_t *__restrict__ fr)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N) return;
+    const DiagGroup d = g[dg_find(g, ng, e)];
+    uint32_t h = hl[e], esc = h & DG_ESC;
+    h &= DG_IDX;
+    int64_t ho = (int64_t)h - d.base;
+    int T = 0;
+    for (int t = 1; t <= L; ++t) {
+        if (ho - t < 0) {
+            if (d.clip_lo) esc = DG_ESC;
+            break;
+        }
+        if (mg[h - t]) T = t;
+    }
+    fl[e] = (h - (uint32_t)T) | esc;
+    h = hr[e];
+    esc = h & DG_ESC;
+    h &= DG_IDX;
+    ho = (int64_t)h - d.base;
+    T = 0;
+    for (int t = 1; t <= L; ++t) {
+        if (ho + t >= d.n) {
+            if (d.clip_hi) esc = DG_ESC;
+            break;
+        }
+        if (mg[h + t]) T = t;
+    }
+    fr[e] = (h + (uint32_t)T) | esc;
+}
+
+// pointer jumping that collects the flags along the path
+__global__ void dg_jump_or_kernel(uint64_t N, const uint32_t *__restrict__ il, const uint32_t *__restrict__ ir,
+                                  uint32_t *__restrict__ ol, uint32_t *__restrict__ orr)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N) return;
+    uint32_t a = il[e], b = il[a & DG_IDX];
+    ol[e] = (b & DG_IDX) | ((a | b) & DG_ESC);
+    a = ir[e];
+    b = ir[a & DG_IDX];
+    orr[e] = (b & DG_IDX) | ((a | b) & DG_ESC);
+}
+
+__global__ void dg_heads_kernel(const uint32_t *__restrict__ long_idx, uint64_t nl, const uint32_t *__restrict__ hoi,
+                                const uint32_t *__restrict__ hoj, const uint64_t *__restrict__ ck0,
+                                const uint32_t *__restrict__ pos0, const uint64_t *__restrict__ ck1,
+                                const uint32_t *__restrict__ pos1, DiagHead *__restrict__ out)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nl) return;
+    const uint32_t h = long_idx[x], i = hoi[h], j = hoj[h];
+    const int32_t s1 = (int32_t)pos1[j] + 1;
+    DiagHead r;
+    r.head = h;
+    r.group = 0;
+    r.s0 = (int32_t)pos0[i] + 1;
+    r.t1 = (ck0[i] & 1ull) != (ck1[j] & 1ull) ? -s1 : s1;
+    out[x] = r;
+}
+
+__global__ void dg_result_kernel(const DiagHead *__restrict__ hd, uint64_t nh, const DiagGroup *__restrict__ g,
+                                 const uint32_t *__restrict__ fl, const uint32_t *__restrict__ fr, int L,
+                                 SeedRec *__restrict__ res, uint8_t *__restrict__ esc)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nh) return;
+    const DiagHead h = hd[x];
+    const DiagGroup d = g[h.group];
+    const uint64_t e = (uint64_t)d.base + (uint64_t)(h.s0 - d.lo);
+    const uint32_t a = fl[e], b = fr[e];
+    esc[x] = ((a | b) & DG_ESC) != 0;
+    const int64_t left = d.lo + ((int64_t)(a & DG_IDX) - d.base);
+    const int64_t last = d.lo + ((int64_t)(b & DG_IDX) - d.base);   // start of the rightmost window
+    const int64_t len = last - left + L;
+    const int64_t s0 = h.s0, s1 = h.t1 < 0 ? -(int64_t)h.t1 : h.t1;
+    SeedRec r;
+    r.s0 = h.s0;
+    r.s1 = h.t1;
+    r.e0 = (int32_t)left;
+    r.e1 = (int32_t)(d.rev ? -(s0 + s1 + L - left - len) : left + (s1 - s0));
+    r.elen = (uint32_t)len;
+    res[h.head] = r;
 }
 
 // ---- solid odd-length seeds ----------------------------------------------
@@ -599,6 +759,216 @@ __global__ void gather_dirty_kernel(const uint32_t *__restrict__ idx, uint64_t m
 }  // namespace
 
 // Seeds in HashMatch order with their extensions; returns the count.
+static int compact_flags(mh_context *ctx, const uint32_t *flag, uint64_t n, uint32_t *out_idx, uint32_t *tiles,
+                         uint32_t *d_count, uint64_t *count);
+
+// The long walks of the heads in long_idx, per diagonal (the kernels above).
+static int resolve_long(mh_context *ctx, const mh_sml *a, const mh_sml *b, const ExtEnv &env,
+                        const uint32_t *long_idx, uint64_t nl, const uint32_t *hoi, const uint32_t *hoj,
+                        SeedRec *hres)
+{
+    DevBuf *g = ctx->diag_scratch;
+    int rc;
+    if ((rc = g[0].ensure(nl * sizeof(DiagHead) + 64)) != MH_OK) return rc;
+    DiagHead *d_heads = g[0].as<DiagHead>();
+    const dim3 blk(256);
+    hipLaunchKernelGGL(dg_heads_kernel, dim3((unsigned)((nl + 255) / 256)), blk, 0, ctx->stream, long_idx, nl, hoi,
+                       hoj, a->d_keys, a->d_pos, b->d_keys, b->d_pos, d_heads);
+    MH_HIP(hipGetLastError());
+    std::vector<DiagHead> hh(nl);
+    MH_HIP(hipMemcpyAsync(hh.data(), d_heads, nl * sizeof(DiagHead), hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t L = env.seed_len;
+    const int64_t last0 = env.L0 - L + 1, last1 = env.L1 - L + 1;   // last window starts
+    auto key = [](const DiagHead &h) {
+        const int64_t s1 = h.t1 < 0 ? -(int64_t)h.t1 : h.t1;
+        return h.t1 < 0 ? std::make_pair(1, h.s0 + s1) : std::make_pair(0, s1 - h.s0);
+    };
+    std::sort(hh.begin(), hh.end(), [&](const DiagHead &x, const DiagHead &y) {
+        const auto kx = key(x), ky = key(y);
+        return kx != ky ? kx < ky : x.s0 < y.s0;
+    });
+    struct Diag {
+        int rev;
+        int64_t dconst, glo, ghi, smin, smax, reach;
+        size_t h0, h1;
+        bool done;
+    };
+    std::vector<Diag> diags;
+    for (size_t i = 0; i < nl;) {
+        size_t j = i + 1;
+        while (j < nl && key(hh[j]) == key(hh[i])) ++j;
+        Diag d;
+        d.rev = key(hh[i]).first;
+        d.dconst = key(hh[i]).second;
+        // window starts p of genome 1 whose partner q is a window start of genome 2
+        if (d.rev) {
+            d.glo = std::max<int64_t>(1, d.dconst - last1);
+            d.ghi = std::min<int64_t>(last0, d.dconst - 1);
+        } else {
+            d.glo = std::max<int64_t>(1, 1 - d.dconst);
+            d.ghi = std::min<int64_t>(last0, last1 - d.dconst);
+        }
+        d.smin = hh[i].s0;
+        d.smax = hh[j - 1].s0;
+        MH_CHECK(d.glo <= d.smin && d.smax <= d.ghi, MH_ERR_DEVICE, "seed window outside its sequences");
+        d.reach = 16384;
+        d.h0 = i;
+        d.h1 = j;
+        d.done = false;
+        diags.push_back(d);
+        i = j;
+    }
+    std::vector<DiagGroup> tab;
+    std::vector<DiagHead> hp;
+    std::vector<size_t> owner;
+    std::vector<uint8_t> esc;
+    for (int iter = 0;; ++iter) {
+        MH_CHECK(iter < 16, MH_ERR_DEVICE, "long seed extensions did not settle");
+        tab.clear();
+        hp.clear();
+        owner.clear();
+        int64_t N = 0, max_n = 1;
+        for (size_t k = 0; k < diags.size(); ++k) {
+            Diag &d = diags[k];
+            if (d.done) continue;
+            DiagGroup t;
+            t.lo = std::max(d.glo, d.smin - d.reach);
+            const int64_t hi = std::min(d.ghi, d.smax + d.reach);
+            t.n = hi - t.lo + 1;
+            t.base = N;
+            t.dconst = d.dconst;
+            t.rev = d.rev;
+            t.clip_lo = t.lo > d.glo;
+            t.clip_hi = hi < d.ghi;
+            t.pad = 0;
+            N += t.n;
+            max_n = std::max(max_n, t.n);
+            for (size_t i = d.h0; i < d.h1; ++i) {
+                DiagHead h = hh[i];
+                h.group = (uint32_t)tab.size();
+                hp.push_back(h);
+            }
+            owner.push_back(k);
+            tab.push_back(t);
+        }
+        if (tab.empty()) break;
+        MH_CHECK(N < (int64_t)DG_IDX, MH_ERR_UNSUPPORTED, "long seed extensions span too many windows");
+        const uint64_t nh = hp.size(), ng = tab.size();
+        if ((rc = g[1].ensure(ng * sizeof(DiagGroup) + 64)) != MH_OK) return rc;
+        if ((rc = g[0].ensure(nh * sizeof(DiagHead) + 64)) != MH_OK) return rc;
+        if ((rc = g[2].ensure((uint64_t)N + 64)) != MH_OK) return rc;
+        for (int q = 3; q < 7; ++q)
+            if ((rc = g[q].ensure((uint64_t)N * 4 + 64)) != MH_OK) return rc;
+        if ((rc = g[7].ensure(nh + 64)) != MH_OK) return rc;
+        DiagGroup *d_tab = g[1].as<DiagGroup>();
+        DiagHead *d_hp = g[0].as<DiagHead>();
+        uint8_t *mg = g[2].as<uint8_t>(), *d_esc = g[7].as<uint8_t>();
+        uint32_t *l0 = g[3].as<uint32_t>(), *r0 = g[4].as<uint32_t>(), *l1 = g[5].as<uint32_t>(),
+                 *r1 = g[6].as<uint32_t>();
+        MH_HIP(hipMemcpyAsync(d_tab, tab.data(), ng * sizeof(DiagGroup), hipMemcpyHostToDevice, ctx->stream));
+        MH_HIP(hipMemcpyAsync(d_hp, hp.data(), nh * sizeof(DiagHead), hipMemcpyHostToDevice, ctx->stream));
+        const dim3 grid((unsigned)((N + 255) / 256));
+        int rounds = 1;
+        while ((1ll << (rounds - 1)) < max_n) ++rounds;
+        hipLaunchKernelGGL(dg_match_kernel, grid, blk, 0, ctx->stream, d_tab, (uint32_t)ng, (uint64_t)N, env, mg);
+        hipLaunchKernelGGL(dg_link_kernel, grid, blk, 0, ctx->stream, d_tab, (uint32_t)ng, (uint64_t)N, (int)L, mg,
+                           l0, r0);
+        for (int r = 0; r < rounds; ++r) {
+            hipLaunchKernelGGL(dg_jump_kernel, grid, blk, 0, ctx->stream, (uint64_t)N, l0, r0, l1, r1);
+            std::swap(l0, l1);
+            std::swap(r0, r1);
+        }
+        hipLaunchKernelGGL(dg_step_kernel, grid, blk, 0, ctx->stream, d_tab, (uint32_t)ng, (uint64_t)N, (int)L, mg,
+                           l0, r0, l1, r1);
+        std::swap(l0, l1);
+        std::swap(r0, r1);
+        for (int r = 0; r < rounds; ++r) {
+            hipLaunchKernelGGL(dg_jump_or_kernel, grid, blk, 0, ctx->stream, (uint64_t)N, l0, r0, l1, r1);
+            std::swap(l0, l1);
+            std::swap(r0, r1);
+        }
+        hipLaunchKernelGGL(dg_result_kernel, dim3((unsigned)((nh + 255) / 256)), blk, 0, ctx->stream, d_hp, nh,
+                           d_tab, l0, r0, (int)L, hres, d_esc);
+        MH_HIP(hipGetLastError());
+        esc.resize(nh);
+        MH_HIP(hipMemcpyAsync(esc.data(), d_esc, nh, hipMemcpyDeviceToHost, ctx->stream));
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<uint8_t> redo(ng, 0);
+        for (uint64_t i = 0; i < nh; ++i)
+            if (esc[i]) redo[hp[i].group] = 1;
+        for (uint64_t k = 0; k < ng; ++k) {
+            Diag &d = diags[owner[k]];
+            if (redo[k]) d.reach *= 8;
+            else d.done = true;
+        }
+    }
+    return MH_OK;
+}
+
+// Spaced-seed extension with one ExtendMatch per grid chain (see the kernels
+// above): sort the seeds by (diagonal, start), link each to its chain
+// predecessor, find chain heads by pointer jumping, extend the heads, copy.
+static int extend_by_chains(mh_context *ctx, const mh_sml *a, const mh_sml *b, const uint32_t *oi,
+                            const uint32_t *oj, uint64_t ns, const ExtEnv &env, SeedRec *d_seeds, uint32_t *cnt)
+{
+    DevBuf *g = ctx->grid_scratch;
+    int rc;
+    for (int q = 0; q < 2; ++q)
+        if ((rc = g[q].ensure(ns * 8 + 64)) != MH_OK) return rc;            // keys
+    for (int q = 2; q < 8; ++q)
+        if ((rc = g[q].ensure(ns * 4 + 64)) != MH_OK) return rc;            // u32 arrays
+    if ((rc = g[8].ensure(((ns + JTILE - 1) / JTILE) * 4 + 64)) != MH_OK) return rc;
+    if ((rc = g[9].ensure(ns * sizeof(SeedRec) + 64)) != MH_OK) return rc;
+    if ((rc = g[10].ensure(ns * 4 + 64)) != MH_OK) return rc;
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) - 1 < a->len) ++bits;
+    const dim3 grid((unsigned)((ns + 255) / 256)), blk(256);
+    uint64_t *k0 = g[0].as<uint64_t>(), *k1 = g[1].as<uint64_t>();
+    uint32_t *v0 = g[2].as<uint32_t>(), *v1 = g[3].as<uint32_t>();
+    int32_t *t1 = g[10].as<int32_t>();
+    hipLaunchKernelGGL(grid_key_kernel, grid, blk, 0, ctx->stream, oi, oj, ns, a->d_keys, a->d_pos, b->d_keys,
+                       b->d_pos, k0, v0, t1);
+    uint64_t *sk = nullptr;
+    uint32_t *sv = nullptr;
+    if ((rc = radix_sort_kv(ctx, k0, v0, k1, v1, ns, bits, &sk, &sv)) != MH_OK) return rc;
+    // the sort's spare value array holds the links, g[4] their jumps
+    uint32_t *la = sv == v0 ? v1 : v0, *lb = g[4].as<uint32_t>();
+    hipLaunchKernelGGL(grid_pred_kernel, grid, blk, 0, ctx->stream, sk, sv, t1, ns, (uint32_t)env.seed_len, 64, la);
+    for (int round = 0;; ++round) {
+        MH_HIP(hipMemsetAsync(cnt + 3, 0, 4, ctx->stream));
+        hipLaunchKernelGGL(grid_jump_kernel, grid, blk, 0, ctx->stream, la, lb, ns, cnt + 3);
+        std::swap(la, lb);
+        uint32_t changed = 0;
+        MH_HIP(hipMemcpyAsync(&changed, cnt + 3, 4, hipMemcpyDeviceToHost, ctx->stream));
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+        if (!changed) break;
+        MH_CHECK(round < 40, MH_ERR_DEVICE, "seed chains did not converge");
+    }
+    uint32_t *is_head = g[5].as<uint32_t>(), *head_of = g[6].as<uint32_t>();
+    hipLaunchKernelGGL(grid_head_kernel, grid, blk, 0, ctx->stream, sv, la, ns, is_head, head_of);
+    uint32_t *heads = g[7].as<uint32_t>();
+    uint64_t nh = 0;
+    if ((rc = compact_flags(ctx, is_head, ns, heads, g[8].as<uint32_t>(), cnt + 3, &nh)) != MH_OK) return rc;
+    // head operands reuse the key buffer the sort left spare (8 ns + 64 bytes)
+    uint32_t *hoi = reinterpret_cast<uint32_t *>(sk == k0 ? k1 : k0), *hoj = hoi + ns + 16;
+    uint32_t *headpos = lb;
+    SeedRec *hres = g[9].as<SeedRec>();
+    const dim3 hgrid((unsigned)((nh + 255) / 256));
+    hipLaunchKernelGGL(grid_gather_kernel, hgrid, blk, 0, ctx->stream, heads, nh, oi, oj, hoi, hoj, headpos);
+    // heads: the step-by-step walk up to a budget, the longer ones per diagonal
+    uint32_t *long_flag = is_head, *long_idx = heads;   // both free by now
+    hipLaunchKernelGGL(seed_kernel, hgrid, blk, 0, ctx->stream, hoi, hoj, nh, a->d_keys, a->d_pos, b->d_keys,
+                       b->d_pos, env, (int64_t)LONG_WALK, hres, cnt + 2, long_flag);
+    uint64_t nl = 0;
+    if ((rc = compact_flags(ctx, long_flag, nh, long_idx, g[8].as<uint32_t>(), cnt + 3, &nl)) != MH_OK) return rc;
+    if (nl && (rc = resolve_long(ctx, a, b, env, long_idx, nl, hoi, hoj, hres)) != MH_OK) return rc;
+    hipLaunchKernelGGL(grid_fill_kernel, grid, blk, 0, ctx->stream, oi, oj, ns, a->d_keys, a->d_pos, b->d_keys,
+                       b->d_pos, head_of, headpos, hres, d_seeds);
+    MH_HIP(hipGetLastError());
+    return MH_OK;
+}
+
 int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const SeedRec **seeds_out, uint64_t *ns_out,
                     uint32_t *repeat_runs)
 {
@@ -664,10 +1034,14 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const See
                            ns, a->d_keys, a->d_pos, b->d_keys, b->d_pos, a->d_words, b->d_words, wbrc,
                            (int64_t)a->len, (int64_t)b->len, a->seed_length, d_seeds);
         MH_HIP(hipGetLastError());
+    } else if (a->len < (1ull << 30) && b->len < (1ull << 30)) {
+        TimedLaunch tl(ctx, "extend");
+        if ((rc = extend_by_chains(ctx, a, b, oi, oj, ns, env, d_seeds, cnt)) != MH_OK) return rc;
     } else {
         TimedLaunch tl(ctx, "extend");
         hipLaunchKernelGGL(seed_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, ctx->stream, oi, oj, ns,
-                           a->d_keys, a->d_pos, b->d_keys, b->d_pos, env, d_seeds, cnt + 2);
+                           a->d_keys, a->d_pos, b->d_keys, b->d_pos, env, (int64_t)0, d_seeds, cnt + 2,
+                           (uint32_t *)nullptr);
         MH_HIP(hipGetLastError());
     }
     uint32_t ext_err = 0;

@@ -103,6 +103,8 @@ struct mh_context {
     mh::DevBuf ml_jobs, ml_y;   // learned band: job table, network outputs + bands
     // seed-matching scratch
     mh::DevBuf mf_scratch[16];
+    mh::DevBuf grid_scratch[11];   // spaced-seed extension by chains
+    mh::DevBuf diag_scratch[8];    // long extensions resolved per diagonal
     // scratch for sorting
     mh::DevBuf sort_tmp[4];
     // pinned host staging for device-to-host transfers
