@@ -133,6 +133,8 @@ typedef struct {
     int64_t collisions;      /* seeds dropped by AddHashEntry                */
     int64_t matches;         /* entries in the match list                    */
     int64_t repeat_runs;     /* runs above MER_REPEAT_LIMIT                  */
+    int64_t walks;           /* seeds extended by the step-by-step walk      */
+    int64_t long_walks;      /* of those, walks finished per diagonal        */
 } mh_match_stats;
 
 /* Replaces PairwiseMatchFinder/MemHash::FindMatches (LM/MemHash.cpp:109-127,

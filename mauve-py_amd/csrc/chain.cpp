@@ -237,41 +237,96 @@ int mh_recursion_gaps(const char *row_a, const char *row_b, uint64_t cols, int64
                       uint64_t min_gap, mh_anchor_gap *gaps, uint64_t cap, uint64_t *n_gaps)
 {
     MH_CHECK(n_gaps && (gaps || cap == 0) && ((row_a && row_b) || cols == 0), MH_ERR_INVALID, "null argument");
-    // the interval's right ends (RightEnd = LeftEnd + residues - 1)
-    uint64_t ra = 0, rb = 0;
-    for (uint64_t c = 0; c < cols; ++c) {
-        ra += row_a[c] != '-';
-        rb += row_b[c] != '-';
-    }
-    const int64_t right_a = left_a + (int64_t)ra - 1, right_b = left_b + (int64_t)rb - 1;
-    uint64_t ci = 0, cj = 0, pi = 0, pj = 0, ng = 0;
-    bool in_gap = false;
-    for (uint64_t col = 0; col <= cols; ++col) {
-        if (col == cols || (row_a[col] != '-' && row_b[col] != '-')) {
-            if (in_gap && ci - pi > min_gap && cj - pj > min_gap) {
-                // zero-length anchors at the previous aligned column and (unless
-                // a count equals the right end, as the reference compares) at
-                // this one; getInterveningCoordinates turns them into the gap
-                mh_anchor_gap g;
-                g.lend[0] = left_a + (int64_t)pi;
-                g.lend[1] = left_b + (int64_t)pj;
-                const bool right = (int64_t)cj != right_b && (int64_t)ci != right_a;
-                g.rend[0] = right ? left_a + (int64_t)ci : right_a + 1;
-                g.rend[1] = right ? left_b + (int64_t)cj : right_b + 1;
-                if (ng < cap) gaps[ng] = g;
-                ++ng;
+    // The scan below keeps, at each column, the residue counts so far (ci, cj),
+    // the counts at the last column aligned in both rows (pi, pj) and whether
+    // a column since then was not (in_gap).  Chunks of columns first summarise
+    // themselves, the state at each chunk start follows from the summaries,
+    // and then every chunk scans with its own start state.
+    const uint64_t chunk = 1 << 18, nc = (cols + chunk - 1) / chunk;
+    struct Sum {
+        uint64_t ca = 0, cb = 0;          // residues in the chunk
+        bool aligned = false;             // a column aligned in both rows
+        uint64_t pa = 0, pb = 0;          // counts (within the chunk) at the last one
+        bool gap_after = false;           // a column not aligned after it (or at all)
+    };
+    std::vector<Sum> sum(nc);
+    host_parallel(nc, [&](size_t k) {
+        Sum &x = sum[k];
+        const uint64_t hi = std::min(cols, (k + 1) * chunk);
+        for (uint64_t c = k * chunk; c < hi; ++c) {
+            const bool ra = row_a[c] != '-', rb = row_b[c] != '-';
+            if (ra && rb) {
+                x.aligned = true;
+                x.pa = x.ca;
+                x.pb = x.cb;
+                x.gap_after = false;
+            } else {
+                x.gap_after = true;
             }
-            pi = ci;
-            pj = cj;
-            in_gap = false;
+            x.ca += ra;
+            x.cb += rb;
+        }
+    });
+    struct State {
+        uint64_t ci = 0, cj = 0, pi = 0, pj = 0;
+        bool in_gap = false;
+    };
+    std::vector<State> at(nc + 1);
+    for (uint64_t k = 0; k < nc; ++k) {
+        State t = at[k];
+        if (sum[k].aligned) {
+            t.pi = t.ci + sum[k].pa;
+            t.pj = t.cj + sum[k].pb;
+            t.in_gap = sum[k].gap_after;
         } else {
-            in_gap = true;
+            t.in_gap = t.in_gap || sum[k].gap_after;
         }
-        if (col < cols) {
-            ci += row_a[col] != '-';
-            cj += row_b[col] != '-';
-        }
+        t.ci += sum[k].ca;
+        t.cj += sum[k].cb;
+        at[k + 1] = t;
     }
+    // the interval's right ends (RightEnd = LeftEnd + residues - 1)
+    const int64_t right_a = left_a + (int64_t)at[nc].ci - 1, right_b = left_b + (int64_t)at[nc].cj - 1;
+    // a gap closes at an aligned column (or the end) after a run of other
+    // columns longer than min_gap in both rows; zero-length anchors at the
+    // previous aligned column and (unless a count equals the right end, as the
+    // reference compares) at this one, which getInterveningCoordinates turns
+    // into the gap
+    auto close = [&](const State &t, std::vector<mh_anchor_gap> &out) {
+        if (!(t.in_gap && t.ci - t.pi > min_gap && t.cj - t.pj > min_gap)) return;
+        mh_anchor_gap g;
+        g.lend[0] = left_a + (int64_t)t.pi;
+        g.lend[1] = left_b + (int64_t)t.pj;
+        const bool right = (int64_t)t.cj != right_b && (int64_t)t.ci != right_a;
+        g.rend[0] = right ? left_a + (int64_t)t.ci : right_a + 1;
+        g.rend[1] = right ? left_b + (int64_t)t.cj : right_b + 1;
+        out.push_back(g);
+    };
+    std::vector<std::vector<mh_anchor_gap>> found(nc + 1);
+    host_parallel(nc, [&](size_t k) {
+        State t = at[k];
+        const uint64_t hi = std::min(cols, (k + 1) * chunk);
+        for (uint64_t c = k * chunk; c < hi; ++c) {
+            const bool ra = row_a[c] != '-', rb = row_b[c] != '-';
+            if (ra && rb) {
+                close(t, found[k]);
+                t.pi = t.ci;
+                t.pj = t.cj;
+                t.in_gap = false;
+            } else {
+                t.in_gap = true;
+            }
+            t.ci += ra;
+            t.cj += rb;
+        }
+    });
+    close(at[nc], found[nc]);
+    uint64_t ng = 0;
+    for (const auto &v : found)
+        for (const mh_anchor_gap &g : v) {
+            if (ng < cap) gaps[ng] = g;
+            ++ng;
+        }
     *n_gaps = ng;
     return MH_OK;
 }

@@ -1196,19 +1196,9 @@ static void make_items(const mh_dp_job *jobs, uint64_t njobs, const uint32_t *ba
     auto fill = [&](uint64_t lo, uint64_t hi) {
         for (uint64_t j = lo; j < hi; ++j) items[j] = make_item(jobs[j], (uint32_t)j, bands ? bands[j] : 0u, force);
     };
-    unsigned nt = njobs >= 8192 ? std::thread::hardware_concurrency() : 1;
-    if (nt > 8) nt = 8;
-    if (nt <= 1) {
-        fill(0, njobs);
-        return;
-    }
-    std::vector<std::thread> pool;
-    const uint64_t chunk = (njobs + nt - 1) / nt;
-    for (unsigned t = 0; t < nt; ++t) {
-        const uint64_t lo = t * chunk, hi = std::min<uint64_t>(njobs, lo + chunk);
-        if (lo < hi) pool.emplace_back(fill, lo, hi);
-    }
-    for (auto &th : pool) th.join();
+    const uint64_t chunk = 8192;
+    host_parallel((size_t)((njobs + chunk - 1) / chunk),
+                  [&](size_t k) { fill(k * chunk, std::min<uint64_t>(njobs, (k + 1) * chunk)); });
 }
 
 uint64_t dp_item_cells(uint32_t la, uint32_t lb, uint32_t band)

@@ -18,14 +18,22 @@
 // not fit is skipped.
 #include "mh_internal.hpp"
 
+#include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <thread>
+#include <vector>
 
 using namespace mh;
 
 namespace {
 
-inline bool is_ws(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\f' || c == '\v'; }
+// ' ', '\t', '\n', '\v', '\f', '\r' (bits 32 and 9-13)
+inline bool is_ws(char c)
+{
+    const uint32_t u = (uint8_t)c;
+    return u <= 32 && ((0x100003E00ull >> u) & 1ull);
+}
 inline bool is_digit(char c) { return c >= '0' && c <= '9'; }
 
 struct Entry {
@@ -67,7 +75,8 @@ const char *parse_entry(const char *p, const char *end, Entry &e)
     if (q < end && *q != '\n') return nullptr;
     // the sequence: everything up to the next '>' or the group's end
     const char *s = q;
-    while (q < end && *q != '>') ++q;
+    const void *gt = std::memchr(q, '>', (size_t)(end - q));
+    q = gt ? static_cast<const char *>(gt) : end;
     if (q == s) return nullptr;
     e.num = v[0];
     e.start = v[1];
@@ -76,27 +85,36 @@ const char *parse_entry(const char *p, const char *end, Entry &e)
     return q;
 }
 
-// The sequence of an entry without whitespace, into out.
-void squeeze(const Entry &e, std::string &out)
-{
-    out.resize((size_t)(e.seq_end - e.seq));
-    char *w = &out[0];
-    for (const char *c = e.seq; c < e.seq_end; ++c) {
-        *w = *c;
-        w += !is_ws(*c);
-    }
-    out.resize((size_t)(w - out.data()));
-}
+template <class F> void parallel_chunks(uint64_t nc, F f) { host_parallel((size_t)nc, std::function<void(size_t)>(f)); }
 
-template <class F> void parallel_chunks(uint64_t nc, F f)
+// The sequence of an entry without whitespace, into out: per chunk a count of
+// the characters kept, then each chunk copies to its offset.
+struct Row {
+    const char *c;
+    uint64_t n;
+};
+
+Row squeeze(const Entry &e, HostBuf &buf)
 {
-    const unsigned nt = std::max(1u, std::min<unsigned>((unsigned)nc, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; ++t)
-        th.emplace_back([&, t]() {
-            for (uint64_t k = t; k < nc; k += nt) f(k);
-        });
-    for (auto &x : th) x.join();
+    const uint64_t n = (uint64_t)(e.seq_end - e.seq), chunk = 1 << 20, nc = (n + chunk - 1) / chunk;
+    std::vector<uint64_t> at(nc + 1, 0);
+    parallel_chunks(nc, [&](uint64_t k) {
+        const char *c = e.seq + k * chunk, *ce = e.seq + std::min(n, (k + 1) * chunk);
+        uint64_t kept = 0;
+        for (; c < ce; ++c) kept += !is_ws(*c);
+        at[k + 1] = kept;
+    });
+    for (uint64_t k = 0; k < nc; ++k) at[k + 1] += at[k];
+    char *out = buf.get<char>(at[nc]);
+    parallel_chunks(nc, [&](uint64_t k) {
+        const char *c = e.seq + k * chunk, *ce = e.seq + std::min(n, (k + 1) * chunk);
+        char *w = out + at[k];
+        for (; c < ce; ++c) {
+            *w = *c;
+            w += !is_ws(*c);
+        }
+    });
+    return Row{out, at[nc]};
 }
 
 // The column walk of one group (PY/buildindex.py:119-129): each index
@@ -105,9 +123,10 @@ template <class F> void parallel_chunks(uint64_t nc, F f)
 // indices each chunk starts at; a chunk's writes before its first residue of
 // sequence 1 (all-gap columns) reuse the previous chunk's last index and are
 // applied afterwards in column order, so later columns still win.
-void walk(const std::string &gs, const std::string &rs, int64_t g0, int64_t r0, uint64_t genome_length, int32_t *lut)
+void walk(Row gr, Row rr, int64_t g0, int64_t r0, uint64_t genome_length, int32_t *lut)
 {
-    const uint64_t n = gs.size(), chunk = 1 << 18, nc = (n + chunk - 1) / chunk;
+    const char *gs = gr.c, *rs = rr.c;
+    const uint64_t n = gr.n, chunk = 1 << 18, nc = (n + chunk - 1) / chunk;
     auto put = [&](int64_t gi, int64_t ri) {
         // index -1 is the last entry (numpy's negative indexing)
         const int64_t at = gi >= 0 ? gi : (int64_t)genome_length + gi;
@@ -160,18 +179,55 @@ void walk(const std::string &gs, const std::string &rs, int64_t g0, int64_t r0, 
 
 // Starting at a gap start idx whose left neighbour lower = idx-1 is mapped:
 // the first upper in (idx, idx + width] with lut[upper] - lut[lower] ==
-// upper - lower gets [lower, upper] filled with consecutive values.
-inline void fill_from(int32_t *lut, uint64_t n, uint64_t idx, int64_t width)
+// upper - lower, or 0 if none.  Filling sets [lower, upper] to consecutive
+// values (its two ends keep theirs).
+inline uint64_t fill_end(const int32_t *lut, uint64_t n, uint64_t idx, int64_t width)
 {
     const uint64_t lower = idx - 1;
     const int64_t lm = lut[lower];
-    for (uint64_t up = idx + 1; up < n; ++up) {
-        if ((int64_t)(up - idx) > width) return;
-        if ((int64_t)lut[up] - lm == (int64_t)(up - lower)) {
-            for (uint64_t k = lower; k <= up; ++k) lut[k] = (int32_t)(lm + (int64_t)(k - lower));
-            return;
+    const uint64_t stop = std::min<uint64_t>(n, idx + (uint64_t)width + 1);
+    for (uint64_t up = idx + 1; up < stop; ++up)
+        if ((int64_t)lut[up] - lm == (int64_t)(up - lower)) return up;
+    return 0;
+}
+
+// A left-to-right pass that, at every idx where trig(lut, idx) holds, fills
+// from idx as above (indexutils.pyx's fillGaps and smoothEdges).  In the
+// sequential pass the entries a fill writes inside its range no longer
+// trigger, and every trigger after a fill's range, and the fill it finds, read
+// only values no fill has changed (the range ends keep their values).  So the
+// fills are found in parallel on the input, kept left to right unless they
+// start inside a kept fill, and written in parallel (the kept ranges overlap
+// at most at their ends).
+template <class Trig> void fill_pass(int32_t *lut, uint64_t n, int64_t width, Trig trig)
+{
+    if (n < 2) return;
+    const uint64_t chunk = 1 << 18, nc = (n + chunk - 1) / chunk;
+    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> found(nc);
+    parallel_chunks(nc, [&](uint64_t k) {
+        const uint64_t hi = std::min(n, (k + 1) * chunk);
+        for (uint64_t idx = std::max<uint64_t>(1, k * chunk); idx < hi; ++idx)
+            if (trig(lut, idx)) {
+                const uint64_t up = fill_end(lut, n, idx, width);
+                if (up) found[k].push_back({idx - 1, up});
+            }
+    });
+    std::vector<std::pair<uint64_t, uint64_t>> keep;
+    uint64_t last_up = 0;
+    for (const auto &v : found)
+        for (const auto &f : v)
+            if (f.first + 1 > last_up) {
+                keep.push_back(f);
+                last_up = f.second;
+            }
+    const uint64_t per = 4096, nk = (keep.size() + per - 1) / per;
+    parallel_chunks(nk, [&](uint64_t k) {
+        for (uint64_t i = k * per; i < std::min<uint64_t>(keep.size(), (k + 1) * per); ++i) {
+            const uint64_t lower = keep[i].first, up = keep[i].second;
+            const int64_t lm = lut[lower];
+            for (uint64_t x = lower + 1; x < up; ++x) lut[x] = (int32_t)(lm + (int64_t)(x - lower));
         }
-    }
+    });
 }
 
 }  // namespace
@@ -181,11 +237,18 @@ extern "C" {
 int mh_index_from_xmfa(const char *xmfa, uint64_t len, uint64_t genome_length, int32_t *lut, uint64_t *n_groups)
 {
     MH_CHECK((xmfa || len == 0) && (lut || genome_length == 0), MH_ERR_INVALID, "null argument");
-    for (uint64_t k = 0; k < genome_length; ++k) lut[k] = -1;
+    const uint64_t fill_chunk = 1 << 20;
+    parallel_chunks((genome_length + fill_chunk - 1) / fill_chunk, [&](uint64_t k) {
+        const uint64_t hi = std::min(genome_length, (k + 1) * fill_chunk);
+        for (uint64_t x = k * fill_chunk; x < hi; ++x) lut[x] = -1;
+    });
     const char *end = xmfa + len;
     const char *p = xmfa;
     uint64_t groups = 0;
-    std::string gs, rs;
+    // the squeezed rows: buffers kept between calls (one call at a time)
+    static std::mutex mu;
+    static HostBuf rows[2];
+    std::lock_guard<std::mutex> lock(mu);
     while (p < end) {
         // next '>' at a line start
         const char *gt = nullptr;
@@ -226,9 +289,8 @@ int mh_index_from_xmfa(const char *xmfa, uint64_t len, uint64_t genome_length, i
             q = static_cast<const char *>(nl) + 1;
         }
         if (!have_g || !have_r) continue;
-        squeeze(g, gs);
-        squeeze(r, rs);
-        MH_CHECK(rs.size() >= gs.size(), MH_ERR_INVALID, "string index out of range (genome 2 row shorter than genome 1 row)");
+        const Row gs = squeeze(g, rows[0]), rs = squeeze(r, rows[1]);
+        MH_CHECK(rs.n >= gs.n, MH_ERR_INVALID, "string index out of range (genome 2 row shorter than genome 1 row)");
         walk(gs, rs, g.start - 1, r.start - 1, genome_length, lut);
     }
     if (n_groups) *n_groups = groups;
@@ -254,17 +316,17 @@ int mh_index_fix_zero(int32_t *lut, uint64_t n, const char *genome, uint64_t gen
 int mh_index_fill_gaps(int32_t *lut, uint64_t n, int max_gap_width)
 {
     MH_CHECK(lut || n == 0, MH_ERR_INVALID, "null argument");
-    for (uint64_t idx = 1; idx < n; ++idx)
-        if (lut[idx] == -1 && lut[idx - 1] != -1) fill_from(lut, n, idx, max_gap_width);
+    fill_pass(lut, n, max_gap_width,
+              [](const int32_t *t, uint64_t idx) { return t[idx] == -1 && t[idx - 1] != -1; });
     return MH_OK;
 }
 
 int mh_index_smooth_edges(int32_t *lut, uint64_t n, int smoothing_radius)
 {
     MH_CHECK(lut || n == 0, MH_ERR_INVALID, "null argument");
-    for (uint64_t idx = 1; idx < n; ++idx)
-        if ((int64_t)lut[idx] != (int64_t)lut[idx - 1] + 1 && lut[idx - 1] != -1)
-            fill_from(lut, n, idx, smoothing_radius);
+    fill_pass(lut, n, smoothing_radius, [](const int32_t *t, uint64_t idx) {
+        return (int64_t)t[idx] != (int64_t)t[idx - 1] + 1 && t[idx - 1] != -1;
+    });
     return MH_OK;
 }
 

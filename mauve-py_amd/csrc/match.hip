@@ -963,6 +963,8 @@ static int extend_by_chains(mh_context *ctx, const mh_sml *a, const mh_sml *b, c
     uint64_t nl = 0;
     if ((rc = compact_flags(ctx, long_flag, nh, long_idx, g[8].as<uint32_t>(), cnt + 3, &nl)) != MH_OK) return rc;
     if (nl && (rc = resolve_long(ctx, a, b, env, long_idx, nl, hoi, hoj, hres)) != MH_OK) return rc;
+    ctx->last_walks = nh;
+    ctx->last_long_walks = nl;
     hipLaunchKernelGGL(grid_fill_kernel, grid, blk, 0, ctx->stream, oi, oj, ns, a->d_keys, a->d_pos, b->d_keys,
                        b->d_pos, head_of, headpos, hres, d_seeds);
     MH_HIP(hipGetLastError());
@@ -976,6 +978,7 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const See
     *seeds_out = nullptr;
     *ns_out = 0;
     *repeat_runs = 0;
+    ctx->last_walks = ctx->last_long_walks = 0;
     if (n0 == 0 || n1 == 0) return MH_OK;
     MH_CHECK(a->len < (1ull << 31) && b->len < (1ull << 31), MH_ERR_UNSUPPORTED,
              "genomes longer than 2^31 are not supported by the seed matcher");
@@ -1034,6 +1037,7 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const See
                            ns, a->d_keys, a->d_pos, b->d_keys, b->d_pos, a->d_words, b->d_words, wbrc,
                            (int64_t)a->len, (int64_t)b->len, a->seed_length, d_seeds);
         MH_HIP(hipGetLastError());
+        ctx->last_walks = ns;
     } else if (a->len < (1ull << 30) && b->len < (1ull << 30)) {
         TimedLaunch tl(ctx, "extend");
         if ((rc = extend_by_chains(ctx, a, b, oi, oj, ns, env, d_seeds, cnt)) != MH_OK) return rc;
@@ -1043,6 +1047,7 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const See
                            a->d_keys, a->d_pos, b->d_keys, b->d_pos, env, (int64_t)0, d_seeds, cnt + 2,
                            (uint32_t *)nullptr);
         MH_HIP(hipGetLastError());
+        ctx->last_walks = ns;
     }
     uint32_t ext_err = 0;
     MH_HIP(hipMemcpyAsync(&ext_err, cnt + 2, 4, hipMemcpyDeviceToHost, ctx->stream));

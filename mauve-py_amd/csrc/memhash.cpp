@@ -99,6 +99,8 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
     if (st) {
         std::memset(st, 0, sizeof(*st));
         st->repeat_runs = reps;
+        st->walks = (int64_t)ctx->last_walks;
+        st->long_walks = (int64_t)ctx->last_long_walks;
     }
     MH_CHECK(reps == 0, MH_ERR_UNSUPPORTED,
              "a seed run exceeds MER_REPEAT_LIMIT (1000); the reference's restart after such runs "
@@ -156,14 +158,8 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
         }
         collisions += coll;
     };
-    unsigned nt = std::thread::hardware_concurrency();
-    if (nt == 0) nt = 1;
-    if (nt > 16) nt = 16;
-    if (nt > busy.size()) nt = (unsigned)std::max<size_t>(1, busy.size());
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
-    work();
-    for (auto &th : pool) th.join();
+    const unsigned nt = (unsigned)std::min<size_t>(host_threads(), std::max<size_t>(1, busy.size()));
+    host_parallel(nt, [&](size_t) { work(); });
     const double tr1 = now_ms();
     host_time(ctx, "host_replay", tr1 - tg1);
     // GetMatchList order: bucket by bucket (LM/MemHash.h:182-203); clean and

@@ -10,6 +10,8 @@
 #include <map>
 #include "mauvehip.h"
 
+#include <functional>
+
 // A match list owned by the caller: matches in GetMatchList order, and the
 // sequence names and lengths a match-list file carries.
 struct mh_match_list {
@@ -105,6 +107,7 @@ struct mh_context {
     mh::DevBuf mf_scratch[16];
     mh::DevBuf grid_scratch[11];   // spaced-seed extension by chains
     mh::DevBuf diag_scratch[8];    // long extensions resolved per diagonal
+    uint64_t last_walks = 0, last_long_walks = 0;   // of the last seed extension
     // scratch for sorting
     mh::DevBuf sort_tmp[4];
     // pinned host staging for device-to-host transfers
@@ -147,6 +150,11 @@ int collect_timers(mh_context *ctx);
 // Host-side stage timing folded into the same table (wall clock, ms).
 void host_time(mh_context *ctx, const char *name, double ms);
 double now_ms();
+// Runs f(0) .. f(n-1) on the library's host thread pool (the caller takes
+// part); tasks are handed out in order as threads free up.  A call made while
+// the pool is busy (from a task, or another thread) runs its tasks itself.
+void host_parallel(size_t n, const std::function<void(size_t)> &f);
+unsigned host_threads();
 // Pinned host staging buffer of at least `bytes` (owned by the context).
 int pinned(mh_context *ctx, size_t bytes, void **out);
 

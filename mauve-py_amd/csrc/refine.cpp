@@ -63,35 +63,52 @@ inline unsigned letter(char c)
     }
 }
 
+// Column classes: 0-3 the letters A C G T (either case), GAPC MUSCLE's gap
+// characters '-' and '.', anything else OTHER.
+constexpr uint8_t GAPC = 4, OTHER = 5;
+struct ColumnCodes {
+    uint8_t v[256];
+    float mm[6][6];   // the letter score of a column (0 unless both are letters)
+    ColumnCodes()
+    {
+        for (int c = 0; c < 256; ++c) {
+            const unsigned l = letter((char)c);
+            v[c] = l < 4 ? (uint8_t)l : mu_gap((char)c) ? GAPC : OTHER;
+        }
+        for (int x = 0; x < 6; ++x)
+            for (int y = 0; y < 6; ++y) mm[x][y] = x < 4 && y < 4 ? (float)HOXD[x][y] + 60.0f : 0.0f;
+    }
+};
+const ColumnCodes CODES;
+
 // LetterObjScoreXP's per-column score for two single-sequence profiles
 // (weights 1): the NUC_SP letter score, plus each gap's penalty spread
-// evenly over the gap's columns (MU/anchoredpp.cpp:16-316).
+// evenly over the gap's columns (MU/anchoredpp.cpp:16-316).  A column is
+// one or the other: letter-score columns carry no gap share, and the columns
+// a gap's share covers (from its first column to the next column of two
+// residues, all-gap columns included) have no letter score.
 void column_scores(const char *r1, const char *r2, unsigned n, std::vector<float> &score)
 {
-    std::vector<float> mm(n, 0.0f), gg(n, 0.0f);
+    score.assign(n, 0.0f);
+    float *out = score.data();
+    const uint8_t *code = CODES.v;
     unsigned start = 0, end = n - 1;
     for (unsigned c = 0; c < n; ++c)
-        if (!(mu_gap(r1[c]) && mu_gap(r2[c]))) {
+        if (!(code[(uint8_t)r1[c]] == GAPC && code[(uint8_t)r2[c]] == GAPC)) {
             start = c;
             break;
         }
     for (int c = (int)n - 1; c >= 0; --c)
-        if (!(mu_gap(r1[c]) && mu_gap(r2[c]))) {
+        if (!(code[(uint8_t)r1[c]] == GAPC && code[(uint8_t)r2[c]] == GAPC)) {
             end = (unsigned)c;
             break;
         }
-    for (unsigned c = start; c <= end; ++c) {
-        const unsigned a = letter(r1[c]);
-        if (a >= 4) continue;
-        const unsigned b = letter(r2[c]);
-        if (b >= 4) continue;
-        mm[c] = (float)HOXD[a][b] + 60.0f;
-    }
     bool g1 = false, g2 = false;
     unsigned left = 0;
     float cur = 0.0f;
     for (unsigned c = start; c <= end; ++c) {
-        const bool x = mu_gap(r1[c]), y = mu_gap(r2[c]);
+        const uint8_t a = code[(uint8_t)r1[c]], b = code[(uint8_t)r2[c]];
+        const bool x = a == GAPC, y = b == GAPC;
         if (x && y) continue;
         if (x || y) {
             bool &gapping = x ? g1 : g2;
@@ -106,20 +123,19 @@ void column_scores(const char *r1, const char *r2, unsigned n, std::vector<float
         }
         if (g1 || g2) {
             const float per = cur / (float)(c - left);
-            for (unsigned k = left; k < c; ++k) gg[k] = per;
+            for (unsigned k = left; k < c; ++k) out[k] = 1.0f * (0.0f + per);
             left = INSANE;
             cur = 0.0f;
         }
         g1 = g2 = false;
+        out[c] = 1.0f * (CODES.mm[a][b] + 0.0f);
     }
     if (g1 || g2) {
         cur -= GAP_OPEN;
         cur += TERM_GAP;
         const float per = cur / (float)(n - left);
-        for (unsigned k = left; k < n; ++k) gg[k] = per;
+        for (unsigned k = left; k < n; ++k) out[k] = 1.0f * (0.0f + per);
     }
-    score.assign(n, 0.0f);
-    for (unsigned c = 0; c < n; ++c) score[c] += 1.0f * (mm[c] + gg[c]);
 }
 
 // WindowSmooth, MU/anchors.cpp:9-48.
@@ -149,10 +165,11 @@ void refine_anchor_cols(const char *r1, const char *r2, unsigned n, std::vector<
 {
     anchors.clear();
     if (n == 0) return;
-    std::vector<float> score, smooth;
+    thread_local std::vector<float> score, smooth;
+    thread_local std::vector<unsigned> best;
+    best.clear();
     column_scores(r1, r2, n, score);
     window_smooth(score, smooth);
-    std::vector<unsigned> best;
     for (unsigned i = 0; i < n; ++i) {
         if ((double)score[i] < MIN_BEST_COL_SCORE) continue;
         if ((double)smooth[i] < MIN_SMOOTH_SCORE) continue;
@@ -196,22 +213,7 @@ struct RefinePiece {
 
 namespace {
 
-template <class F> void parallel_for(size_t n, F f)
-{
-    if (n == 0) return;
-    const size_t nt = std::max<size_t>(1, std::min<size_t>(n, std::max(1u, std::thread::hardware_concurrency())));
-    if (nt == 1) {
-        for (size_t i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::vector<std::thread> th;
-    std::atomic<size_t> next(0);
-    for (size_t t = 0; t < nt; ++t)
-        th.emplace_back([&]() {
-            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-        });
-    for (auto &x : th) x.join();
-}
+template <class F> void parallel_for(size_t n, F f) { host_parallel(n, std::function<void(size_t)>(f)); }
 
 // Counts of residues ('-' is the only gap in Mauve's GappedAlignment) in
 // each row before every column, so a piece's lengths are two subtractions.

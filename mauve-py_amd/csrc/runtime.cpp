@@ -2,6 +2,10 @@
 // C-ABI entry points that are not tied to one kernel family.
 #include "mh_internal.hpp"
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <cstring>
 #include <cmath>
@@ -180,6 +184,90 @@ int seed_length(int64_t seed)
 }
 
 int seed_weight(int64_t seed) { return __builtin_popcountll((uint64_t)seed); }
+
+namespace {
+
+// Worker threads wait for a job generation; each takes task indices from a
+// shared counter.  The caller clears the job only once no worker is inside it,
+// and a worker that wakes after that sees no job.
+class ThreadPool {
+public:
+    explicit ThreadPool(unsigned nt)
+    {
+        for (unsigned t = 1; t < nt; ++t) workers_.emplace_back([this]() { loop(); });
+    }
+    ~ThreadPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &w : workers_) w.join();
+    }
+    unsigned size() const { return (unsigned)workers_.size() + 1; }
+    void run(size_t n, const std::function<void(size_t)> &f)
+    {
+        std::unique_lock<std::mutex> busy(call_mu_, std::try_to_lock);
+        if (!busy.owns_lock() || workers_.empty() || n < 2) {
+            for (size_t i = 0; i < n; ++i) f(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &f;
+            n_ = n;
+            next_.store(0);
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (size_t i; (i = next_.fetch_add(1)) < n;) f(i);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&]() { return active_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void loop()
+    {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&]() { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            const std::function<void(size_t)> *f = job_;
+            if (!f) continue;
+            const size_t n = n_;
+            ++active_;
+            lk.unlock();
+            for (size_t i; (i = next_.fetch_add(1)) < n;) (*f)(i);
+            lk.lock();
+            if (--active_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_, call_mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(size_t)> *job_ = nullptr;
+    size_t n_ = 0;
+    std::atomic<size_t> next_{0};
+    int active_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+ThreadPool &pool()
+{
+    static ThreadPool p(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    return p;
+}
+
+}  // namespace
+
+void host_parallel(size_t n, const std::function<void(size_t)> &f) { pool().run(n, f); }
+
+unsigned host_threads() { return pool().size(); }
 
 }  // namespace mh
 
@@ -369,19 +457,9 @@ int mh_dp_band_cells(const uint32_t *a_len, const uint32_t *b_len, const uint32_
     auto fill = [&](uint64_t lo, uint64_t hi) {
         for (uint64_t x = lo; x < hi; ++x) cells[x] = dp_item_cells(a_len[x], b_len[x], band ? band[x] : 0u);
     };
-    unsigned nt = n >= 65536 ? std::thread::hardware_concurrency() : 1;
-    if (nt > 16) nt = 16;
-    if (nt <= 1) {
-        fill(0, n);
-        return MH_OK;
-    }
-    std::vector<std::thread> pool;
-    const uint64_t chunk = (n + nt - 1) / nt;
-    for (unsigned t = 0; t < nt; ++t) {
-        const uint64_t lo = t * chunk, hi = std::min<uint64_t>(n, lo + chunk);
-        if (lo < hi) pool.emplace_back(fill, lo, hi);
-    }
-    for (auto &th : pool) th.join();
+    const uint64_t chunk = 65536;
+    mh::host_parallel((size_t)((n + chunk - 1) / chunk),
+                      [&](size_t k) { fill(k * chunk, std::min<uint64_t>(n, (k + 1) * chunk)); });
     return MH_OK;
 }
 

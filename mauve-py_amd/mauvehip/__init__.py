@@ -98,7 +98,7 @@ class Match(C.Structure):
 
 class MatchStats(C.Structure):
     _fields_ = [("seed_tuples", C.c_int64), ("collisions", C.c_int64), ("matches", C.c_int64),
-                ("repeat_runs", C.c_int64)]
+                ("repeat_runs", C.c_int64), ("walks", C.c_int64), ("long_walks", C.c_int64)]
 
 
 DP_JOB_DTYPE = np.dtype([("a_off", "<u8"), ("b_off", "<u8"), ("a_len", "<u4"), ("b_len", "<u4")])

@@ -60,9 +60,12 @@ def align_pair(ctx, d_seqs, seqs, seed, xmfa_path, names, recursive=True, refine
     t0 = mark("chain", t0)
     n_rec = 0
     if recursive:
+        t1 = t()
         gaps = recursion_gaps(ra, rb, 1, 1, MIN_RECURSIVE_GAP)
+        t1 = mark("recurse.gaps", t1)
         if len(gaps):
             gs, gl, _ = anchor_gaps_device(ctx, d0, L0, d1, L1, gaps)
+            t1 = mark("recurse.search", t1)
             n_rec = len(gl)
             if n_rec:
                 extra = np.zeros(n_rec, dtype=MATCH_DTYPE)
@@ -71,6 +74,7 @@ def align_pair(ctx, d_seqs, seqs, seed, xmfa_path, names, recursive=True, refine
                 mt = np.concatenate([mt, extra])
                 chain, _ = chain_gap_jobs_raw(mt, L0, L1, 0)
                 ra, rb = chain_rows(mt, chain, s0, s1)
+                t1 = mark("recurse.chain", t1)
         t0 = mark("recurse", t0)
     if refine:
         (ra, rb), = refine_pairwise(ctx, [(ra, rb)], stats=refine_stats)

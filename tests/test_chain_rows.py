@@ -63,3 +63,21 @@ def test_recursion_gaps_match_oracle(seed):
     got = mauvehip.recursion_gaps(ra, rb, la, lb, mg)
     want = po.recursion_gaps(ra, rb, la, lb, mg)
     assert [(tuple(int(x) for x in g["lend"]), tuple(int(x) for x in g["rend"])) for g in got] == want
+
+
+def test_recursion_gaps_across_chunks():
+    """Rows longer than the library's 2^18-column scan chunks: gaps that open
+    in one chunk and close in another, one spanning a whole chunk, and an
+    aligned column right at a chunk border."""
+    n = 700_000
+    ra = bytearray(b"A" * n)
+    rb = bytearray(b"A" * n)
+    for lo, hi, row in [(262_000, 262_500, rb), (262_500, 263_000, ra), (300_000, 524_400, rb),
+                        (524_400, 524_700, ra), (600_000, 600_250, ra), (600_250, 600_500, rb)]:
+        row[lo:hi] = b"-" * (hi - lo)
+    ra[524_288] = ord("-")
+    ra, rb = bytes(ra), bytes(rb)
+    got = mauvehip.recursion_gaps(ra, rb, 5, 9, 200)
+    want = po.recursion_gaps(ra.decode(), rb.decode(), 5, 9, 200)
+    assert len(want) >= 3
+    assert [((g["lend"][0], g["lend"][1]), (g["rend"][0], g["rend"][1])) for g in got] == want

@@ -24,6 +24,7 @@ def _compare(ctx, a, b, seed):
     assert len(gl) == len(ol)
     assert np.array_equal(gs, os_)
     assert np.array_equal(gl, ol)
+    _compare.last_stats = gst
     return len(gl)
 
 
@@ -68,6 +69,10 @@ def test_match_recoded_4_6Mb(ctx):
     ref, mut, _ = synth.pair_recoded()
     n = _compare(ctx, ref.tobytes(), mut.tobytes(), po.get_seed(17, 3))
     assert n > 10000
+    # the coding seed bridges recoded codons, so walks over whole ORFs: both
+    # the shared walks along seed chains and the per-diagonal finish are used
+    st = _compare.last_stats
+    assert 0 < st.long_walks < st.walks < st.seed_tuples
 
 
 @pytest.mark.parametrize("seed", [po.get_seed(17, 3), po.get_seed(15, 3)])

@@ -187,3 +187,25 @@ def test_long_rows_walked_in_chunks():
     body = lambda s: "\n".join(s[k:k + 80] for k in range(0, len(s), 80))
     text = f"> 1:5-{n} + g\n{body(gs)}\n> 2:9-{n} + r\n{body(rs)}\n=\n"
     _both_walks(text, n + 10)
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_passes_across_chunks(seed):
+    """fillGaps and smoothEdges over more than 2^18 entries (the library finds
+    fills in parallel chunks and keeps them in order): gaps and stray entries
+    placed at and around the chunk borders, where a fill found in one chunk
+    covers the first triggers of the next."""
+    rng = np.random.default_rng(300 + seed)
+    n = 700_000
+    lut = _random_lut(rng, n)
+    for b in (262144, 524288):
+        lut[b - 5:b + 7] = -1
+        lut[b + 9] = int(rng.integers(0, 5 * n))
+    a = lut.copy()
+    mi.fillGaps(a)
+    b = lut.tolist()
+    io.fill_gaps(b, 300)
+    assert a.tolist() == b
+    mi.smoothEdges(a)
+    io.smooth_edges(b, 20)
+    assert a.tolist() == b
