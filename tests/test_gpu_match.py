@@ -69,8 +69,26 @@ def test_match_recoded_4_6Mb(ctx):
     ref, mut, _ = synth.pair_recoded()
     n = _compare(ctx, ref.tobytes(), mut.tobytes(), po.get_seed(17, 3))
     assert n > 10000
-    # the coding seed bridges recoded codons, so walks over whole ORFs: both
-    # the shared walks along seed chains and the per-diagonal finish are used
+    st = _compare.last_stats          # a solid seed: every seed walks on its own
+    assert st.walks == st.seed_tuples and st.long_walks == 0
+
+
+@pytest.mark.parametrize("seed", [po.get_seed(15, 0), po.get_seed(11, 0)])
+def test_match_spaced_long_walks(ctx, seed):
+    """Spaced seeds over long identical and near-identical stretches, forward
+    and inverted: the seeds of one chain (one seed length apart on a diagonal)
+    share one walk, and walks past the step budget finish per diagonal."""
+    rng = np.random.default_rng(seed & 0xFFFF)
+    a = synth.BASES[rng.integers(0, 4, 120_000)]
+    b = a.copy()
+    snp = rng.random(len(b)) < 0.02
+    snp[:30_000] = False                                  # identical
+    snp[60_000:90_000] = False                            # identical, inverted below
+    b[snp] = synth.BASES[(np.searchsorted(synth.BASES, b[snp]) + rng.integers(1, 4, snp.sum())) % 4]
+    b[60_000:120_000] = synth.revcomp(b[60_000:120_000])
+    b = np.concatenate([b[:45_000], b[45_012:100_000], synth.BASES[rng.integers(0, 4, 7)], b[100_000:]])
+    n = _compare(ctx, a.tobytes(), b.tobytes(), seed)
+    assert n >= 5
     st = _compare.last_stats
     assert 0 < st.long_walks < st.walks < st.seed_tuples
 
