@@ -33,7 +33,7 @@ constexpr int JT = 256;              // threads per join/compaction tile
 constexpr int JITEMS = 16;
 constexpr int JTILE = JT * JITEMS;
 constexpr uint32_t MER_REPEAT_LIMIT = 1000;
-constexpr int64_t LONG_WALK = 256;   // steps of a head's walk before it is resolved per diagonal
+constexpr int64_t LONG_WALK = 64;   // steps of a head's walk before it is resolved per diagonal
 
 __device__ __forceinline__ uint64_t lower_bound_m(const uint64_t *__restrict__ ck, uint64_t n, uint64_t m)
 {

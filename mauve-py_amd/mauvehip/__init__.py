@@ -642,9 +642,10 @@ def anchor_gaps_device(ctx, d_a, len_a, d_b, len_b, gaps):
     return out["start"][:, :2].copy(), out["length"].copy(), first
 
 
-def chain_rows(matches, chain, seq_a, seq_b):
-    """mh_chain_rows: (row_a, row_b) as bytes for a chain of MATCH_DTYPE
-    matches (chain = indices, as chain_gap_jobs_raw returns them)."""
+def chain_rows(matches, chain, seq_a, seq_b, as_array=False):
+    """mh_chain_rows: (row_a, row_b) as bytes (uint8 arrays if as_array) for a
+    chain of MATCH_DTYPE matches (chain = indices, as chain_gap_jobs_raw
+    returns them)."""
     L = load_library()
     m = np.ascontiguousarray(matches, dtype=MATCH_DTYPE)
     ch = np.ascontiguousarray(chain, dtype=np.uint32)
@@ -657,19 +658,21 @@ def chain_rows(matches, chain, seq_a, seq_b):
     _check(L.mh_chain_rows(m.ctypes.data, ch.ctypes.data, len(ch), a, len(a), b, len(b), ra.ctypes.data,
                            rb.ctypes.data, cap, C.byref(cols)))
     n = cols.value
+    if as_array:
+        return ra[:n], rb[:n]
     return ra[:n].tobytes(), rb[:n].tobytes()
 
 
 def recursion_gaps(row_a, row_b, left_a=1, left_b=1, min_gap=200):
     """mh_recursion_gaps: ANCHOR_GAP_DTYPE records of the gaps searched recursively."""
     L = load_library()
-    a, b = _row_bytes(row_a), _row_bytes(row_b)
-    if len(a) != len(b):
+    (a, na, ka), (b, nb, kb) = _row_ptr(row_a), _row_ptr(row_b)
+    if na != nb:
         raise ValueError("rows differ in length")
     n = C.c_uint64()
     # each gap spans more than min_gap columns: this many always suffice
-    out = np.zeros(len(a) // (min_gap + 1) + 2, dtype=ANCHOR_GAP_DTYPE)
-    _check(L.mh_recursion_gaps(a, b, len(a), left_a, left_b, min_gap, out.ctypes.data, len(out), C.byref(n)))
+    out = np.zeros(na // (min_gap + 1) + 2, dtype=ANCHOR_GAP_DTYPE)
+    _check(L.mh_recursion_gaps(a, b, na, left_a, left_b, min_gap, out.ctypes.data, len(out), C.byref(n)))
     return out[:n.value]
 
 
@@ -699,6 +702,15 @@ def _row_bytes(r):
     return r.encode() if isinstance(r, str) else bytes(r)
 
 
+def _row_ptr(r):
+    """(char pointer, length, object to keep alive) for a row given as str,
+    bytes or a contiguous uint8 array (arrays are passed without a copy)."""
+    if isinstance(r, np.ndarray) and r.dtype == np.uint8 and r.ndim == 1 and r.flags.c_contiguous:
+        return C.c_char_p(r.ctypes.data if len(r) else None), len(r), r
+    b = _row_bytes(r)
+    return b, len(b), b
+
+
 def refine_anchor_cols(row1, row2):
     """FindAnchorColsPP over two aligned rows (host part of mh_refine_pairwise)."""
     L = load_library()
@@ -724,25 +736,29 @@ def refine_pieces(row1, row2):
     return [(int(out[3 * k]), int(out[3 * k + 1]), bool(out[3 * k + 2])) for k in range(n.value)]
 
 
-def refine_pairwise(ctx, intervals, passes=REFINE_PROFILE | REFINE_WINDOWS, stats=None):
-    """mh_refine_pairwise over [(row1, row2)]: the refined row pairs (bytes).
-    A RefineStats passed as stats receives the DP cells of each pass."""
+def refine_pairwise(ctx, intervals, passes=REFINE_PROFILE | REFINE_WINDOWS, stats=None, as_array=False):
+    """mh_refine_pairwise over [(row1, row2)]: the refined row pairs (bytes;
+    uint8 arrays if as_array).  A RefineStats passed as stats receives the DP
+    cells of each pass."""
     L = load_library()
     n = len(intervals)
-    r1 = [_row_bytes(a) for a, _ in intervals]
-    r2 = [_row_bytes(b) for _, b in intervals]
+    r1 = [_row_ptr(a) for a, _ in intervals]
+    r2 = [_row_ptr(b) for _, b in intervals]
     for a, b in zip(r1, r2):
-        if len(a) != len(b):
+        if a[1] != b[1]:
             raise ValueError("rows of an interval differ in length")
-    cols = (C.c_uint64 * max(1, n))(*[len(a) for a in r1])
-    caps = (C.c_uint64 * max(1, n))(*[2 * len(a) + 1 for a in r1])
-    o1 = [np.empty(2 * len(a) + 1, dtype=np.uint8) for a in r1]
-    o2 = [np.empty(2 * len(a) + 1, dtype=np.uint8) for a in r1]
+    cols = (C.c_uint64 * max(1, n))(*[a[1] for a in r1])
+    caps = (C.c_uint64 * max(1, n))(*[2 * a[1] + 1 for a in r1])
+    o1 = [np.empty(2 * a[1] + 1, dtype=np.uint8) for a in r1]
+    o2 = [np.empty(2 * a[1] + 1, dtype=np.uint8) for a in r1]
     outc = (C.c_uint64 * max(1, n))()
-    _check(L.mh_refine_pairwise(ctx.h, n, (C.c_char_p * max(1, n))(*r1), (C.c_char_p * max(1, n))(*r2), cols,
+    _check(L.mh_refine_pairwise(ctx.h, n, (C.c_char_p * max(1, n))(*[a[0] for a in r1]),
+                                (C.c_char_p * max(1, n))(*[b[0] for b in r2]), cols,
                                 passes, (C.c_void_p * max(1, n))(*[b.ctypes.data for b in o1]),
                                 (C.c_void_p * max(1, n))(*[b.ctypes.data for b in o2]), caps, outc,
                                 C.byref(stats) if stats is not None else None))
+    if as_array:
+        return [(o1[i][:outc[i]], o2[i][:outc[i]]) for i in range(n)]
     return [(o1[i][:outc[i]].tobytes(), o2[i][:outc[i]].tobytes()) for i in range(n)]
 
 
@@ -763,9 +779,9 @@ def write_xmfa(path, intervals, seq_files, backbone_file=None):
         if len(iv) != k:
             raise ValueError("every interval needs one row per genome")
         for s, (start, length, row) in enumerate(iv):
-            b = row.encode() if isinstance(row, str) else bytes(row)
-            keep.append(b)
-            rows[i * k + s] = XmfaRow(int(start), int(length), b, len(b))
+            b, nb, kb = _row_ptr(row)
+            keep.append(kb)
+            rows[i * k + s] = XmfaRow(int(start), int(length), b, nb)
     files = (C.c_char_p * k)(*[os.fsencode(f) for f in seq_files])
     _check(L.mh_xmfa_write(os.fsencode(path), k, files, os.fsencode(backbone_file) if backbone_file else None,
                            C.cast(rows, C.c_void_p), len(intervals)))

@@ -36,8 +36,9 @@ def align_pair(ctx, d_seqs, seqs, seed, xmfa_path, names, recursive=True, refine
                refine_stats=None):
     """Align genome 1 (seqs[0]) with genome 2 (seqs[1]); d_seqs are the same
     genomes in device memory (ASCII).  Writes xmfa_path and returns
-    (row_1, row_2, lut or None, info); timings (dict) accumulates seconds per
-    stage, refine_stats (RefineStats) receives the DP cells of the passes."""
+    (row_1, row_2, lut or None, info), the rows as uint8 arrays; timings
+    (dict) accumulates seconds per stage, refine_stats (RefineStats) receives
+    the DP cells of the passes."""
     t = time.perf_counter
     tm = timings if timings is not None else {}
 
@@ -56,7 +57,7 @@ def align_pair(ctx, d_seqs, seqs, seed, xmfa_path, names, recursive=True, refine
     sb.free()
     t0 = mark("match", t0)
     chain, _ = chain_gap_jobs_raw(mt, L0, L1, 0)
-    ra, rb = chain_rows(mt, chain, s0, s1)
+    ra, rb = chain_rows(mt, chain, s0, s1, as_array=True)
     t0 = mark("chain", t0)
     n_rec = 0
     if recursive:
@@ -73,11 +74,11 @@ def align_pair(ctx, d_seqs, seqs, seed, xmfa_path, names, recursive=True, refine
                 extra["length"] = gl
                 mt = np.concatenate([mt, extra])
                 chain, _ = chain_gap_jobs_raw(mt, L0, L1, 0)
-                ra, rb = chain_rows(mt, chain, s0, s1)
+                ra, rb = chain_rows(mt, chain, s0, s1, as_array=True)
                 t1 = mark("recurse.chain", t1)
         t0 = mark("recurse", t0)
     if refine:
-        (ra, rb), = refine_pairwise(ctx, [(ra, rb)], stats=refine_stats)
+        (ra, rb), = refine_pairwise(ctx, [(ra, rb)], stats=refine_stats, as_array=True)
         t0 = mark("refine", t0)
     write_xmfa(xmfa_path, [[(1, L0, ra), (1, L1, rb)]], names)
     t0 = mark("xmfa", t0)

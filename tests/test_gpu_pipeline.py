@@ -39,10 +39,12 @@ def test_pipeline_stages(ctx, pair, tmp_path):
     names = ["mut.fa", "ref.fa"]
     ua, ub, _, info0 = pipeline.align_pair(ctx, d, (g, r), seed, str(tmp_path / "u.xmfa"), names, refine=False,
                                            lut=False)
+    ua, ub = ua.tobytes(), ub.tobytes()
     assert ua.replace(b"-", b"") == g and ub.replace(b"-", b"") == r
     assert info0["chain_anchors"] > 100
     tm = {}
     ra, rb, lut, info = pipeline.align_pair(ctx, d, (g, r), seed, str(tmp_path / "a.xmfa"), names, timings=tm)
+    ra, rb = ra.tobytes(), rb.tobytes()
     assert {k for k in tm if "." not in k} == {"sml", "match", "chain", "recurse", "refine", "xmfa", "lut"}
     assert {k for k in tm if "." in k} <= {"recurse.gaps", "recurse.search", "recurse.chain"}
     assert ra.replace(b"-", b"") == g and rb.replace(b"-", b"") == r
