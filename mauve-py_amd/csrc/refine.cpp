@@ -215,23 +215,43 @@ namespace {
 
 template <class F> void parallel_for(size_t n, F f) { host_parallel(n, std::function<void(size_t)>(f)); }
 
-// Counts of residues ('-' is the only gap in Mauve's GappedAlignment) in
-// each row before every column, so a piece's lengths are two subtractions.
-// Built in parallel chunks.
+// Gaps ('-', the only gap in Mauve's GappedAlignment) among the 8 bytes at p.
+inline unsigned gaps8(const char *p)
+{
+    uint64_t v;
+    std::memcpy(&v, p, 8);
+    v ^= 0x2D2D2D2D2D2D2D2Dull;                       // '-' bytes become zero
+    const uint64_t lo7 = 0x7F7F7F7F7F7F7F7Full;
+    const uint64_t t = ~(((v & lo7) + lo7) | v | lo7);  // high bit of each zero byte
+    return (unsigned)__builtin_popcountll(t);
+}
+
+// Residue counts of the two rows before any column, from counts kept every 64
+// columns and a count inside the block.  Built in parallel chunks.
 struct Prefix {
-    uint32_t *c1 = nullptr, *c2 = nullptr;    // n + 1 entries each, caller's memory
+    const char *r[2] = {nullptr, nullptr};
+    uint32_t *blk[2] = {nullptr, nullptr};    // n / 64 + 1 entries each, caller's memory
+    static uint64_t entries(uint64_t n) { return n / 64 + 2; }
     void build(const char *r1, const char *r2, uint64_t n, uint32_t *m1, uint32_t *m2)
     {
-        c1 = m1;
-        c2 = m2;
-        const uint64_t chunk = 1 << 18, nc = (n + chunk - 1) / chunk;
+        r[0] = r1;
+        r[1] = r2;
+        blk[0] = m1;
+        blk[1] = m2;
+        const uint64_t nb = n / 64, chunk = 1 << 12, nc = (nb + chunk - 1) / chunk;   // chunks of blocks
         std::vector<uint32_t> t1(nc + 1, 0), t2(nc + 1, 0);
         parallel_for(nc, [&](size_t k) {
-            const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
             uint32_t a = 0, b = 0;
-            for (uint64_t c = lo; c < hi; ++c) {
-                a += r1[c] != '-';
-                b += r2[c] != '-';
+            for (uint64_t q = k * chunk; q < std::min(nb, (k + 1) * chunk); ++q) {
+                m1[q] = a;
+                m2[q] = b;
+                unsigned g1 = 0, g2 = 0;
+                for (int x = 0; x < 64; x += 8) {
+                    g1 += gaps8(r1 + q * 64 + x);
+                    g2 += gaps8(r2 + q * 64 + x);
+                }
+                a += 64 - g1;
+                b += 64 - g2;
             }
             t1[k + 1] = a;
             t2[k + 1] = b;
@@ -241,20 +261,25 @@ struct Prefix {
             t2[k + 1] += t2[k];
         }
         parallel_for(nc, [&](size_t k) {
-            const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
-            uint32_t a = t1[k], b = t2[k];
-            for (uint64_t c = lo; c < hi; ++c) {
-                c1[c] = a;
-                c2[c] = b;
-                a += r1[c] != '-';
-                b += r2[c] != '-';
+            for (uint64_t q = k * chunk; q < std::min(nb, (k + 1) * chunk); ++q) {
+                m1[q] += t1[k];
+                m2[q] += t2[k];
             }
         });
-        c1[n] = t1[nc];
-        c2[n] = t2[nc];
+        m1[nb] = t1[nc];
+        m2[nb] = t2[nc];
     }
-    uint64_t l1(uint64_t lo, uint64_t hi) const { return c1[hi] - c1[lo]; }
-    uint64_t l2(uint64_t lo, uint64_t hi) const { return c2[hi] - c2[lo]; }
+    uint64_t count(int g, uint64_t x) const
+    {
+        const uint64_t q = x / 64;
+        const char *p = r[g] + q * 64;
+        unsigned m = (unsigned)(x - q * 64), res = 0;
+        for (; m >= 8; m -= 8, p += 8) res += 8 - gaps8(p);
+        for (; m > 0; --m, ++p) res += *p != '-';
+        return blk[g][q] + res;
+    }
+    uint64_t l1(uint64_t lo, uint64_t hi) const { return count(0, hi) - count(0, lo); }
+    uint64_t l2(uint64_t lo, uint64_t hi) const { return count(1, hi) - count(1, lo); }
 };
 
 // IsDenseEnough (LM/ProgressiveAligner.cpp:1015-1038) from a piece's residue
@@ -464,14 +489,14 @@ void plan_piece(const char *r1, const char *r2, const Prefix &pf, bool profile, 
 // Residue codes of row r in [lo, hi) into out; false on a letter outside ACGT.
 bool put_codes(const char *r, uint64_t lo, uint64_t hi, uint8_t *out)
 {
-    bool ok = true;
+    unsigned bad = 0;
     for (uint64_t c = lo; c < hi; ++c) {
         if (r[c] == '-') continue;
-        const unsigned l = letter(r[c]);
-        ok &= l < 4;
+        const uint8_t l = CODES.v[(uint8_t)r[c]];
+        bad |= l >= 4;
         *out++ = (uint8_t)(l & 3);
     }
-    return ok;
+    return !bad;
 }
 
 // One pass over all intervals: plan (in parallel over pieces), one batched DP
@@ -493,13 +518,13 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
     std::vector<char> whole(n, 0);
     {
         uint64_t tot = 0;
-        for (size_t i = 0; i < n; ++i) tot += lens[i] + 1;
+        for (size_t i = 0; i < n; ++i) tot += Prefix::entries(lens[i]);
         uint32_t *m1 = ctx->refine_prefix[0].get<uint32_t>(tot), *m2 = ctx->refine_prefix[1].get<uint32_t>(tot);
         MH_CHECK(m1 && m2, MH_ERR_NOMEM, "host allocation failed");
         for (size_t i = 0; i < n; ++i) {
             pf[i].build(in1[i], in2[i], lens[i], m1, m2);
-            m1 += lens[i] + 1;
-            m2 += lens[i] + 1;
+            m1 += Prefix::entries(lens[i]);
+            m2 += Prefix::entries(lens[i]);
         }
     }
     parallel_for(n, [&](size_t i) {
@@ -770,7 +795,7 @@ int mh_refine_pieces(const char *row1, const char *row2, uint64_t cols, uint64_t
 {
     MH_CHECK(row1 && row2 && n_pieces && (pieces || cap == 0), MH_ERR_INVALID, "null argument");
     std::vector<RefinePiece> p;
-    std::vector<uint32_t> m1(cols + 1), m2(cols + 1);
+    std::vector<uint32_t> m1(Prefix::entries(cols)), m2(Prefix::entries(cols));
     Prefix pf;
     pf.build(row1, row2, cols, m1.data(), m2.data());
     pieces_fast(row1, row2, cols, pf, p);

@@ -44,3 +44,17 @@ def test_pieces_across_scan_chunks():
         side[lo:lo + ln] = ord("-")
     a, b = r1.tobytes().decode(), r2.tobytes().decode()
     assert mauvehip.refine_pieces(a, b) == ro.pieces_of(a, b)
+
+
+def test_pieces_across_prefix_chunks():
+    """An alignment longer than the library's residue-count chunks (2^12
+    blocks of 64 columns), with long one-sided gaps placed across a chunk
+    border: the cuts and the density halving read counts at arbitrary columns."""
+    rng = np.random.default_rng(91)
+    r1, r2 = alignment(rng, 600_000, div=0.05, marble=0.3)
+    c = 262_144 - 3000
+    r1 = r1[:c] + "-" * 6000 + r1[c:]
+    r2 = r2[:c] + "".join(rng.choice(list("ACGT"), 6000)) + r2[c:]
+    got = mauvehip.refine_pieces(r1, r2)
+    assert got == ro.pieces_of(r1, r2)
+    assert len(got) > 20
