@@ -57,6 +57,19 @@ void radix_sort_pairs(std::vector<uint64_t> &key, std::vector<uint32_t> &val)
     }
 }
 
+// High bit of each of the 8 bytes at p that holds a residue (not '-').
+inline uint64_t residues8(const char *p)
+{
+    uint64_t v;
+    std::memcpy(&v, p, 8);
+    v ^= 0x2D2D2D2D2D2D2D2Dull;
+    const uint64_t lo7 = 0x7F7F7F7F7F7F7F7Full;
+    return (((v & lo7) + lo7) | v) & 0x8080808080808080ull;
+}
+
+// Bits of the bytes [0, b) of a group of 8 (b <= 8).
+inline uint64_t bytes_below(int b) { return b ? ~0ull >> (64 - 8 * b) : 0ull; }
+
 }  // namespace
 
 extern "C" {
@@ -250,10 +263,27 @@ int mh_recursion_gaps(const char *row_a, const char *row_b, uint64_t cols, int64
         bool gap_after = false;           // a column not aligned after it (or at all)
     };
     std::vector<Sum> sum(nc);
+    const uint64_t HIGH = 0x8080808080808080ull;
     host_parallel(nc, [&](size_t k) {
         Sum &x = sum[k];
         const uint64_t hi = std::min(cols, (k + 1) * chunk);
-        for (uint64_t c = k * chunk; c < hi; ++c) {
+        uint64_t c = k * chunk;
+        // groups of 8 columns at once (residue and aligned-column bit masks)
+        for (; c + 8 <= hi; c += 8) {
+            const uint64_t ra = residues8(row_a + c), rb = residues8(row_b + c), al = ra & rb;
+            if (al) {
+                const int l = (63 - __builtin_clzll(al)) >> 3;
+                x.aligned = true;
+                x.pa = x.ca + (uint64_t)__builtin_popcountll(ra & bytes_below(l));
+                x.pb = x.cb + (uint64_t)__builtin_popcountll(rb & bytes_below(l));
+                x.gap_after = (~al & HIGH & ~bytes_below(l + 1)) != 0;
+            } else {
+                x.gap_after = true;
+            }
+            x.ca += (uint64_t)__builtin_popcountll(ra);
+            x.cb += (uint64_t)__builtin_popcountll(rb);
+        }
+        for (; c < hi; ++c) {
             const bool ra = row_a[c] != '-', rb = row_b[c] != '-';
             if (ra && rb) {
                 x.aligned = true;
@@ -306,7 +336,30 @@ int mh_recursion_gaps(const char *row_a, const char *row_b, uint64_t cols, int64
     host_parallel(nc, [&](size_t k) {
         State t = at[k];
         const uint64_t hi = std::min(cols, (k + 1) * chunk);
-        for (uint64_t c = k * chunk; c < hi; ++c) {
+        uint64_t c = k * chunk;
+        // groups of 8 columns at once: only a group's first aligned column can
+        // close a gap when a gap needs more than 7 residues (the others follow an
+        // aligned column less than 8 columns before them)
+        if (min_gap >= 8)
+            for (; c + 8 <= hi; c += 8) {
+                const uint64_t ra = residues8(row_a + c), rb = residues8(row_b + c), al = ra & rb;
+                if (al) {
+                    const int f = __builtin_ctzll(al) >> 3, l = (63 - __builtin_clzll(al)) >> 3;
+                    State u = t;
+                    u.ci += (uint64_t)__builtin_popcountll(ra & bytes_below(f));
+                    u.cj += (uint64_t)__builtin_popcountll(rb & bytes_below(f));
+                    if (f > 0) u.in_gap = true;
+                    close(u, found[k]);
+                    t.pi = t.ci + (uint64_t)__builtin_popcountll(ra & bytes_below(l));
+                    t.pj = t.cj + (uint64_t)__builtin_popcountll(rb & bytes_below(l));
+                    t.in_gap = (~al & HIGH & ~bytes_below(l + 1)) != 0;
+                } else {
+                    t.in_gap = true;
+                }
+                t.ci += (uint64_t)__builtin_popcountll(ra);
+                t.cj += (uint64_t)__builtin_popcountll(rb);
+            }
+        for (; c < hi; ++c) {
             const bool ra = row_a[c] != '-', rb = row_b[c] != '-';
             if (ra && rb) {
                 close(t, found[k]);

@@ -265,14 +265,11 @@ __global__ void grid_pred_kernel(const uint64_t *__restrict__ sk, const uint32_t
     link[r] = out;
 }
 
-__global__ void grid_jump_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t ns,
-                                 uint32_t *__restrict__ changed)
+__global__ void grid_jump_kernel(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint64_t ns)
 {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= ns) return;
-    const uint32_t a = in[r], b = in[a];
-    out[r] = b;
-    if (b != a) atomicOr(changed, 1u);
+    out[r] = in[in[r]];
 }
 
 // per seed (original order): is it a chain head, and which seed heads its chain
@@ -935,15 +932,13 @@ static int extend_by_chains(mh_context *ctx, const mh_sml *a, const mh_sml *b, c
     // the sort's spare value array holds the links, g[4] their jumps
     uint32_t *la = sv == v0 ? v1 : v0, *lb = g[4].as<uint32_t>();
     hipLaunchKernelGGL(grid_pred_kernel, grid, blk, 0, ctx->stream, sk, sv, t1, ns, (uint32_t)env.seed_len, 64, la);
-    for (int round = 0;; ++round) {
-        MH_HIP(hipMemsetAsync(cnt + 3, 0, 4, ctx->stream));
-        hipLaunchKernelGGL(grid_jump_kernel, grid, blk, 0, ctx->stream, la, lb, ns, cnt + 3);
+    // links point to earlier sorted entries, so a chain has at most ns seeds and
+    // this many doublings reach every head without a check in between
+    int rounds = 1;
+    while ((1ull << (rounds - 1)) < ns) ++rounds;
+    for (int round = 0; round < rounds; ++round) {
+        hipLaunchKernelGGL(grid_jump_kernel, grid, blk, 0, ctx->stream, la, lb, ns);
         std::swap(la, lb);
-        uint32_t changed = 0;
-        MH_HIP(hipMemcpyAsync(&changed, cnt + 3, 4, hipMemcpyDeviceToHost, ctx->stream));
-        MH_HIP(hipStreamSynchronize(ctx->stream));
-        if (!changed) break;
-        MH_CHECK(round < 40, MH_ERR_DEVICE, "seed chains did not converge");
     }
     uint32_t *is_head = g[5].as<uint32_t>(), *head_of = g[6].as<uint32_t>();
     hipLaunchKernelGGL(grid_head_kernel, grid, blk, 0, ctx->stream, sv, la, ns, is_head, head_of);

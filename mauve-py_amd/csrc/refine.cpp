@@ -226,6 +226,19 @@ inline unsigned gaps8(const char *p)
     return (unsigned)__builtin_popcountll(t);
 }
 
+// High bit of each of the 8 columns at c that holds residues in both rows.
+inline uint64_t aligned8(const char *r1, const char *r2)
+{
+    uint64_t v1, v2;
+    std::memcpy(&v1, r1, 8);
+    std::memcpy(&v2, r2, 8);
+    v1 ^= 0x2D2D2D2D2D2D2D2Dull;
+    v2 ^= 0x2D2D2D2D2D2D2D2Dull;
+    const uint64_t lo7 = 0x7F7F7F7F7F7F7F7Full;
+    const uint64_t g1 = ~(((v1 & lo7) + lo7) | v1 | lo7), g2 = ~(((v2 & lo7) + lo7) | v2 | lo7);
+    return ~(g1 | g2) & 0x8080808080808080ull;
+}
+
 // Residue counts of the two rows before any column, from counts kept every 64
 // columns and a count inside the block.  Built in parallel chunks.
 struct Prefix {
@@ -318,13 +331,20 @@ void pieces_fast(const char *r1, const char *r2, uint64_t n, const Prefix &pf, s
     parallel_for(nc, [&](size_t k) {
         const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
         int64_t prev = -1;
-        for (uint64_t c = lo; c < hi; ++c) {
-            if (r1[c] != '-' && r2[c] != '-') {
-                if (prev < 0) fst[k] = (int64_t)c;
-                else if ((int64_t)c - prev - 1 > (int64_t)MAX_GAP_LENGTH) inner[k].push_back(Run{prev, (int64_t)c});
-                prev = (int64_t)c;
-            }
+        // an aligned column from first to last of a group of 8 (runs inside a
+        // group are shorter than MAX_GAP_LENGTH)
+        auto seen = [&](int64_t first, int64_t last) {
+            if (prev < 0) fst[k] = first;
+            else if (first - prev - 1 > (int64_t)MAX_GAP_LENGTH) inner[k].push_back(Run{prev, first});
+            prev = last;
+        };
+        uint64_t c = lo;
+        for (; c + 8 <= hi; c += 8) {
+            const uint64_t al = aligned8(r1 + c, r2 + c);
+            if (al) seen((int64_t)c + (__builtin_ctzll(al) >> 3), (int64_t)c + ((63 - __builtin_clzll(al)) >> 3));
         }
+        for (; c < hi; ++c)
+            if (r1[c] != '-' && r2[c] != '-') seen((int64_t)c, (int64_t)c);
         lst[k] = prev;
     });
     std::vector<Run> runs;

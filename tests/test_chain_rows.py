@@ -54,12 +54,12 @@ def test_recursion_gaps_known():
     assert len(mauvehip.recursion_gaps(ra, rb2, 1, 1)) == 0 == len(po.recursion_gaps(ra, rb2, 1, 1))
 
 
-@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("seed", range(16))
 def test_recursion_gaps_match_oracle(seed):
     rng = np.random.default_rng(70 + seed)
     ra, rb = alignment(rng, int(rng.integers(500, 20000)), div=0.1, marble=float(rng.choice([0.3, 1.0, 3.0])))
     la, lb = int(rng.integers(1, 1000)), int(rng.integers(1, 1000))
-    mg = int(rng.choice([0, 50, 200]))
+    mg = int(rng.choice([0, 7, 8, 9, 50, 200]))
     got = mauvehip.recursion_gaps(ra, rb, la, lb, mg)
     want = po.recursion_gaps(ra, rb, la, lb, mg)
     assert [(tuple(int(x) for x in g["lend"]), tuple(int(x) for x in g["rend"])) for g in got] == want
