@@ -56,8 +56,11 @@ def align_pair(ctx, d_seqs, seqs, seed, xmfa_path, names, recursive=True, refine
     sa.free()
     sb.free()
     t0 = mark("match", t0)
+    t1 = t()
     chain, _ = chain_gap_jobs_raw(mt, L0, L1, 0)
+    t1 = mark("chain.dp", t1)
     ra, rb = chain_rows(mt, chain, s0, s1, as_array=True)
+    mark("chain.rows", t1)
     t0 = mark("chain", t0)
     n_rec = 0
     if recursive:
