@@ -220,10 +220,38 @@ int mh_chain_rows(const mh_match *matches, const uint32_t *chain, uint64_t n_cha
     *cols = n;
     MH_CHECK(cap >= n, MH_ERR_INVALID, "row capacity too small (cols holds the size)");
     MH_CHECK(row_a && row_b, MH_ERR_INVALID, "null argument");
-    uint64_t c = 0;
-    pa = pb = 0;
-    auto stretch = [&](uint64_t ea, uint64_t eb) {
-        // genome 1's residues against gaps, then gaps against genome 2's
+    // every anchor's stretch (the unaligned residues before it, genome 1's
+    // against gaps, then gaps against genome 2's) and the anchor itself start
+    // at a column known from the lengths before it, so blocks of anchors are
+    // written in parallel; the last stretch runs to the genome ends
+    std::vector<uint64_t> at(n_chain + 1);
+    {
+        uint64_t c = 0, qa = 0, qb = 0;
+        for (uint64_t q = 0; q < n_chain; ++q) {
+            const mh_match &m = matches[chain[q]];
+            const uint64_t s0 = (uint64_t)m.start[0] - 1, s1 = (uint64_t)m.start[1] - 1;
+            at[q] = c;
+            c += (s0 - qa) + (s1 - qb) + m.length;
+            qa = s0 + m.length;
+            qb = s1 + m.length;
+        }
+        at[n_chain] = c;
+    }
+    auto piece = [&](uint64_t q) {   // stretch + anchor q, or the final stretch for q == n_chain
+        uint64_t pa = 0, pb = 0;
+        if (q > 0) {
+            const mh_match &m = matches[chain[q - 1]];
+            pa = (uint64_t)m.start[0] - 1 + m.length;
+            pb = (uint64_t)m.start[1] - 1 + m.length;
+        }
+        uint64_t ea = len_a, eb = len_b, len = 0;
+        if (q < n_chain) {
+            const mh_match &m = matches[chain[q]];
+            ea = (uint64_t)m.start[0] - 1;
+            eb = (uint64_t)m.start[1] - 1;
+            len = m.length;
+        }
+        uint64_t c = at[q];
         const uint64_t ga = ea - pa, gb = eb - pb;
         std::memcpy(row_a + c, seq_a + pa, ga);
         std::memset(row_b + c, '-', ga);
@@ -231,18 +259,13 @@ int mh_chain_rows(const mh_match *matches, const uint32_t *chain, uint64_t n_cha
         std::memset(row_a + c, '-', gb);
         std::memcpy(row_b + c, seq_b + pb, gb);
         c += gb;
+        std::memcpy(row_a + c, seq_a + ea, len);
+        std::memcpy(row_b + c, seq_b + eb, len);
     };
-    for (uint64_t q = 0; q < n_chain; ++q) {
-        const mh_match &m = matches[chain[q]];
-        const uint64_t s0 = (uint64_t)m.start[0] - 1, s1 = (uint64_t)m.start[1] - 1;
-        stretch(s0, s1);
-        std::memcpy(row_a + c, seq_a + s0, m.length);
-        std::memcpy(row_b + c, seq_b + s1, m.length);
-        c += m.length;
-        pa = s0 + m.length;
-        pb = s1 + m.length;
-    }
-    stretch(len_a, len_b);
+    const uint64_t per = 512, ntask = (n_chain + 1 + per - 1) / per;
+    host_parallel(ntask, [&](size_t k) {
+        for (uint64_t q = k * per; q < std::min<uint64_t>(n_chain + 1, (k + 1) * per); ++q) piece(q);
+    });
     return MH_OK;
 }
 

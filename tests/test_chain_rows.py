@@ -29,6 +29,32 @@ def test_chain_rows_layout():
     assert ra.replace(b"-", b"") == A and rb.replace(b"-", b"") == B
 
 
+def test_chain_rows_long_chain():
+    """A chain of 3000 anchors (the rows are written in blocks of anchors in
+    parallel): each stretch is genome 1's residues against gaps, then gaps
+    against genome 2's, then the anchor; the last stretch runs to the ends."""
+    rng = np.random.default_rng(5)
+    A = rng.choice(list(b"ACGT"), 400_000).astype(np.uint8).tobytes()
+    B = rng.choice(list(b"ACGT"), 410_000).astype(np.uint8).tobytes()
+    rows, pa, pb = [], 0, 0
+    for _ in range(3000):
+        a = pa + int(rng.integers(0, 40))
+        b = pb + int(rng.integers(0, 40))
+        n = int(rng.integers(1, 60))
+        rows.append((a + 1, b + 1, n))
+        pa, pb = a + n, b + n
+    m = _matches(rows)
+    ra, rb = mauvehip.chain_rows(m, np.arange(len(rows)), A, B)
+    wa, wb, pa, pb = [], [], 0, 0
+    for a, b, n in rows + [(len(A) + 1, len(B) + 1, 0)]:
+        a, b = a - 1, b - 1
+        wa += [A[pa:a], b"-" * (b - pb), A[a:a + n]]
+        wb += [b"-" * (a - pa), B[pb:b], B[b:b + n]]
+        pa, pb = a + n, b + n
+    want_a, want_b = b"".join(wa), b"".join(wb)
+    assert ra == want_a and rb == want_b
+
+
 def test_chain_rows_rejects_bad_chains():
     m = _matches([(5, 5, 10), (8, 30, 4)])      # overlapping in genome 1
     with pytest.raises(mauvehip.MhError):
