@@ -86,19 +86,19 @@ __host__ __device__ inline void stripe_rows(uint32_t la, uint32_t lb, uint32_t b
     r1 = (int)a1;
 }
 
-// Trace bytes of one job: per non-empty stripe (rows + 63) steps of 64 lanes
-// x cols/2 bytes.
-uint64_t dp_trace_bytes(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols)
+// Trace bytes of one job run by g lanes: per non-empty stripe (rows + g - 1)
+// steps of g lanes x cols/2 bytes.
+uint64_t dp_trace_bytes(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols, uint32_t g)
 {
-    const uint32_t sw = 64 * cols;
+    const uint32_t sw = g * cols;
     const uint32_t ns = (lb + sw - 1) / sw;
     uint64_t steps = 0;
     for (uint32_t s = 0; s < ns; ++s) {
         int r0, r1;
         stripe_rows(la, lb, band, sw, s, r0, r1);
-        if (r1 >= r0) steps += (uint64_t)(r1 - r0 + 1) + 63;
+        if (r1 >= r0) steps += (uint64_t)(r1 - r0 + 1) + (g - 1);
     }
-    return steps * 64 * (cols / 2);
+    return steps * g * (cols / 2);
 }
 
 namespace {
@@ -107,6 +107,14 @@ __device__ __forceinline__ int shr1_old(int v, int old)
 {
     // wave_shr:1 -- lane k receives lane k-1's value, lane 0 receives `old`
     return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xF, 0xF, false);
+}
+
+// The same shift within the job's lanes: the whole wavefront (G = 64) or each
+// 16-lane DPP row (G = 16, row_shr:1; the row's first lane receives `old`).
+template <int G> __device__ __forceinline__ int shr1_group(int v, int old)
+{
+    if constexpr (G == 64) return shr1_old(v, old);
+    else return __builtin_amdgcn_update_dpp(old, v, 0x111, 0xF, 0xF, false);
 }
 
 template <int... Cs, class F>
@@ -122,7 +130,10 @@ __device__ __forceinline__ int max3i(int a, int b, int c)
     return r;
 }
 
-// One wavefront per job.  D and I are kept shifted by the close half (goh):
+// One job per G lanes: a wavefront (G = 64), or each 16-lane DPP row of a
+// wavefront (G = 16: four jobs at once, for jobs at most 16*COLS columns wide,
+// whose skew and column padding would leave most of a wavefront idle).  In
+// what follows "64" is G.  D and I are kept shifted by the close half (goh):
 // D' = D + goh and I' = I + goh, so the M recurrence reads them without an
 // add; every comparison is between two values shifted alike, so ties and
 // trace bits are unchanged.  Per row a lane runs two passes over its COLS
@@ -139,17 +150,18 @@ __device__ __forceinline__ int max3i(int a, int b, int c)
 // bit 3 MI; each lane stores COLS/2 bytes per step, so a wavefront stores
 // 32*COLS contiguous bytes per step.  Cell (i, j) of lane k, column c is at
 // byte  stripe_base + ((i - r0 + k)*64 + k)*COLS/2 + c/2, nibble c%2.
-template <int COLS>
+template <int COLS, int G>
 __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restrict__ A,
                                        const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
                                        int *__restrict__ bnd, DpOut *__restrict__ out, uint32_t sr0,
                                        uint32_t sr1, uint32_t sr2, uint32_t sr3, int goh, int (*lds_in)[64],
                                        int (*lds_out)[64])
 {
-    constexpr int STRIPE_W = LANES * COLS;
+    constexpr int STRIPE_W = G * COLS;
     constexpr int NW = COLS >= 8 ? COLS / 8 : 1;   // 32-bit trace words per lane per step
     constexpr int LANE_BYTES = COLS / 2;
-    const int lane = threadIdx.x;
+    const int lane = (int)threadIdx.x & (G - 1);
+    const int gb = (int)threadIdx.x & ~(G - 1);    // the job's slots in the LDS rows
     const int la = (int)jb.la, lb = (int)jb.lb;
     const uint8_t *a = A + jb.a_off;
     const uint8_t *b = B + jb.b_off;
@@ -180,7 +192,7 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
         const int rem = lb - (jl - 1);
         const int ncols = rem < 0 ? 0 : (rem > COLS ? COLS : rem);
         const int span = lb - s * STRIPE_W;
-        const int nl = span >= STRIPE_W ? LANES : (span + COLS - 1) / COLS;
+        const int nl = span >= STRIPE_W ? G : (span + COLS - 1) / COLS;
         const bool last_stripe = (s == nstripes - 1);
         const bool top0 = (r0 == 1);
 
@@ -213,7 +225,7 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
         int outM = NEG, outD = NEG, outI = NEG, acode = 0;
         const int ob2_0 = (jl == 1) ? goh : go2;      // open_B[j-1] + goh for the lane's first column
 
-        // lane 0's inputs, 64 rows per chunk: column jl-1 of the previous stripe
+        // lane 0's inputs, G rows per chunk: column jl-1 of the previous stripe
         // where it computed that row (else NEG), or column 0 (M = NEG,
         // D' = 0 + goh, I' = NEG + goh); and the A codes.  The next chunk is
         // loaded one chunk ahead.
@@ -233,18 +245,18 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
 
         const int nsteps = nrows + nl - 1;
         for (int t = 0; t < nsteps; ++t) {
-            const int q = t & 63;
+            const int q = t & (G - 1);
             if (q == 0) {
-                lds_in[0][lane] = nM;
-                lds_in[1][lane] = nD;
-                lds_in[2][lane] = nI;
-                lds_in[3][lane] = nA;
-                left_in(t + 64 + lane, nM, nD, nI, nA);
+                lds_in[0][gb + lane] = nM;
+                lds_in[1][gb + lane] = nD;
+                lds_in[2][gb + lane] = nI;
+                lds_in[3][gb + lane] = nA;
+                left_in(t + G + lane, nM, nD, nI, nA);
             }
-            const int rM = shr1_old(outM, lds_in[0][q]);
-            const int rD = shr1_old(outD, lds_in[1][q]);
-            const int rI = shr1_old(outI, lds_in[2][q]);
-            const int ra = shr1_old(acode, lds_in[3][q]);
+            const int rM = shr1_group<G>(outM, lds_in[0][gb + q]);
+            const int rD = shr1_group<G>(outD, lds_in[1][gb + q]);
+            const int rI = shr1_group<G>(outI, lds_in[2][gb + q]);
+            const int ra = shr1_group<G>(acode, lds_in[3][gb + q]);
             acode = ra;
 
             const int i = r0 + t - lane;
@@ -295,7 +307,7 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
             lpM = rM;
             lpD = rD;
             lpI = rI;
-            uint8_t *dst = tbj + tbs + ((size_t)t * 64 + lane) * LANE_BYTES;
+            uint8_t *dst = tbj + tbs + ((size_t)t * G + lane) * LANE_BYTES;
             if constexpr (COLS == 16) {
                 *reinterpret_cast<uint2 *>(dst) = make_uint2(word[0], word[1]);
             } else if constexpr (COLS == 8) {
@@ -310,28 +322,28 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
             }
             }
             if (!last_stripe) {
-                // lane 63 just finished relative row t-63 of the stripe's last column
-                const int x63 = t - 63;
-                if (lane == 63 && x63 >= 0 && x63 < nrows) {
-                    const int slot = x63 & 63;
+                // the last lane just finished relative row t-(G-1) of the stripe's last column
+                const int xl = t - (G - 1);
+                if (lane == G - 1 && xl >= 0 && xl < nrows) {
+                    const int slot = gb + (xl & (G - 1));
                     lds_out[0][slot] = outM;
                     lds_out[1][slot] = outD;
                     lds_out[2][slot] = outI;
                 }
-                if (x63 >= 0 && x63 < nrows && ((x63 & 63) == 63 || x63 == nrows - 1)) {
-                    const int x = (x63 & ~63) + lane;
+                if (xl >= 0 && xl < nrows && ((xl & (G - 1)) == G - 1 || xl == nrows - 1)) {
+                    const int x = (xl & ~(G - 1)) + lane;
                     if (x < nrows) {
                         const int r = r0 - 1 + x;
-                        bM[r] = lds_out[0][lane];
-                        bD[r] = lds_out[1][lane];
-                        bI[r] = lds_out[2][lane];
+                        bM[r] = lds_out[0][gb + lane];
+                        bD[r] = lds_out[1][gb + lane];
+                        bI[r] = lds_out[2][gb + lane];
                     }
                 }
             }
         }
         prev_r0 = r0;
         prev_r1 = r1;
-        tbs += (uint64_t)(nrows + 63) * 64 * LANE_BYTES;
+        tbs += (uint64_t)(nrows + G - 1) * G * LANE_BYTES;
         __threadfence();
     }
     if (lane == fin_lane) {
@@ -355,11 +367,29 @@ __global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs
     __shared__ int lds_out[3][64];
     const DevJob jb = jobs[blockIdx.x];
     if (jb.cols == 16)
-        dp_job<16>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+        dp_job<16, 64>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
     else if (jb.cols == 8)
-        dp_job<8>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+        dp_job<8, 64>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
     else
-        dp_job<4>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+        dp_job<4, 64>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+}
+
+// Four jobs of one shape per wavefront, one per 16-lane row (DevJob::cols =
+// GROUP_TAG + COLS); a row past the n jobs has nothing to do.
+constexpr uint32_t GROUP_TAG = 32;
+constexpr uint32_t ROWS_MAX_LB = 256;   // widest job run four to a wavefront
+template <int COLS>
+__global__ __launch_bounds__(64) void dp_forward_rows(const DevJob *__restrict__ jobs, int n,
+                                                      const uint8_t *__restrict__ A, const uint8_t *__restrict__ B,
+                                                      uint8_t *__restrict__ tb, int *__restrict__ bnd,
+                                                      DpOut *__restrict__ outs, uint32_t sr0, uint32_t sr1,
+                                                      uint32_t sr2, uint32_t sr3, int goh)
+{
+    __shared__ int lds_in[4][64];
+    __shared__ int lds_out[3][64];
+    const int x = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 4);
+    if (x >= n) return;
+    dp_job<COLS, 16>(jobs[x], A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
 }
 
 // ---- sliding-window banded DP ---------------------------------------------
@@ -725,7 +755,10 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
     if (x >= njobs) return;
     const DevJob jb = jobs[x];
     const int la = (int)jb.la, lb = (int)jb.lb;
-    const int cols = (int)jb.cols, sw = 64 * cols, lane_bytes = cols / 2;
+    // stripe layouts: G lanes per job (16 for the row-packed jobs) of cols columns
+    const bool rows16 = jb.cols > GROUP_TAG && jb.cols < WINDOW_TAG;
+    const int G = rows16 ? 16 : 64;
+    const int cols = rows16 ? (int)(jb.cols - GROUP_TAG) : (int)jb.cols, sw = G * cols, lane_bytes = cols / 2;
     const uint8_t *tj = tb + jb.tb_off;
     uint8_t *slot = paths + jb.path_off;
     // trace block of the last stripe: the sum over the stripes before it
@@ -734,7 +767,7 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
     for (int s = 0; s < cur; ++s) {
         int a0, a1;
         stripe_rows(jb.la, jb.lb, jb.band, sw, s, a0, a1);
-        if (a1 >= a0) base += (uint64_t)(a1 - a0 + 64) * 64 * lane_bytes;
+        if (a1 >= a0) base += (uint64_t)(a1 - a0 + G) * G * lane_bytes;
     }
     stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
     // stripe layout: column j-1 = cur*sw + k*cols + c, tracked without
@@ -762,7 +795,7 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
                 nib = (w >> (4 * (C * ((i - 1) % rpd) + xw % C))) & 15u;
             } else {
                 if (i < r0 || i > r1) { ok = false; break; }
-                const uint8_t byte = tj[base + ((uint64_t)(i - r0 + k) * 64 + k) * lane_bytes + (c >> 1)];
+                const uint8_t byte = tj[base + ((uint64_t)(i - r0 + k) * G + k) * lane_bytes + (c >> 1)];
                 nib = (byte >> (4 * (c & 1))) & 15u;
             }
         }
@@ -774,10 +807,10 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
         if (stripes && left && j > 0 && --c < 0) {
             c = cols - 1;
             if (--k < 0) {   // into the previous stripe: step back over its trace block
-                k = 63;
+                k = G - 1;
                 --cur;
                 stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
-                if (r1 >= r0) base -= (uint64_t)(r1 - r0 + 64) * 64 * lane_bytes;
+                if (r1 >= r0) base -= (uint64_t)(r1 - r0 + G) * G * lane_bytes;
             }
         }
         maxdev = max(maxdev, dnum < 0 ? -dnum : dnum);
@@ -1027,7 +1060,8 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
     }
     host_time(ctx, "host_dp_sort", now_ms() - ts0);
     auto bnd_ints = [&](const DpItem &it) {
-        if (it.cols == TINY_COLS_TAG || it.cols > WINDOW_TAG) return 0ull;
+        // one-lane, row-packed (a single stripe) and window jobs hand no column over
+        if (it.cols == TINY_COLS_TAG || it.cols > GROUP_TAG) return 0ull;
         const uint64_t ns = (jobs[it.job].b_len + 64 * it.cols - 1) / (64 * it.cols);
         return ns > 1 ? 3ull * ((jobs[it.job].a_len + 63) & ~63ull) : 0ull;
     };
@@ -1075,13 +1109,31 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
                               [](const DpItem &it) { return it.cols < WINDOW_TAG; });
         uint64_t nstripe = 0;
         while (nstripe < nbig && items[first + nstripe].cols < WINDOW_TAG) ++nstripe;
-        // stripe jobs with long trace-back walks first: they get a wavefront each
+        // wavefront stripes, then the row-packed jobs by shape (16, 8, 4 columns a lane)
         std::stable_partition(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)(first + nstripe),
+                              [](const DpItem &it) { return it.cols < GROUP_TAG; });
+        std::stable_sort(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)(first + nstripe),
+                         [](const DpItem &x, const DpItem &y) {
+                             const uint32_t kx = x.cols < GROUP_TAG ? 0u : 100u - x.cols;
+                             const uint32_t ky = y.cols < GROUP_TAG ? 0u : 100u - y.cols;
+                             return kx < ky;
+                         });
+        uint64_t n64 = 0;
+        while (n64 < nstripe && items[first + n64].cols < GROUP_TAG) ++n64;
+        uint64_t rows_at[4] = {n64, n64, n64, n64};   // ranges of the 16-, 8- and 4-column row jobs
+        for (int q = 0; q < 3; ++q) {
+            uint64_t e = rows_at[q];
+            while (e < nstripe && items[first + e].cols == GROUP_TAG + (16u >> q)) ++e;
+            rows_at[q + 1] = e;
+        }
+        MH_CHECK(rows_at[3] == nstripe, MH_ERR_DEVICE, "DP job shapes out of order");
+        // stripe jobs with long trace-back walks first: they get a wavefront each
+        std::stable_partition(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)(first + n64),
                               [&](const DpItem &it) {
                                   return jobs[it.job].a_len + jobs[it.job].b_len >= WAVE_WALK_MIN;
                               });
         uint64_t nwave = 0;
-        while (nwave < nstripe &&
+        while (nwave < n64 &&
                jobs[items[first + nwave].job].a_len + jobs[items[first + nwave].job].b_len >= WAVE_WALK_MIN)
             ++nwave;
         dj.resize(n);
@@ -1114,10 +1166,18 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         MH_HIP(hipMemcpyAsync(d_jobs, dj.data(), n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
         {
             TimedLaunch tl(ctx, "dp_forward");
-            if (nstripe)
-                hipLaunchKernelGGL(dp_forward, dim3((unsigned)nstripe), dim3(64), 0, ctx->stream, d_jobs, d_a, d_b,
+            if (n64)
+                hipLaunchKernelGGL(dp_forward, dim3((unsigned)n64), dim3(64), 0, ctx->stream, d_jobs, d_a, d_b,
                                    ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs, sr[0], sr[1], sr[2],
                                    sr[3], -200);
+            for (int q = 0; q < 3; ++q) {
+                const uint64_t a0 = rows_at[q], nr = rows_at[q + 1] - a0;
+                if (!nr) continue;
+                auto kern = q == 0 ? dp_forward_rows<16> : (q == 1 ? dp_forward_rows<8> : dp_forward_rows<4>);
+                hipLaunchKernelGGL(kern, dim3((unsigned)((nr + 3) / 4)), dim3(64), 0, ctx->stream, d_jobs + a0,
+                                   (int)nr, d_a, d_b, ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs + a0,
+                                   sr[0], sr[1], sr[2], sr[3], -200);
+            }
             if (nbig > nstripe)
                 hipLaunchKernelGGL(dp_window, dim3((unsigned)(nbig - nstripe)), dim3(64), 0, ctx->stream,
                                    d_jobs + nstripe, d_a, d_b, ctx->dp_tb.as<uint8_t>(), d_outs + nstripe, sr[0],
@@ -1176,7 +1236,14 @@ static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band, int fo
         it.tb_bytes = (uint64_t)jb.a_len * 8;
         return it;
     }
-    it.tb_bytes = dp_trace_bytes(jb.a_len, jb.b_len, it.band, it.cols);
+    if (it.band == 0 && !force && jb.b_len <= ROWS_MAX_LB && jb.a_len + jb.b_len < WAVE_WALK_MIN) {
+        // four jobs per wavefront, one stripe of 64, 128 or 256 columns each
+        const uint32_t c = jb.b_len > 128 ? 16u : (jb.b_len > 64 ? 8u : 4u);
+        it.cols = GROUP_TAG + c;
+        it.tb_bytes = dp_trace_bytes(jb.a_len, jb.b_len, 0, c, 16);
+        return it;
+    }
+    it.tb_bytes = dp_trace_bytes(jb.a_len, jb.b_len, it.band, it.cols, 64);
     if (it.band) {
         const uint32_t wc = dp_window_cols(jb.a_len, jb.b_len, it.band, dp_force_kind());
         if (wc) {

@@ -81,6 +81,28 @@ def test_dp_multi_stripe_long(ctx):
     assert _check_against_oracle(ctx, pairs) == []
 
 
+@pytest.mark.parametrize("extra", [0, 1, 2, 3])
+def test_dp_row_packed_shapes(ctx, extra):
+    """Jobs up to 256 columns wide with short trace-back walks run four to a
+    wavefront (one per 16-lane row, stripes of 64, 128 or 256 columns): the
+    shape boundaries, walks either side of the wavefront trace-back limit
+    (a + b = 1024), related and unrelated pairs, and job counts that leave
+    the last wavefront part empty."""
+    rng = np.random.default_rng(40 + extra)
+    pairs = []
+    for lb in (17, 18, 63, 64, 65, 127, 128, 129, 255, 256, 257):
+        for la in (1, 2, 16, 17, 100, 766, 767, 768, 1000):
+            a = rng.integers(0, 4, la, dtype=np.uint8)
+            if rng.random() < 0.5:
+                b = _mutate(rng, a, 0.05)
+                b = np.concatenate([b, rng.integers(0, 4, max(0, lb - len(b)), dtype=np.uint8)])[:lb]
+            else:
+                b = rng.integers(0, 4, lb, dtype=np.uint8)
+            pairs.append((a, b))
+    pairs = pairs[:len(pairs) - (len(pairs) % 4) + extra] if extra else pairs[:len(pairs) - (len(pairs) % 4)]
+    assert _check_against_oracle(ctx, pairs) == []
+
+
 def test_dp_zero_length_is_an_error(ctx):
     jobs = mauvehip.make_jobs([0], [0], [0], [5])
     with pytest.raises(mauvehip.MhError) as e:
