@@ -435,6 +435,10 @@ int mh_xmfa_write(const char *path, int seq_count, const char *const *seq_files,
  * shorter than its sequence-1 row (the reference's IndexError). */
 int mh_index_from_xmfa(const char *xmfa, uint64_t len, uint64_t genome_length, int32_t *lut,
                        uint64_t *n_groups);
+/* The same walk over the XMFA file at path, as buildIndex opens it
+ * (PY/buildindex.py:110-118); the file is read into a buffer the library keeps
+ * between calls.  MH_ERR_IO if it cannot be read. */
+int mh_index_from_xmfa_file(const char *path, uint64_t genome_length, int32_t *lut, uint64_t *n_groups);
 /* fixZeroIdx, PY/indexutils.pyx:27-43: an unmapped lut[0] takes
  * lut[k] - k from the first mapped k >= 10 when the first bases agree.
  * MH_ERR_INVALID when no such k exists (the reference's IndexError). */

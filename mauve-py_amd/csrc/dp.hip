@@ -1089,6 +1089,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
 
     uint64_t first = 0;
     std::vector<DevJob> dj;
+    std::vector<DpItem> shaped;
     while (first < nitems) {
         // one round: as many jobs (in schedule order) as fit the trace budget
         uint64_t tbb = 0, bi = 0, last = first;
@@ -1109,24 +1110,21 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
                               [](const DpItem &it) { return it.cols < WINDOW_TAG; });
         uint64_t nstripe = 0;
         while (nstripe < nbig && items[first + nstripe].cols < WINDOW_TAG) ++nstripe;
-        // wavefront stripes, then the row-packed jobs by shape (16, 8, 4 columns a lane)
-        std::stable_partition(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)(first + nstripe),
-                              [](const DpItem &it) { return it.cols < GROUP_TAG; });
-        std::stable_sort(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)(first + nstripe),
-                         [](const DpItem &x, const DpItem &y) {
-                             const uint32_t kx = x.cols < GROUP_TAG ? 0u : 100u - x.cols;
-                             const uint32_t ky = y.cols < GROUP_TAG ? 0u : 100u - y.cols;
-                             return kx < ky;
-                         });
-        uint64_t n64 = 0;
-        while (n64 < nstripe && items[first + n64].cols < GROUP_TAG) ++n64;
-        uint64_t rows_at[4] = {n64, n64, n64, n64};   // ranges of the 16-, 8- and 4-column row jobs
-        for (int q = 0; q < 3; ++q) {
-            uint64_t e = rows_at[q];
-            while (e < nstripe && items[first + e].cols == GROUP_TAG + (16u >> q)) ++e;
-            rows_at[q + 1] = e;
+        // wavefront stripes, then the row-packed jobs by shape (16, 8, 4 columns
+        // a lane), each in order: a stable four-bucket pass
+        auto shape = [](const DpItem &it) {
+            return it.cols < GROUP_TAG ? 0 : (it.cols == GROUP_TAG + 16 ? 1 : (it.cols == GROUP_TAG + 8 ? 2 : 3));
+        };
+        uint64_t rows_at[5] = {0, 0, 0, 0, 0};
+        for (uint64_t x = 0; x < nstripe; ++x) ++rows_at[shape(items[first + x]) + 1];
+        for (int q = 0; q < 4; ++q) rows_at[q + 1] += rows_at[q];
+        {
+            uint64_t put[4] = {rows_at[0], rows_at[1], rows_at[2], rows_at[3]};
+            shaped.resize(nstripe);
+            for (uint64_t x = 0; x < nstripe; ++x) shaped[put[shape(items[first + x])]++] = items[first + x];
+            std::copy(shaped.begin(), shaped.end(), items.begin() + (ptrdiff_t)first);
         }
-        MH_CHECK(rows_at[3] == nstripe, MH_ERR_DEVICE, "DP job shapes out of order");
+        const uint64_t n64 = rows_at[1];   // rows_at[1..4]: the 16-, 8- and 4-column row jobs
         // stripe jobs with long trace-back walks first: they get a wavefront each
         std::stable_partition(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)(first + n64),
                               [&](const DpItem &it) {
@@ -1171,7 +1169,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
                                    ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs, sr[0], sr[1], sr[2],
                                    sr[3], -200);
             for (int q = 0; q < 3; ++q) {
-                const uint64_t a0 = rows_at[q], nr = rows_at[q + 1] - a0;
+                const uint64_t a0 = rows_at[q + 1], nr = rows_at[q + 2] - a0;
                 if (!nr) continue;
                 auto kern = q == 0 ? dp_forward_rows<16> : (q == 1 ? dp_forward_rows<8> : dp_forward_rows<4>);
                 hipLaunchKernelGGL(kern, dim3((unsigned)((nr + 3) / 4)), dim3(64), 0, ctx->stream, d_jobs + a0,

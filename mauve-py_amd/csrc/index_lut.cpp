@@ -19,6 +19,10 @@
 #include "mh_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -295,6 +299,45 @@ int mh_index_from_xmfa(const char *xmfa, uint64_t len, uint64_t genome_length, i
     }
     if (n_groups) *n_groups = groups;
     return MH_OK;
+}
+
+int mh_index_from_xmfa_file(const char *path, uint64_t genome_length, int32_t *lut, uint64_t *n_groups)
+{
+    MH_CHECK(path, MH_ERR_INVALID, "null argument");
+    const int fd = ::open(path, O_RDONLY);
+    MH_CHECK(fd >= 0, MH_ERR_IO, std::string("cannot open ") + path);
+    struct stat st;
+    if (::fstat(fd, &st) != 0) {
+        ::close(fd);
+        MH_CHECK(false, MH_ERR_IO, std::string("cannot stat ") + path);
+    }
+    const uint64_t n = (uint64_t)st.st_size;
+    static std::mutex mu;
+    static HostBuf text;
+    std::lock_guard<std::mutex> lock(mu);
+    char *buf = text.get<char>(n);
+    if (!buf) {
+        ::close(fd);
+        MH_CHECK(false, MH_ERR_NOMEM, "out of host memory");
+    }
+    // read in parallel 4 MiB pieces
+    const uint64_t piece = 4ull << 20, np = (n + piece - 1) / piece;
+    std::atomic<int> bad{0};
+    parallel_chunks(np, [&](uint64_t k) {
+        uint64_t off = k * piece;
+        const uint64_t end = std::min(n, off + piece);
+        while (off < end) {
+            const ssize_t got = ::pread(fd, buf + off, (size_t)(end - off), (off_t)off);
+            if (got <= 0) {
+                bad = 1;
+                return;
+            }
+            off += (uint64_t)got;
+        }
+    });
+    ::close(fd);
+    MH_CHECK(!bad, MH_ERR_IO, std::string("cannot read ") + path);
+    return mh_index_from_xmfa(buf, n, genome_length, lut, n_groups);
 }
 
 int mh_index_fix_zero(int32_t *lut, uint64_t n, const char *genome, uint64_t genome_len, const char *ref,

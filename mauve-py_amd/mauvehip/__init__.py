@@ -44,7 +44,7 @@ EXPORTS = [
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free", "mh_match_list_seq_count",
     "mh_match_list_seq_file", "mh_match_list_seq_length", "mh_match_list_write", "mh_match_list_read",
-    "mh_index_from_xmfa", "mh_index_fix_zero", "mh_index_fill_gaps", "mh_index_smooth_edges", "mh_xmfa_write",
+    "mh_index_from_xmfa", "mh_index_from_xmfa_file", "mh_index_fix_zero", "mh_index_fill_gaps", "mh_index_smooth_edges", "mh_xmfa_write",
     "mh_refine_pairwise", "mh_refine_anchor_cols", "mh_refine_pieces", "mh_anchor_gaps", "mh_chain_rows",
     "mh_recursion_gaps",
     "mh_dp_align_banded_device", "mh_dp_band_region", "mh_dp_band_cells",
@@ -174,6 +174,7 @@ def load_library(path=LIB_PATH):
         "mh_match_list_write": (C.c_int, [C.c_char_p, vp, u64, C.c_int, C.POINTER(C.c_char_p), C.POINTER(u64)]),
         "mh_match_list_read": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
         "mh_index_from_xmfa": (C.c_int, [C.c_char_p, u64, u64, vp, C.POINTER(u64)]),
+        "mh_index_from_xmfa_file": (C.c_int, [C.c_char_p, u64, vp, C.POINTER(u64)]),
         "mh_index_fix_zero": (C.c_int, [vp, u64, C.c_char_p, u64, C.c_char_p, u64]),
         "mh_index_fill_gaps": (C.c_int, [vp, u64, C.c_int]),
         "mh_index_smooth_edges": (C.c_int, [vp, u64, C.c_int]),

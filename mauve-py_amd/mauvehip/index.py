@@ -12,6 +12,7 @@ Sequences are read as the first record of a FASTA file, whitespace removed
 files buildIndex is documented for).
 """
 import ctypes as C
+import os
 import re
 
 import numpy as np
@@ -59,6 +60,15 @@ def lut_from_xmfa(xmfa, genome_length):
     return lut
 
 
+def lut_from_xmfa_file(path, genome_length):
+    """lut_from_xmfa of the XMFA file at path (read by the library)."""
+    L = load_library()
+    lut = np.empty(genome_length, dtype=IDX_ARRAY_DTYPE)
+    ng = C.c_uint64()
+    _check(L.mh_index_from_xmfa_file(os.fsencode(path), genome_length, lut.ctypes.data, C.byref(ng)))
+    return lut
+
+
 def fixZeroIdx(idx_lut, genome, ref_genome):
     """PY/indexutils.pyx:27-43, in place.  IndexError as in the reference
     when no mapped entry exists at or after index 10."""
@@ -94,9 +104,7 @@ def build_index_from_xmfa(xmfa_fp, genome_fp, ref_genome_fp, genome_seq=None, re
                           max_gap_width=300, smooth_edges=True, smoothing_radius=20):
     """buildIndex (PY/buildindex.py:90-138) from the XMFA alignment of
     genome_fp (sequence 1) against ref_genome_fp (sequence 2)."""
-    with open(xmfa_fp, "rb") as f:
-        text = f.read()
-    lut = lut_from_xmfa(text, genome_length(genome_fp))
+    lut = lut_from_xmfa_file(xmfa_fp, genome_length(genome_fp))
     if fill_gaps:
         genome_seq = genome_seq or seq_from_file(genome_fp)
         ref_genome_seq = ref_genome_seq or seq_from_file(ref_genome_fp)

@@ -87,9 +87,7 @@ def align_pair(ctx, d_seqs, seqs, seed, xmfa_path, names, recursive=True, refine
     t0 = mark("xmfa", t0)
     table = None
     if lut:
-        with open(xmfa_path, "rb") as f:
-            text = f.read()
-        table = mindex.lut_from_xmfa(text, L0)
+        table = mindex.lut_from_xmfa_file(xmfa_path, L0)
         mindex.fixZeroIdx(table, s0, s1)
         mindex.fillGaps(table)
         mindex.smoothEdges(table)

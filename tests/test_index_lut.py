@@ -209,3 +209,24 @@ def test_passes_across_chunks(seed):
     mi.smoothEdges(a)
     io.smooth_edges(b, 20)
     assert a.tolist() == b
+
+
+def test_lut_from_file_matches_text(tmp_path):
+    """The library's own read of the file gives the walk of the text, for a
+    file longer than one 4 MiB read piece; a missing file is an I/O error."""
+    rng = np.random.default_rng(31)
+    n = 3_000_000
+    alpha = np.array(list("ACGT-"))
+    g = alpha[rng.choice(5, n, p=[.24, .24, .24, .24, .04])]
+    r = g.copy()
+    flip = rng.random(n) < 0.05
+    r[flip] = alpha[rng.integers(0, 5, int(flip.sum()))]
+    gs, rs = "".join(g), "".join(r)
+    body = lambda s: "\n".join(s[k:k + 80] for k in range(0, len(s), 80))
+    text = f"#FormatVersion Mauve1\n> 1:1-{n} + g\n{body(gs)}\n> 2:1-{n} + r\n{body(rs)}\n=\n"
+    path = tmp_path / "a.xmfa"
+    path.write_text(text)
+    assert path.stat().st_size > (4 << 20)
+    assert np.array_equal(mi.lut_from_xmfa_file(str(path), n + 5), mi.lut_from_xmfa(text, n + 5))
+    with pytest.raises(mi.MhError):
+        mi.lut_from_xmfa_file(str(tmp_path / "missing.xmfa"), 10)
