@@ -15,6 +15,8 @@
 #include "mh_internal.hpp"
 
 #include <algorithm>
+#include <string>
+#include <vector>
 #include <cstdlib>
 
 using namespace mh;
@@ -97,60 +99,121 @@ void eliminate_overlaps(std::vector<mh_match> &ml)
     ml.swap(out);
 }
 
+// One gap: its pieces' sorted mer lists and pairwise search on c, then the
+// clean-up and the shift to genome coordinates, into out.
+int search_gap(mh_context *c, const uint8_t *const seq[2], const uint64_t len[2], const mh_anchor_gap &gp,
+               std::vector<mh_match> &out)
+{
+    out.clear();
+    uint64_t off[2] = {0, 0}, plen[2] = {0, 0};
+    for (int g = 0; g < 2; ++g) {
+        int64_t diff = gp.rend[g] - gp.lend[g];
+        diff = diff > 0 ? diff - 1 : 0;
+        if (diff > 0 && gp.lend[g] >= 1 && (uint64_t)(gp.lend[g] + diff - 1) <= len[g]) {
+            off[g] = (uint64_t)gp.lend[g] - 1;
+            plen[g] = (uint64_t)diff;
+        }
+    }
+    const uint32_t w = mh_default_seed_weight((plen[0] + plen[1]) / 2);
+    if ((int)w < MIN_DNA_SEED_WEIGHT) return MH_OK;
+    const uint64_t seed = (uint64_t)get_seed((int)w, 0);
+    // a piece shorter than the seed has an empty sorted mer list: no seeds pair
+    if (std::min(plen[0], plen[1]) < (uint64_t)seed_length((int64_t)seed)) return MH_OK;
+    mh_sml *sm[2] = {nullptr, nullptr};
+    int rc = MH_OK;
+    for (int g = 0; g < 2 && rc == MH_OK; ++g) rc = mh_sml_create_device(c, seq[g] + off[g], plen[g], seed, &sm[g]);
+    std::vector<mh_match> found;
+    if (rc == MH_OK) rc = find_matches_pairwise(c, sm, 2, found, nullptr);
+    mh_sml_free(sm[0]);
+    mh_sml_free(sm[1]);
+    if (rc != MH_OK) return rc;
+    eliminate_overlaps(found);
+    for (mh_match m : found) {
+        if (m.length < ANCHOR_MIN_LENGTH) continue;
+        for (int g = 0; g < 2; ++g) {
+            if (m.start[g] == 0) continue;   // LeftEnd NO_MATCH: the reference's shift of 0 is not reached here
+            const int64_t le = left_end(m, g) + gp.lend[g] - 1;
+            m.start[g] = m.start[g] < 0 ? -le : le;
+        }
+        out.push_back(m);
+    }
+    return MH_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
+// The gaps are independent searches: up to MAX_HELPERS of them run at once,
+// each on a helper context of its own (a stream and scratch of its own), so
+// that one search's host steps overlap another's kernels.  The results are
+// kept per gap and returned in gap order.
 int mh_anchor_gaps(mh_context *ctx, const void *d_seq_a, uint64_t len_a, const void *d_seq_b, uint64_t len_b,
                    const mh_anchor_gap *gaps, uint64_t n_gaps, mh_match_list **out, uint64_t *first)
 {
     MH_CHECK(ctx && out && (n_gaps == 0 || (gaps && d_seq_a && d_seq_b && first)), MH_ERR_INVALID, "null argument");
     MH_HIP(hipSetDevice(ctx->device));
+    const uint8_t *seq[2] = {static_cast<const uint8_t *>(d_seq_a), static_cast<const uint8_t *>(d_seq_b)};
+    const uint64_t len[2] = {len_a, len_b};
+    std::vector<std::vector<mh_match>> per(n_gaps);
+    constexpr uint64_t MAX_HELPERS = 4;
+    const uint64_t nh = std::min<uint64_t>(MAX_HELPERS, n_gaps);
+    if (nh <= 1) {
+        for (uint64_t gi = 0; gi < n_gaps; ++gi) {
+            const int rc = search_gap(ctx, seq, len, gaps[gi], per[gi]);
+            if (rc != MH_OK) return rc;
+        }
+    } else {
+        // the sequences may have been written on the caller's stream
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+        while (ctx->helpers.size() < nh) {
+            mh_context *h = nullptr;
+            const int rc = mh_create(ctx->device, &h);
+            if (rc != MH_OK) return rc;
+            ctx->helpers.push_back(h);
+        }
+        std::vector<int> rcs(nh, MH_OK);
+        std::vector<std::string> msgs(nh);
+        host_parallel(nh, [&](size_t t) {
+            mh_context *h = ctx->helpers[t];
+            h->timing = ctx->timing;
+            if (hipSetDevice(ctx->device) != hipSuccess) {
+                rcs[t] = MH_ERR_DEVICE;
+                msgs[t] = "hipSetDevice failed";
+                return;
+            }
+            for (uint64_t gi = t; gi < n_gaps; gi += nh) {
+                const int rc = search_gap(h, seq, len, gaps[gi], per[gi]);
+                if (rc != MH_OK) {
+                    rcs[t] = rc;
+                    msgs[t] = mh_last_error();
+                    return;
+                }
+            }
+        });
+        // the helpers' kernel and host timers count as this context's
+        for (mh_context *h : ctx->helpers) {
+            collect_timers(h);
+            for (auto &kv : h->timers) {
+                KernelTimer &k = ctx->timers[kv.first];
+                k.total_ms += kv.second.total_ms;
+                k.launches += kv.second.launches;
+            }
+            h->timers.clear();
+        }
+        for (uint64_t t = 0; t < nh; ++t)
+            if (rcs[t] != MH_OK) {
+                set_error(msgs[t]);
+                return rcs[t];
+            }
+    }
     mh_match_list *l = new mh_match_list();
     l->seq_count = 2;
     l->files.assign(2, std::string());
     l->lengths = {len_a, len_b};
-    const uint8_t *seq[2] = {static_cast<const uint8_t *>(d_seq_a), static_cast<const uint8_t *>(d_seq_b)};
-    const uint64_t len[2] = {len_a, len_b};
-    std::vector<mh_match> found;
     for (uint64_t gi = 0; gi < n_gaps; ++gi) {
-        if (first) first[gi] = l->v.size();
-        const mh_anchor_gap &gp = gaps[gi];
-        uint64_t off[2] = {0, 0}, plen[2] = {0, 0};
-        for (int g = 0; g < 2; ++g) {
-            int64_t diff = gp.rend[g] - gp.lend[g];
-            diff = diff > 0 ? diff - 1 : 0;
-            if (diff > 0 && gp.lend[g] >= 1 && (uint64_t)(gp.lend[g] + diff - 1) <= len[g]) {
-                off[g] = (uint64_t)gp.lend[g] - 1;
-                plen[g] = (uint64_t)diff;
-            }
-        }
-        const uint32_t w = mh_default_seed_weight((plen[0] + plen[1]) / 2);
-        if ((int)w < MIN_DNA_SEED_WEIGHT) continue;
-        const uint64_t seed = (uint64_t)get_seed((int)w, 0);
-        // a piece shorter than the seed has an empty sorted mer list: no seeds pair
-        if (std::min(plen[0], plen[1]) < (uint64_t)seed_length((int64_t)seed)) continue;
-        mh_sml *sm[2] = {nullptr, nullptr};
-        int rc = MH_OK;
-        for (int g = 0; g < 2 && rc == MH_OK; ++g) rc = mh_sml_create_device(ctx, seq[g] + off[g], plen[g], seed, &sm[g]);
-        found.clear();
-        if (rc == MH_OK) rc = find_matches_pairwise(ctx, sm, 2, found, nullptr);
-        mh_sml_free(sm[0]);
-        mh_sml_free(sm[1]);
-        if (rc != MH_OK) {
-            delete l;
-            return rc;
-        }
-        eliminate_overlaps(found);
-        for (mh_match m : found) {
-            if (m.length < ANCHOR_MIN_LENGTH) continue;
-            for (int g = 0; g < 2; ++g) {
-                if (m.start[g] == 0) continue;   // LeftEnd NO_MATCH: the reference's shift of 0 is not reached here
-                const int64_t le = left_end(m, g) + gp.lend[g] - 1;
-                m.start[g] = m.start[g] < 0 ? -le : le;
-            }
-            l->v.push_back(m);
-        }
+        first[gi] = l->v.size();
+        l->v.insert(l->v.end(), per[gi].begin(), per[gi].end());
     }
     if (first) first[n_gaps] = l->v.size();
     *out = l;

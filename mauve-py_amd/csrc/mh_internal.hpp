@@ -108,6 +108,9 @@ struct mh_context {
     mh::DevBuf grid_scratch[11];   // spaced-seed extension by chains
     mh::DevBuf diag_scratch[8];    // long extensions resolved per diagonal
     uint64_t last_walks = 0, last_long_walks = 0;   // of the last seed extension
+    // contexts (own streams and scratch) that run independent searches
+    // concurrently for this one, created on first use
+    std::vector<mh_context *> helpers;
     // scratch for sorting
     mh::DevBuf sort_tmp[4];
     // pinned host staging for device-to-host transfers

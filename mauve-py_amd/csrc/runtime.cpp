@@ -300,6 +300,7 @@ void mh_destroy(mh_context *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (mh_context *h : ctx->helpers) mh_destroy(h);
     collect_timers(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     ctx->dp_tb.release();
