@@ -45,6 +45,27 @@ struct Entry {
     const char *seq = nullptr, *seq_end = nullptr;   // raw sequence text (with whitespace)
 };
 
+// The first c in [p, end), or end: one memchr for short ranges, else the
+// lowest hit among 1 MiB pieces searched in parallel.
+const char *find_char(const char *p, const char *end, char c)
+{
+    const uint64_t n = (uint64_t)(end - p), piece = 1 << 20;
+    if (n <= 2 * piece) {
+        const void *hit = std::memchr(p, c, (size_t)n);
+        return hit ? static_cast<const char *>(hit) : end;
+    }
+    const uint64_t np = (n + piece - 1) / piece;
+    std::vector<uint64_t> at(np, n);
+    host_parallel(np, [&](size_t k) {
+        const uint64_t lo = k * piece, len = std::min(piece, n - lo);
+        const void *hit = std::memchr(p + lo, c, (size_t)len);
+        if (hit) at[k] = (uint64_t)(static_cast<const char *>(hit) - p);
+    });
+    for (uint64_t k = 0; k < np; ++k)
+        if (at[k] < n) return p + at[k];
+    return end;
+}
+
 // Parses the entry whose '>' is at p (a line start inside [.., end)); on a
 // match sets e and returns the position after the entry's sequence, else
 // nullptr.
@@ -79,8 +100,7 @@ const char *parse_entry(const char *p, const char *end, Entry &e)
     if (q < end && *q != '\n') return nullptr;
     // the sequence: everything up to the next '>' or the group's end
     const char *s = q;
-    const void *gt = std::memchr(q, '>', (size_t)(end - q));
-    q = gt ? static_cast<const char *>(gt) : end;
+    q = find_char(q, end, '>');
     if (q == s) return nullptr;
     e.num = v[0];
     e.start = v[1];
@@ -266,8 +286,8 @@ int mh_index_from_xmfa(const char *xmfa, uint64_t len, uint64_t genome_length, i
             q = static_cast<const char *>(nl);
         }
         if (!gt) break;
-        const char *eq = static_cast<const char *>(std::memchr(gt, '=', (size_t)(end - gt)));
-        if (!eq) break;
+        const char *eq = find_char(gt, end, '=');
+        if (eq == end) break;
         if (eq == gt + 1) {         // '>' then '=' directly: no group starts here
             p = gt + 1;
             continue;
