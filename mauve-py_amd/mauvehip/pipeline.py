@@ -87,10 +87,13 @@ def align_pair(ctx, d_seqs, seqs, seed, xmfa_path, names, recursive=True, refine
     t0 = mark("xmfa", t0)
     table = None
     if lut:
+        t1 = t()
         table = mindex.lut_from_xmfa_file(xmfa_path, L0)
+        t1 = mark("lut.walk", t1)
         mindex.fixZeroIdx(table, s0, s1)
         mindex.fillGaps(table)
         mindex.smoothEdges(table)
+        mark("lut.passes", t1)
         t0 = mark("lut", t0)
     info = {"matches": int(len(mt)) - n_rec, "seed_tuples": int(stats.seed_tuples), "chain_anchors": int(len(chain)),
             "recursive_anchors": n_rec, "columns": len(ra)}

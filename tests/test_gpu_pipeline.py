@@ -47,7 +47,7 @@ def test_pipeline_stages(ctx, pair, tmp_path):
     ra, rb = ra.tobytes(), rb.tobytes()
     assert {k for k in tm if "." not in k} == {"sml", "match", "chain", "recurse", "refine", "xmfa", "lut"}
     assert {k for k in tm if "." in k} <= {"chain.dp", "chain.rows", "recurse.gaps", "recurse.search",
-                                           "recurse.chain"}
+                                           "recurse.chain", "lut.walk", "lut.passes"}
     assert ra.replace(b"-", b"") == g and rb.replace(b"-", b"") == r
     want = ro.gapped_alignment(ua.decode(), ub.decode())
     assert (ra.decode(), rb.decode()) == want
