@@ -1088,7 +1088,6 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
     packed_rows(sr);
 
     uint64_t first = 0;
-    std::vector<DevJob> dj;
     std::vector<DpItem> shaped;
     while (first < nitems) {
         // one round: as many jobs (in schedule order) as fit the trace budget
@@ -1134,7 +1133,11 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         while (nwave < n64 &&
                jobs[items[first + nwave].job].a_len + jobs[items[first + nwave].job].b_len >= WAVE_WALK_MIN)
             ++nwave;
-        dj.resize(n);
+        DevJob *dj = nullptr;
+        {
+            const int rc0 = pinned_jobs(ctx, n * sizeof(DevJob), (void **)&dj);
+            if (rc0 != MH_OK) return rc0;
+        }
         uint64_t tw = 0, bo = 0;
         for (uint64_t x = 0; x < n; ++x) {
             const DpItem &it = items[first + x];
@@ -1161,7 +1164,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         if ((rc = ctx->dp_jobs.ensure(n * sizeof(DevJob) + n * sizeof(DpOut) + 256)) != MH_OK) return rc;
         DevJob *d_jobs = ctx->dp_jobs.as<DevJob>();
         DpOut *d_outs = reinterpret_cast<DpOut *>(d_jobs + n);
-        MH_HIP(hipMemcpyAsync(d_jobs, dj.data(), n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
+        MH_HIP(hipMemcpyAsync(d_jobs, dj, n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
         {
             TimedLaunch tl(ctx, "dp_forward");
             if (n64)
@@ -1203,6 +1206,30 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         MH_HIP(hipStreamSynchronize(ctx->stream));
         first = last;
     }
+    return MH_OK;
+}
+
+__global__ void pack_paths_kernel(const uint8_t *__restrict__ p, uint64_t n, uint8_t *__restrict__ out)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k * 4 >= n) return;
+    uint32_t v = 0;
+    for (int x = 0; x < 4; ++x) {
+        const uint64_t at = k * 4 + x;
+        const uint8_t e = at < n ? p[at] : 0;
+        const uint32_t c = e == 'M' ? 0u : (e == 'D' ? 1u : (e == 'I' ? 2u : 3u));
+        v |= c << (2 * x);
+    }
+    out[k] = (uint8_t)v;
+}
+
+int dp_pack_paths(mh_context *ctx, const uint8_t *d_paths, uint64_t n, uint8_t *d_out)
+{
+    const uint64_t nb = (n + 3) / 4;
+    if (!nb) return MH_OK;
+    hipLaunchKernelGGL(pack_paths_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, ctx->stream, d_paths, n,
+                       d_out);
+    MH_HIP(hipGetLastError());
     return MH_OK;
 }
 

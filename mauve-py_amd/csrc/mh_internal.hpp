@@ -113,9 +113,13 @@ struct mh_context {
     std::vector<mh_context *> helpers;
     // scratch for sorting
     mh::DevBuf sort_tmp[4];
-    // pinned host staging for device-to-host transfers
+    // pinned host staging for device-to-host transfers; a second one for the
+    // DP job tables
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
+    void *pinned_jobs = nullptr;
+    size_t pinned_jobs_bytes = 0;
+    mh::DevBuf dp_packed;   // trace-back paths at two bits an edge
     // sorted mer lists: live ones (orphaned when the context goes first) and
     // released buffer sets kept for the next build
     std::set<mh_sml *> smls;
@@ -160,10 +164,14 @@ void host_parallel(size_t n, const std::function<void(size_t)> &f);
 unsigned host_threads();
 // Pinned host staging buffer of at least `bytes` (owned by the context).
 int pinned(mh_context *ctx, size_t bytes, void **out);
+int pinned_jobs(mh_context *ctx, size_t bytes, void **out);   // a second pinned buffer, for DP job tables
 
 // Launchers (return MH_OK or an error code); all run on ctx->stream.
 int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
            uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results);
+// paths[0, n) ('M', 'D', 'I') packed four to a byte into out: edge x at bits
+// 2(x mod 4) of byte x / 4, codes M 0, D 1, I 2 (3 for anything else)
+int dp_pack_paths(mh_context *ctx, const uint8_t *d_paths, uint64_t n, uint8_t *d_out);
 // Banded DP with edge fallback (bands null: all full); d_dev may be null.
 int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs, uint64_t njobs,
                   const uint32_t *bands, uint32_t flags, uint8_t *d_paths, mh_dp_result *d_results,

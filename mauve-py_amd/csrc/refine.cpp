@@ -640,7 +640,12 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
         rc = dp_run(ctx, ctx->dp_a.as<uint8_t>(), ctx->dp_b.as<uint8_t>(), jobs.data(), njobs,
                     ctx->dp_paths.as<uint8_t>(), ctx->dp_res.as<mh_dp_result>());
         if (rc != MH_OK) return rc;
-        MH_HIP(hipMemcpyAsync(stage + abytes + bbytes, ctx->dp_paths.ptr, pbytes, hipMemcpyDeviceToHost, ctx->stream));
+        // the paths come back at two bits an edge
+        if ((rc = ctx->dp_packed.ensure(pbytes / 4 + 64)) != MH_OK) return rc;
+        if ((rc = dp_pack_paths(ctx, ctx->dp_paths.as<uint8_t>(), pbytes, ctx->dp_packed.as<uint8_t>())) != MH_OK)
+            return rc;
+        MH_HIP(hipMemcpyAsync(stage + abytes + bbytes, ctx->dp_packed.ptr, (pbytes + 3) / 4, hipMemcpyDeviceToHost,
+                              ctx->stream));
         MH_HIP(hipMemcpyAsync(stage + res_off, ctx->dp_res.ptr, njobs * sizeof(mh_dp_result), hipMemcpyDeviceToHost,
                               ctx->stream));
         MH_HIP(hipStreamSynchronize(ctx->stream));
@@ -718,13 +723,13 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
                 case S_JOB: {
                     const Job &jb = t.jobs[sg.lo];
                     const mh_dp_result &r = res[jbase[k] + sg.lo];
-                    const uint8_t *e = paths + r.path_start;
                     const char *src = in[q];
                     uint64_t c = jb.lo;
                     // row 0 takes a residue on M and D, row 1 on M and I
-                    const uint8_t skip = q == 0 ? 'I' : 'D';
-                    for (uint32_t x = 0; x < r.path_len; ++x) {
-                        if (e[x] == skip) {
+                    // (two-bit codes M 0, D 1, I 2)
+                    const uint32_t skip = q == 0 ? 2u : 1u;
+                    for (uint64_t at = r.path_start, e = r.path_start + r.path_len; at < e; ++at) {
+                        if (((paths[at >> 2] >> (2 * (at & 3))) & 3u) == skip) {
                             *w++ = '-';
                             continue;
                         }
@@ -756,13 +761,31 @@ int mh_refine_pairwise(mh_context *ctx, uint64_t n, const char *const *rows1, co
     if (stats) std::memset(stats, 0, sizeof(*stats));
     std::vector<const char *> in1(n), in2(n);
     std::vector<uint64_t> lens(n);
+    // 1 MiB pieces of every row, for the '.' check and the final copy
+    struct Span {
+        uint64_t iv, lo, hi;
+    };
+    std::vector<Span> spans;
+    auto pieces_of = [&](const std::vector<uint64_t> &len) {
+        spans.clear();
+        for (uint64_t i = 0; i < n; ++i)
+            for (uint64_t lo = 0; lo < len[i]; lo += 1 << 20) spans.push_back({i, lo, std::min(len[i], lo + (1 << 20))});
+    };
     for (uint64_t i = 0; i < n; ++i) {
         MH_CHECK(rows1[i] && rows2[i], MH_ERR_INVALID, "null row");
-        MH_CHECK(!std::memchr(rows1[i], '.', cols[i]) && !std::memchr(rows2[i], '.', cols[i]), MH_ERR_INVALID,
-                 "'.' as a gap character is not supported (use '-')");
         in1[i] = rows1[i];
         in2[i] = rows2[i];
         lens[i] = cols[i];
+    }
+    {
+        pieces_of(lens);
+        std::atomic<int> dot(0);
+        parallel_for(spans.size(), [&](size_t k) {
+            const Span &p = spans[k];
+            if (std::memchr(rows1[p.iv] + p.lo, '.', p.hi - p.lo) || std::memchr(rows2[p.iv] + p.lo, '.', p.hi - p.lo))
+                dot = 1;
+        });
+        MH_CHECK(!dot, MH_ERR_INVALID, "'.' as a gap character is not supported (use '-')");
     }
     std::vector<char *> o1, o2;
     std::vector<uint64_t> ol;
@@ -791,9 +814,11 @@ int mh_refine_pairwise(mh_context *ctx, uint64_t n, const char *const *rows1, co
     for (uint64_t i = 0; i < n; ++i)
         MH_CHECK(lens[i] <= caps[i], MH_ERR_INVALID,
                  "output capacity too small for interval " + std::to_string(i) + " (out_cols holds the sizes)");
-    parallel_for(n, [&](size_t i) {
-        std::memcpy(out1[i], in1[i], lens[i]);
-        std::memcpy(out2[i], in2[i], lens[i]);
+    pieces_of(lens);
+    parallel_for(spans.size(), [&](size_t k) {
+        const Span &p = spans[k];
+        std::memcpy(out1[p.iv] + p.lo, in1[p.iv] + p.lo, p.hi - p.lo);
+        std::memcpy(out2[p.iv] + p.lo, in2[p.iv] + p.lo, p.hi - p.lo);
     });
     return MH_OK;
 }

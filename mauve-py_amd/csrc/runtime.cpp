@@ -114,6 +114,7 @@ int pinned(mh_context *ctx, size_t bytes, void **out)
 {
     if (bytes > ctx->pinned_bytes) {
         if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->pinned_jobs) (void)hipHostFree(ctx->pinned_jobs);
         ctx->pinned = nullptr;
         ctx->pinned_bytes = 0;
         const size_t want = bytes + bytes / 4 + 4096;
@@ -126,6 +127,25 @@ int pinned(mh_context *ctx, size_t bytes, void **out)
         ctx->pinned_bytes = want;
     }
     *out = ctx->pinned;
+    return MH_OK;
+}
+
+int pinned_jobs(mh_context *ctx, size_t bytes, void **out)
+{
+    if (bytes > ctx->pinned_jobs_bytes) {
+        if (ctx->pinned_jobs) (void)hipHostFree(ctx->pinned_jobs);
+        ctx->pinned_jobs = nullptr;
+        ctx->pinned_jobs_bytes = 0;
+        const size_t want = bytes + bytes / 4 + 4096;
+        if (hipHostMalloc(&ctx->pinned_jobs, want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->pinned_jobs = nullptr;
+            set_error("pinned host allocation failed");
+            return MH_ERR_NOMEM;
+        }
+        ctx->pinned_jobs_bytes = want;
+    }
+    *out = ctx->pinned_jobs;
     return MH_OK;
 }
 
