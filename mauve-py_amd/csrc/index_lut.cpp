@@ -40,6 +40,16 @@ inline bool is_ws(char c)
 }
 inline bool is_digit(char c) { return c >= '0' && c <= '9'; }
 
+// '-' among the 8 bytes at p (exact zero-byte test of p ^ '-').
+inline unsigned gaps8(const char *p)
+{
+    uint64_t v;
+    std::memcpy(&v, p, 8);
+    v ^= 0x2D2D2D2D2D2D2D2Dull;
+    const uint64_t lo7 = 0x7F7F7F7F7F7F7F7Full;
+    return (unsigned)__builtin_popcountll(~(((v & lo7) + lo7) | v | lo7));
+}
+
 struct Entry {
     int64_t num = 0, start = 0;
     const char *seq = nullptr, *seq_end = nullptr;   // raw sequence text (with whitespace)
@@ -118,21 +128,47 @@ struct Row {
     uint64_t n;
 };
 
+// True when none of the 8 bytes at p is below 33 (so none is whitespace).
+inline bool no_ws8(const char *p)
+{
+    uint64_t v;
+    std::memcpy(&v, p, 8);
+    return ((v - 0x2121212121212121ull) & ~v & 0x8080808080808080ull) == 0;
+}
+
 Row squeeze(const Entry &e, HostBuf &buf)
 {
-    const uint64_t n = (uint64_t)(e.seq_end - e.seq), chunk = 1 << 20, nc = (n + chunk - 1) / chunk;
+    const uint64_t n = (uint64_t)(e.seq_end - e.seq), chunk = 1 << 18, nc = (n + chunk - 1) / chunk;
     std::vector<uint64_t> at(nc + 1, 0);
     parallel_chunks(nc, [&](uint64_t k) {
         const char *c = e.seq + k * chunk, *ce = e.seq + std::min(n, (k + 1) * chunk);
         uint64_t kept = 0;
+        for (; c + 8 <= ce; c += 8) {
+            if (no_ws8(c)) {
+                kept += 8;
+                continue;
+            }
+            for (int x = 0; x < 8; ++x) kept += !is_ws(c[x]);
+        }
         for (; c < ce; ++c) kept += !is_ws(*c);
         at[k + 1] = kept;
     });
     for (uint64_t k = 0; k < nc; ++k) at[k + 1] += at[k];
-    char *out = buf.get<char>(at[nc]);
+    char *out = buf.get<char>(at[nc] + 8);
     parallel_chunks(nc, [&](uint64_t k) {
         const char *c = e.seq + k * chunk, *ce = e.seq + std::min(n, (k + 1) * chunk);
         char *w = out + at[k];
+        for (; c + 8 <= ce; c += 8) {
+            if (no_ws8(c)) {
+                std::memcpy(w, c, 8);
+                w += 8;
+                continue;
+            }
+            for (int x = 0; x < 8; ++x) {
+                *w = c[x];
+                w += !is_ws(c[x]);
+            }
+        }
         for (; c < ce; ++c) {
             *w = *c;
             w += !is_ws(*c);
@@ -170,7 +206,12 @@ void walk(Row gr, Row rr, int64_t g0, int64_t r0, uint64_t genome_length, int32_
     parallel_chunks(nc, [&](uint64_t k) {
         const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
         int64_t a = 0, b = 0;
-        for (uint64_t c = lo; c < hi; ++c) {
+        uint64_t c = lo;
+        for (; c + 8 <= hi; c += 8) {
+            a += 8 - (int64_t)gaps8(gs + c);
+            b += 8 - (int64_t)gaps8(rs + c);
+        }
+        for (; c < hi; ++c) {
             a += gs[c] != '-';
             b += rs[c] != '-';
         }
@@ -341,7 +382,7 @@ int mh_index_from_xmfa_file(const char *path, uint64_t genome_length, int32_t *l
         MH_CHECK(false, MH_ERR_NOMEM, "out of host memory");
     }
     // read in parallel 4 MiB pieces
-    const uint64_t piece = 4ull << 20, np = (n + piece - 1) / piece;
+    const uint64_t piece = 1ull << 20, np = (n + piece - 1) / piece;
     std::atomic<int> bad{0};
     parallel_chunks(np, [&](uint64_t k) {
         uint64_t off = k * piece;

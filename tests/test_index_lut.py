@@ -230,3 +230,27 @@ def test_lut_from_file_matches_text(tmp_path):
     assert np.array_equal(mi.lut_from_xmfa_file(str(path), n + 5), mi.lut_from_xmfa(text, n + 5))
     with pytest.raises(mi.MhError):
         mi.lut_from_xmfa_file(str(tmp_path / "missing.xmfa"), 10)
+
+
+def test_rows_with_scattered_whitespace():
+    """Sequence text with every whitespace character the grammar removes
+    (space, tab, CR, VT, FF, LF) at random places and in runs, over more than
+    one squeeze chunk: the library removes them eight bytes at a time where a
+    group holds none."""
+    rng = np.random.default_rng(77)
+    n = 400_000
+    alpha = np.array(list("ACGT-"))
+    g = alpha[rng.choice(5, n, p=[.24, .24, .24, .24, .04])]
+    r = g.copy()
+    flip = rng.random(n) < 0.05
+    r[flip] = alpha[rng.integers(0, 5, int(flip.sum()))]
+    ws = np.array([" ", "\t", "\r", "\v", "\f", "\n", "  \t", "\r\n"])
+
+    def body(seq):
+        out = list(seq)
+        for at in sorted(rng.choice(len(out), 30_000, replace=False), reverse=True):
+            out.insert(int(at), ws[rng.integers(0, len(ws))])
+        return "".join(out)
+
+    text = f"> 1:3-{n + 2} + g\n{body(g)}\n> 2:7-{n + 6} + r\n{body(r)}\n=\n"
+    _both_walks(text, n + 10)
