@@ -228,7 +228,22 @@ void walk(Row gr, Row rr, int64_t g0, int64_t r0, uint64_t genome_length, int32_
     parallel_chunks(nc, [&](uint64_t k) {
         const uint64_t lo = k * chunk, hi = std::min(n, lo + chunk);
         int64_t gi = cg[k], ri = cr[k];
-        for (uint64_t c = lo; c < hi; ++c) {
+        uint64_t c = lo;
+        while (c < hi) {
+            // eight equal columns without gaps: both indices step by one each,
+            // and every column maps (none is before the chunk's first residue)
+            if (c + 8 <= hi && gi >= -1 && gi + 8 < (int64_t)genome_length) {
+                uint64_t va, vb;
+                std::memcpy(&va, gs + c, 8);
+                std::memcpy(&vb, rs + c, 8);
+                if (va == vb && gaps8(gs + c) == 0) {
+                    for (int x = 1; x <= 8; ++x) lut[gi + x] = (int32_t)(ri + x);
+                    gi += 8;
+                    ri += 8;
+                    c += 8;
+                    continue;
+                }
+            }
             const char a = gs[c], b = rs[c];
             gi += a != '-';
             ri += b != '-';
@@ -236,6 +251,7 @@ void walk(Row gr, Row rr, int64_t g0, int64_t r0, uint64_t genome_length, int32_
                 if (gi == cg[k]) early[k].push_back({gi, ri});
                 else put(gi, ri);
             }
+            ++c;
         }
     });
     for (uint64_t k = 0; k < nc; ++k)
