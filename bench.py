@@ -298,8 +298,8 @@ def full_leg(ctx, g, r, seed, steps, warmup):
     cells = int(rs.profile_cells + rs.window_cells)
     kern = {}
     for name in ("sml_pack", "sml_keys", "sml_sort", "join", "extend", "partition", "dp_forward", "dp_traceback",
-                 "host_replay", "host_dp_plan", "host_refine_pieces", "host_refine_plan", "host_refine_dp",
-                 "host_refine_splice"):
+                 "host_replay", "host_dp_plan", "host_refine_prefix", "host_refine_pieces", "host_refine_plan",
+                 "host_refine_dp", "host_refine_splice"):
         ms, n = ctx.kernel_time(name)
         kern[name] = {"avg_ms": round(ms, 4), "launches_per_step": n / steps, "ms_per_step": round(ms * n / steps, 3)}
     fwd = kern["dp_forward"]["ms_per_step"]

@@ -547,6 +547,7 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
             m2 += Prefix::entries(lens[i]);
         }
     }
+    lap("host_refine_prefix");
     parallel_for(n, [&](size_t i) {
         const uint64_t len = lens[i];
         // an interval that holds one genome only is not refined (the
