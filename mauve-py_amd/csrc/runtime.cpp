@@ -228,11 +228,22 @@ public:
     unsigned size() const { return (unsigned)workers_.size() + 1; }
     void run(size_t n, const std::function<void(size_t)> &f)
     {
-        std::unique_lock<std::mutex> busy(call_mu_, std::try_to_lock);
-        if (!busy.owns_lock() || workers_.empty() || n < 2) {
+        // a single task, or a call made from inside a task (a worker, or the
+        // caller while it takes part), runs here; so does a call made while
+        // another thread's job holds the pool
+        if (n < 2 || workers_.empty() || inside_) {
             for (size_t i = 0; i < n; ++i) f(i);
             return;
         }
+        std::unique_lock<std::mutex> busy(call_mu_, std::try_to_lock);
+        if (!busy.owns_lock()) {
+            for (size_t i = 0; i < n; ++i) f(i);
+            return;
+        }
+        struct Inside {
+            Inside() { inside_ = true; }
+            ~Inside() { inside_ = false; }
+        } mark;
         {
             std::lock_guard<std::mutex> lk(mu_);
             job_ = &f;
@@ -250,6 +261,7 @@ public:
 private:
     void loop()
     {
+        inside_ = true;
         uint64_t seen = 0;
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
@@ -266,6 +278,7 @@ private:
             if (--active_ == 0) done_.notify_all();
         }
     }
+    static thread_local bool inside_;
     std::vector<std::thread> workers_;
     std::mutex mu_, call_mu_;
     std::condition_variable cv_, done_;
@@ -276,6 +289,8 @@ private:
     uint64_t gen_ = 0;
     bool stop_ = false;
 };
+
+thread_local bool ThreadPool::inside_ = false;
 
 ThreadPool &pool()
 {
