@@ -114,7 +114,6 @@ int pinned(mh_context *ctx, size_t bytes, void **out)
 {
     if (bytes > ctx->pinned_bytes) {
         if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    if (ctx->pinned_jobs) (void)hipHostFree(ctx->pinned_jobs);
         ctx->pinned = nullptr;
         ctx->pinned_bytes = 0;
         const size_t want = bytes + bytes / 4 + 4096;
@@ -350,6 +349,7 @@ void mh_destroy(mh_context *ctx)
     for (mh_sml *s : ctx->smls) s->ctx = nullptr;   // lists outliving the context free their own buffers
     for (auto &b : ctx->sml_pool) b.release();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->pinned_jobs) (void)hipHostFree(ctx->pinned_jobs);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

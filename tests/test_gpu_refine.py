@@ -51,3 +51,24 @@ def test_refine_rejects_other_residues(ctx):
     with pytest.raises(mauvehip.MhError) as e:
         mauvehip.refine_pairwise(ctx, [("ACGTNACGT" * 20, "ACGTAACGT" * 20)])
     assert e.value.code == -6
+
+
+def test_refine_after_dp_batches_and_growing_staging():
+    """A fresh context that first runs a batched DP (its job table in pinned
+    memory), then refinement batches whose pinned staging grows from one call
+    to the next: every result equals the oracle's.  (Growing the staging once
+    released the job table's pinned memory, which the next DP call then wrote
+    into.)"""
+    from mauvehip import synth
+    c = mauvehip.Context(0)
+    try:
+        A, ao, al, B, bo, bl = synth.dp_jobs(n=3000, seed=301, max_ratio=300)
+        mauvehip.global_align(c, A, B, mauvehip.make_jobs(ao, al, bo, bl))
+        for seed, count in ((61, 2), (62, 6), (63, 12)):
+            ivs = _cases(seed, count)
+            got = [(a.decode(), b.decode()) for a, b in mauvehip.refine_pairwise(c, ivs)]
+            for (r1, r2), g in zip(ivs, got):
+                assert g == ro.gapped_alignment(r1, r2)
+            mauvehip.global_align(c, A, B, mauvehip.make_jobs(ao, al, bo, bl))
+    finally:
+        c.close()
