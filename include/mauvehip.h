@@ -141,7 +141,8 @@ typedef struct {
  * LM/PairwiseMatchFinder.cpp:37-71, LM/MatchFinder.cpp:172-340,
  * LM/MemHash.cpp:167-251, LM/MatchFinder.h:218-374): all matches between the
  * k sorted mer lists, in GetMatchList order (LM/MemHash.h:182-203).
- * This build supports k == 2.  If *n_out exceeds cap, only cap entries are
+ * k is 2 to 4 (MH_MAX_SEQ), as progressiveMauve uses this finder for up to four
+ * genomes (MA/progressiveMauve.cpp:493-504).  If *n_out exceeds cap, only cap entries are
  * written and the call still returns MH_OK with the full count. */
 int mh_find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k,
                              mh_match *out, uint64_t cap, uint64_t *n_out,

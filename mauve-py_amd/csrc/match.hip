@@ -57,6 +57,42 @@ __device__ __forceinline__ uint64_t upper_bound_m(const uint64_t *__restrict__ c
     return lo;
 }
 
+// the masked mer of each seed's genome-a start
+__global__ void seed_mer_kernel(const SeedRec *__restrict__ seeds, uint64_t ns, const uint64_t *__restrict__ kp,
+                                uint64_t *__restrict__ out)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ns) return;
+    out[x] = kp[seeds[x].s0 - 1] >> 1;
+}
+
+// Up to four sorted lists.
+struct KLists {
+    const uint64_t *ck[4];
+    uint64_t n[4];
+    int k;
+};
+
+// Runs of one masked mer over all k lists above MER_REPEAT_LIMIT, each counted
+// from the first list that holds it.
+__global__ void repeat_k_kernel(KLists L, int i, uint32_t *__restrict__ runs)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= L.n[i]) return;
+    const uint64_t *ci = L.ck[i];
+    const uint64_t m = ci[x] >> 1;
+    if (x > 0 && (ci[x - 1] >> 1) == m) return;
+    uint64_t total = 0;
+    for (int j = 0; j < L.k; ++j) {
+        const uint64_t lo = lower_bound_m(L.ck[j], L.n[j], m);
+        const bool hit = lo < L.n[j] && (L.ck[j][lo] >> 1) == m;
+        if (!hit) continue;
+        if (j < i) return;   // counted from list j
+        total += upper_bound_m(L.ck[j], L.n[j], m) - lo;
+    }
+    if (total > MER_REPEAT_LIMIT) atomicAdd(runs, 1u);
+}
+
 // flag[i] = 1 if genome-0 entry i forms a seed; partner[i] = its genome-1 index
 __global__ void join_kernel(const uint64_t *__restrict__ ck0, uint64_t n0, const uint64_t *__restrict__ ck1,
                             uint64_t n1, uint32_t *__restrict__ flag, uint32_t *__restrict__ partner,
@@ -963,6 +999,38 @@ static int extend_by_chains(mh_context *ctx, const mh_sml *a, const mh_sml *b, c
     hipLaunchKernelGGL(grid_fill_kernel, grid, blk, 0, ctx->stream, oi, oj, ns, a->d_keys, a->d_pos, b->d_keys,
                        b->d_pos, head_of, headpos, hres, d_seeds);
     MH_HIP(hipGetLastError());
+    return MH_OK;
+}
+
+int seed_mers(mh_context *ctx, const mh_sml *a, const SeedRec *d_seeds, uint64_t ns, uint64_t *d_out)
+{
+    if (!ns) return MH_OK;
+    hipLaunchKernelGGL(seed_mer_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, ctx->stream, d_seeds, ns,
+                       a->d_keys_pos, d_out);
+    MH_HIP(hipGetLastError());
+    return MH_OK;
+}
+
+int repeat_runs_k(mh_context *ctx, const mh_sml *const *smls, int k, uint32_t *runs)
+{
+    KLists L;
+    std::memset(&L, 0, sizeof(L));
+    L.k = k;
+    for (int i = 0; i < k; ++i) {
+        L.ck[i] = smls[i]->d_keys;
+        L.n[i] = smls[i]->n;
+    }
+    int rc;
+    if ((rc = ctx->mf_scratch[3].ensure(64)) != MH_OK) return rc;
+    uint32_t *d = ctx->mf_scratch[3].as<uint32_t>();
+    MH_HIP(hipMemsetAsync(d, 0, 4, ctx->stream));
+    for (int i = 0; i < k; ++i)
+        if (L.n[i])
+            hipLaunchKernelGGL(repeat_k_kernel, dim3((unsigned)((L.n[i] + 255) / 256)), dim3(256), 0, ctx->stream, L,
+                               i, d);
+    MH_HIP(hipGetLastError());
+    MH_HIP(hipMemcpyAsync(runs, d, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
     return MH_OK;
 }
 

@@ -82,12 +82,218 @@ inline size_t lower_bound_ent(const std::vector<Ent> &v, const Ent &q)
     return first;
 }
 
+// A MatchHashEntry of up to four genomes (start 0: absent), first = the
+// index of its first present genome (FirstStart).
+struct EntK {
+    int64_t s[4];
+    uint64_t len;
+    int64_t off;
+    uint64_t mersize;
+    int first;
+};
+
+// MatchHashEntry::Contains (a contains b), LM/MatchHashEntry.cpp:164-200
+inline bool contains_k(const EntK &a, const EntK &b, int k)
+{
+    if (a.off != b.off) return false;
+    int i = b.first;
+    const int64_t diff = b.s[i] - a.s[i];
+    if (a.s[i] == 0) return false;
+    if (diff < 0 || a.len < b.len + (uint64_t)diff) return false;
+    const int64_t diff_rc = (int64_t)b.len - (int64_t)a.len + diff;
+    for (++i; i < k; ++i) {
+        const int64_t di = b.s[i] - a.s[i];
+        if (b.s[i] == 0 && a.s[i] == 0) continue;
+        if (b.s[i] < 0 && diff_rc == di) continue;
+        if (diff != di) return false;
+    }
+    return true;
+}
+
+// strict_start_lessthan_ptr, LM/MatchHashEntry.cpp:48-67
+inline bool strict_less_k(const EntK &a, const EntK &b, int k)
+{
+    if (a.first == b.first)
+        for (int i = 0; i < k; ++i) {
+            int64_t as = a.s[i], bs = b.s[i];
+            if (as < 0) as = (int64_t)((uint64_t)(-as) + a.len - a.mersize);
+            if (bs < 0) bs = (int64_t)((uint64_t)(-bs) + b.len - b.mersize);
+            if (as != bs) return as < bs;
+        }
+    return a.first < b.first;
+}
+
+// MheCompare, LM/MatchHashEntry.h:121-143
+inline bool mhe_less_k(const EntK &a, const EntK &b, int k)
+{
+    if (a.first > b.first) return true;
+    if (a.first != b.first) return false;
+    for (int i = a.first; i < k; ++i) {
+        if (a.s[i] == 0 && b.s[i] != 0) return true;
+        if (a.s[i] != 0 && b.s[i] == 0) return false;
+    }
+    if (contains_k(a, b, k) || contains_k(b, a, k)) return false;
+    return strict_less_k(a, b, k);
+}
+
+inline size_t lower_bound_k(const std::vector<EntK> &v, const EntK &q, int k)
+{
+    size_t first = 0, len = v.size();
+    while (len > 0) {
+        const size_t half = len >> 1, mid = first + half;
+        if (mhe_less_k(v[mid], q, k)) {
+            first = mid + 1;
+            len = len - half - 1;
+        } else {
+            len = half;
+        }
+    }
+    return first;
+}
+
+// CalculateOffset, LM/MatchHashEntry.cpp:141-160 (Length is the seed length
+// when HashMatch calls it)
+inline int64_t offset_k(const EntK &e, int k)
+{
+    int64_t off = 0;
+    const int64_t ref = e.s[e.first];
+    for (int i = e.first + 1; i < k; ++i)
+        if (e.s[i] != 0) off += e.s[i] - ref - (e.s[i] < 0 ? (int64_t)e.len : 0);
+    return off;
+}
+
+// PairwiseMatchFinder over three or four genomes (LM/PairwiseMatchFinder.cpp:
+// 37-71 with LM/MatchFinder.cpp:172-340 and LM/MemHash.cpp:167-251): every
+// run of one masked mer hashes each pair of genomes that hold it exactly once,
+// pairs in genome order, runs in mer order.  A pair's seeds and their
+// extensions are the two-genome search's (ExtendMatch reads only the genomes
+// a seed holds); the insertion into the 40000 buckets is replayed on the
+// host for every seed, buckets in parallel, and the list comes out bucket by
+// bucket (LM/MemHash.h:182-203).
+int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::vector<mh_match> &out,
+                       mh_match_stats *st)
+{
+    for (int i = 0; i < k; ++i) {
+        MH_CHECK(smls[i], MH_ERR_INVALID, "null sorted mer list");
+        MH_CHECK(smls[i]->seed == smls[0]->seed, MH_ERR_INVALID, "Different seed patterns.");
+    }
+    uint32_t reps = 0;
+    int rc;
+    if ((rc = repeat_runs_k(ctx, smls, k, &reps)) != MH_OK) return rc;
+    if (st) std::memset(st, 0, sizeof(*st));
+    if (st) st->repeat_runs = reps;
+    MH_CHECK(reps == 0, MH_ERR_UNSUPPORTED,
+             "a seed run exceeds MER_REPEAT_LIMIT (1000); the reference's restart after such runs "
+             "(LM/MatchFinder.cpp:253-276) is not implemented");
+    const uint64_t sl = (uint64_t)smls[0]->seed_length;
+    struct Tup {
+        uint64_t mer;
+        int a, b;
+        SeedRec r;
+    };
+    std::vector<Tup> tup;
+    std::vector<SeedRec> hs;
+    std::vector<uint64_t> hm;
+    int64_t walks = 0, long_walks = 0;
+    for (int a = 0; a < k; ++a)
+        for (int b = a + 1; b < k; ++b) {
+            const SeedRec *d_seeds = nullptr;
+            uint64_t ns = 0;
+            uint32_t pr = 0;
+            if ((rc = join_and_extend(ctx, smls[a], smls[b], &d_seeds, &ns, &pr)) != MH_OK) return rc;
+            walks += (int64_t)ctx->last_walks;
+            long_walks += (int64_t)ctx->last_long_walks;
+            if (!ns) continue;
+            if ((rc = ctx->mf_scratch[0].ensure(ns * 8 + 64)) != MH_OK) return rc;
+            if ((rc = seed_mers(ctx, smls[a], d_seeds, ns, ctx->mf_scratch[0].as<uint64_t>())) != MH_OK) return rc;
+            hs.resize(ns);
+            hm.resize(ns);
+            MH_HIP(hipMemcpyAsync(hs.data(), d_seeds, ns * sizeof(SeedRec), hipMemcpyDeviceToHost, ctx->stream));
+            MH_HIP(hipMemcpyAsync(hm.data(), ctx->mf_scratch[0].ptr, ns * 8, hipMemcpyDeviceToHost, ctx->stream));
+            MH_HIP(hipStreamSynchronize(ctx->stream));
+            for (uint64_t x = 0; x < ns; ++x) tup.push_back(Tup{hm[x], a, b, hs[x]});
+        }
+    const double tg1 = now_ms();
+    // HashMatch order: runs by mer; within a run, pairs in genome order (the
+    // pair lists were appended in that order, so a stable sort keeps it)
+    std::stable_sort(tup.begin(), tup.end(), [](const Tup &x, const Tup &y) { return x.mer < y.mer; });
+    // buckets of the generalized offsets, each in HashMatch order
+    std::vector<uint32_t> bucket(tup.size()), cnt(TABLE_SIZE + 1, 0);
+    std::vector<EntK> seed(tup.size());
+    for (size_t x = 0; x < tup.size(); ++x) {
+        EntK c;
+        std::memset(&c, 0, sizeof(c));
+        c.s[tup[x].a] = tup[x].r.s0;
+        c.s[tup[x].b] = tup[x].r.s1;
+        c.len = sl;
+        c.mersize = sl;
+        c.first = tup[x].a;
+        c.off = offset_k(c, k);
+        seed[x] = c;
+        const int64_t m = c.off % (int64_t)TABLE_SIZE;
+        bucket[x] = (uint32_t)(m < 0 ? m + TABLE_SIZE : m);
+        ++cnt[bucket[x] + 1];
+    }
+    for (uint32_t q = 0; q < TABLE_SIZE; ++q) cnt[q + 1] += cnt[q];
+    std::vector<uint32_t> order(tup.size());
+    {
+        std::vector<uint32_t> at(cnt.begin(), cnt.end() - 1);
+        for (size_t x = 0; x < tup.size(); ++x) order[at[bucket[x]]++] = (uint32_t)x;
+    }
+    std::vector<std::vector<EntK>> table(TABLE_SIZE);
+    std::atomic<int64_t> collisions{0};
+    const uint32_t per = 64;
+    host_parallel((TABLE_SIZE + per - 1) / per, [&](size_t t) {
+        int64_t coll = 0;
+        for (uint32_t q = (uint32_t)t * per; q < std::min<uint32_t>(TABLE_SIZE, (uint32_t)(t + 1) * per); ++q) {
+            std::vector<EntK> &v = table[q];
+            for (uint32_t y = cnt[q]; y < cnt[q + 1]; ++y) {
+                const uint32_t x = order[y];
+                const EntK &c = seed[x];
+                size_t it = lower_bound_k(v, c, k);
+                if (it != v.size() && !mhe_less_k(c, v[it], k)) {   // !comp(*it, c) holds at lower_bound
+                    ++coll;
+                    continue;
+                }
+                EntK e = c;
+                const SeedRec &r = tup[x].r;
+                e.s[tup[x].a] = r.e0;
+                e.s[tup[x].b] = r.e1;
+                e.len = r.elen;
+                e.mersize = 0;       // MatchHashEntry::operator= zeroes m_mersize; the offset is kept
+                it = lower_bound_k(v, e, k);
+                v.insert(v.begin() + (ptrdiff_t)it, e);
+            }
+        }
+        collisions += coll;
+    });
+    host_time(ctx, "host_replay", now_ms() - tg1);
+    out.clear();
+    for (uint32_t q = 0; q < TABLE_SIZE; ++q)
+        for (const EntK &e : table[q]) {
+            mh_match m;
+            std::memset(&m, 0, sizeof(m));
+            for (int i = 0; i < k; ++i) m.start[i] = e.s[i];
+            m.length = e.len;
+            out.push_back(m);
+        }
+    if (st) {
+        st->seed_tuples = (int64_t)tup.size();
+        st->collisions = collisions.load();
+        st->matches = (int64_t)out.size();
+        st->walks = walks;
+        st->long_walks = long_walks;
+    }
+    return MH_OK;
+}
+
 }  // namespace
 
 int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std::vector<mh_match> &out,
                           mh_match_stats *st)
 {
-    MH_CHECK(k == 2, MH_ERR_UNSUPPORTED, "this build matches exactly two genomes");
+    MH_CHECK(k >= 2 && k <= MH_MAX_SEQ, MH_ERR_UNSUPPORTED, "this build matches two to four genomes");
+    if (k > 2) return find_matches_multi(ctx, smls, k, out, st);
     const mh_sml *a = smls[0], *b = smls[1];
     MH_CHECK(a && b, MH_ERR_INVALID, "null sorted mer list");
     MH_CHECK(a->seed == b->seed, MH_ERR_INVALID, "Different seed patterns.");   // LM/MatchFinder.cpp:190-194

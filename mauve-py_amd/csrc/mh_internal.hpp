@@ -207,6 +207,11 @@ struct SeedRec {
 // match search); *d_seeds points there.
 int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const SeedRec **d_seeds, uint64_t *ns,
                     uint32_t *repeat_runs);
+// The merge order of the seeds of one genome pair: genome a's sorted-list key
+// without its strand bit (the masked mer) at each seed's start.
+int seed_mers(mh_context *ctx, const mh_sml *a, const SeedRec *d_seeds, uint64_t ns, uint64_t *d_out);
+// Runs of one masked mer over the k lists longer than MER_REPEAT_LIMIT.
+int repeat_runs_k(mh_context *ctx, const mh_sml *const *smls, int k, uint32_t *runs);
 // A distinct extension of a clean bucket, and a seed of any other bucket.
 struct CleanEnt {
     uint32_t bucket;

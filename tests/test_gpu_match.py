@@ -110,6 +110,39 @@ def test_match_bucket_alias(ctx, seed):
     assert n > 20
 
 
+def _relatives(rng, ref, count, snp, inv):
+    out = [ref]
+    for _ in range(count - 1):
+        m = ref.copy()
+        hit = rng.random(len(m)) < snp
+        m[hit] = synth.BASES[(np.searchsorted(synth.BASES, m[hit]) + rng.integers(1, 4, hit.sum())) % 4]
+        at = int(rng.integers(0, len(m) - inv))
+        m[at:at + inv] = synth.revcomp(m[at:at + inv])
+        cut = int(rng.integers(0, len(m) - 50))
+        m = np.concatenate([m[:cut], m[cut + int(rng.integers(1, 30)):]])
+        out.append(m)
+    return out
+
+
+@pytest.mark.parametrize("ng,seed", [(3, po.get_seed(15, 3)), (3, po.get_seed(13, 0)), (4, po.get_seed(17, 3)),
+                                     (4, po.get_seed(15, 0))])
+def test_match_three_and_four_genomes(ctx, ng, seed):
+    """PairwiseMatchFinder over three and four genomes (solid and spaced seeds,
+    relatives with SNPs, an inversion and a deletion each): identical match
+    lists (all k starts, lengths, order) and seed/collision counts."""
+    rng = np.random.default_rng(ng * 1000 + (seed & 0xFF))
+    seqs = [s.tobytes() for s in _relatives(rng, synth.BASES[rng.integers(0, 4, 60_000)], ng, 0.03, 6000)]
+    smls = [mauvehip.SortedMerList(ctx, s, seed) for s in seqs]
+    gs, gl, gst = mauvehip.find_matches(ctx, smls)
+    os_, ol, ost = po.find_matches(seqs, seed)
+    assert gst.seed_tuples == ost.hash_calls
+    assert gst.collisions == ost.collisions
+    assert len(gl) == len(ol) > 50
+    assert np.array_equal(gs, os_)
+    assert np.array_equal(gl, ol)
+    assert (gs != 0).sum(axis=1).max() == 2      # pairwise matches: two genomes each
+
+
 def test_match_repeat_limit_refused(ctx):
     rng = np.random.default_rng(9)
     ref = np.concatenate([synth.BASES[rng.integers(0, 4, 5000)], np.full(1200, ord("A"), np.uint8),

@@ -160,3 +160,49 @@ def test_clean_bucket_closed_form(k):
         assert got[bk] == ext, bk
         clean_rc += next(iter(ls))[1]
     assert clean_rc > 0   # reverse-strand clean buckets were exercised
+
+
+@pytest.mark.parametrize("k,ng", [(11, 3), (13, 3), (11, 4)])
+def test_multi_genome_brute_force(k, ng):
+    """Three or four genomes (PairwiseMatchFinder): every run hashes each pair
+    of genomes holding the mer exactly once, so with a solid seed the seed
+    tuples are the union over genome pairs of the two-genome closed form (the
+    first genome of the pair leading, the others absent), the list is each
+    distinct extension of each pair once (entries of different pairs never
+    compare equal: MheCompare orders genome presence first), in bucket order,
+    and the collisions are counted per pair."""
+    rng = np.random.default_rng(300 + k + ng)
+    base, _ = _pair(rng, 4000)
+    seqs = [base]
+    for _ in range(ng - 1):
+        # a relative of base: its own SNPs and inversion, on base's sequence
+        t = list(base)
+        for i in np.nonzero(rng.random(len(t)) < 0.03)[0]:
+            t[i] = "ACGT"[("ACGT".index(t[i]) + int(rng.integers(1, 4))) % 4]
+        t = "".join(t)
+        at = int(rng.integers(0, 2000))
+        seqs.append(t[:at] + t[at:at + 700][::-1].translate(COMP) + t[at + 700:])
+    seed = (1 << k) - 1
+    starts, lengths, stats = po.find_matches(seqs, seed)
+    want, tuples = set(), 0
+    for x in range(ng):
+        for y in range(x + 1, ng):
+            tup = brute_tuples(seqs[x], seqs[y], k)
+            tuples += len(tup)
+            for s0, s1 in tup:
+                n0, n1, L = brute_extend(seqs[x], seqs[y], s0, s1, k)
+                st = [0] * ng
+                st[x], st[y] = n0, n1
+                want.add((tuple(st), L))
+    got = [(tuple(int(v) for v in s), int(L)) for s, L in zip(starts, lengths)]
+    assert len(got) == len(set(got))
+    assert set(got) == want
+    assert stats.hash_calls == tuples
+    assert stats.collisions == tuples - len(want)
+
+    def bucket(m):
+        st, L = m
+        first = next(i for i, v in enumerate(st) if v)
+        off = sum(v - st[first] - (L if v < 0 else 0) for v in st[first + 1:] if v)
+        return ((off % 40000) + 40000) % 40000
+    assert [bucket(m) for m in got] == sorted(bucket(m) for m in got)
