@@ -224,6 +224,9 @@ def main():
     if bandset is not None:
         out["learned_band"] = learned_band_leg(ctx, bandset)
 
+    if not args.whole_only:
+        out["multi_genome"] = multi_leg(ctx, ref, seed, args.steps, args.warmup, not args.no_cpu_baseline)
+
     # the whole path (stand-in LCB chain, recursive anchoring, refinement
     # passes, XMFA and index LUT) is the headline
     full = full_leg(ctx, mut, ref, seed, args.steps, args.warmup)
@@ -266,6 +269,67 @@ def main():
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def multi_leg(ctx, ref, seed, steps, warmup, cpu):
+    """Seed matching over four genomes (PairwiseMatchFinder): the first 1 Mb of
+    the reference and three relatives of it (1% SNPs, a 60 kb inversion and a
+    short deletion each), resident in HBM; a step builds the four sorted mer
+    lists and finds all pairwise matches.  The oracle times the same search on
+    one core and its list is compared."""
+    import mauvehip
+    from mauvehip import synth
+    rng = np.random.default_rng(77)
+    base = np.asarray(ref[:1_000_000])
+    seqs = [base]
+    for _ in range(3):
+        m = base.copy()
+        hit = rng.random(len(m)) < 0.01
+        m[hit] = synth.BASES[(np.searchsorted(synth.BASES, m[hit]) + rng.integers(1, 4, hit.sum())) % 4]
+        at = int(rng.integers(0, len(m) - 60_000))
+        m[at:at + 60_000] = synth.revcomp(m[at:at + 60_000])
+        cut = int(rng.integers(0, len(m) - 50))
+        seqs.append(np.concatenate([m[:cut], m[cut + int(rng.integers(1, 30)):]]))
+    dev = []
+    for q in seqs:
+        d = ctx.alloc(len(q) + 16)
+        ctx.h2d(d, q)
+        dev.append(d)
+    total = sum(len(q) for q in seqs)
+
+    def step():
+        smls = [mauvehip.SortedMerList.from_device(ctx, d, len(q), seed) for d, q in zip(dev, seqs)]
+        out = mauvehip.find_matches_raw(ctx, smls)
+        for x in smls:
+            x.free()
+        return out
+
+    for _ in range(max(1, warmup)):
+        mt, st = step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mt, st = step()
+    ctx.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    res = {"genomes": 4, "genome_bp": [len(q) for q in seqs], "seed": hex(seed),
+           "value": round(total / 1e6 / el, 3), "unit": "Mb/s", "ms_per_step": round(1e3 * el, 3),
+           "seed_tuples": int(st.seed_tuples), "matches": int(st.matches), "collisions": int(st.collisions)}
+    log(f"[bench] four-genome matching {res['value']:.1f} Mb/s, {res['ms_per_step']:.1f} ms/step")
+    if cpu:
+        from oracle import pyoracle as po
+        t0 = time.perf_counter()
+        os_, ol, _ = po.find_matches([q.tobytes() for q in seqs], seed)
+        cs = time.perf_counter() - t0
+        same = (len(ol) == len(mt) and np.array_equal(ol, mt["length"])
+                and np.array_equal(os_, mt["start"][:, :4]))
+        res["cpu_baseline"] = {"value": round(total / 1e6 / cs, 4), "unit": "Mb/s", "cores": 1, "kind": "port",
+                               "seconds": round(cs, 2), "sample": "the same four-genome search (oracle sorted mer "
+                               "lists and PairwiseMatchFinder), single thread", "matches_identical": bool(same)}
+        log(f"[bench] CPU oracle four-genome matching {cs:.2f}s")
+    for d in dev:
+        ctx.free(d)
+    return res
 
 
 def full_leg(ctx, g, r, seed, steps, warmup):

@@ -57,13 +57,14 @@ __device__ __forceinline__ uint64_t upper_bound_m(const uint64_t *__restrict__ c
     return lo;
 }
 
-// the masked mer of each seed's genome-a start
+// the masked mer of each seed's genome-a start (the top 2 * weight bits of its
+// position key), shifted up by three bits for the pair number
 __global__ void seed_mer_kernel(const SeedRec *__restrict__ seeds, uint64_t ns, const uint64_t *__restrict__ kp,
-                                uint64_t *__restrict__ out)
+                                int shift, uint32_t pair, uint64_t *__restrict__ out)
 {
     const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= ns) return;
-    out[x] = kp[seeds[x].s0 - 1] >> 1;
+    out[x] = ((kp[seeds[x].s0 - 1] >> shift) << 3) | pair;
 }
 
 // Up to four sorted lists.
@@ -1002,12 +1003,52 @@ static int extend_by_chains(mh_context *ctx, const mh_sml *a, const mh_sml *b, c
     return MH_OK;
 }
 
-int seed_mers(mh_context *ctx, const mh_sml *a, const SeedRec *d_seeds, uint64_t ns, uint64_t *d_out)
+int seed_mers(mh_context *ctx, const mh_sml *a, const SeedRec *d_seeds, uint64_t ns, uint32_t pair,
+              uint64_t *d_out)
 {
+    MH_CHECK(a->seed_weight <= 30 && pair < 8, MH_ERR_UNSUPPORTED,
+             "seed weight above 30 with more than two genomes");
     if (!ns) return MH_OK;
     hipLaunchKernelGGL(seed_mer_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, ctx->stream, d_seeds, ns,
-                       a->d_keys_pos, d_out);
+                       a->d_keys_pos, 64 - 2 * a->seed_weight, pair, d_out);
     MH_HIP(hipGetLastError());
+    return MH_OK;
+}
+
+__global__ void gather_seeds_kernel(const SeedRec *__restrict__ in, const uint32_t *__restrict__ idx, uint64_t n,
+                                    SeedRec *__restrict__ out)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    out[x] = in[idx[x]];
+}
+
+__global__ void iota_kernel(uint32_t *__restrict__ v, uint64_t n)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) v[x] = (uint32_t)x;
+}
+
+int merge_pair_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t *d_keys, uint64_t n, SeedRec **d_seeds_out,
+                     uint64_t **d_keys_out)
+{
+    DevBuf *m = ctx->multi_scratch;
+    int rc;
+    if ((rc = m[2].ensure(n * 8 + 64)) != MH_OK) return rc;
+    if ((rc = m[3].ensure(n * 4 + 64)) != MH_OK) return rc;
+    if ((rc = m[4].ensure(n * 4 + 64)) != MH_OK) return rc;
+    if ((rc = m[5].ensure(n * sizeof(SeedRec) + 64)) != MH_OK) return rc;
+    const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+    hipLaunchKernelGGL(iota_kernel, grid, blk, 0, ctx->stream, m[3].as<uint32_t>(), n);
+    uint64_t *sk = nullptr;
+    uint32_t *sv = nullptr;
+    if ((rc = radix_sort_kv(ctx, d_keys, m[3].as<uint32_t>(), m[2].as<uint64_t>(), m[4].as<uint32_t>(), n, 64, &sk,
+                            &sv)) != MH_OK)
+        return rc;
+    hipLaunchKernelGGL(gather_seeds_kernel, grid, blk, 0, ctx->stream, d_seeds, sv, n, m[5].as<SeedRec>());
+    MH_HIP(hipGetLastError());
+    *d_seeds_out = m[5].as<SeedRec>();
+    *d_keys_out = sk;
     return MH_OK;
 }
 

@@ -107,6 +107,7 @@ struct mh_context {
     mh::DevBuf mf_scratch[16];
     mh::DevBuf grid_scratch[11];   // spaced-seed extension by chains
     mh::DevBuf diag_scratch[8];    // long extensions resolved per diagonal
+    mh::DevBuf multi_scratch[6];   // the seeds of every genome pair, merged
     uint64_t last_walks = 0, last_long_walks = 0;   // of the last seed extension
     // contexts (own streams and scratch) that run independent searches
     // concurrently for this one, created on first use
@@ -207,9 +208,14 @@ struct SeedRec {
 // match search); *d_seeds points there.
 int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const SeedRec **d_seeds, uint64_t *ns,
                     uint32_t *repeat_runs);
-// The merge order of the seeds of one genome pair: genome a's sorted-list key
-// without its strand bit (the masked mer) at each seed's start.
-int seed_mers(mh_context *ctx, const mh_sml *a, const SeedRec *d_seeds, uint64_t ns, uint64_t *d_out);
+// The merge key of the seeds of one genome pair: the masked mer at each
+// seed's genome-a start, shifted up three bits, or'ed with the pair number.
+int seed_mers(mh_context *ctx, const mh_sml *a, const SeedRec *d_seeds, uint64_t ns, uint32_t pair,
+              uint64_t *d_out);
+// Orders n seeds (concatenated over genome pairs) by key = mer << 3 | pair:
+// d_seeds_out[i] = d_seeds[sorted index i], d_keys_out[i] = its key.
+int merge_pair_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t *d_keys, uint64_t n, SeedRec **d_seeds_out,
+                     uint64_t **d_keys_out);
 // Runs of one masked mer over the k lists longer than MER_REPEAT_LIMIT.
 int repeat_runs_k(mh_context *ctx, const mh_sml *const *smls, int k, uint32_t *runs);
 // A distinct extension of a clean bucket, and a seed of any other bucket.
