@@ -203,6 +203,9 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
     // every pair's seeds on the device, with key = mer << 3 | pair (pairs
     // numbered in genome order), then one sort: HashMatch order
     DevBuf *mb = ctx->multi_scratch;
+    const bool solid_odd = smls[0]->seed_length == smls[0]->seed_weight && (smls[0]->seed_length & 1);
+    std::vector<CleanEnt> clean_p[6];
+    std::vector<uint8_t> replay_bucket(TABLE_SIZE, 0);
     uint64_t total = 0;
     int64_t walks = 0, long_walks = 0;
     int pair_a[6], pair_b[6], np = 0;
@@ -217,6 +220,13 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
             pair_a[np] = a;
             pair_b[np] = b;
             if (ns) {
+                // the pair's buckets that the two-genome closed form settles
+                std::vector<DirtySeed> dirty_p;
+                uint64_t ncs = 0;
+                if ((rc = partition_seeds(ctx, d_seeds, ns, (int)sl, solid_odd, smls[a]->len, clean_p[np], dirty_p,
+                                          &ncs)) != MH_OK)
+                    return rc;
+                for (const DirtySeed &d : dirty_p) replay_bucket[d.bucket] = 1;
                 if ((rc = grow_keep(ctx, mb[0], total * sizeof(SeedRec), (total + ns) * sizeof(SeedRec) + 64)) != MH_OK)
                     return rc;
                 if ((rc = grow_keep(ctx, mb[1], total * 8, (total + ns) * 8 + 64)) != MH_OK) return rc;
@@ -250,9 +260,17 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
         const uint32_t p = (uint32_t)(hk[x] & 7u);
         tup[x] = Tup{pair_a[p], pair_b[p], hs[x]};
     }
-    // buckets of the generalized offsets, each in HashMatch order
+    // A bucket is settled in closed form when every genome pair's seeds in it
+    // are (a single line, odd solid seed): MheCompare orders entries by
+    // genome presence before anything else, so each pair's entries form a
+    // block (pairs with the later first genome first, then the later second
+    // genome), the comparisons a search makes against other blocks are
+    // consistent, and within its block a seed meets only its own line, where
+    // the two-genome result holds.  Any other bucket is replayed literally with
+    // the seeds of every pair, in HashMatch order.
     std::vector<uint32_t> bucket(tup.size()), cnt(TABLE_SIZE + 1, 0);
     std::vector<EntK> seed(tup.size());
+    uint64_t settled_seeds = 0;
     for (size_t x = 0; x < tup.size(); ++x) {
         EntK c;
         std::memset(&c, 0, sizeof(c));
@@ -265,13 +283,15 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
         seed[x] = c;
         const int64_t m = c.off % (int64_t)TABLE_SIZE;
         bucket[x] = (uint32_t)(m < 0 ? m + TABLE_SIZE : m);
-        ++cnt[bucket[x] + 1];
+        if (replay_bucket[bucket[x]]) ++cnt[bucket[x] + 1];
+        else ++settled_seeds;
     }
     for (uint32_t q = 0; q < TABLE_SIZE; ++q) cnt[q + 1] += cnt[q];
-    std::vector<uint32_t> order(tup.size());
+    std::vector<uint32_t> order(cnt[TABLE_SIZE]);
     {
         std::vector<uint32_t> at(cnt.begin(), cnt.end() - 1);
-        for (size_t x = 0; x < tup.size(); ++x) order[at[bucket[x]]++] = (uint32_t)x;
+        for (size_t x = 0; x < tup.size(); ++x)
+            if (replay_bucket[bucket[x]]) order[at[bucket[x]]++] = (uint32_t)x;
     }
     std::vector<std::vector<EntK>> table(TABLE_SIZE);
     std::atomic<int64_t> collisions{0};
@@ -301,15 +321,40 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
         collisions += coll;
     });
     host_time(ctx, "host_replay", now_ms() - tg1);
+    // pair blocks of a settled bucket in MheCompare order
+    std::vector<int> block(np);
+    for (int p = 0; p < np; ++p) block[p] = p;
+    std::sort(block.begin(), block.end(), [&](int x, int y) {
+        return pair_a[x] != pair_a[y] ? pair_a[x] > pair_a[y] : pair_b[x] > pair_b[y];
+    });
+    std::vector<size_t> ci(np, 0);
+    uint64_t settled_entries = 0;
     out.clear();
-    for (uint32_t q = 0; q < TABLE_SIZE; ++q)
-        for (const EntK &e : table[q]) {
-            mh_match m;
-            std::memset(&m, 0, sizeof(m));
-            for (int i = 0; i < k; ++i) m.start[i] = e.s[i];
-            m.length = e.len;
-            out.push_back(m);
+    auto emit = [&](const int64_t *s4, uint64_t len) {
+        mh_match m;
+        std::memset(&m, 0, sizeof(m));
+        for (int i = 0; i < k; ++i) m.start[i] = s4[i];
+        m.length = len;
+        out.push_back(m);
+    };
+    for (uint32_t q = 0; q < TABLE_SIZE; ++q) {
+        for (int p = 0; p < np; ++p)   // each pair's entries of this bucket (used only when settled)
+            while (ci[p] < clean_p[p].size() && clean_p[p][ci[p]].bucket < q) ++ci[p];
+        if (replay_bucket[q]) {
+            for (const EntK &e : table[q]) emit(e.s, e.len);
+            continue;
         }
+        for (int p : block) {
+            for (size_t y = ci[p]; y < clean_p[p].size() && clean_p[p][y].bucket == q; ++y) {
+                int64_t s4[4] = {0, 0, 0, 0};
+                s4[pair_a[p]] = clean_p[p][y].e0;
+                s4[pair_b[p]] = clean_p[p][y].e1;
+                emit(s4, clean_p[p][y].elen);
+                ++settled_entries;
+            }
+        }
+    }
+    collisions += (int64_t)(settled_seeds - settled_entries);
     if (st) {
         st->seed_tuples = (int64_t)tup.size();
         st->collisions = collisions.load();
