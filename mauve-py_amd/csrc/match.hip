@@ -1052,6 +1052,69 @@ int merge_pair_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t *d_keys, 
     return MH_OK;
 }
 
+static int compact_flags(mh_context *ctx, const uint32_t *flag, uint64_t n, uint32_t *out_idx, uint32_t *tiles,
+                         uint32_t *d_count, uint64_t *count);
+
+__device__ __forceinline__ uint32_t bucket_of_seed(const SeedRec &r, int sl)
+{
+    const int64_t off = (int64_t)r.s1 - r.s0 - (r.s1 < 0 ? sl : 0);   // CalculateOffset, LM/MatchHashEntry.cpp:141-160
+    return (uint32_t)(((off % 40000) + 40000) % 40000);
+}
+
+__global__ void bucket_mask_kernel(const SeedRec *__restrict__ seeds, uint64_t n, const uint8_t *__restrict__ mask,
+                                   int sl, uint32_t *__restrict__ flag)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    flag[x] = mask[bucket_of_seed(seeds[x], sl)];
+}
+
+__global__ void gather_marked_kernel(const SeedRec *__restrict__ seeds, const uint64_t *__restrict__ keys,
+                                     const uint32_t *__restrict__ idx, uint64_t m, SeedRec *__restrict__ out,
+                                     uint8_t *__restrict__ pair)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= m) return;
+    const uint32_t i = idx[x];
+    out[x] = seeds[i];
+    pair[x] = (uint8_t)(keys[i] & 7u);
+}
+
+int select_bucket_seeds(mh_context *ctx, const SeedRec *d_seeds, const uint64_t *d_keys, uint64_t n,
+                        const uint8_t *mask, int seed_len, std::vector<SeedRec> &seeds, std::vector<uint8_t> &pairs)
+{
+    seeds.clear();
+    pairs.clear();
+    if (!n) return MH_OK;
+    DevBuf *m = ctx->multi_scratch;
+    int rc;
+    if ((rc = m[6].ensure(40000 + 64)) != MH_OK) return rc;
+    if ((rc = m[7].ensure(n * 4 + 64)) != MH_OK) return rc;
+    if ((rc = m[8].ensure(n * 4 + 64)) != MH_OK) return rc;
+    if ((rc = m[9].ensure(((n + JTILE - 1) / JTILE) * 4 + 128)) != MH_OK) return rc;
+    MH_HIP(hipMemcpyAsync(m[6].ptr, mask, 40000, hipMemcpyHostToDevice, ctx->stream));
+    const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+    hipLaunchKernelGGL(bucket_mask_kernel, grid, blk, 0, ctx->stream, d_seeds, n, m[6].as<uint8_t>(), seed_len,
+                       m[7].as<uint32_t>());
+    uint32_t *tiles = m[9].as<uint32_t>();
+    uint32_t *d_count = tiles + (n + JTILE - 1) / JTILE + 8;
+    uint64_t cnt = 0;
+    if ((rc = compact_flags(ctx, m[7].as<uint32_t>(), n, m[8].as<uint32_t>(), tiles, d_count, &cnt)) != MH_OK)
+        return rc;
+    if (!cnt) return MH_OK;
+    if ((rc = m[0].ensure(cnt * sizeof(SeedRec) + 64)) != MH_OK) return rc;
+    if ((rc = m[10].ensure(cnt + 64)) != MH_OK) return rc;
+    hipLaunchKernelGGL(gather_marked_kernel, dim3((unsigned)((cnt + 255) / 256)), blk, 0, ctx->stream, d_seeds,
+                       d_keys, m[8].as<uint32_t>(), cnt, m[0].as<SeedRec>(), m[10].as<uint8_t>());
+    MH_HIP(hipGetLastError());
+    seeds.resize(cnt);
+    pairs.resize(cnt);
+    MH_HIP(hipMemcpyAsync(seeds.data(), m[0].ptr, cnt * sizeof(SeedRec), hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(pairs.data(), m[10].ptr, cnt, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    return MH_OK;
+}
+
 int repeat_runs_k(mh_context *ctx, const mh_sml *const *smls, int k, uint32_t *runs)
 {
     KLists L;

@@ -239,27 +239,25 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
             ++np;
         }
     const double tg1 = now_ms();
-    std::vector<SeedRec> hs(total);
-    std::vector<uint64_t> hk(total);
+    // only the seeds of buckets to replay come back to the host
+    std::vector<SeedRec> hs;
+    std::vector<uint8_t> hp;
     if (total) {
         SeedRec *d_sorted = nullptr;
         uint64_t *d_skeys = nullptr;
         if ((rc = merge_pair_seeds(ctx, mb[0].as<SeedRec>(), mb[1].as<uint64_t>(), total, &d_sorted, &d_skeys)) !=
             MH_OK)
             return rc;
-        MH_HIP(hipMemcpyAsync(hs.data(), d_sorted, total * sizeof(SeedRec), hipMemcpyDeviceToHost, ctx->stream));
-        MH_HIP(hipMemcpyAsync(hk.data(), d_skeys, total * 8, hipMemcpyDeviceToHost, ctx->stream));
-        MH_HIP(hipStreamSynchronize(ctx->stream));
+        if ((rc = select_bucket_seeds(ctx, d_sorted, d_skeys, total, replay_bucket.data(), (int)sl, hs, hp)) != MH_OK)
+            return rc;
     }
     struct Tup {
         int a, b;
         SeedRec r;
     };
-    std::vector<Tup> tup(total);
-    for (uint64_t x = 0; x < total; ++x) {
-        const uint32_t p = (uint32_t)(hk[x] & 7u);
-        tup[x] = Tup{pair_a[p], pair_b[p], hs[x]};
-    }
+    std::vector<Tup> tup(hs.size());
+    for (uint64_t x = 0; x < hs.size(); ++x) tup[x] = Tup{pair_a[hp[x]], pair_b[hp[x]], hs[x]};
+    const uint64_t settled_seeds = total - hs.size();
     // A bucket is settled in closed form when every genome pair's seeds in it
     // are (a single line, odd solid seed): MheCompare orders entries by
     // genome presence before anything else, so each pair's entries form a
@@ -270,7 +268,6 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
     // the seeds of every pair, in HashMatch order.
     std::vector<uint32_t> bucket(tup.size()), cnt(TABLE_SIZE + 1, 0);
     std::vector<EntK> seed(tup.size());
-    uint64_t settled_seeds = 0;
     for (size_t x = 0; x < tup.size(); ++x) {
         EntK c;
         std::memset(&c, 0, sizeof(c));
@@ -283,15 +280,13 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
         seed[x] = c;
         const int64_t m = c.off % (int64_t)TABLE_SIZE;
         bucket[x] = (uint32_t)(m < 0 ? m + TABLE_SIZE : m);
-        if (replay_bucket[bucket[x]]) ++cnt[bucket[x] + 1];
-        else ++settled_seeds;
+        ++cnt[bucket[x] + 1];
     }
     for (uint32_t q = 0; q < TABLE_SIZE; ++q) cnt[q + 1] += cnt[q];
     std::vector<uint32_t> order(cnt[TABLE_SIZE]);
     {
         std::vector<uint32_t> at(cnt.begin(), cnt.end() - 1);
-        for (size_t x = 0; x < tup.size(); ++x)
-            if (replay_bucket[bucket[x]]) order[at[bucket[x]]++] = (uint32_t)x;
+        for (size_t x = 0; x < tup.size(); ++x) order[at[bucket[x]]++] = (uint32_t)x;
     }
     std::vector<std::vector<EntK>> table(TABLE_SIZE);
     std::atomic<int64_t> collisions{0};
@@ -356,7 +351,7 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
     }
     collisions += (int64_t)(settled_seeds - settled_entries);
     if (st) {
-        st->seed_tuples = (int64_t)tup.size();
+        st->seed_tuples = (int64_t)total;
         st->collisions = collisions.load();
         st->matches = (int64_t)out.size();
         st->walks = walks;

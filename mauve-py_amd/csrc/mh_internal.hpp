@@ -107,7 +107,7 @@ struct mh_context {
     mh::DevBuf mf_scratch[16];
     mh::DevBuf grid_scratch[11];   // spaced-seed extension by chains
     mh::DevBuf diag_scratch[8];    // long extensions resolved per diagonal
-    mh::DevBuf multi_scratch[6];   // the seeds of every genome pair, merged
+    mh::DevBuf multi_scratch[11];  // the seeds of every genome pair, merged and filtered
     uint64_t last_walks = 0, last_long_walks = 0;   // of the last seed extension
     // contexts (own streams and scratch) that run independent searches
     // concurrently for this one, created on first use
@@ -216,6 +216,10 @@ int seed_mers(mh_context *ctx, const mh_sml *a, const SeedRec *d_seeds, uint64_t
 // d_seeds_out[i] = d_seeds[sorted index i], d_keys_out[i] = its key.
 int merge_pair_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t *d_keys, uint64_t n, SeedRec **d_seeds_out,
                      uint64_t **d_keys_out);
+// The merged seeds (d_seeds, keys as merge_pair_seeds leaves them) whose
+// MemHash bucket is marked in mask[40000], in order, with their pair numbers.
+int select_bucket_seeds(mh_context *ctx, const SeedRec *d_seeds, const uint64_t *d_keys, uint64_t n,
+                        const uint8_t *mask, int seed_len, std::vector<SeedRec> &seeds, std::vector<uint8_t> &pairs);
 // Runs of one masked mer over the k lists longer than MER_REPEAT_LIMIT.
 int repeat_runs_k(mh_context *ctx, const mh_sml *const *smls, int k, uint32_t *runs);
 // A distinct extension of a clean bucket, and a seed of any other bucket.
