@@ -26,8 +26,9 @@ struct Anc {
     uint32_t idx;
 };
 
-// Stable LSD radix sort of (key, value) pairs, 11-bit digits, as many passes
-// as the largest key needs.
+// Stable LSD radix sort of (key, value) pairs: as few passes of at most 13
+// bits as the largest key needs, digits of equal width, every pass's histogram
+// from one read of the keys.
 void radix_sort_pairs(std::vector<uint64_t> &key, std::vector<uint32_t> &val)
 {
     const size_t n = key.size();
@@ -35,22 +36,30 @@ void radix_sort_pairs(std::vector<uint64_t> &key, std::vector<uint32_t> &val)
     for (uint64_t k : key) mx |= k;
     int bits = 0;
     while (bits < 64 && (mx >> bits)) ++bits;
+    if (!bits || n < 2) return;
+    const int passes = (bits + 12) / 13, width = (bits + passes - 1) / passes;
+    const uint64_t radix = 1ull << width, mask = radix - 1;
+    std::vector<uint32_t> cnt((size_t)passes * radix, 0u);
+    for (size_t i = 0; i < n; ++i)
+        for (int p = 0; p < passes; ++p) ++cnt[(size_t)p * radix + ((key[i] >> (p * width)) & mask)];
     std::vector<uint64_t> k2(n);
     std::vector<uint32_t> v2(n);
-    std::vector<uint32_t> cnt(2048);
-    for (int sh = 0; sh < bits; sh += 11) {
-        std::fill(cnt.begin(), cnt.end(), 0u);
-        for (size_t i = 0; i < n; ++i) ++cnt[(key[i] >> sh) & 2047u];
+    for (int p = 0; p < passes; ++p) {
+        uint32_t *c = cnt.data() + (size_t)p * radix;
         uint32_t acc = 0;
-        for (uint32_t &c : cnt) {
-            const uint32_t t = c;
-            c = acc;
+        bool one = false;
+        for (uint64_t d = 0; d < radix; ++d) {
+            if (c[d] == n) one = true;   // every key has this digit: nothing moves
+            const uint32_t t = c[d];
+            c[d] = acc;
             acc += t;
         }
+        if (one) continue;
+        const int sh = p * width;
         for (size_t i = 0; i < n; ++i) {
-            const uint32_t d = (uint32_t)((key[i] >> sh) & 2047u);
-            k2[cnt[d]] = key[i];
-            v2[cnt[d]++] = val[i];
+            const uint32_t d = (uint32_t)((key[i] >> sh) & mask);
+            k2[c[d]] = key[i];
+            v2[c[d]++] = val[i];
         }
         key.swap(k2);
         val.swap(v2);
@@ -85,9 +94,14 @@ int mh_chain_gap_jobs(const mh_match *matches, uint64_t n, uint64_t len_a, uint6
     std::vector<uint32_t> val;
     key.reserve(n);
     val.reserve(n);
+    uint64_t top1 = len_b;   // the largest start in genome 2 fixes the key's low field
+    for (uint64_t i = 0; i < n; ++i)
+        if (matches[i].start[1] > 0 && (uint64_t)matches[i].start[1] > top1) top1 = (uint64_t)matches[i].start[1];
+    int b1 = 1;
+    while (b1 < 32 && (1ull << b1) <= top1) ++b1;
     for (uint64_t i = 0; i < n; ++i)
         if (matches[i].start[0] > 0 && matches[i].start[1] > 0) {
-            key.push_back(((uint64_t)(matches[i].start[0] - 1) << 32) | (uint64_t)(matches[i].start[1] - 1));
+            key.push_back(((uint64_t)(matches[i].start[0] - 1) << b1) | (uint64_t)(matches[i].start[1] - 1));
             val.push_back((uint32_t)i);
         }
     radix_sort_pairs(key, val);

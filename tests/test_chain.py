@@ -65,3 +65,34 @@ def test_chain_empty_and_reverse_strand_ignored():
     chain, jobs = mauvehip.chain_gap_jobs(starts, np.array([20, 20], dtype=np.uint64), 100, 100, 5000)
     assert len(chain) == 0
     assert [(int(j["a_off"]), int(j["b_off"]), int(j["a_len"]), int(j["b_len"])) for j in jobs] == [(0, 0, 100, 100)]
+
+
+def _vector_best(s0, s1, ln):
+    """The O(n^2) chain DP of _brute_best, one numpy pass per anchor."""
+    order = np.lexsort((np.arange(len(s0)), s1, s0))
+    best = np.zeros(len(s0), dtype=np.int64)
+    e0, e1 = s0 + ln, s1 + ln
+    for i in order:
+        ok = (e0 <= s0[i]) & (e1 <= s1[i])
+        best[i] = ln[i] + (best[ok].max() if ok.any() else 0)
+    return int(best.max()) if len(best) else 0
+
+
+def test_chain_large_genomes_and_repeated_starts():
+    """Starts of more than 20 bits in both genomes (the sort keys then take
+    several radix passes) and anchors sharing starts or ends."""
+    rng = np.random.default_rng(11)
+    for La, Lb, n in [(3_000_000, 3_100_000, 1500), (40_000_000, 70_000, 900), (900, 30_000_000, 700)]:
+        ln = rng.integers(5, 400, size=n)
+        s0 = (np.sort(rng.integers(0, La - 400, size=n)) + rng.integers(-2000, 2000, size=n)).clip(0, La - 400)
+        s1 = (s0 * (Lb / La)).astype(np.int64) + rng.integers(-500, 500, size=n)
+        s1 = s1.clip(0, Lb - 400)
+        s0[rng.integers(0, n, size=n // 10)] = s0[0]      # shared starts in genome 1
+        s1[rng.integers(0, n, size=n // 10)] = s1[1]      # and in genome 2
+        starts = np.stack([s0 + 1, s1 + 1], 1).astype(np.int64)
+        chain, _ = mauvehip.chain_gap_jobs(starts, ln.astype(np.uint64), La, Lb, 5000)
+        assert int(ln[chain].sum()) == _vector_best(s0, s1, ln)
+        pa = pb = 0
+        for q in chain:
+            assert s0[q] >= pa and s1[q] >= pb
+            pa, pb = s0[q] + ln[q], s1[q] + ln[q]
