@@ -84,13 +84,15 @@ def test_dp_multi_stripe_long(ctx):
 @pytest.mark.parametrize("extra", [0, 1, 2, 3])
 def test_dp_row_packed_shapes(ctx, extra):
     """Jobs up to 256 columns wide with short trace-back walks run four to a
-    wavefront (one per 16-lane row, stripes of 64, 128 or 256 columns): the
-    shape boundaries, walks either side of the wavefront trace-back limit
+    wavefront (one per 16-lane row, stripes of 64 to 256 columns in steps of
+    32, 4 to 16 columns a lane): the shape boundaries, walks either side of the
+    wavefront trace-back limit
     (a + b = 1024), related and unrelated pairs, and job counts that leave
     the last wavefront part empty."""
     rng = np.random.default_rng(40 + extra)
     pairs = []
-    for lb in (17, 18, 63, 64, 65, 127, 128, 129, 255, 256, 257):
+    for lb in (17, 18, 63, 64, 65, 95, 96, 97, 127, 128, 129, 159, 160, 161, 191, 192, 193, 223, 224, 225, 255, 256,
+               257):
         for la in (1, 2, 16, 17, 100, 766, 767, 768, 1000):
             a = rng.integers(0, 4, la, dtype=np.uint8)
             if rng.random() < 0.5:
