@@ -368,21 +368,6 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
     }
 }
 
-__global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs, const uint8_t *__restrict__ A,
-                                                 const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
-                                                 int *__restrict__ bnd, DpOut *__restrict__ outs, uint32_t sr0,
-                                                 uint32_t sr1, uint32_t sr2, uint32_t sr3, int goh)
-{
-    __shared__ int lds_in[4][64];
-    __shared__ int lds_out[3][64];
-    const DevJob jb = jobs[blockIdx.x];
-    if (jb.cols == 16)
-        dp_job<16, 64>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
-    else if (jb.cols == 8)
-        dp_job<8, 64>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
-    else
-        dp_job<4, 64>(jb, A, B, tb, bnd, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
-}
 
 // Row-packed jobs: four jobs of one shape per wavefront, one per 16-lane row
 // (DevJob::cols = GROUP_TAG + COLS, COLS even from 4 to 16: the narrowest
@@ -390,9 +375,11 @@ __global__ __launch_bounds__(64) void dp_forward(const DevJob *__restrict__ jobs
 constexpr uint32_t GROUP_TAG = 32;
 constexpr uint32_t ROWS_MAX_LB = 256;   // widest job run four to a wavefront
 
-// Every row-packed job of a batch in one launch (one launch per shape would
-// pay a job's duration of drain per shape).  Shape q (16 - 2q columns a lane,
-// widest first) holds jobs [start[q], start[q+1]) in waves [wave[q], wave[q+1]).
+// Every wavefront-stripe job and every row-packed job of a batch in one launch
+// (one launch per shape would pay a job's duration of drain per shape).  Waves
+// [0, n_stripe) take stripe job w (64 lanes); after them, row shape q (16 - 2q
+// columns a lane, widest first) holds jobs [start[q], start[q+1]) in waves
+// n_stripe + [wave[q], wave[q+1]).
 constexpr int ROW_SHAPES = 7;
 struct RowShapes {
     uint32_t start[ROW_SHAPES + 1];
@@ -401,7 +388,7 @@ struct RowShapes {
 
 // At most 168 VGPRs (3 waves a SIMD; the 16-column shape alone would take 182
 // and 2 waves): measured 3% faster on the whole-path batches.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dp_forward_rows_all(const DevJob *__restrict__ jobs, RowShapes sh,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dp_forward_all(const DevJob *__restrict__ jobs, uint32_t n_stripe, RowShapes sh,
                                                           const uint8_t *__restrict__ A,
                                                           const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
                                                           int *__restrict__ bnd, DpOut *__restrict__ outs,
@@ -410,7 +397,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dp
 {
     __shared__ int lds_in[4][64];
     __shared__ int lds_out[3][64];
-    const uint32_t w = blockIdx.x;
+    if (blockIdx.x < n_stripe) {
+        const DevJob jb = jobs[blockIdx.x];
+        DpOut *o = &outs[blockIdx.x];
+        if (jb.cols == 16)
+            dp_job<16, 64>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+        else if (jb.cols == 8)
+            dp_job<8, 64>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+        else
+            dp_job<4, 64>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+        return;
+    }
+    const uint32_t w = blockIdx.x - n_stripe;
     int q = 0;
     while (q < ROW_SHAPES - 1 && w >= sh.wave[q + 1]) ++q;
     const uint32_t x = sh.start[q] + (w - sh.wave[q]) * 4 + (threadIdx.x >> 4);
@@ -1202,10 +1200,6 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         MH_HIP(hipMemcpyAsync(d_jobs, dj, n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
         {
             TimedLaunch tl(ctx, "dp_forward");
-            if (n64)
-                hipLaunchKernelGGL(dp_forward, dim3((unsigned)n64), dim3(64), 0, ctx->stream, d_jobs, d_a, d_b,
-                                   ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs, sr[0], sr[1], sr[2],
-                                   sr[3], -200);
             RowShapes sh;
             uint32_t waves = 0;
             for (int q = 0; q < ROW_SHAPES; ++q) {
@@ -1215,10 +1209,10 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             }
             sh.start[ROW_SHAPES] = (uint32_t)rows_at[ROW_SHAPES + 1];
             sh.wave[ROW_SHAPES] = waves;
-            if (waves)
-                hipLaunchKernelGGL(dp_forward_rows_all, dim3(waves), dim3(64), 0, ctx->stream, d_jobs, sh, d_a, d_b,
-                                   ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs, sr[0], sr[1], sr[2],
-                                   sr[3], -200);
+            if (n64 + waves)
+                hipLaunchKernelGGL(dp_forward_all, dim3((unsigned)(n64 + waves)), dim3(64), 0, ctx->stream, d_jobs,
+                                   (uint32_t)n64, sh, d_a, d_b, ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs,
+                                   sr[0], sr[1], sr[2], sr[3], -200);
             if (nbig > nstripe)
                 hipLaunchKernelGGL(dp_window, dim3((unsigned)(nbig - nstripe)), dim3(64), 0, ctx->stream,
                                    d_jobs + nstripe, d_a, d_b, ctx->dp_tb.as<uint8_t>(), d_outs + nstripe, sr[0],
