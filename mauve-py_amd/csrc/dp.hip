@@ -39,7 +39,6 @@
 namespace mh {
 namespace {
 
-constexpr int LANES = 64;
 constexpr int NEG = -(1 << 29);
 constexpr int SBIAS = 65;           // smallest NUC_SP entry is -125 + 60 = -65
 
