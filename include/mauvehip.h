@@ -132,7 +132,8 @@ typedef struct {
     int64_t seed_tuples;     /* HashMatch calls                              */
     int64_t collisions;      /* seeds dropped by AddHashEntry                */
     int64_t matches;         /* entries in the match list                    */
-    int64_t repeat_runs;     /* runs above MER_REPEAT_LIMIT                  */
+    int64_t repeat_runs;     /* SearchRange restarts after runs above
+                                MER_REPEAT_LIMIT (LM/MatchFinder.cpp:253-276)  */
     int64_t walks;           /* seeds extended by the step-by-step walk      */
     int64_t long_walks;      /* of those, walks finished per diagonal        */
 } mh_match_stats;
@@ -147,6 +148,18 @@ typedef struct {
 int mh_find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k,
                              mh_match *out, uint64_t cap, uint64_t *n_out,
                              mh_match_stats *stats);
+
+/* The index ranges of k sorted lists that MatchFinder::SearchRange reads when
+ * runs above MER_REPEAT_LIMIT (1000) make it restart (LM/MatchFinder.cpp:
+ * 166-340 with GetBreakpoint 89-126 and FindMatchSeeds 139-164): each restart
+ * drops the run and jumps every list to new start points; the matches found
+ * are the runs of the ranges read.  The device search plans the same ranges;
+ * this host-only entry point plans them from host copies of the sorted 64-bit
+ * keys (mh_sml_read order; len = sequence lengths), for checking.  ranges
+ * receives (list, lo, hi) triples, list by list, ascending; *restarts the
+ * number of restarts.  MH_ERR_INVALID with *n_ranges set when cap is short. */
+int mh_search_live_ranges(int k, const uint64_t *const *keys, const uint64_t *n, const uint64_t *len,
+                          uint64_t seed, uint64_t *ranges, uint64_t cap, uint64_t *n_ranges, int64_t *restarts);
 
 /* Same search, result kept in a library-owned list (one call, no size guess). */
 typedef struct mh_match_list mh_match_list;

@@ -16,9 +16,12 @@
 // sequences.  Extension runs for every seed in parallel; which extended seeds
 // enter the match table is decided by the host replay of AddHashEntry.
 //
-// Runs longer than MER_REPEAT_LIMIT (LM/MatchFinder.cpp:166) make the
-// reference restart its merge from recomputed start points (253-276); such
-// runs are counted so the host can refuse inputs that contain them.
+// Runs longer than MER_REPEAT_LIMIT (LM/MatchFinder.cpp:166) can make the
+// reference restart its merge from recomputed start points (253-276).
+// heavy_kernel finds every masked mer with more than MER_REPEAT_LIMIT entries
+// over all lists; when there is one, the host plans the restarts
+// (restart.cpp) and the join reads only the index ranges the reference's
+// search reads (live ranges).
 #include "extend.hpp"
 #include "mh_internal.hpp"
 
@@ -67,73 +70,100 @@ __global__ void seed_mer_kernel(const SeedRec *__restrict__ seeds, uint64_t ns, 
     out[x] = ((kp[seeds[x].s0 - 1] >> shift) << 3) | pair;
 }
 
-// Up to four sorted lists.
+// Up to PLAN_MAXK sorted lists.
 struct KLists {
-    const uint64_t *ck[4];
-    uint64_t n[4];
+    const uint64_t *ck[PLAN_MAXK];
+    uint64_t n[PLAN_MAXK];
     int k;
 };
 
-// Runs of one masked mer over all k lists above MER_REPEAT_LIMIT, each counted
-// from the first list that holds it.
-__global__ void repeat_k_kernel(KLists L, int i, uint32_t *__restrict__ runs)
+// Masked mers of more than MER_REPEAT_LIMIT entries over all k lists.  Such a
+// run holds more than MER_REPEAT_LIMIT / k entries in some list, so only run
+// starts of list i with at least `thresh` = MER_REPEAT_LIMIT / k + 1 entries
+// count the others (binary searches); a mer may be reported from several
+// lists (the host removes duplicates).
+__global__ void heavy_kernel(KLists L, int i, uint32_t thresh, uint64_t *__restrict__ out, uint32_t cap,
+                             uint32_t *__restrict__ count)
 {
     const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L.n[i]) return;
     const uint64_t *ci = L.ck[i];
     const uint64_t m = ci[x] >> 1;
     if (x > 0 && (ci[x - 1] >> 1) == m) return;
+    if (x + thresh - 1 >= L.n[i] || (ci[x + thresh - 1] >> 1) != m) return;
     uint64_t total = 0;
     for (int j = 0; j < L.k; ++j) {
         const uint64_t lo = lower_bound_m(L.ck[j], L.n[j], m);
-        const bool hit = lo < L.n[j] && (L.ck[j][lo] >> 1) == m;
-        if (!hit) continue;
-        if (j < i) return;   // counted from list j
-        total += upper_bound_m(L.ck[j], L.n[j], m) - lo;
+        if (lo < L.n[j] && (L.ck[j][lo] >> 1) == m) total += upper_bound_m(L.ck[j], L.n[j], m) - lo;
     }
-    if (total > MER_REPEAT_LIMIT) atomicAdd(runs, 1u);
+    if (total > MER_REPEAT_LIMIT) {
+        const uint32_t at = atomicAdd(count, 1u);
+        if (at < cap) out[at] = m;
+    }
 }
 
-// flag[i] = 1 if genome-0 entry i forms a seed; partner[i] = its genome-1 index
+// Live ranges of the two lists of a join: n == 0 means the whole list.
+struct LivePair {
+    const uint64_t *lo0, *hi0, *lo1, *hi1;
+    uint32_t c0, c1;
+};
+
+// the live range [ilo, ihi) holding index i (false when i is in none)
+__device__ __forceinline__ bool live_at(const uint64_t *__restrict__ lo, const uint64_t *__restrict__ hi, uint32_t c,
+                                        uint64_t n, uint64_t i, uint64_t &ilo, uint64_t &ihi)
+{
+    if (c == 0) {
+        ilo = 0;
+        ihi = n;
+        return true;
+    }
+    uint32_t a = 0, len = c;    // last range with lo <= i
+    while (len > 0) {
+        const uint32_t half = len >> 1, mid = a + half;
+        if (lo[mid] <= i) { a = mid + 1; len -= half + 1; }
+        else len = half;
+    }
+    if (a == 0) return false;
+    ilo = lo[a - 1];
+    ihi = hi[a - 1];
+    return i < ihi;
+}
+
+// flag[i] = 1 if genome-0 entry i forms a seed; partner[i] = its genome-1
+// index.  A seed is a masked mer with exactly one live entry in each list.
 __global__ void join_kernel(const uint64_t *__restrict__ ck0, uint64_t n0, const uint64_t *__restrict__ ck1,
-                            uint64_t n1, uint32_t *__restrict__ flag, uint32_t *__restrict__ partner,
-                            uint32_t *__restrict__ repeat_runs)
+                            uint64_t n1, LivePair live, uint32_t *__restrict__ flag, uint32_t *__restrict__ partner)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n0) return;
-    const uint64_t m = ck0[i] >> 1;
-    const bool first = i == 0 || (ck0[i - 1] >> 1) != m;
-    const bool last = i + 1 == n0 || (ck0[i + 1] >> 1) != m;
     uint32_t f = 0, pj = 0;
-    if (first) {
-        const uint64_t lo = lower_bound_m(ck1, n1, m);
-        const bool hit = lo < n1 && (ck1[lo] >> 1) == m;
-        if (last && hit && (lo + 1 == n1 || (ck1[lo + 1] >> 1) != m)) {
-            f = 1;
-            pj = (uint32_t)lo;
-        }
-        if (!last || hit) {
-            const uint64_t r0 = last ? 1 : upper_bound_m(ck0, n0, m) - i;
-            const uint64_t r1 = hit ? upper_bound_m(ck1, n1, m) - lo : 0;
-            if (r0 + r1 > MER_REPEAT_LIMIT) atomicAdd(repeat_runs, 1u);
+    uint64_t ilo, ihi;
+    if (live_at(live.lo0, live.hi0, live.c0, n0, i, ilo, ihi)) {
+        const uint64_t m = ck0[i] >> 1;
+        const bool first = i == ilo || (ck0[i - 1] >> 1) != m;
+        const bool last = i + 1 == ihi || (ck0[i + 1] >> 1) != m;
+        if (first && last) {
+            const uint64_t lo = lower_bound_m(ck1, n1, m);
+            if (lo < n1 && (ck1[lo] >> 1) == m) {
+                uint64_t jlo, jhi;
+                if (lo + 1 == n1 || (ck1[lo + 1] >> 1) != m) {
+                    if (live_at(live.lo1, live.hi1, live.c1, n1, lo, jlo, jhi)) {
+                        f = 1;
+                        pj = (uint32_t)lo;
+                    }
+                } else if (live.c1) {
+                    // a run the last restart cut: one live entry left?
+                    const uint64_t hi = upper_bound_m(ck1, n1, m);
+                    if (live_at(live.lo1, live.hi1, live.c1, n1, hi - 1, jlo, jhi) && (lo > jlo ? lo : jlo) == hi - 1) {
+                        f = 1;
+                        pj = (uint32_t)(hi - 1);
+                    }
+                }
+            }
         }
     }
     flag[i] = f;
     partner[i] = pj;
-}
-
-// runs of genome 1 alone above the limit
-__global__ void repeat1_kernel(const uint64_t *__restrict__ ck1, uint64_t n1, const uint64_t *__restrict__ ck0,
-                               uint64_t n0, uint32_t *__restrict__ repeat_runs)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n1) return;
-    const uint64_t m = ck1[i] >> 1;
-    if (i > 0 && (ck1[i - 1] >> 1) == m) return;
-    if (i + MER_REPEAT_LIMIT >= n1 || (ck1[i + MER_REPEAT_LIMIT] >> 1) != m) return;
-    const uint64_t lo = lower_bound_m(ck0, n0, m);
-    if (lo < n0 && (ck0[lo] >> 1) == m) return;   // counted from genome 0's side
-    atomicAdd(repeat_runs, 1u);
 }
 
 __global__ __launch_bounds__(JT) void tile_count_kernel(const uint32_t *__restrict__ flag, uint64_t n,
@@ -1115,8 +1145,14 @@ int select_bucket_seeds(mh_context *ctx, const SeedRec *d_seeds, const uint64_t 
     return MH_OK;
 }
 
-int repeat_runs_k(mh_context *ctx, const mh_sml *const *smls, int k, uint32_t *runs)
+int search_live_ranges(mh_context *ctx, const mh_sml *const *smls, int k, LiveRanges *out)
 {
+    MH_CHECK(k >= 1 && k <= PLAN_MAXK, MH_ERR_INVALID, "list count out of range");
+    out->restarts = 0;
+    for (int g = 0; g < k; ++g) {
+        out->lo[g].assign(1, 0);
+        out->hi[g].assign(1, smls[g]->n);
+    }
     KLists L;
     std::memset(&L, 0, sizeof(L));
     L.k = k;
@@ -1124,27 +1160,54 @@ int repeat_runs_k(mh_context *ctx, const mh_sml *const *smls, int k, uint32_t *r
         L.ck[i] = smls[i]->d_keys;
         L.n[i] = smls[i]->n;
     }
+    const uint32_t thresh = (uint32_t)(MER_REPEAT_LIMIT / (uint32_t)k + 1);
     int rc;
-    if ((rc = ctx->mf_scratch[3].ensure(64)) != MH_OK) return rc;
-    uint32_t *d = ctx->mf_scratch[3].as<uint32_t>();
-    MH_HIP(hipMemsetAsync(d, 0, 4, ctx->stream));
-    for (int i = 0; i < k; ++i)
-        if (L.n[i])
-            hipLaunchKernelGGL(repeat_k_kernel, dim3((unsigned)((L.n[i] + 255) / 256)), dim3(256), 0, ctx->stream, L,
-                               i, d);
-    MH_HIP(hipGetLastError());
-    MH_HIP(hipMemcpyAsync(runs, d, 4, hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t cap = 4096, count = 0;
+    for (;;) {
+        if ((rc = ctx->heavy_buf.ensure((size_t)cap * 8 + 64)) != MH_OK) return rc;
+        uint64_t *d_out = ctx->heavy_buf.as<uint64_t>();
+        uint32_t *d_cnt = reinterpret_cast<uint32_t *>(d_out + cap);
+        MH_HIP(hipMemsetAsync(d_cnt, 0, 4, ctx->stream));
+        for (int i = 0; i < k; ++i)
+            if (L.n[i])
+                hipLaunchKernelGGL(heavy_kernel, dim3((unsigned)((L.n[i] + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   L, i, thresh, d_out, cap, d_cnt);
+        MH_HIP(hipGetLastError());
+        MH_HIP(hipMemcpyAsync(&count, d_cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+        if (count <= cap) break;
+        cap = count;
+    }
+    if (count == 0) return MH_OK;
+    std::vector<uint64_t> heavy(count);
+    MH_HIP(hipMemcpyAsync(heavy.data(), ctx->heavy_buf.ptr, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
+    // the sorted keys on the host, for the planner's searches
+    std::vector<std::vector<uint64_t>> keys(k);
+    PlanLists P;
+    P.k = k;
+    P.sw = smls[0]->seed_weight;
+    P.sl = smls[0]->seed_length;
+    for (int g = 0; g < k; ++g) {
+        keys[g].resize(smls[g]->n);
+        if (smls[g]->n)
+            MH_HIP(hipMemcpyAsync(keys[g].data(), smls[g]->d_keys, smls[g]->n * 8, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        P.ck[g] = keys[g].data();
+        P.n[g] = smls[g]->n;
+        P.len[g] = smls[g]->len;
+    }
     MH_HIP(hipStreamSynchronize(ctx->stream));
-    return MH_OK;
+    std::sort(heavy.begin(), heavy.end());
+    heavy.erase(std::unique(heavy.begin(), heavy.end()), heavy.end());
+    return plan_restarts(P, heavy, out);
 }
 
-int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const SeedRec **seeds_out, uint64_t *ns_out,
-                    uint32_t *repeat_runs)
+int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const LiveRanges *live, int ga, int gb,
+                    const SeedRec **seeds_out, uint64_t *ns_out)
 {
     const uint64_t n0 = a->n, n1 = b->n;
     *seeds_out = nullptr;
     *ns_out = 0;
-    *repeat_runs = 0;
     ctx->last_walks = ctx->last_long_walks = 0;
     if (n0 == 0 || n1 == 0) return MH_OK;
     MH_CHECK(a->len < (1ull << 31) && b->len < (1ull << 31), MH_ERR_UNSUPPORTED,
@@ -1158,14 +1221,41 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const See
     uint32_t *flag = ctx->mf_scratch[0].as<uint32_t>();
     uint32_t *partner = ctx->mf_scratch[1].as<uint32_t>();
     uint32_t *tiles = ctx->mf_scratch[2].as<uint32_t>();
-    uint32_t *cnt = ctx->mf_scratch[3].as<uint32_t>();   // [0] repeat runs, [1] total seeds, [2] ext errors
+    uint32_t *cnt = ctx->mf_scratch[3].as<uint32_t>();   // [1] total seeds, [2] ext errors
+    // live ranges other than the whole list go to the device
+    LivePair lp;
+    std::memset(&lp, 0, sizeof(lp));
+    const bool cut_a = live && !(live->lo[ga].size() == 1 && live->lo[ga][0] == 0 && live->hi[ga][0] == n0);
+    const bool cut_b = live && !(live->lo[gb].size() == 1 && live->lo[gb][0] == 0 && live->hi[gb][0] == n1);
+    if (cut_a || cut_b) {
+        const size_t ca = cut_a ? live->lo[ga].size() : 0, cb = cut_b ? live->lo[gb].size() : 0;
+        std::vector<uint64_t> h(2 * ca + 2 * cb + 2, 0);
+        for (size_t t = 0; t < ca; ++t) {
+            h[t] = live->lo[ga][t];
+            h[ca + t] = live->hi[ga][t];
+        }
+        for (size_t t = 0; t < cb; ++t) {
+            h[2 * ca + t] = live->lo[gb][t];
+            h[2 * ca + cb + t] = live->hi[gb][t];
+        }
+        if ((rc = ctx->live_buf.ensure(h.size() * 8 + 64)) != MH_OK) return rc;
+        uint64_t *d = ctx->live_buf.as<uint64_t>();
+        MH_HIP(hipMemcpyAsync(d, h.data(), h.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        MH_HIP(hipStreamSynchronize(ctx->stream));
+        lp.lo0 = d;
+        lp.hi0 = d + ca;
+        lp.lo1 = d + 2 * ca;
+        lp.hi1 = d + 2 * ca + cb;
+        lp.c0 = (uint32_t)ca;
+        lp.c1 = (uint32_t)cb;
+        // a list that reads nothing gives no seeds
+        if ((cut_a && ca == 0) || (cut_b && cb == 0)) return MH_OK;
+    }
     MH_HIP(hipMemsetAsync(cnt, 0, 16, ctx->stream));
     {
         TimedLaunch tl(ctx, "join");
         hipLaunchKernelGGL(join_kernel, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, ctx->stream, a->d_keys, n0,
-                           b->d_keys, n1, flag, partner, cnt);
-        hipLaunchKernelGGL(repeat1_kernel, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, ctx->stream, b->d_keys,
-                           n1, a->d_keys, n0, cnt);
+                           b->d_keys, n1, lp, flag, partner);
         hipLaunchKernelGGL(tile_count_kernel, dim3((unsigned)ntiles), dim3(JT), 0, ctx->stream, flag, n0, tiles);
         hipLaunchKernelGGL(excl_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, tiles, ntiles, cnt + 1);
         MH_HIP(hipGetLastError());
@@ -1173,7 +1263,6 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const See
     uint32_t h[2] = {0, 0};
     MH_HIP(hipMemcpyAsync(h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
     MH_HIP(hipStreamSynchronize(ctx->stream));
-    *repeat_runs = h[0];
     const uint64_t ns = h[1];
     if (ns == 0) return MH_OK;
     if ((rc = ctx->mf_scratch[4].ensure(ns * 4 + 64)) != MH_OK) return rc;

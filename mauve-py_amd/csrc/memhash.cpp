@@ -191,14 +191,11 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
         MH_CHECK(smls[i], MH_ERR_INVALID, "null sorted mer list");
         MH_CHECK(smls[i]->seed == smls[0]->seed, MH_ERR_INVALID, "Different seed patterns.");
     }
-    uint32_t reps = 0;
     int rc;
-    if ((rc = repeat_runs_k(ctx, smls, k, &reps)) != MH_OK) return rc;
+    LiveRanges live;
+    if ((rc = search_live_ranges(ctx, smls, k, &live)) != MH_OK) return rc;
     if (st) std::memset(st, 0, sizeof(*st));
-    if (st) st->repeat_runs = reps;
-    MH_CHECK(reps == 0, MH_ERR_UNSUPPORTED,
-             "a seed run exceeds MER_REPEAT_LIMIT (1000); the reference's restart after such runs "
-             "(LM/MatchFinder.cpp:253-276) is not implemented");
+    if (st) st->repeat_runs = live.restarts;
     const uint64_t sl = (uint64_t)smls[0]->seed_length;
     // every pair's seeds on the device, with key = mer << 3 | pair (pairs
     // numbered in genome order), then one sort: HashMatch order
@@ -213,8 +210,7 @@ int find_matches_multi(mh_context *ctx, const mh_sml *const *smls, int k, std::v
         for (int b = a + 1; b < k; ++b) {
             const SeedRec *d_seeds = nullptr;
             uint64_t ns = 0;
-            uint32_t pr = 0;
-            if ((rc = join_and_extend(ctx, smls[a], smls[b], &d_seeds, &ns, &pr)) != MH_OK) return rc;
+            if ((rc = join_and_extend(ctx, smls[a], smls[b], &live, a, b, &d_seeds, &ns)) != MH_OK) return rc;
             walks += (int64_t)ctx->last_walks;
             long_walks += (int64_t)ctx->last_long_walks;
             pair_a[np] = a;
@@ -372,18 +368,16 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
     MH_CHECK(a->seed == b->seed, MH_ERR_INVALID, "Different seed patterns.");   // LM/MatchFinder.cpp:190-194
     const SeedRec *d_seeds = nullptr;
     uint64_t nseeds = 0;
-    uint32_t reps = 0;
-    int rc = join_and_extend(ctx, a, b, &d_seeds, &nseeds, &reps);
+    LiveRanges live;
+    int rc = search_live_ranges(ctx, smls, 2, &live);
     if (rc != MH_OK) return rc;
+    if ((rc = join_and_extend(ctx, a, b, &live, 0, 1, &d_seeds, &nseeds)) != MH_OK) return rc;
     if (st) {
         std::memset(st, 0, sizeof(*st));
-        st->repeat_runs = reps;
+        st->repeat_runs = live.restarts;
         st->walks = (int64_t)ctx->last_walks;
         st->long_walks = (int64_t)ctx->last_long_walks;
     }
-    MH_CHECK(reps == 0, MH_ERR_UNSUPPORTED,
-             "a seed run exceeds MER_REPEAT_LIMIT (1000); the reference's restart after such runs "
-             "(LM/MatchFinder.cpp:253-276) is not implemented");
     const int64_t sl = a->seed_length;
     const bool solid_odd = a->seed_length == a->seed_weight && (a->seed_length & 1);
     std::vector<CleanEnt> clean;

@@ -108,6 +108,7 @@ struct mh_context {
     mh::DevBuf grid_scratch[11];   // spaced-seed extension by chains
     mh::DevBuf diag_scratch[8];    // long extensions resolved per diagonal
     mh::DevBuf multi_scratch[11];  // the seeds of every genome pair, merged and filtered
+    mh::DevBuf heavy_buf, live_buf; // masked mers above the repeat limit; live ranges of a join
     uint64_t last_walks = 0, last_long_walks = 0;   // of the last seed extension
     // contexts (own streams and scratch) that run independent searches
     // concurrently for this one, created on first use
@@ -206,8 +207,11 @@ struct SeedRec {
 };
 // Seeds stay in device memory owned by the context (valid until the next
 // match search); *d_seeds points there.
-int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const SeedRec **d_seeds, uint64_t *ns,
-                    uint32_t *repeat_runs);
+// live (nullable) gives the index ranges of each list the search reads
+// (search_live_ranges); ga, gb are a's and b's list numbers in it.
+struct LiveRanges;
+int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const LiveRanges *live, int ga, int gb,
+                    const SeedRec **d_seeds, uint64_t *ns);
 // The merge key of the seeds of one genome pair: the masked mer at each
 // seed's genome-a start, shifted up three bits, or'ed with the pair number.
 int seed_mers(mh_context *ctx, const mh_sml *a, const SeedRec *d_seeds, uint64_t ns, uint32_t pair,
@@ -220,8 +224,6 @@ int merge_pair_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t *d_keys, 
 // MemHash bucket is marked in mask[40000], in order, with their pair numbers.
 int select_bucket_seeds(mh_context *ctx, const SeedRec *d_seeds, const uint64_t *d_keys, uint64_t n,
                         const uint8_t *mask, int seed_len, std::vector<SeedRec> &seeds, std::vector<uint8_t> &pairs);
-// Runs of one masked mer over the k lists longer than MER_REPEAT_LIMIT.
-int repeat_runs_k(mh_context *ctx, const mh_sml *const *smls, int k, uint32_t *runs);
 // A distinct extension of a clean bucket, and a seed of any other bucket.
 struct CleanEnt {
     uint32_t bucket;
@@ -247,6 +249,31 @@ uint64_t sml_compact_key(uint64_t k, int sw);
 // pair of buffers holding the result.  Runs on ctx->stream.
 int radix_sort_kv(mh_context *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, uint64_t n, int bits,
                   uint64_t **out_k, uint32_t **out_v);
+
+// Sorted lists as the restart planner reads them: compact keys
+// ((chars << 1) | strand, ordered as the 64-bit keys), host memory.
+constexpr int PLAN_MAXK = 16;
+struct PlanLists {
+    int k = 0, sw = 0, sl = 0;              // list count, seed weight, seed length
+    const uint64_t *ck[PLAN_MAXK] = {};     // sorted compact keys
+    uint64_t n[PLAN_MAXK] = {};             // SMLLength
+    uint64_t len[PLAN_MAXK] = {};           // sequence length (FindMer's bound)
+};
+// The index ranges of each list that SearchRange reads, restart by restart
+// (ascending, disjoint), and the number of restarts.
+struct LiveRanges {
+    std::vector<uint64_t> lo[PLAN_MAXK], hi[PLAN_MAXK];
+    int64_t restarts = 0;
+};
+// Plans SearchRange's repeat-limit restarts (LM/MatchFinder.cpp:166-340) given
+// every masked mer with more than MER_REPEAT_LIMIT entries over all lists
+// (ascending, distinct).  restart.cpp.
+int plan_restarts(const PlanLists &L, const std::vector<uint64_t> &heavy, LiveRanges *out);
+std::vector<uint64_t> heavy_mers_host(const PlanLists &L);
+// The live ranges of k device lists: heavy mers found on the device; when there
+// are any, the keys come to the host and the restarts are planned.  A list
+// with no restart reads [0, n).
+int search_live_ranges(mh_context *ctx, const mh_sml *const *smls, int k, LiveRanges *out);
 
 // host-side seed helpers (LM/SeedMasks.h)
 int64_t get_seed(int weight, int rank);

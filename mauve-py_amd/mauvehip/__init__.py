@@ -39,7 +39,8 @@ EXPORTS = [
     "mh_kernel_time_reset", "mh_set_timing", "mh_get_seed", "mh_get_seed_length",
     "mh_get_seed_weight", "mh_default_seed_weight", "mh_sml_create", "mh_sml_create_device",
     "mh_sml_free", "mh_sml_get_header", "mh_sml_read", "mh_sml_get_seed_mers", "mh_sml_find_mer",
-    "mh_sml_get_sequence", "mh_sml_save", "mh_sml_load", "mh_find_matches_pairwise", "mh_dp_align", "mh_dp_align_device",
+    "mh_sml_get_sequence", "mh_sml_save", "mh_sml_load", "mh_find_matches_pairwise", "mh_search_live_ranges",
+    "mh_dp_align", "mh_dp_align_device",
     "mh_device_alloc", "mh_device_free", "mh_memcpy_h2d", "mh_memcpy_d2h",
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free", "mh_match_list_seq_count",
@@ -142,6 +143,7 @@ def load_library(path=LIB_PATH):
         "mh_sml_load": (C.c_int, [vp, C.c_char_p, C.POINTER(vp), C.POINTER(C.c_int)]),
         "mh_find_matches_pairwise": (C.c_int, [vp, C.POINTER(vp), C.c_int, vp, u64, C.POINTER(u64),
                                                C.POINTER(MatchStats)]),
+        "mh_search_live_ranges": (C.c_int, [C.c_int, vp, vp, vp, u64, vp, u64, vp, vp]),
         "mh_dp_align": (C.c_int, [vp, vp, u64, vp, u64, vp, u64, vp, vp]),
         "mh_dp_align_device": (C.c_int, [vp, vp, vp, vp, u64, vp, vp]),
         "mh_dp_align_banded_device": (C.c_int, [vp, vp, vp, vp, u64, vp, C.c_uint32, vp, vp, vp,
@@ -614,6 +616,34 @@ def find_matches_raw(ctx, smls):
     finally:
         ctx.L.mh_match_list_free(h)
     return out, st
+
+
+def search_live_ranges(keys, lens, seed):
+    """Host-only plan of SearchRange's repeat-limit restarts over sorted
+    64-bit key arrays (one per list) of sequences of the given lengths:
+    (list of [(lo, hi), ...] per list, number of restarts)."""
+    L = load_library()
+    k = len(keys)
+    ks = [np.ascontiguousarray(x, dtype=np.uint64) for x in keys]
+    ptrs = (C.c_void_p * k)(*[x.ctypes.data if len(x) else None for x in ks])
+    n = (C.c_uint64 * k)(*[len(x) for x in ks])
+    ln = (C.c_uint64 * k)(*[int(x) for x in lens])
+    cap = 64
+    while True:
+        buf = np.zeros(3 * cap, dtype=np.uint64)
+        nr = C.c_uint64()
+        rs = C.c_int64()
+        rc = L.mh_search_live_ranges(k, ptrs, n, ln, C.c_uint64(seed), buf.ctypes.data, cap, C.byref(nr),
+                                     C.byref(rs))
+        if rc == -1 and nr.value > cap:
+            cap = nr.value
+            continue
+        _check(rc)
+        break
+    out = [[] for _ in range(k)]
+    for t in range(nr.value):
+        out[int(buf[3 * t])].append((int(buf[3 * t + 1]), int(buf[3 * t + 2])))
+    return out, rs.value
 
 
 def find_matches(ctx, smls):

@@ -360,6 +360,18 @@ static int sml_find_mer(const sml_t *s, uint64_t q, uint64_t *result)
     return s->keys[*result] == q;
 }
 
+/* FindMer over a sorted key list of a sequence of `len` bases (seed length
+ * sl): 1 found, 0 not found, -1 list too short (result untouched). */
+int or_find_mer_keys(const uint64_t *keys, uint64_t len, int sl, uint64_t q, uint64_t *result)
+{
+    sml_t s;
+    memset(&s, 0, sizeof(s));
+    s.keys = (uint64_t *)keys;
+    s.len = len;
+    s.sl = sl;
+    return sml_find_mer(&s, q, result);
+}
+
 /* ---- match entries (LM/MatchHashEntry.{h,cpp}, LM/HybridAbstractMatch.h) ---- */
 typedef struct {
     int64_t  start[OR_MAX_SEQ];

@@ -56,6 +56,10 @@ void     or_fill_dna_seed_sml(const uint32_t *seq, uint64_t n, uint64_t seed,
 
 /* Build a sorted mer list: packed words, keys sorted ascending (ties by
  * position) and the matching positions.  Returns number of entries, or -1. */
+/* SortedMerList::FindMer (LM/SortedMerList.cpp:170-179) on a sorted key list:
+ * 1 found, 0 not found (result = where the bsearch stopped), -1 a list shorter
+ * than the seed (result untouched). */
+int      or_find_mer_keys(const uint64_t *keys, uint64_t len, int sl, uint64_t q, uint64_t *result);
 int64_t  or_sml_build(const char *ascii, uint64_t len, uint64_t seed,
                       uint32_t *packed_out, uint64_t *keys_out, uint32_t *pos_out);
 

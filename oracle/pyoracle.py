@@ -55,6 +55,8 @@ def lib():
         L.or_fill_dna_seed_sml.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_void_p]
         L.or_sml_build.restype = C.c_int64
         L.or_sml_build.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_find_mer_keys.restype = C.c_int
+        L.or_find_mer_keys.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_uint64, C.POINTER(C.c_uint64)]
         L.or_find_matches_pairwise.restype = C.c_int64
         L.or_find_matches_pairwise.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
                                                C.c_uint64, C.POINTER(OrMatch), C.c_int64, C.POINTER(OrStats)]
@@ -154,6 +156,15 @@ def sml_build(seq, seed):
     if r < 0:
         raise ValueError("SML build failed")
     return words, keys[:r], pos[:r]
+
+
+def find_mer(keys, seq_len, seed, q):
+    """SortedMerList::FindMer on the sorted keys of a sequence of seq_len
+    bases: (1 found / 0 not found / -1 too short, result or None if unset)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    r = C.c_uint64(0xFFFFFFFFFFFFFFFF)
+    f = lib().or_find_mer_keys(k.ctypes.data if len(k) else None, seq_len, seed_length(seed), int(q), C.byref(r))
+    return f, (None if f < 0 else r.value)
 
 
 def _seq_args(seqs):

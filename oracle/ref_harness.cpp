@@ -11,6 +11,12 @@
 //       GetDnaSeedMer per position (LM/SortedMerList.cpp:764-769), and the
 //       sorted positions with their keys as the reference std::sort left them.
 //
+//   ref_harness findmer <seed> <in.seq> <queries.bin> <out.bin>
+//       Runs DNAMemorySML::Create on the sequence, then SortedMerList::FindMer
+//       (LM/SortedMerList.cpp:170-179, bsearch 380-394) for each uint64 query
+//       of queries.bin, and writes per query the found flag (uint64) and the
+//       index it left in result (uint64; UINT64_MAX if it left it unset).
+//
 //   ref_harness nw <in.txt> <out.txt>
 //       For each line "A B" runs ProfileProfile (MU/profile.cpp:68-93) on two
 //       one-sequence MSAs with the globals set as MuscleInterface::
@@ -86,6 +92,29 @@ int do_keys(uint64_t seed, const char* in, const char* out)
     return 0;
 }
 
+int do_findmer(uint64_t seed, const char* in, const char* qpath, const char* out)
+{
+    ifstream is(in);
+    string s((istreambuf_iterator<char>(is)), istreambuf_iterator<char>());
+    while (!s.empty() && (s.back() == '\n' || s.back() == '\r')) s.pop_back();
+    genome::gnSequence seq(s);
+    ProbeSML sml;
+    sml.Create(seq, seed);
+    FILE* q = fopen(qpath, "rb");
+    FILE* f = fopen(out, "wb");
+    if (!q || !f) return 2;
+    uint64_t v;
+    while (fread(&v, 8, 1, q) == 1) {
+        gnSeqI r = ~(gnSeqI)0;
+        const bool found = sml.FindMer(v, r);
+        put64(f, found ? 1 : 0);
+        put64(f, (uint64_t)r);
+    }
+    fclose(q);
+    fclose(f);
+    return 0;
+}
+
 void msa_from_string(muscle::MSA& msa, const string& s, unsigned id_base)
 {
     // same steps as msaFromSeqTable (LM/MuscleInterface.cpp:808-820)
@@ -143,8 +172,11 @@ int main(int argc, char** argv)
 {
     if (argc >= 5 && !strcmp(argv[1], "keys"))
         return do_keys(strtoull(argv[2], 0, 0), argv[3], argv[4]);
+    if (argc >= 6 && !strcmp(argv[1], "findmer"))
+        return do_findmer(strtoull(argv[2], 0, 0), argv[3], argv[4], argv[5]);
     if (argc >= 4 && !strcmp(argv[1], "nw"))
         return do_nw(argv[2], argv[3]);
-    fprintf(stderr, "usage: ref_harness keys <seed> <in.seq> <out.bin> | nw <in.txt> <out.txt>\n");
+    fprintf(stderr, "usage: ref_harness keys <seed> <in.seq> <out.bin> | findmer <seed> <in.seq> <queries.bin> "
+                    "<out.bin> | nw <in.txt> <out.txt>\n");
     return 1;
 }

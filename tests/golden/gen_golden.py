@@ -10,6 +10,10 @@ DNAMemorySML.cpp) in this container, and stores its outputs as data:
 * keys_golden.npz -- per case: the sequence, seed, packed words, the fill keys
                      in position order, GetDnaSeedMer per position and the sorted
                      positions exactly as the reference std::sort left them.
+* findmer_golden.npz -- per case: the sequence, seed, query keys and
+                     SortedMerList::FindMer's answer to each (found flag, and
+                     the index its bsearch left, LM/SortedMerList.cpp:170-179,
+                     380-394), as the repeat-limit restart calls it.
 
 Usage: python tests/golden/gen_golden.py   (needs the reference tree)
 """
@@ -130,12 +134,71 @@ def write_keys(path):
     return len(KEY_CASES)
 
 
+FINDMER_CASES = [
+    # (sequence builder, seed)
+    ("random", 3000, 0x16df6d),
+    ("polya", 3000, 0x1FFFF),
+    ("tandem", 4000, 0x16df6d),
+    ("random", 1200, 0x7954f),
+    ("lowcomp", 2000, 0x1FFFF),
+    ("short", 10, 0x1FFFF),       # shorter than the seed: FindMer leaves result unset
+    ("exact", 17, 0x1FFFF),       # one mer
+]
+
+
+def _findmer_seq(rng, kind, L):
+    r = lambda n: "".join("ACGT"[x] for x in rng.integers(0, 4, n))
+    if kind == "polya":
+        return r(L // 3) + "A" * (L // 3) + r(L - 2 * (L // 3)) + "T" * 40
+    if kind == "tandem":
+        return r(L // 4) + "AC" * (L // 8) + r(L // 4) + "ACG" * (L // 12) + r(L - L // 2 - 2 * (L // 8) - 3 * (L // 12))
+    if kind == "lowcomp":
+        return "".join("AAAAAACG"[x] for x in rng.integers(0, 8, L))
+    return r(L)
+
+
+def write_findmer(path):
+    import ctypes
+    rng = np.random.default_rng(4242)
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for ci, (kind, L, seed) in enumerate(FINDMER_CASES):
+            seq = _findmer_seq(rng, kind, L)
+            fs, fb = os.path.join(td, "s.seq"), os.path.join(td, "k.bin")
+            open(fs, "w").write(seq)
+            subprocess.check_call([HARNESS, "keys", hex(seed), fs, fb])
+            _, _, seedmer, pos = read_keys_bin(fb)
+            keys = np.sort(seedmer[pos]) if len(pos) else np.zeros(0, np.uint64)
+            w = bin(seed).count("1")
+            unit = np.uint64(1) << np.uint64(64 - 2 * w)
+            masked = keys & ~(unit - np.uint64(1)) if len(keys) else keys
+            q = [keys, keys + np.uint64(1), masked + unit, masked | np.uint64(1), masked - unit,
+                 np.array([0, 1, unit, 0xFFFFFFFFFFFFFFFF, 0xFFFFFFFFFFFFFFFF - unit + 1], dtype=np.uint64),
+                 (rng.integers(0, 2 ** 63, 200, dtype=np.int64).astype(np.uint64) << np.uint64(1))
+                 & ~(unit - np.uint64(1))]
+            qs = np.unique(np.concatenate(q).astype(np.uint64))
+            if len(qs) > 3000:
+                qs = np.sort(rng.choice(qs, 3000, replace=False))
+            fq, fo = os.path.join(td, "q.bin"), os.path.join(td, "o.bin")
+            qs.tofile(fq)
+            subprocess.check_call([HARNESS, "findmer", hex(seed), fs, fq, fo])
+            res = np.fromfile(fo, dtype=np.uint64).reshape(-1, 2)
+            out[f"c{ci}_seq"] = np.frombuffer(seq.encode(), np.uint8)
+            out[f"c{ci}_seed"] = np.array([seed], np.uint64)
+            out[f"c{ci}_query"] = qs
+            out[f"c{ci}_found"] = res[:, 0].astype(np.uint8)
+            out[f"c{ci}_result"] = res[:, 1]
+    np.savez_compressed(path, n_cases=np.array([len(FINDMER_CASES)]), **out)
+    return len(FINDMER_CASES)
+
+
 def main():
     if not os.path.exists(HARNESS):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
     n1 = write_dp(os.path.join(HERE, "dp_golden.txt"))
     n2 = write_keys(os.path.join(HERE, "keys_golden.npz"))
-    print(f"wrote {n1} DP cases and {n2} key cases")
+    n3 = write_findmer(os.path.join(HERE, "findmer_golden.npz"))
+    print(f"wrote {n1} DP cases, {n2} key cases and {n3} FindMer cases")
 
 
 if __name__ == "__main__":

@@ -143,14 +143,59 @@ def test_match_three_and_four_genomes(ctx, ng, seed):
     assert (gs != 0).sum(axis=1).max() == 2      # pairwise matches: two genomes each
 
 
-def test_match_repeat_limit_refused(ctx):
-    rng = np.random.default_rng(9)
-    ref = np.concatenate([synth.BASES[rng.integers(0, 4, 5000)], np.full(1200, ord("A"), np.uint8),
-                          synth.BASES[rng.integers(0, 4, 5000)]])
-    mut = ref.copy()
-    seed = po.get_seed(15, 3)
-    sa = mauvehip.SortedMerList(ctx, ref.tobytes(), seed)
-    sb = mauvehip.SortedMerList(ctx, mut.tobytes(), seed)
-    with pytest.raises(mauvehip.MhError) as e:
-        mauvehip.find_matches(ctx, [sa, sb])
-    assert e.value.code == -6
+def _tandem(unit, copies):
+    return np.frombuffer((unit * copies).encode(), dtype=np.uint8).copy()
+
+
+@pytest.mark.parametrize("seed", [po.get_seed(15, 3), po.get_seed(17, 3), po.get_seed(11, 0)])
+def test_match_repeat_restart(ctx, seed):
+    """Runs above MER_REPEAT_LIMIT (poly-A, (AC)n, (ACG)n, (AGCT)n) in both
+    genomes: the search restarts as the reference's SearchRange does
+    (LM/MatchFinder.cpp:253-276), and the match lists and restart counts
+    equal the oracle's."""
+    rng = np.random.default_rng(seed & 0xFFF)
+    r = lambda n: synth.BASES[rng.integers(0, 4, n)]
+    a = np.concatenate([r(6000), np.full(1300, ord("A"), np.uint8), r(5000), _tandem("AC", 1500), r(3000),
+                        _tandem("ACG", 1200), r(3000), _tandem("AGCT", 1100), r(8000)])
+    b = _mutant(rng, a, snp=0.01, indel=0.0005, inv=3000)
+    b = np.concatenate([r(2000), b[:9000], _tandem("AC", 900), b[9000:], np.full(800, ord("T"), np.uint8), r(500)])
+    print(f"[repeat] seed {seed:#x} lengths {len(a)} {len(b)}", flush=True)
+    _compare(ctx, a.tobytes(), b.tobytes(), seed)
+    _, _, ost = po.find_matches([a.tobytes(), b.tobytes()], seed)
+    assert _compare.last_stats.repeat_runs == ost.restarts >= 1
+
+
+@pytest.mark.parametrize("trial", [32, 60, 2, 16])
+def test_match_repeat_restart_at_buffer_ends(ctx, trial):
+    """Poly-C runs whose sorted entries straddle a 10,000-entry buffer end: the
+    restart fires (32, 60) or not (2, 16) on the refill check."""
+    rng = np.random.default_rng(10000 + trial)
+    seed = [po.get_seed(15, 3), po.get_seed(17, 3)][trial % 2]
+    a = synth.BASES[rng.integers(0, 4, int(rng.integers(15000, 45000)))]
+    b = synth.BASES[rng.integers(0, 4, int(rng.integers(15000, 45000)))]
+    for s in (a, b):
+        n = int(rng.integers(450, 700))
+        at = int(rng.integers(0, len(s) - n))
+        s[at:at + n] = ord("C")
+    _compare(ctx, a.tobytes(), b.tobytes(), seed)
+    assert (_compare.last_stats.repeat_runs > 0) == (trial in (32, 60))
+
+
+@pytest.mark.parametrize("ng,seed", [(3, po.get_seed(15, 3)), (4, po.get_seed(17, 3)), (4, po.get_seed(11, 0))])
+def test_match_repeat_restart_multi(ctx, ng, seed):
+    """Repeat-limit restarts with three and four genomes (PairwiseMatchFinder):
+    identical match lists and counts."""
+    rng = np.random.default_rng(ng * 77 + (seed & 0xFF))
+    seqs = _relatives(rng, synth.BASES[rng.integers(0, 4, 40_000)], ng, 0.02, 4000)
+    seqs = [np.concatenate([s[:10000], np.full(600 + 400 * g, ord("A"), np.uint8), s[10000:20000],
+                            _tandem("AC", 300 + 250 * (ng - g)), s[20000:]]) for g, s in enumerate(seqs)]
+    seqs = [s.tobytes() for s in seqs]
+    smls = [mauvehip.SortedMerList(ctx, s, seed) for s in seqs]
+    gs, gl, gst = mauvehip.find_matches(ctx, smls)
+    os_, ol, ost = po.find_matches(seqs, seed)
+    assert gst.seed_tuples == ost.hash_calls
+    assert gst.collisions == ost.collisions
+    assert gst.repeat_runs == ost.restarts
+    assert len(gl) == len(ol) > 50
+    assert np.array_equal(gs, os_)
+    assert np.array_equal(gl, ol)
