@@ -225,6 +225,23 @@ int mh_dp_align_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
                        const mh_dp_job *jobs, uint64_t njobs,
                        uint8_t *d_paths, mh_dp_result *d_results);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same DP as mh_dp_align_device (same schedule, kernels
+ * and results), with every scratch buffer -- each round's job table and
+ * results, stripe boundary columns and trace arena -- in one fixed device array
+ * of mh_dp_noalloc_scratch_bytes() (64 MiB) that the library holds on each GPU
+ * until the process ends; no hipMalloc, hipFree or pinned allocation is made.
+ * Jobs run in as many rounds as the scratch needs; a single job whose trace
+ * and boundaries exceed it is refused with MH_ERR_INVALID (about 100M cells,
+ * e.g. 10,000 x 10,000).  The array is shared by every call on one GPU and
+ * keeps the last call's contents: calls that overlap on one GPU (several
+ * contexts or threads) race and must be serialised by the caller. */
+int mh_dp_align_device_noalloc(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
+                               const mh_dp_job *jobs, uint64_t njobs,
+                               uint8_t *d_paths, mh_dp_result *d_results);
+uint64_t mh_dp_noalloc_scratch_bytes(void);
+
 /* Banded mode (no counterpart in the reference, whose NWSmall always fills
  * the whole matrix; the full DP stays the default and the judge of it).
  * bands[j] = W > 0 restricts job j to the region of band 2W around the

@@ -35,6 +35,7 @@
 #include <type_traits>
 #include <cstring>
 #include <thread>
+#include <mutex>
 
 namespace mh {
 namespace {
@@ -988,6 +989,37 @@ constexpr double COST_PER_CELL[3] = {1.30, 1.10, 1.00};   // cols 4, 8, 16
 
 }  // namespace
 
+// Scratch of the no-allocation entry point (mh_dp_align_device_noalloc): the
+// round's job table, stripe boundary columns and trace arena, in one array
+// that every GPU the library runs on holds until the process ends.  Host code
+// never uses the array's name: each device's address comes from
+// hipGetSymbolAddress, resolved once per device by dp_scratch_init.
+constexpr uint64_t DP_SCRATCH_BYTES = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[DP_SCRATCH_BYTES];
+
+namespace {
+constexpr int DP_SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[DP_SCRATCH_MAX_DEVICES] = {};
+}  // namespace
+
+int dp_scratch_init(int device)
+{
+    MH_CHECK(device >= 0 && device < DP_SCRATCH_MAX_DEVICES, MH_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MH_OK;
+    int cur = -1;
+    MH_HIP(hipGetDevice(&cur));
+    MH_CHECK(cur == device, MH_ERR_INVALID, "dp_scratch_init: device is not current");
+    void *ptr = nullptr;
+    MH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    MH_CHECK(ptr != nullptr, MH_ERR_DEVICE, "hipGetSymbolAddress gave a null address for the DP scratch");
+    g_scratch_ptr[device] = static_cast<uint8_t *>(ptr);
+    return MH_OK;
+}
+
+uint64_t dp_scratch_bytes() { return DP_SCRATCH_BYTES; }
+
 int dp_force_cols()
 {
     if (const char *e = getenv("MH_DP_COLS")) {
@@ -1061,9 +1093,20 @@ uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t col
     return (l0 == 1 && f1 == (int)la) ? 0u : band;
 }
 
+// A fixed device scratch (the no-allocation entry point): each round's job
+// table, results, boundary columns and trace arena are laid out in it, and
+// no device or pinned memory is allocated.
+struct FixedScratch {
+    uint8_t *ptr = nullptr;
+    uint64_t bytes = 0;
+};
+
+static inline uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
+
 static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
                         const std::vector<uint64_t> &slot, std::vector<DpItem> &items, uint8_t *d_paths,
-                        mh_dp_result *d_results, uint32_t *d_dev, uint64_t *cells)
+                        mh_dp_result *d_results, uint32_t *d_dev, uint64_t *cells,
+                        const FixedScratch *fx = nullptr)
 {
     const uint64_t nitems = items.size();
     if (nitems == 0) return MH_OK;
@@ -1096,15 +1139,30 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         const uint64_t ns = (jobs[it.job].b_len + 64 * it.cols - 1) / (64 * it.cols);
         return ns > 1 ? 3ull * ((jobs[it.job].a_len + 63) & ~63ull) : 0ull;
     };
+    // footprint in a fixed scratch of a round of `cnt` jobs with trace bytes
+    // tbb (each rounded up to 16) and bi boundary ints
+    auto fixed_need = [&](uint64_t cnt, uint64_t tbb, uint64_t bi) {
+        return align256(cnt * (sizeof(DevJob) + sizeof(DpOut))) + align256(bi * 4) + tbb + 256;
+    };
+    if (fx) {
+        for (const DpItem &it : items)
+            MH_CHECK(fixed_need(1, (it.tb_bytes + 15) & ~15ull, bnd_ints(it)) <= fx->bytes, MH_ERR_INVALID,
+                     "DP job " + std::to_string(it.job) + " (" + std::to_string(jobs[it.job].a_len) + " x " +
+                         std::to_string(jobs[it.job].b_len) + ") needs more than the " +
+                         std::to_string(fx->bytes >> 20) + " MiB scratch of the no-allocation entry point");
+    }
     // trace arena budget: 70% of the memory that is free or already held by
     // this context's arena (at most 192 GB); once grown the arena is kept
-    size_t free_b = 0, total_b = 0;
-    MH_HIP(hipMemGetInfo(&free_b, &total_b));
-    uint64_t budget = (uint64_t)((free_b + ctx->dp_tb.bytes + ctx->dp_bnd.bytes) * 0.7);
-    const uint64_t cap = 192ull << 30;
-    if (budget > cap) budget = cap;
-    if (const char *e = getenv("MH_DP_TB_BUDGET_GB")) budget = (uint64_t)(atof(e) * (1ull << 30));
-    {
+    uint64_t budget = 0;
+    if (!fx) {
+        size_t free_b = 0, total_b = 0;
+        MH_HIP(hipMemGetInfo(&free_b, &total_b));
+        budget = (uint64_t)((free_b + ctx->dp_tb.bytes + ctx->dp_bnd.bytes) * 0.7);
+        const uint64_t cap = 192ull << 30;
+        if (budget > cap) budget = cap;
+        if (const char *e = getenv("MH_DP_TB_BUDGET_GB")) budget = (uint64_t)(atof(e) * (1ull << 30));
+    }
+    if (!fx) {
         // size the arena once for the whole call: the largest round fits the budget
         uint64_t need = 0;
         for (const DpItem &it : items) need += it.tb_bytes + bnd_ints(it) * 4;
@@ -1125,8 +1183,14 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         uint64_t tbb = 0, bi = 0, last = first;
         while (last < nitems) {
             const uint64_t w = items[last].tb_bytes, b = bnd_ints(items[last]);
-            if (last > first && tbb + w + (bi + b) * 4 > budget) break;
-            tbb += w;
+            if (fx) {
+                const uint64_t w16 = (w + 15) & ~15ull;
+                if (last > first && fixed_need(last - first + 1, tbb + w16, bi + b) > fx->bytes) break;
+                tbb += w16;
+            } else {
+                if (last > first && tbb + w + (bi + b) * 4 > budget) break;
+                tbb += w;
+            }
             bi += b;
             ++last;
         }
@@ -1166,7 +1230,11 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
                jobs[items[first + nwave].job].a_len + jobs[items[first + nwave].job].b_len >= WAVE_WALK_MIN)
             ++nwave;
         DevJob *dj = nullptr;
-        {
+        std::vector<DevJob> host_jobs;
+        if (fx) {
+            host_jobs.resize(n);
+            dj = host_jobs.data();
+        } else {
             const int rc0 = pinned_jobs(ctx, n * sizeof(DevJob), (void **)&dj);
             if (rc0 != MH_OK) return rc0;
         }
@@ -1190,12 +1258,30 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             bo += bnd_ints(it);
             if (cells) *cells += 2 * it.tb_bytes;
         }
-        int rc;
-        if ((rc = ctx->dp_tb.ensure(tw + 256)) != MH_OK) return rc;
-        if ((rc = ctx->dp_bnd.ensure(bo * 4 + 256)) != MH_OK) return rc;
-        if ((rc = ctx->dp_jobs.ensure(n * sizeof(DevJob) + n * sizeof(DpOut) + 256)) != MH_OK) return rc;
-        DevJob *d_jobs = ctx->dp_jobs.as<DevJob>();
-        DpOut *d_outs = reinterpret_cast<DpOut *>(d_jobs + n);
+        DevJob *d_jobs;
+        DpOut *d_outs;
+        uint8_t *d_tb;
+        int *d_bnd;
+        if (fx) {
+            // [jobs | results] [boundary ints] [trace]
+            uint8_t *q = fx->ptr;
+            d_jobs = reinterpret_cast<DevJob *>(q);
+            d_outs = reinterpret_cast<DpOut *>(d_jobs + n);
+            q += align256(n * (sizeof(DevJob) + sizeof(DpOut)));
+            d_bnd = reinterpret_cast<int *>(q);
+            q += align256(bo * 4);
+            d_tb = q;
+            MH_CHECK((uint64_t)(q - fx->ptr) + tw <= fx->bytes, MH_ERR_INVALID, "DP round exceeds the scratch");
+        } else {
+            int rc;
+            if ((rc = ctx->dp_tb.ensure(tw + 256)) != MH_OK) return rc;
+            if ((rc = ctx->dp_bnd.ensure(bo * 4 + 256)) != MH_OK) return rc;
+            if ((rc = ctx->dp_jobs.ensure(n * sizeof(DevJob) + n * sizeof(DpOut) + 256)) != MH_OK) return rc;
+            d_jobs = ctx->dp_jobs.as<DevJob>();
+            d_outs = reinterpret_cast<DpOut *>(d_jobs + n);
+            d_tb = ctx->dp_tb.as<uint8_t>();
+            d_bnd = ctx->dp_bnd.as<int>();
+        }
         MH_HIP(hipMemcpyAsync(d_jobs, dj, n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
         {
             TimedLaunch tl(ctx, "dp_forward");
@@ -1210,15 +1296,15 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             sh.wave[ROW_SHAPES] = waves;
             if (n64 + waves)
                 hipLaunchKernelGGL(dp_forward_all, dim3((unsigned)(n64 + waves)), dim3(64), 0, ctx->stream, d_jobs,
-                                   (uint32_t)n64, sh, d_a, d_b, ctx->dp_tb.as<uint8_t>(), ctx->dp_bnd.as<int>(), d_outs,
+                                   (uint32_t)n64, sh, d_a, d_b, d_tb, d_bnd, d_outs,
                                    sr[0], sr[1], sr[2], sr[3], -200);
             if (nbig > nstripe)
                 hipLaunchKernelGGL(dp_window, dim3((unsigned)(nbig - nstripe)), dim3(64), 0, ctx->stream,
-                                   d_jobs + nstripe, d_a, d_b, ctx->dp_tb.as<uint8_t>(), d_outs + nstripe, sr[0],
+                                   d_jobs + nstripe, d_a, d_b, d_tb, d_outs + nstripe, sr[0],
                                    sr[1], sr[2], sr[3], -200);
             if (n > nbig)
                 hipLaunchKernelGGL(dp_tiny, dim3((unsigned)((n - nbig + 63) / 64)), dim3(64), 0, ctx->stream,
-                                   d_jobs + nbig, (int)(n - nbig), d_a, d_b, ctx->dp_tb.as<uint8_t>(),
+                                   d_jobs + nbig, (int)(n - nbig), d_a, d_b, d_tb,
                                    d_outs + nbig, sr[0], sr[1], sr[2], sr[3], -200);
             MH_HIP(hipGetLastError());
         }
@@ -1228,10 +1314,10 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             TimedLaunch tl(ctx, "dp_traceback");
             if (nwave)
                 hipLaunchKernelGGL(dp_traceback_wave, dim3((unsigned)nwave), dim3(64), 0, ctx->stream, d_jobs,
-                                   d_outs, ctx->dp_tb.as<uint8_t>(), d_paths, d_results, d_dev);
+                                   d_outs, d_tb, d_paths, d_results, d_dev);
             if (n > nwave)
                 hipLaunchKernelGGL(dp_traceback, dim3((unsigned)((n - nwave + 63) / 64)), dim3(64), 0, ctx->stream,
-                                   d_jobs + nwave, d_outs + nwave, ctx->dp_tb.as<uint8_t>(), d_paths, d_results,
+                                   d_jobs + nwave, d_outs + nwave, d_tb, d_paths, d_results,
                                    d_dev, (int)(n - nwave));
             MH_HIP(hipGetLastError());
         }
@@ -1348,6 +1434,27 @@ int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_
     make_items(jobs, njobs, nullptr, items);
     host_time(ctx, "host_dp_plan", now_ms() - tp0);
     return dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, nullptr, nullptr);
+}
+
+int dp_run_noalloc(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs, uint64_t njobs,
+                   uint8_t *d_paths, mh_dp_result *d_results)
+{
+    uint8_t *base = nullptr;
+    if (ctx->device >= 0 && ctx->device < DP_SCRATCH_MAX_DEVICES) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = g_scratch_ptr[ctx->device];
+    }
+    MH_CHECK(base, MH_ERR_INVALID, "the DP scratch of this device was not set up (mh_create does it)");
+    if (njobs == 0) return MH_OK;
+    std::vector<uint64_t> slot;
+    int rc;
+    if ((rc = dp_check_jobs(jobs, njobs, slot)) != MH_OK) return rc;
+    std::vector<DpItem> items;
+    make_items(jobs, njobs, nullptr, items);
+    FixedScratch fx;
+    fx.ptr = base;
+    fx.bytes = DP_SCRATCH_BYTES;
+    return dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, nullptr, nullptr, &fx);
 }
 
 int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs, uint64_t njobs,

@@ -171,6 +171,13 @@ int pinned_jobs(mh_context *ctx, size_t bytes, void **out);   // a second pinned
 // Launchers (return MH_OK or an error code); all run on ctx->stream.
 int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
            uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results);
+// The no-allocation DP: the same schedule and kernels with every scratch
+// buffer in the library's fixed per-device array (dp_scratch_bytes() bytes,
+// address resolved by dp_scratch_init for the current device).
+int dp_scratch_init(int device);
+uint64_t dp_scratch_bytes();
+int dp_run_noalloc(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs, uint64_t njobs,
+                   uint8_t *d_paths, mh_dp_result *d_results);
 // paths[0, n) ('M', 'D', 'I') packed four to a byte into out: edge x at bits
 // 2(x mod 4) of byte x / 4, codes M 0, D 1, I 2 (3 for anything else)
 int dp_pack_paths(mh_context *ctx, const uint8_t *d_paths, uint64_t n, uint8_t *d_out);

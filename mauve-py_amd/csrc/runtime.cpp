@@ -318,6 +318,11 @@ int mh_create(int device, mh_context **out)
     MH_HIP(hipGetDeviceCount(&n));
     MH_CHECK(device >= 0 && device < n, MH_ERR_INVALID, "no such HIP device");
     MH_HIP(hipSetDevice(device));
+    {
+        // the no-allocation DP's scratch address on this device (once per device)
+        const int rc = mh::dp_scratch_init(device);
+        if (rc != MH_OK) return rc;
+    }
     mh_context *c = new mh_context();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -458,6 +463,17 @@ int mh_dp_align_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, 
     MH_HIP(hipSetDevice(ctx->device));
     return dp_run(ctx, d_a, d_b, jobs, njobs, d_paths, d_results);
 }
+
+int mh_dp_align_device_noalloc(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
+                               uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results)
+{
+    MH_CHECK(ctx, MH_ERR_INVALID, "null context");
+    MH_CHECK(njobs == 0 || (jobs && d_a && d_b && d_paths && d_results), MH_ERR_INVALID, "null argument");
+    MH_HIP(hipSetDevice(ctx->device));
+    return dp_run_noalloc(ctx, d_a, d_b, jobs, njobs, d_paths, d_results);
+}
+
+uint64_t mh_dp_noalloc_scratch_bytes(void) { return mh::dp_scratch_bytes(); }
 
 int mh_dp_align_banded_device(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
                               uint64_t njobs, const uint32_t *bands, uint32_t flags, uint8_t *d_paths,

@@ -40,7 +40,7 @@ EXPORTS = [
     "mh_get_seed_weight", "mh_default_seed_weight", "mh_sml_create", "mh_sml_create_device",
     "mh_sml_free", "mh_sml_get_header", "mh_sml_read", "mh_sml_get_seed_mers", "mh_sml_find_mer",
     "mh_sml_get_sequence", "mh_sml_save", "mh_sml_load", "mh_find_matches_pairwise", "mh_search_live_ranges",
-    "mh_dp_align", "mh_dp_align_device",
+    "mh_dp_align", "mh_dp_align_device", "mh_dp_align_device_noalloc", "mh_dp_noalloc_scratch_bytes",
     "mh_device_alloc", "mh_device_free", "mh_memcpy_h2d", "mh_memcpy_d2h",
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free", "mh_match_list_seq_count",
@@ -146,6 +146,8 @@ def load_library(path=LIB_PATH):
         "mh_search_live_ranges": (C.c_int, [C.c_int, vp, vp, vp, u64, vp, u64, vp, vp]),
         "mh_dp_align": (C.c_int, [vp, vp, u64, vp, u64, vp, u64, vp, vp]),
         "mh_dp_align_device": (C.c_int, [vp, vp, vp, vp, u64, vp, vp]),
+        "mh_dp_align_device_noalloc": (C.c_int, [vp, vp, vp, vp, u64, vp, vp]),
+        "mh_dp_noalloc_scratch_bytes": (u64, []),
         "mh_dp_align_banded_device": (C.c_int, [vp, vp, vp, vp, u64, vp, C.c_uint32, vp, vp, vp,
                                                 C.POINTER(BandStats)]),
         "mh_dp_band_region": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
