@@ -17,6 +17,23 @@
 //       of queries.bin, and writes per query the found flag (uint64) and the
 //       index it left in result (uint64; UINT64_MAX if it left it unset).
 //
+//   ref_harness app <in.txt> <out.txt>
+//       For each line "row1 row2" (two gapped rows of one length) sets up
+//       MSAs as MuscleInterface::ProfileAlignFast(anchored = true) does
+//       (LM/MuscleInterface.cpp:1053-1106) and writes three lines:
+//         "S <n> <score bits ...>"  LetterObjScoreXP's column scores after
+//                                   PrepareMSAforScoring (MU/anchoredpp.cpp:
+//                                   256-316, 430-441), float bits in hex;
+//         "A <k> <cols ...>"        FindAnchorColsPP (MU/anchoredpp.cpp:354-404);
+//         "R <row1> <row2>"         AnchoredProfileProfile's output rows
+//                                   (MU/anchoredpp.cpp:443-552).
+//
+//   ref_harness refinew <window> <in.txt> <out.txt>
+//       For each line "row1 row2" runs MuscleInterface::RefineFast(ga,
+//       window) for two sequences (LM/MuscleInterface.cpp:823-894):
+//       the globals, the guide tree, RefineW (MU/refinew.cpp:75-170), and
+//       writes the two output rows.
+//
 //   ref_harness nw <in.txt> <out.txt>
 //       For each line "A B" runs ProfileProfile (MU/profile.cpp:68-93) on two
 //       one-sequence MSAs with the globals set as MuscleInterface::
@@ -38,8 +55,17 @@
 #include "libMUSCLE/msa.h"
 #include "libMUSCLE/profile.h"
 #include "libMUSCLE/params.h"
+#include "libMUSCLE/tree.h"
 
 using namespace std;
+
+// libMUSCLE functions with external linkage that no header declares
+namespace muscle {
+SCORE LetterObjScoreXP(const MSA &msa1, const MSA &msa2, SCORE MatchScore[]);
+void FindAnchorColsPP(const MSA &msa1, const MSA &msa2, unsigned AnchorCols[], unsigned *ptruAnchorColCount);
+void PrepareMSAforScoring(MSA &msa);
+void RefineW(const MSA &msaIn, MSA &msaOut);
+}
 
 namespace {
 
@@ -126,6 +152,140 @@ void msa_from_string(muscle::MSA& msa, const string& s, unsigned id_base)
     for (size_t i = 0; i < s.size(); ++i) msa.SetChar(0, i, s[i]);
 }
 
+void set_pairwise_globals()
+{
+    using namespace muscle;
+    // as ProfileAlignFast / RefineFast set them (LM/MuscleInterface.cpp:837-846, 1088-1097)
+    g_SeqType.get() = SEQTYPE_DNA;
+    g_uMaxIters.get() = 1;
+    g_bStable.get() = true;
+    g_bQuiet.get() = true;
+    g_SeqWeight1.get() = SEQWEIGHT_ClustalW;
+    SetMaxIters(g_uMaxIters.get());
+    SetSeqWeightMethod(g_SeqWeight1.get());
+}
+
+// output row of msaOut for input sequence id `id` (GetSeqIndex order)
+string row_of(const muscle::MSA& msa, unsigned id)
+{
+    for (unsigned r = 0; r < msa.GetSeqCount(); ++r)
+        if (msa.GetSeqIndex(r) == id) {
+            const char* buf = msa.GetSeqBuffer(msa.GetSeqIndex(r));
+            return string(buf, msa.GetColCount());
+        }
+    return string();
+}
+
+int do_app(const char* in, const char* out)
+{
+    using namespace muscle;
+    ifstream is(in);
+    FILE* f = fopen(out, "w");
+    if (!f) return 2;
+    string line;
+    while (getline(is, line)) {
+        istringstream ls(line);
+        string a, b;
+        if (!(ls >> a >> b)) continue;
+        set_pairwise_globals();
+        MSA::SetIdCount(2);
+        {
+            // column scores and anchor columns on MSAs prepared as
+            // AnchoredProfileProfile prepares them
+            MSA m1, m2;
+            msa_from_string(m1, a, 0);
+            msa_from_string(m2, b, 1);
+            SetAlpha(ALPHA_DNA);
+            m1.FixAlpha();
+            m2.FixAlpha();
+            SetPPScore(PPSCORE_SPN);
+            PrepareMSAforScoring(m1);
+            PrepareMSAforScoring(m2);
+            vector<SCORE> sc(a.size());
+            LetterObjScoreXP(m1, m2, sc.data());
+            fprintf(f, "S %zu", sc.size());
+            for (SCORE x : sc) {
+                uint32_t u;
+                memcpy(&u, &x, 4);
+                fprintf(f, " %08x", u);
+            }
+            fprintf(f, "\n");
+            MSA p1, p2;
+            msa_from_string(p1, a, 0);
+            msa_from_string(p2, b, 1);
+            p1.FixAlpha();
+            p2.FixAlpha();
+            PrepareMSAforScoring(p1);
+            PrepareMSAforScoring(p2);
+            vector<unsigned> cols(a.size() + 1);
+            unsigned nc = 0;
+            FindAnchorColsPP(p1, p2, cols.data(), &nc);
+            fprintf(f, "A %u", nc);
+            for (unsigned i = 0; i < nc; ++i) fprintf(f, " %u", cols[i]);
+            fprintf(f, "\n");
+        }
+        set_pairwise_globals();
+        MSA::SetIdCount(2);
+        MSA msa1, msa2, msaOut;
+        msa_from_string(msa1, a, 0);
+        msa_from_string(msa2, b, 1);
+        SetAlpha(ALPHA_DNA);
+        msa1.FixAlpha();
+        msa2.FixAlpha();
+        SetPPScore(PPSCORE_SPN);
+        AnchoredProfileProfile(msa1, msa2, msaOut);
+        fprintf(f, "R %s %s\n", row_of(msaOut, 0).c_str(), row_of(msaOut, 1).c_str());
+    }
+    fclose(f);
+    return 0;
+}
+
+int do_refinew(unsigned window, const char* in, const char* out)
+{
+    using namespace muscle;
+    ifstream is(in);
+    FILE* f = fopen(out, "w");
+    if (!f) return 2;
+    string line;
+    while (getline(is, line)) {
+        istringstream ls(line);
+        string a, b;
+        if (!(ls >> a >> b)) continue;
+        set_pairwise_globals();
+        g_uRefineWindow.get() = window;
+        g_uWindowTo.get() = 0;
+        MSA::SetIdCount(2);
+        MSA msa;
+        msa.SetSize(2, a.size());
+        const string* rows[2] = {&a, &b};
+        for (unsigned r = 0; r < 2; ++r) {
+            char name[32];
+            snprintf(name, sizeof(name), "seq%u", r);
+            msa.SetSeqName(r, name);
+            msa.SetSeqId(r, r);
+            for (size_t i = 0; i < a.size(); ++i) msa.SetChar(r, i, (*rows[r])[i]);
+        }
+        SetAlpha(ALPHA_DNA);
+        msa.FixAlpha();
+        SetPPScore(PPSCORE_SPN);
+        Tree GuideTree;
+        TreeFromMSA(msa, GuideTree, g_Cluster2.get(), g_Distance2.get(), g_Root2.get());
+        SetMuscleTree(GuideTree);
+        MSA msaOut;
+        RefineW(msa, msaOut);
+        // RefineFast reads the rows by GetSeqIndex (LM/MuscleInterface.cpp:879-886)
+        string r0, r1;
+        for (unsigned r = 0; r < msaOut.GetSeqCount(); ++r) {
+            const unsigned idx = msaOut.GetSeqIndex(r);
+            string row(msaOut.GetSeqBuffer(idx), msaOut.GetColCount());
+            (r == 0 ? r0 : r1) = row;
+        }
+        fprintf(f, "%s %s\n", r0.c_str(), r1.c_str());
+    }
+    fclose(f);
+    return 0;
+}
+
 int do_nw(const char* in, const char* out)
 {
     using namespace muscle;
@@ -174,9 +334,13 @@ int main(int argc, char** argv)
         return do_keys(strtoull(argv[2], 0, 0), argv[3], argv[4]);
     if (argc >= 6 && !strcmp(argv[1], "findmer"))
         return do_findmer(strtoull(argv[2], 0, 0), argv[3], argv[4], argv[5]);
+    if (argc >= 4 && !strcmp(argv[1], "app"))
+        return do_app(argv[2], argv[3]);
+    if (argc >= 5 && !strcmp(argv[1], "refinew"))
+        return do_refinew((unsigned)strtoul(argv[2], 0, 0), argv[3], argv[4]);
     if (argc >= 4 && !strcmp(argv[1], "nw"))
         return do_nw(argv[2], argv[3]);
     fprintf(stderr, "usage: ref_harness keys <seed> <in.seq> <out.bin> | findmer <seed> <in.seq> <queries.bin> "
-                    "<out.bin> | nw <in.txt> <out.txt>\n");
+                    "<out.bin> | app <in.txt> <out.txt> | refinew <window> <in.txt> <out.txt> | nw <in.txt> <out.txt>\n");
     return 1;
 }

@@ -10,6 +10,15 @@ DNAMemorySML.cpp) in this container, and stores its outputs as data:
 * keys_golden.npz -- per case: the sequence, seed, packed words, the fill keys
                      in position order, GetDnaSeedMer per position and the sorted
                      positions exactly as the reference std::sort left them.
+* refine_golden.txt -- libMUSCLE's own profile pass and refinement windows on
+                     two-row gapped alignments (gaps of many lengths, so the
+                     column scores carry fractional gap shares; terminal gaps;
+                     lower case; wildcard letters in the score-only cases):
+                     "APP row1 row2" followed by LetterObjScoreXP's column
+                     scores (float bits), FindAnchorColsPP's anchor columns
+                     and AnchoredProfileProfile's rows (MU/anchoredpp.cpp),
+                     and "RW window row1 row2" followed by RefineW's rows
+                     (MU/refinew.cpp) under RefineFast's globals.
 * findmer_golden.npz -- per case: the sequence, seed, query keys and
                      SortedMerList::FindMer's answer to each (found flag, and
                      the index its bsearch left, LM/SortedMerList.cpp:170-179,
@@ -192,13 +201,96 @@ def write_findmer(path):
     return len(FINDMER_CASES)
 
 
+def _app_safe(r1, r2):
+    """True when every range between the oracle's anchor columns holds
+    residues of both rows (otherwise libMUSCLE's NWSmall quits the process)."""
+    sys.path.insert(0, ROOT)
+    from oracle import refine_oracle as ro
+    for lo, hi in ro.cols_to_ranges(ro.anchor_cols(r1, r2), len(r1)):
+        if hi == lo:
+            continue
+        if not ro._residues(r1[lo:hi]) or not ro._residues(r2[lo:hi]):
+            return False
+    return True
+
+
+def _refine_inputs():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refine_cases import alignment
+    rng = np.random.default_rng(99)
+    app, rw, score_only = [], [], []
+    while len(app) < 50:
+        n = int(rng.choice([40, 120, 300, 600]))
+        r1, r2 = alignment(rng, n, div=float(rng.choice([0.02, 0.1, 0.25, 0.4])),
+                           marble=float(rng.choice([0.0, 0.1, 0.5])), lower=bool(rng.random() < 0.2))
+        if len(r1) < 2 or not _app_safe(r1, r2):
+            continue
+        app.append((r1, r2))
+    # hand-placed gaps of 1..9 columns (shares of -400 by 1..9), terminal gaps
+    for k in range(1, 10):
+        core = "".join("ACGT"[x] for x in rng.integers(0, 4, 80))
+        r1 = core[:30] + "-" * k + core[30:]
+        r2 = core[:30] + "".join("ACGT"[x] for x in rng.integers(0, 4, k)) + core[30:]
+        r1 = "-" * (k % 3) + r1 + "-" * (k % 2)
+        r2 = "".join("ACGT"[x] for x in rng.integers(0, 4, k % 3)) + r2 + "".join("ACGT"[x] for x in rng.integers(0, 4, k % 2))
+        if _app_safe(r1, r2):
+            app.append((r1, r2))
+    # wildcards: column scores and anchor columns only
+    for _ in range(12):
+        r1, r2 = alignment(rng, int(rng.choice([60, 200, 400])), div=0.1, marble=0.1)
+        b1, b2 = bytearray(r1.encode()), bytearray(r2.encode())
+        for b in (b1, b2):
+            for i in rng.integers(0, len(b), max(1, len(b) // 30)):
+                if b[i] != ord("-"):
+                    b[i] = ord(rng.choice(list("NRYKMSWBDHVX")))
+        if _app_safe(b1.decode(), b2.decode()):
+            score_only.append((b1.decode(), b2.decode()))
+    while len(rw) < 24:
+        n = int(rng.choice([150, 600, 1200]))
+        r1, r2 = alignment(rng, n, div=float(rng.choice([0.02, 0.1, 0.3])), marble=float(rng.choice([0.0, 0.2])),
+                           lower=bool(rng.random() < 0.2))
+        w = int(rng.choice([200, 500]))
+        # RefineW needs residues in every window
+        ok = all(any(c != "-" for c in r1[f:f + w] + r2[f:f + w]) for f in range(0, len(r1), w))
+        if ok:
+            rw.append((w, r1, r2))
+    return app, score_only, rw
+
+
+def write_refine(path):
+    app, wild, rw = _refine_inputs()
+    lines = ["# libMUSCLE profile pass (APP, WILD with wildcard letters) and refinement windows (RW); see gen_golden.py"]
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.txt"), os.path.join(td, "out.txt")
+        for tag, cases in (("APP", app), ("WILD", wild)):
+            with open(fin, "w") as f:
+                for r1, r2 in cases:
+                    f.write(f"{r1} {r2}\n")
+            subprocess.check_call([HARNESS, "app", fin, fout])
+            res = open(fout).read().splitlines()
+            assert len(res) == 3 * len(cases)
+            for (r1, r2), k in zip(cases, range(0, len(res), 3)):
+                lines.append(f"{tag} {r1} {r2}")
+                lines.extend(res[k:k + 3])
+        for w, r1, r2 in rw:
+            with open(fin, "w") as f:
+                f.write(f"{r1} {r2}\n")
+            subprocess.check_call([HARNESS, "refinew", str(w), fin, fout])
+            res = open(fout).read().split()
+            lines.append(f"RW {w} {r1} {r2}")
+            lines.append(f"O {res[0]} {res[1]}")
+    open(path, "w").write("\n".join(lines) + "\n")
+    return len(app) + len(wild), len(rw)
+
+
 def main():
     if not os.path.exists(HARNESS):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
     n1 = write_dp(os.path.join(HERE, "dp_golden.txt"))
     n2 = write_keys(os.path.join(HERE, "keys_golden.npz"))
     n3 = write_findmer(os.path.join(HERE, "findmer_golden.npz"))
-    print(f"wrote {n1} DP cases, {n2} key cases and {n3} FindMer cases")
+    n4, n5 = write_refine(os.path.join(HERE, "refine_golden.txt"))
+    print(f"wrote {n1} DP cases, {n2} key cases, {n3} FindMer cases, {n4} profile-pass and {n5} window cases")
 
 
 if __name__ == "__main__":
