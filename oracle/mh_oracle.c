@@ -931,6 +931,75 @@ static const float NUC_SP[4][4] = {
     {-123 + 60.f, -31 + 60.f, -114 + 60.f, 91 + 60.f},
 };
 
+/* Match scores of single-sequence profile positions for the six operand
+ * classes A, C, G, T, X and any other letter, as libMUSCLE computes them:
+ * fractional counts (MU/msa2.cpp:19-90: a letter counts 1; a wildcard's
+ * DNA branch tests the amino values AX_R and AX_Y, so 'X' (NX_X == AX_R)
+ * takes half to G and half to A, and every other wildcard, with any letter
+ * FixAlpha turns into 'N' (MU/msa.cpp:795-813), 1/20 to each base; counts are
+ * not normalised for DNA, MU/params.cpp:296-310), m_AAScores = sum_j counts[j]
+ * * NUC_SP[i][j] (MU/profilefrommsa.cpp:279-285), SortCounts' bubble order
+ * (159-183) and ScoreProfPos2SPN (MU/scorepp.cpp:80-92, centre 0), each step
+ * rounded to float. */
+float OR_SCORE6[OR_NCLASS][OR_NCLASS];
+static int score6_ready;
+
+static void score6_init(void)
+{
+    if (score6_ready) return;
+    const float w = 1.0f;
+    float cnt[OR_NCLASS][4];
+    memset(cnt, 0, sizeof(cnt));
+    for (int c = 0; c < 4; ++c) cnt[c][c] = w;
+    cnt[4][2] += w / 2;
+    cnt[4][0] += w / 2;
+    const float f = w / 20;
+    for (int l = 0; l < 4; ++l) cnt[5][l] += f;
+    float aa[OR_NCLASS][4];
+    int order[OR_NCLASS][4];
+    for (int c = 0; c < OR_NCLASS; ++c) {
+        for (int i = 0; i < 4; ++i) {
+            float sum = 0;
+            for (int j = 0; j < 4; ++j) {
+                const float t = cnt[c][j] * NUC_SP[i][j];
+                sum += t;
+            }
+            aa[c][i] = sum;
+        }
+        for (int i = 0; i < 4; ++i) order[c][i] = i;
+        int any = 1;
+        while (any) {
+            any = 0;
+            for (int n = 0; n < 3; ++n)
+                if (cnt[c][order[c][n]] < cnt[c][order[c][n + 1]]) {
+                    const int t = order[c][n];
+                    order[c][n] = order[c][n + 1];
+                    order[c][n + 1] = t;
+                    any = 1;
+                }
+        }
+    }
+    for (int a = 0; a < OR_NCLASS; ++a)
+        for (int b = 0; b < OR_NCLASS; ++b) {
+            float score = 0;
+            for (int n = 0; n < 4; ++n) {
+                const int l = order[a][n];
+                const float fc = cnt[a][l];
+                if (fc == 0) break;
+                const float t = fc * aa[b][l];
+                score += t;
+            }
+            OR_SCORE6[a][b] = score - 0.0f;
+        }
+    score6_ready = 1;
+}
+
+void or_score6(float *out)
+{
+    score6_init();
+    memcpy(out, OR_SCORE6, sizeof(OR_SCORE6));
+}
+
 typedef struct { float open, close; } gp_t;
 
 /* Gap scores of a one-sequence profile (MU/profilefrommsa.cpp:285-292:
@@ -957,10 +1026,13 @@ int64_t or_nwsmall_params(const uint8_t *PAc, uint32_t LA, const uint8_t *PBc, u
                           float gap_open, float e, char *path_out, float *score_out)
 {
     if (LA == 0 || LB == 0) return -1;          /* MU/nwsmall.cpp:503-504 Quit */
+    for (uint32_t i = 0; i < LA; ++i) if (PAc[i] >= OR_NCLASS) return -3;
+    for (uint32_t j = 0; j < LB; ++j) if (PBc[j] >= OR_NCLASS) return -3;
+    score6_init();
     gp_t *GA = (gp_t *)malloc(LA * sizeof(gp_t)), *GB = (gp_t *)malloc(LB * sizeof(gp_t));
     make_gaps(GA, LA, gap_open);
     make_gaps(GB, LB, gap_open);
-#define S(a, b) (NUC_SP[PAc[a]][PBc[b]])
+#define S(a, b) (OR_SCORE6[PAc[a]][PBc[b]])
     const uint32_t PCA = LA + 1, PCB = LB + 1;
     float *MCurr = (float *)malloc(PCB * sizeof(float)), *MNext = (float *)malloc(PCB * sizeof(float));
     float *MPrev = (float *)malloc(PCB * sizeof(float)), *DRow = (float *)malloc(PCB * sizeof(float));

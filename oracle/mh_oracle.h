@@ -100,6 +100,12 @@ int or_extend_seed(int k, const char *const *seqs, const uint64_t *lens, uint64_
                    or_match *m);
 
 /* ---- inter-anchor DP (MU/nwsmall.cpp, MU/bittraceback.cpp) ---- */
+/* DP operand classes: 0-3 A,C,G,T (either case; U counts as T), 4 'X', 5 any
+ * other letter (libMUSCLE's wildcard counts; see or_score6). */
+#define OR_NCLASS 6
+/* The 6 x 6 float match scores of the classes (row = A's class). */
+void or_score6(float *out);
+
 /* A, B: letter codes 0..3 (A,C,G,T).  path_out receives LA+LB bytes max of
  * 'M'/'D'/'I' in alignment order; returns path length, or -1 if LA==0 or
  * LB==0 (the reference Quit()s there).  *score_out gets the best terminal
@@ -107,7 +113,8 @@ int or_extend_seed(int k, const char *const *seqs, const uint64_t *lens, uint64_
 int64_t or_nwsmall(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
                    char *path_out, float *score_out);
 
-/* Same DP with explicit gap parameters (open is the full open penalty, split
+/* Operands are classes 0..5 (OR_NCLASS); -3 for a code outside them.
+ * Same DP with explicit gap parameters (open is the full open penalty, split
  * in half between open and close as MU/profilefrommsa.cpp:291-292 does). */
 int64_t or_nwsmall_params(const uint8_t *A, uint32_t LA, const uint8_t *B, uint32_t LB,
                           float gap_open, float gap_extend,

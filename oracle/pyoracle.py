@@ -55,6 +55,7 @@ def lib():
         L.or_fill_dna_seed_sml.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_void_p]
         L.or_sml_build.restype = C.c_int64
         L.or_sml_build.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_score6.argtypes = [C.c_void_p]
         L.or_find_mer_keys.restype = C.c_int
         L.or_find_mer_keys.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_uint64, C.POINTER(C.c_uint64)]
         L.or_find_matches_pairwise.restype = C.c_int64
@@ -292,9 +293,24 @@ def extend_seed(seqs, seed, starts, length):
     return [m.start[s] for s in range(k)], m.length
 
 
+_CLASS = np.full(256, 5, dtype=np.uint8)
+for _i, _c in enumerate(b"ACGTX"):
+    _CLASS[_c] = _i
+    _CLASS[ord(chr(_c).lower())] = _i
+_CLASS[ord("U")] = _CLASS[ord("u")] = 3
+
+
 def codes(seq):
-    return np.frombuffer(_b(seq).upper().translate(bytes.maketrans(b"ACGT", b"\x00\x01\x02\x03")),
-                         dtype=np.uint8).copy()
+    """DP operand classes: A,C,G,T (U as T) -> 0..3, X -> 4, any other letter
+    -> 5 (libMUSCLE's wildcard counts, see or_score6; U's class is the one
+    under which libMUSCLE's paths in tests/golden/dp_wild_golden.txt come out)."""
+    return _CLASS[np.frombuffer(_b(seq), dtype=np.uint8)]
+
+
+def score6():
+    out = np.zeros(36, dtype=np.float32)
+    lib().or_score6(out.ctypes.data)
+    return out.reshape(6, 6)
 
 
 def nwsmall(a, b):

@@ -7,6 +7,10 @@ DNAMemorySML.cpp) in this container, and stores its outputs as data:
 * dp_golden.txt   -- "A B rowA rowB" per line: ProfileProfile (MU/profile.cpp:68-93,
                      driving GlobalAlign -> NWSmall -> BitTraceBack) on two
                      one-sequence profiles set up as ProfileAlignFast does.
+* dp_wild_golden.txt -- the same for operands with wildcard and other
+                     letters (X, N, R, Y, ..., lower case, U, invalid letters),
+                     which libMUSCLE scores with fractional profile counts in
+                     float (MU/msa2.cpp:19-90).
 * keys_golden.npz -- per case: the sequence, seed, packed words, the fill keys
                      in position order, GetDnaSeedMer per position and the sorted
                      positions exactly as the reference std::sort left them.
@@ -86,6 +90,49 @@ def write_dp(path):
     assert len(rows) == len(pairs)
     with open(path, "w") as f:
         f.write("# A B rowA rowB  (reference ProfileProfile/NWSmall output)\n")
+        for (a, b), (ra, rb) in zip(pairs, rows):
+            f.write(f"{a} {b} {ra} {rb}\n")
+    return len(pairs)
+
+
+def dp_wild_cases():
+    rng = np.random.default_rng(20261203)
+    alph = "ACGT"
+    wild = "XNRYKMSWBDHVnxuUZ"
+    pairs = []
+
+    def seq(n, pw):
+        return "".join(wild[rng.integers(0, len(wild))] if rng.random() < pw else alph[rng.integers(0, 4)]
+                       for _ in range(n))
+    for la in (1, 2, 3, 5):
+        for lb in (1, 2, 4, 9):
+            pairs.append((seq(la, 0.5), seq(lb, 0.5)))
+    for _ in range(120):
+        la = int(rng.choice([4, 12, 40, 90, 160]))
+        a = seq(la, float(rng.choice([0.02, 0.1, 0.4])))
+        if rng.random() < 0.7:
+            b = _mutate(rng, a, float(rng.choice([0.02, 0.1, 0.25])))
+            b = "".join(wild[rng.integers(0, len(wild))] if rng.random() < 0.05 else c for c in b)
+        else:
+            b = seq(int(rng.integers(1, 150)), 0.2)
+        pairs.append((a, b))
+    pairs.append(("N" * 30, "N" * 25))
+    pairs.append(("X" * 20 + "ACGT", "AGAG" * 6))
+    return pairs
+
+
+def write_dp_wild(path):
+    pairs = dp_wild_cases()
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.txt"), os.path.join(td, "out.txt")
+        with open(fin, "w") as f:
+            for a, b in pairs:
+                f.write(f"{a} {b}\n")
+        subprocess.check_call([HARNESS, "nw", fin, fout], stderr=subprocess.DEVNULL)   # MUSCLE warns of invalid letters
+        rows = [l.split() for l in open(fout)]
+    assert len(rows) == len(pairs)
+    with open(path, "w") as f:
+        f.write("# A B rowA rowB  (reference ProfileProfile/NWSmall output; wildcard letters)\n")
         for (a, b), (ra, rb) in zip(pairs, rows):
             f.write(f"{a} {b} {ra} {rb}\n")
     return len(pairs)
@@ -287,6 +334,7 @@ def main():
     if not os.path.exists(HARNESS):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
     n1 = write_dp(os.path.join(HERE, "dp_golden.txt"))
+    n1 += write_dp_wild(os.path.join(HERE, "dp_wild_golden.txt"))
     n2 = write_keys(os.path.join(HERE, "keys_golden.npz"))
     n3 = write_findmer(os.path.join(HERE, "findmer_golden.npz"))
     n4, n5 = write_refine(os.path.join(HERE, "refine_golden.txt"))

@@ -7,7 +7,8 @@ sets):
 * LetterObjScoreXP column scores, float bit for bit: gaps of 1 to 9 columns
   spread -400 in fractional shares, terminal gaps -200, wildcard letters;
 * FindAnchorColsPP (smoothing in float, best columns, MergeBestCols);
-* AnchoredProfileProfile's rows (ranges re-aligned by ProfileProfile);
+* AnchoredProfileProfile's rows (ranges re-aligned by ProfileProfile, in
+  float where a range holds wildcard letters);
 * RefineW's rows for windows of 200 and 500 columns.
 """
 import os
@@ -67,7 +68,7 @@ def test_anchor_columns(i):
     assert mauvehip.refine_anchor_cols(r1, r2) == want
 
 
-@pytest.mark.parametrize("i", [i for i in range(len(APP)) if APP[i][0] == "APP"])
+@pytest.mark.parametrize("i", range(len(APP)))
 def test_anchored_profile_profile_rows(i):
     _, r1, r2, _, _, o1, o2 = APP[i]
     assert ro.anchored_profile_profile(r1, r2) == (o1, o2)
