@@ -191,8 +191,13 @@ int mh_match_list_read(const char *path, mh_match_list **out);
 
 /* ---- inter-anchor gapped DP ----------------------------------------------- */
 
-/* One DP job: A and B as letter codes 0..3 (A,C,G,T) at the given offsets of
- * the A and B code buffers. */
+/* One DP job: A and B as operand codes at the given offsets of the A and B
+ * code buffers.  Codes: 0-3 A,C,G,T (U as T), 4 'X', 5 any other letter --
+ * the classes libMUSCLE's profile code gives a position (MU/msa2.cpp:19-90:
+ * 'X' counts half to A and half to G, every other wildcard and every letter
+ * FixAlpha turns into 'N' 1/20 to each base).  A job whose operands are all
+ * A,C,G,T runs the int32 DP; any other runs NWSmall in float, as the
+ * reference does. */
 typedef struct {
     uint64_t a_off;
     uint64_t b_off;
@@ -201,7 +206,8 @@ typedef struct {
 } mh_dp_job;
 
 typedef struct {
-    int32_t  score;        /* terminal score of the chosen edge (NWSmall 641-656) */
+    int32_t  score;        /* terminal score of the chosen edge (NWSmall 641-656),
+                              rounded to the nearest integer for float jobs    */
     uint32_t path_len;     /* number of edges                                     */
     uint64_t path_start;   /* offset of the first edge in the path buffer         */
 } mh_dp_result;
@@ -334,9 +340,9 @@ int  mh_mlp_train(mh_context *ctx, mh_mlp *mlp, const float *d_x, const float *d
 int  mh_band_predict(mh_context *ctx, mh_mlp *mlp, const float *d_feat, uint64_t rows, float delta,
                      uint32_t min_band, uint32_t *bands);
 
-/* ASCII genome (device) -> letter codes 0..3 (A,C,G,T, either case) for the DP.
- * Any other byte is refused with MH_ERR_UNSUPPORTED (the DP implements the
- * four-letter alphabet only). */
+/* ASCII genome (device) -> DP operand codes (mh_dp_job): A,C,G,T,U in either
+ * case 0..3, X 4, any other byte 5; the gap characters '-' and '.' are refused
+ * with MH_ERR_GAP_CHAR. */
 int mh_ascii_to_codes_device(mh_context *ctx, const void *d_ascii, uint64_t len, uint8_t *d_codes);
 
 /* Inter-anchor DP jobs from a match list.  No counterpart in the reference:
@@ -414,7 +420,7 @@ int mh_recursion_gaps(const char *row_a, const char *row_b, uint64_t cols, int64
  * MU/refinew.cpp:75-200).  Each re-alignment is one NWSmall of genome 1's
  * residues against genome 2's; all of a pass's jobs, over all n intervals,
  * run as one mh_dp_align batch.  Rows are ASCII with '-' gaps, residues
- * A,C,G,T in either case (others: MH_ERR_UNSUPPORTED, as for the DP); an
+ * in either case, wildcard letters scored as libMUSCLE scores them; an
  * anchored range with residues of one genome only is MH_ERR_ZERO_LENGTH (the
  * reference's NWSmall quits).  out1[i]/out2[i] receive out_cols[i] columns;
  * caps[i] >= 2 * cols[i] always suffices (MH_ERR_INVALID with out_cols set

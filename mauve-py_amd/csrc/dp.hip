@@ -1093,14 +1093,6 @@ uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t col
     return (l0 == 1 && f1 == (int)la) ? 0u : band;
 }
 
-// A fixed device scratch (the no-allocation entry point): each round's job
-// table, results, boundary columns and trace arena are laid out in it, and
-// no device or pinned memory is allocated.
-struct FixedScratch {
-    uint8_t *ptr = nullptr;
-    uint64_t bytes = 0;
-};
-
 static inline uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
 
 static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
@@ -1422,18 +1414,60 @@ uint64_t dp_item_cells(uint32_t la, uint32_t lb, uint32_t band)
     return 2 * it.tb_bytes;
 }
 
+// The jobs whose operands hold letters other than A, C, G, T (class > 3):
+// from the caller's flags, or a device scan; codes above 5 are refused.
+static int split_wild(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
+                      uint64_t njobs, const uint8_t *wild, const FixedScratch *fx, std::vector<uint32_t> &out)
+{
+    out.clear();
+    if (wild) {
+        for (uint64_t j = 0; j < njobs; ++j)
+            if (wild[j]) out.push_back((uint32_t)j);
+        return MH_OK;
+    }
+    std::vector<uint8_t> mx;
+    const int rc = dp_code_max(ctx, d_a, d_b, jobs, njobs, fx, mx);
+    if (rc != MH_OK) return rc;
+    for (uint64_t j = 0; j < njobs; ++j) {
+        MH_CHECK(mx[j] < DP_NCLASS, MH_ERR_INVALID,
+                 "DP operand code above 5 in job " + std::to_string(j) + " (codes: 0-3 A,C,G,T, 4 X, 5 other)");
+        if (mx[j] > 3) out.push_back((uint32_t)j);
+    }
+    return MH_OK;
+}
+
+// drops the items of the float-DP jobs (items[j] is job j's)
+static void drop_wild(std::vector<DpItem> &items, const std::vector<uint32_t> &wild)
+{
+    if (wild.empty()) return;
+    size_t w = 0, k = 0;
+    for (size_t j = 0; j < items.size(); ++j) {
+        if (k < wild.size() && wild[k] == j) {
+            ++k;
+            continue;
+        }
+        items[w++] = items[j];
+    }
+    items.resize(w);
+}
+
 int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
-           uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results)
+           uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results, const uint8_t *wild)
 {
     if (njobs == 0) return MH_OK;
     std::vector<uint64_t> slot;
     int rc;
     if ((rc = dp_check_jobs(jobs, njobs, slot)) != MH_OK) return rc;
+    std::vector<uint32_t> wj;
+    if ((rc = split_wild(ctx, d_a, d_b, jobs, njobs, wild, nullptr, wj)) != MH_OK) return rc;
     std::vector<DpItem> items;
     const double tp0 = now_ms();
     make_items(jobs, njobs, nullptr, items);
+    drop_wild(items, wj);
     host_time(ctx, "host_dp_plan", now_ms() - tp0);
-    return dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, nullptr, nullptr);
+    if ((rc = dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, nullptr, nullptr)) != MH_OK)
+        return rc;
+    return dp_run_float(ctx, d_a, d_b, jobs, slot, wj, d_paths, d_results, nullptr, nullptr, nullptr);
 }
 
 int dp_run_noalloc(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs, uint64_t njobs,
@@ -1449,12 +1483,17 @@ int dp_run_noalloc(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, cons
     std::vector<uint64_t> slot;
     int rc;
     if ((rc = dp_check_jobs(jobs, njobs, slot)) != MH_OK) return rc;
-    std::vector<DpItem> items;
-    make_items(jobs, njobs, nullptr, items);
     FixedScratch fx;
     fx.ptr = base;
     fx.bytes = DP_SCRATCH_BYTES;
-    return dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, nullptr, nullptr, &fx);
+    std::vector<uint32_t> wj;
+    if ((rc = split_wild(ctx, d_a, d_b, jobs, njobs, nullptr, &fx, wj)) != MH_OK) return rc;
+    std::vector<DpItem> items;
+    make_items(jobs, njobs, nullptr, items);
+    drop_wild(items, wj);
+    if ((rc = dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, nullptr, nullptr, &fx)) != MH_OK)
+        return rc;
+    return dp_run_float(ctx, d_a, d_b, jobs, slot, wj, d_paths, d_results, nullptr, nullptr, &fx);
 }
 
 int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs, uint64_t njobs,
@@ -1481,18 +1520,31 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
         const uint64_t w = bands ? bands[j] : 0u;
         region[j] = 2 * w > 0xFFFFFFFFull ? 0u : (uint32_t)(2 * w);
     }
+    // jobs with letters other than A,C,G,T run the full DP in float, unbanded
+    std::vector<uint32_t> wj;
+    if ((rc = split_wild(ctx, d_a, d_b, jobs, njobs, nullptr, nullptr, wj)) != MH_OK) return rc;
     std::vector<DpItem> items;
     make_items(jobs, njobs, region.data(), items);
     std::vector<uint32_t> accept1(njobs, 0);
-    for (uint64_t j = 0; j < njobs; ++j) {
-        if (items[j].band) {
-            accept1[j] = bands[j];
-            ++st.banded;
+    {
+        size_t k = 0;
+        for (uint64_t j = 0; j < njobs; ++j) {
+            if (k < wj.size() && wj[k] == j) {
+                ++k;
+                continue;
+            }
+            if (items[j].band) {
+                accept1[j] = bands[j];
+                ++st.banded;
+            }
         }
     }
+    drop_wild(items, wj);
     host_time(ctx, "host_dp_plan", now_ms() - tp0);
     const int force = dp_force_cols();
     if ((rc = dp_run_items(ctx, d_a, d_b, jobs, slot, items, d_paths, d_results, d_dev, &st.cells)) != MH_OK)
+        return rc;
+    if ((rc = dp_run_float(ctx, d_a, d_b, jobs, slot, wj, d_paths, d_results, d_dev, &st.cells, nullptr)) != MH_OK)
         return rc;
     if (!(flags & MH_BAND_NO_FALLBACK) && st.banded) {
         // edge fallback: a banded path that reaches distance W is rerun with

@@ -102,6 +102,7 @@ struct mh_context {
     std::vector<hipEvent_t> event_pool;
     // DP scratch
     mh::DevBuf dp_tb, dp_bnd, dp_jobs, dp_a, dp_b, dp_paths, dp_res, dp_dev;
+    mh::DevBuf dp_wild, dp_fl;   // operand scan; the float DP's job table and rows
     mh::DevBuf ml_jobs, ml_y;   // learned band: job table, network outputs + bands
     // seed-matching scratch
     mh::DevBuf mf_scratch[16];
@@ -169,8 +170,29 @@ int pinned(mh_context *ctx, size_t bytes, void **out);
 int pinned_jobs(mh_context *ctx, size_t bytes, void **out);   // a second pinned buffer, for DP job tables
 
 // Launchers (return MH_OK or an error code); all run on ctx->stream.
+// wild (nullable): per job, nonzero when its operands hold a class above 3
+// (the caller knows); otherwise the operands are scanned on the device.
 int dp_run(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
-           uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results);
+           uint64_t njobs, uint8_t *d_paths, mh_dp_result *d_results, const uint8_t *wild = nullptr);
+// DP operand classes: 0-3 A,C,G,T, 4 'X', 5 any other letter (libMUSCLE's
+// wildcard counts); jobs with a class above 3 run the float DP (dp_float.hip).
+constexpr int DP_NCLASS = 6;
+// A fixed device scratch (the no-allocation entry point): job tables and
+// arenas are laid out in it and nothing is allocated.
+struct FixedScratch {
+    uint8_t *ptr = nullptr;
+    uint64_t bytes = 0;
+};
+// out[j] = the largest operand code of job j (device scan, synchronises).
+int dp_code_max(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs, uint64_t njobs,
+                const FixedScratch *fx, std::vector<uint8_t> &out);
+// NWSmall in float for the jobs `which` (indices into jobs; slot = path slot
+// starts); cells (nullable) accumulates la * lb.
+int dp_run_float(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
+                 const std::vector<uint64_t> &slot, const std::vector<uint32_t> &which, uint8_t *d_paths,
+                 mh_dp_result *d_results, uint32_t *d_dev, uint64_t *cells, const FixedScratch *fx);
+// the 6 x 6 float match scores of the classes (row = A's class)
+void dp_score6(float *out);
 // The no-allocation DP: the same schedule and kernels with every scratch
 // buffer in the library's fixed per-device array (dp_scratch_bytes() bytes,
 // address resolved by dp_scratch_init for the current device).

@@ -58,7 +58,7 @@ inline unsigned letter(char c)
     case 'A': case 'a': return 0;
     case 'C': case 'c': return 1;
     case 'G': case 'g': return 2;
-    case 'T': case 't': return 3;
+    case 'T': case 't': case 'U': case 'u': return 3;   // libMUSCLE scores U as T
     default: return 99;
     }
 }
@@ -506,17 +506,32 @@ void plan_piece(const char *r1, const char *r2, const Prefix &pf, bool profile, 
     }
 }
 
-// Residue codes of row r in [lo, hi) into out; false on a letter outside ACGT.
+// DP operand classes of the residues (A,C,G,T with U as T -> 0..3, X -> 4,
+// any other letter -> 5; libMUSCLE's wildcard counts, see dp_float.hip).
+struct OperandCodes {
+    uint8_t v[256];
+    OperandCodes()
+    {
+        for (int c = 0; c < 256; ++c) {
+            const unsigned l = letter((char)c);
+            v[c] = l < 4 ? (uint8_t)l : (c == 'X' || c == 'x') ? 4 : 5;
+        }
+    }
+};
+const OperandCodes OPCODES;
+
+// Operand classes of the residues of row r in [lo, hi) into out; true when
+// any is not A, C, G, T (the job takes the float DP).
 bool put_codes(const char *r, uint64_t lo, uint64_t hi, uint8_t *out)
 {
-    unsigned bad = 0;
+    unsigned wild = 0;
     for (uint64_t c = lo; c < hi; ++c) {
         if (r[c] == '-') continue;
-        const uint8_t l = CODES.v[(uint8_t)r[c]];
-        bad |= l >= 4;
-        *out++ = (uint8_t)(l & 3);
+        const uint8_t l = OPCODES.v[(uint8_t)r[c]];
+        wild |= l > 3;
+        *out++ = l;
     }
-    return !bad;
+    return wild != 0;
 }
 
 // One pass over all intervals: plan (in parallel over pieces), one batched DP
@@ -607,11 +622,10 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
         int rc = pinned(ctx, res_off + njobs * sizeof(mh_dp_result) + 64, (void **)&stage);
         if (rc != MH_OK) return rc;
     }
-    std::atomic<int> bad(0);
+    std::vector<uint8_t> wild(njobs, 0);
     parallel_for(nt, [&](size_t k) {
         const Task &t = tasks[k];
         uint64_t pa = abase[k], pb = bbase[k];
-        bool ok = true;
         for (size_t j = 0; j < t.jobs.size(); ++j) {
             const Job &jb = t.jobs[j];
             mh_dp_job &dj = jobs[jbase[k] + j];
@@ -619,14 +633,13 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
             dj.b_off = pb;
             dj.a_len = jb.la;
             dj.b_len = jb.lb;
-            ok &= put_codes(in1[t.iv], jb.lo, jb.hi, stage + pa);
-            ok &= put_codes(in2[t.iv], jb.lo, jb.hi, stage + abytes + pb);
+            const bool w1 = put_codes(in1[t.iv], jb.lo, jb.hi, stage + pa);
+            const bool w2 = put_codes(in2[t.iv], jb.lo, jb.hi, stage + abytes + pb);
+            wild[jbase[k] + j] = (uint8_t)(w1 || w2);
             pa += jb.la;
             pb += jb.lb;
         }
-        if (!ok) bad = 1;
     });
-    MH_CHECK(!bad, MH_ERR_UNSUPPORTED, "residue other than A,C,G,T in a DP operand");
     lap("host_refine_plan");
     const uint8_t *paths = stage + abytes + bbytes;
     const mh_dp_result *res = reinterpret_cast<const mh_dp_result *>(stage + res_off);
@@ -639,7 +652,7 @@ int refine_pass(mh_context *ctx, const std::vector<const char *> &in1, const std
         MH_HIP(hipMemcpyAsync(ctx->dp_a.ptr, stage, abytes, hipMemcpyHostToDevice, ctx->stream));
         MH_HIP(hipMemcpyAsync(ctx->dp_b.ptr, stage + abytes, bbytes, hipMemcpyHostToDevice, ctx->stream));
         rc = dp_run(ctx, ctx->dp_a.as<uint8_t>(), ctx->dp_b.as<uint8_t>(), jobs.data(), njobs,
-                    ctx->dp_paths.as<uint8_t>(), ctx->dp_res.as<mh_dp_result>());
+                    ctx->dp_paths.as<uint8_t>(), ctx->dp_res.as<mh_dp_result>(), wild.data());
         if (rc != MH_OK) return rc;
         // the paths come back at two bits an edge
         if ((rc = ctx->dp_packed.ensure(pbytes / 4 + 64)) != MH_OK) return rc;

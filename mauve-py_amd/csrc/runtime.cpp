@@ -529,9 +529,9 @@ int mh_dp_align(mh_context *ctx, const uint8_t *a_codes, uint64_t a_bytes, const
         total += (uint64_t)jobs[j].a_len + jobs[j].b_len;
     }
     for (uint64_t x = 0; x < a_bytes; ++x)
-        MH_CHECK(a_codes[x] < 4, MH_ERR_UNSUPPORTED, "A code outside 0..3 (only A,C,G,T residues)");
+        MH_CHECK(a_codes[x] < DP_NCLASS, MH_ERR_INVALID, "A code outside 0..5");
     for (uint64_t x = 0; x < b_bytes; ++x)
-        MH_CHECK(b_codes[x] < 4, MH_ERR_UNSUPPORTED, "B code outside 0..3 (only A,C,G,T residues)");
+        MH_CHECK(b_codes[x] < DP_NCLASS, MH_ERR_INVALID, "B code outside 0..5");
     MH_HIP(hipSetDevice(ctx->device));
     int rc;
     if ((rc = ctx->dp_a.ensure(a_bytes + 64)) != MH_OK) return rc;
@@ -540,8 +540,16 @@ int mh_dp_align(mh_context *ctx, const uint8_t *a_codes, uint64_t a_bytes, const
     if ((rc = ctx->dp_res.ensure(njobs * sizeof(mh_dp_result) + 64)) != MH_OK) return rc;
     MH_HIP(hipMemcpyAsync(ctx->dp_a.ptr, a_codes, a_bytes, hipMemcpyHostToDevice, ctx->stream));
     MH_HIP(hipMemcpyAsync(ctx->dp_b.ptr, b_codes, b_bytes, hipMemcpyHostToDevice, ctx->stream));
+    // jobs with letters other than A,C,G,T (classes 4, 5) take the float DP
+    std::vector<uint8_t> wild(njobs, 0);
+    for (uint64_t j = 0; j < njobs; ++j) {
+        uint8_t m = 0;
+        for (uint32_t x = 0; x < jobs[j].a_len; ++x) m |= a_codes[jobs[j].a_off + x];
+        for (uint32_t x = 0; x < jobs[j].b_len; ++x) m |= b_codes[jobs[j].b_off + x];
+        wild[j] = m > 3;
+    }
     rc = dp_run(ctx, ctx->dp_a.as<uint8_t>(), ctx->dp_b.as<uint8_t>(), jobs, njobs, ctx->dp_paths.as<uint8_t>(),
-                ctx->dp_res.as<mh_dp_result>());
+                ctx->dp_res.as<mh_dp_result>(), wild.data());
     if (rc != MH_OK) return rc;
     MH_HIP(hipMemcpyAsync(paths, ctx->dp_paths.ptr, total, hipMemcpyDeviceToHost, ctx->stream));
     MH_HIP(hipMemcpyAsync(results, ctx->dp_res.ptr, njobs * sizeof(mh_dp_result), hipMemcpyDeviceToHost,

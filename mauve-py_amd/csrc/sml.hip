@@ -311,19 +311,22 @@ __global__ void check_sorted_kernel(const uint64_t *__restrict__ keys, uint64_t 
     if (keys[x] > keys[x + 1]) atomicOr(bad, 1);
 }
 
-// ASCII -> DP letter codes (A,C,G,T either case -> 0..3; anything else flags)
+// ASCII -> DP operand classes: A,C,G,T (U as T), either case -> 0..3, X -> 4,
+// any other letter -> 5 (libMUSCLE's wildcard counts, see dp_float.hip); the
+// gap characters '-' and '.' are not residues and flag
 __global__ void codes_kernel(const uint8_t *__restrict__ ascii, uint64_t len, uint8_t *__restrict__ codes,
                              int *__restrict__ bad)
 {
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= len) return;
-    const uint32_t c = ascii[p] | 0x20u;
-    uint32_t v = 255;
+    const uint32_t raw = ascii[p], c = raw | 0x20u;
+    uint32_t v = 5;
     if (c == 'a') v = 0;
     else if (c == 'c') v = 1;
     else if (c == 'g') v = 2;
-    else if (c == 't') v = 3;
-    if (v == 255) atomicOr(bad, 1);
+    else if (c == 't' || c == 'u') v = 3;
+    else if (c == 'x') v = 4;
+    if (raw == '-' || raw == '.') atomicOr(bad, 1);
     codes[p] = (uint8_t)v;
 }
 
@@ -539,7 +542,7 @@ int ascii_to_codes(mh_context *ctx, const void *d_ascii, uint64_t len, uint8_t *
     int bad = 0;
     MH_HIP(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, ctx->stream));
     MH_HIP(hipStreamSynchronize(ctx->stream));
-    MH_CHECK(bad == 0, MH_ERR_UNSUPPORTED, "residue other than A,C,G,T in a DP operand");
+    MH_CHECK(bad == 0, MH_ERR_GAP_CHAR, "gap character in a DP operand");
     return MH_OK;
 }
 

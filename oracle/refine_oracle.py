@@ -71,8 +71,11 @@ def _mu_gap(c):
 
 
 def _letter(c):
-    i = "ACGT".find(c.upper())
-    return i if i >= 0 else 99
+    """Letter index of a residue for the column score: A,C,G,T (U as T, as
+    libMUSCLE scores it, tests/golden/refine_golden.txt), else 99 (no letter
+    score: wildcards)."""
+    i = "ACGTU".find(c.upper())
+    return min(i, 3) if i >= 0 else 99
 
 
 def column_scores(r1, r2):

@@ -289,7 +289,7 @@ def _refine_inputs():
         for b in (b1, b2):
             for i in rng.integers(0, len(b), max(1, len(b) // 30)):
                 if b[i] != ord("-"):
-                    b[i] = ord(rng.choice(list("NRYKMSWBDHVX")))
+                    b[i] = ord(rng.choice(list("NRYKMSWBDHVXUu")))
         if _app_safe(b1.decode(), b2.decode()):
             score_only.append((b1.decode(), b2.decode()))
     while len(rw) < 24:

@@ -47,10 +47,30 @@ def test_refine_single_interval_and_corners(ctx):
     assert got[1] == cases[1]                       # one genome only: left as it is
 
 
-def test_refine_rejects_other_residues(ctx):
-    with pytest.raises(mauvehip.MhError) as e:
-        mauvehip.refine_pairwise(ctx, [("ACGTNACGT" * 20, "ACGTAACGT" * 20)])
-    assert e.value.code == -6
+def _with_wildcards(rng, r, rate=0.01):
+    b = bytearray(r.encode())
+    for i in np.nonzero(rng.random(len(b)) < rate)[0]:
+        if b[i] != ord("-"):
+            b[i] = ord(rng.choice(list("NNNNXRYUunZ")))
+    return b.decode()
+
+
+@pytest.mark.parametrize("passes", [mauvehip.REFINE_PROFILE, mauvehip.REFINE_PROFILE | mauvehip.REFINE_WINDOWS])
+def test_refine_wildcards_match_oracle(ctx, passes):
+    """Rows with N, X, R, Y, U, lower case and letters outside the alphabet:
+    the anchored ranges and windows that hold them run NWSmall in float with
+    libMUSCLE's wildcard scores; rows equal the oracle's (itself pinned to
+    libMUSCLE on wildcard pairs and profile passes)."""
+    rng = np.random.default_rng(90 + passes)
+    ivs = []
+    for r1, r2 in _cases(91 + passes, 8):
+        ivs.append((_with_wildcards(rng, r1), _with_wildcards(rng, r2)))
+    ivs.append(("ACGTN" * 60, "ACGTA" * 60))
+    ivs.append(("NNNNNNNNNNACGTACGTACNNNN" * 10, "ACGTACGTACGTACGTACGTACGT" * 10))
+    got = [(a.decode(), b.decode()) for a, b in mauvehip.refine_pairwise(ctx, ivs, passes)]
+    for (r1, r2), g in zip(ivs, got):
+        want = ro.refine_alignment(r1, r2, True) if passes == mauvehip.REFINE_PROFILE else ro.gapped_alignment(r1, r2)
+        assert g == want
 
 
 def test_refine_after_dp_batches_and_growing_staging():
