@@ -166,6 +166,105 @@ __global__ void join_kernel(const uint64_t *__restrict__ ck0, uint64_t n0, const
     partner[i] = pj;
 }
 
+// ---- tiled join (every list-0 entry live) ---------------------------------
+// List 0 is cut into tiles of TJ_I entries.  A tile's masked mers lie in
+// [m(first), m(last)]; the list-1 entries with those mers, [jlo, jhi), are
+// found once per tile and staged in LDS with the tile, so each entry's search
+// in list 1 is a binary search in LDS instead of ~22 dependent global loads.
+// The tile is joined twice: once to count its seeds, once (after a scan of
+// the counts) to write them at their place in list-0 (HashMatch) order.
+constexpr int TJ_T = 256;                 // threads per tile
+constexpr int TJ_ITEMS = 8;               // list-0 entries per thread (consecutive)
+constexpr int TJ_I = TJ_T * TJ_ITEMS;     // list-0 entries per tile
+constexpr int TJ_CAP = 4096;              // list-1 entries a tile can stage
+constexpr uint64_t TJ_NONE = ~0ull;       // not a masked mer (those fit 62 bits)
+
+__global__ void tile_bounds_kernel(const uint64_t *__restrict__ ck0, uint64_t n0, const uint64_t *__restrict__ ck1,
+                                   uint64_t n1, uint64_t ntiles, uint64_t *__restrict__ jb)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const uint64_t i0 = t * TJ_I, i1 = min(n0, i0 + TJ_I);
+    jb[2 * t] = lower_bound_m(ck1, n1, ck0[i0] >> 1);
+    jb[2 * t + 1] = upper_bound_m(ck1, n1, ck0[i1 - 1] >> 1);
+}
+
+// write == 0: cnt[tile] = the tile's seed count; write == 1: the seeds
+// (list-0 index, list-1 index) at off[tile] onwards, in list-0 order
+__global__ __launch_bounds__(TJ_T) void tile_join_kernel(const uint64_t *__restrict__ ck0, uint64_t n0,
+                                                         const uint64_t *__restrict__ ck1, uint64_t n1,
+                                                         const uint64_t *__restrict__ jb, int write,
+                                                         uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
+                                                         uint32_t *__restrict__ out_i, uint32_t *__restrict__ out_j)
+{
+    __shared__ uint64_t s0[TJ_I + 2];
+    __shared__ uint64_t s1[TJ_CAP];
+    __shared__ uint32_t sc[TJ_T];
+    const int t = threadIdx.x;
+    const uint64_t tile = blockIdx.x;
+    const uint64_t i0 = tile * TJ_I, i1 = min(n0, i0 + TJ_I), ni = i1 - i0;
+    const uint64_t jlo = jb[2 * tile], jhi = jb[2 * tile + 1], nj = jhi - jlo;
+    const bool staged = nj <= (uint64_t)TJ_CAP;
+    // s0[k] = masked mer of list-0 entry i0 - 1 + k (TJ_NONE outside the list)
+    for (uint64_t k = t; k < ni + 2; k += TJ_T) {
+        const int64_t idx = (int64_t)i0 - 1 + (int64_t)k;
+        s0[k] = (idx >= 0 && (uint64_t)idx < n0) ? ck0[idx] >> 1 : TJ_NONE;
+    }
+    if (staged)
+        for (uint64_t k = t; k < nj; k += TJ_T) s1[k] = ck1[jlo + k] >> 1;
+    __syncthreads();
+    uint32_t part[TJ_ITEMS];
+    uint32_t c = 0;
+    // the thread's entries are consecutive, so one binary search finds the
+    // first one's lower bound in list 1 and the others advance from there
+    auto v1 = [&](uint64_t q) { return staged ? s1[q] : (ck1[jlo + q] >> 1); };
+    uint64_t lo = 0;
+    {
+        const uint64_t k0 = (uint64_t)t * TJ_ITEMS;
+        const uint64_t m0 = k0 < ni ? s0[k0 + 1] : TJ_NONE;
+        uint64_t len = nj;
+        while (len > 0) {
+            const uint64_t half = len >> 1, mid = lo + half;
+            if (v1(mid) < m0) { lo = mid + 1; len -= half + 1; }
+            else len = half;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < TJ_ITEMS; ++r) {
+        const uint64_t k = (uint64_t)t * TJ_ITEMS + r;
+        part[r] = ~0u;
+        if (k >= ni) continue;
+        const uint64_t m = s0[k + 1];
+        while (lo < nj && v1(lo) < m) ++lo;               // lower bound of m in list-1 entries [jlo, jhi)
+        if (s0[k] == m || s0[k + 2] == m) continue;      // not unique in list 0
+        if (lo >= nj || v1(lo) != m) continue;
+        if (lo + 1 < nj && v1(lo + 1) == m) continue;    // not unique in list 1
+        part[r] = (uint32_t)(jlo + lo);
+        ++c;
+    }
+    // exclusive scan of the per-thread counts (threads hold consecutive entries)
+    sc[t] = c;
+    __syncthreads();
+    for (int o = 1; o < TJ_T; o <<= 1) {
+        const uint32_t w = t >= o ? sc[t - o] : 0u;
+        __syncthreads();
+        sc[t] += w;
+        __syncthreads();
+    }
+    if (!write) {
+        if (t == TJ_T - 1) cnt[tile] = sc[t];
+        return;
+    }
+    uint32_t o = off[tile] + sc[t] - c;
+#pragma unroll
+    for (int r = 0; r < TJ_ITEMS; ++r)
+        if (part[r] != ~0u) {
+            out_i[o] = (uint32_t)(i0 + (uint64_t)t * TJ_ITEMS + r);
+            out_j[o] = part[r];
+            ++o;
+        }
+}
+
 __global__ __launch_bounds__(JT) void tile_count_kernel(const uint32_t *__restrict__ flag, uint64_t n,
                                                         uint32_t *__restrict__ tile_sum)
 {
@@ -633,44 +732,243 @@ __global__ void rc_pack_kernel(const uint32_t *__restrict__ w, uint64_t len, uin
     out[ow] = v;
 }
 
-__global__ void seed_kernel_solid(const uint32_t *__restrict__ pi, const uint32_t *__restrict__ pj, uint64_t ns,
-                                  const uint64_t *__restrict__ ck0, const uint32_t *__restrict__ pos0,
-                                  const uint64_t *__restrict__ ck1, const uint32_t *__restrict__ pos1,
-                                  const uint32_t *__restrict__ wa, const uint32_t *__restrict__ wb,
-                                  const uint32_t *__restrict__ wbrc, int64_t La, int64_t Lb, int k,
-                                  SeedRec *__restrict__ out)
+// ---- odd solid seeds by runs ------------------------------------------------
+// Two seeds one position apart on one diagonal (forward: (p0, p1) and (p0+1,
+// p1+1); reverse, in genome 1's forward coordinates: (p0, p1) and (p0+1,
+// p1-1)) are exact matches that overlap in k-1 characters, so they extend to
+// the same maximal exact match.  A run of such seeds is found through
+// genome-0 positions (each position holds at most one seed: its mer is
+// unique in list 0): only the run's first seed walks left and its last seed
+// walks right, and every seed of the run takes their ends.  The first and
+// last seed of the run at each position come from a segmented scan over the
+// positions (the nearest head at or before, the nearest tail at or after).
+
+constexpr uint32_t NO_SEED = 0xFFFFFFFFu;
+constexpr int RS_T = 256, RS_ITEMS = 16, RS_P = RS_T * RS_ITEMS;   // positions per scan block
+constexpr uint8_t RF_SEED = 4, RF_FOLLOW = 1, RF_NEXT = 2;       // position flags
+
+// seed order -> genome-0 position order: pp[p0] = (genome-1 forward position
+// with the strand relation in bit 31, seed index)
+__global__ void run_scatter_kernel(const uint32_t *__restrict__ pi, const uint32_t *__restrict__ pj, uint64_t ns,
+                                   const uint64_t *__restrict__ ck0, const uint32_t *__restrict__ pos0,
+                                   const uint64_t *__restrict__ ck1, const uint32_t *__restrict__ pos1,
+                                   uint2 *__restrict__ pp)
 {
     const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= ns) return;
     const uint32_t i = pi[x], j = pj[x];
-    const int64_t p0 = pos0[i];
-    const int64_t p1f = pos1[j];
+    const uint32_t p0 = pos0[i];
     const bool rev = (ck0[i] & 1ull) != (ck1[j] & 1ull);
-    SeedRec r;
-    r.s0 = (int32_t)(p0 + 1);
-    if (!rev) {
-        const int64_t left = match_left(wa, p0, wb, p1f, p0 < p1f ? p0 : p1f);
-        const int64_t ra = La - (p0 + k), rb = Lb - (p1f + k);
-        const int64_t right = match_right(wa, p0 + k, wb, p1f + k, ra < rb ? ra : rb);
-        r.s1 = (int32_t)(p1f + 1);
-        r.e0 = (int32_t)(p0 - left + 1);
-        r.e1 = (int32_t)(p1f - left + 1);
-        r.elen = (uint32_t)(k + left + right);
-    } else {
-        // genome 1 window in reverse-complement coordinates
-        const int64_t p1 = Lb - p1f - k;
-        const int64_t left = match_left(wa, p0, wbrc, p1, p0 < p1 ? p0 : p1);
-        const int64_t ra = La - (p0 + k), rb = Lb - (p1 + k);
-        const int64_t right = match_right(wa, p0 + k, wbrc, p1 + k, ra < rb ? ra : rb);
-        const int64_t L = k + left + right;
-        r.s1 = (int32_t)(-(p1f + 1));
-        r.e0 = (int32_t)(p0 - left + 1);
-        r.e1 = (int32_t)(-(Lb - (p1 - left + L) + 1));
-        r.elen = (uint32_t)L;
-    }
-    out[x] = r;
+    pp[p0] = make_uint2(pos1[j] | (rev ? 0x80000000u : 0u), (uint32_t)x);
 }
 
+// per position: a seed (RF_SEED), continued from the seed at p - 1
+// (RF_FOLLOW: not a head), continued into the seed at p + 1 (RF_NEXT: not a
+// tail); forward seeds continue at (p+1, p1+1), reverse ones at (p+1, p1-1)
+__global__ void run_flags_kernel(const uint2 *__restrict__ pp, uint64_t L0, uint8_t *__restrict__ flag)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= L0) return;
+    const uint32_t q = pp[p].x;
+    uint8_t f = 0;
+    if (q != NO_SEED) {
+        f = RF_SEED;
+        const uint32_t rev = q & 0x80000000u, p1 = q & 0x7FFFFFFFu;
+        if (p > 0) {
+            const uint32_t y = pp[p - 1].x;
+            if (y != NO_SEED && (y & 0x80000000u) == rev && (y & 0x7FFFFFFFu) == (rev ? p1 + 1 : p1 - 1))
+                f |= RF_FOLLOW;
+        }
+        if (p + 1 < L0) {
+            const uint32_t z = pp[p + 1].x;
+            if (z != NO_SEED && (z & 0x80000000u) == rev && (z & 0x7FFFFFFFu) == (rev ? p1 - 1 : p1 + 1))
+                f |= RF_NEXT;
+        }
+    }
+    flag[p] = f;
+}
+
+__device__ __forceinline__ bool rf_head(uint8_t f) { return (f & (RF_SEED | RF_FOLLOW)) == RF_SEED; }
+__device__ __forceinline__ bool rf_tail(uint8_t f) { return (f & (RF_SEED | RF_NEXT)) == RF_SEED; }
+
+// per block of RS_P positions: the last head and the first tail in it (-1 / L0)
+__global__ __launch_bounds__(RS_T) void run_block_kernel(const uint8_t *__restrict__ flag, uint64_t L0,
+                                                         int64_t *__restrict__ bhead, int64_t *__restrict__ btail)
+{
+    __shared__ int64_t h[RS_T], tl[RS_T];
+    const int t = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * RS_P + (uint64_t)t * RS_ITEMS;
+    int64_t lh = -1, ft = (int64_t)L0;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const uint64_t p = base + r;
+        if (p >= L0) break;
+        const uint8_t f = flag[p];
+        if (rf_head(f)) lh = (int64_t)p;
+        if (rf_tail(f) && ft == (int64_t)L0) ft = (int64_t)p;
+    }
+    h[t] = lh;
+    tl[t] = ft;
+    __syncthreads();
+    for (int o = RS_T / 2; o > 0; o >>= 1) {
+        if (t < o) {
+            h[t] = max(h[t], h[t + o]);
+            tl[t] = min(tl[t], tl[t + o]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        bhead[blockIdx.x] = h[0];
+        btail[blockIdx.x] = tl[0];
+    }
+}
+
+// carry-in of every block: the last head before it, the first tail after it
+__global__ __launch_bounds__(1024) void run_carry_kernel(const int64_t *__restrict__ bhead,
+                                                         const int64_t *__restrict__ btail, uint64_t nb, uint64_t L0,
+                                                         int64_t *__restrict__ chead, int64_t *__restrict__ ctail)
+{
+    __shared__ int64_t ph[1024], pt[1024];
+    const int t = threadIdx.x;
+    const uint64_t chunk = (nb + 1023) / 1024;
+    const uint64_t lo = min((uint64_t)t * chunk, nb), hi = min(lo + chunk, nb);
+    int64_t mh = -1, mt = (int64_t)L0;
+    for (uint64_t b = lo; b < hi; ++b) mh = max(mh, bhead[b]);
+    for (uint64_t b = lo; b < hi; ++b) mt = min(mt, btail[b]);
+    ph[t] = mh;
+    pt[t] = mt;
+    __syncthreads();
+    // inclusive max scan forward, inclusive min scan backward
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int64_t a = t >= o ? ph[t - o] : -1;
+        const int64_t b = t + o < 1024 ? pt[t + o] : (int64_t)L0;
+        __syncthreads();
+        ph[t] = max(ph[t], a);
+        pt[t] = min(pt[t], b);
+        __syncthreads();
+    }
+    int64_t run = t > 0 ? ph[t - 1] : -1;
+    for (uint64_t b = lo; b < hi; ++b) {
+        chead[b] = run;
+        run = max(run, bhead[b]);
+    }
+    int64_t rt = t + 1 < 1024 ? pt[t + 1] : (int64_t)L0;
+    for (uint64_t b = hi; b > lo; --b) {
+        ctail[b - 1] = rt;
+        rt = min(rt, btail[b - 1]);
+    }
+}
+
+// heads walk left, tails walk right (position order)
+__global__ void run_walk_kernel(const uint8_t *__restrict__ flag, const uint2 *__restrict__ pp, uint64_t L0,
+                                const uint32_t *__restrict__ wa, const uint32_t *__restrict__ wb,
+                                const uint32_t *__restrict__ wbrc, int64_t La, int64_t Lb, int k,
+                                uint32_t *__restrict__ lft, uint32_t *__restrict__ rgt)
+{
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= L0) return;
+    const uint8_t f = flag[p];
+    const bool hd = rf_head(f), tl = rf_tail(f);
+    if (!hd && !tl) return;
+    const int64_t p0 = (int64_t)p;
+    const uint32_t q = pp[p].x;
+    const bool rev = (q & 0x80000000u) != 0;
+    const int64_t p1f = q & 0x7FFFFFFFu;
+    const int64_t p1 = rev ? Lb - p1f - k : p1f;
+    const uint32_t *w1 = rev ? wbrc : wb;
+    if (hd) lft[p] = (uint32_t)match_left(wa, p0, w1, p1, p0 < p1 ? p0 : p1);
+    if (tl) {
+        const int64_t ra = La - (p0 + k), rb = Lb - (p1 + k);
+        rgt[p] = (uint32_t)match_right(wa, p0 + k, w1, p1 + k, ra < rb ? ra : rb);
+    }
+}
+
+// every seed's record from its run's head (left end) and tail (right end),
+// written at its seed index.  The scan runs over RS_ITEMS consecutive
+// positions per thread; each seed's head and tail go through LDS (padded one
+// word per 16) so that the reads of pp run over consecutive positions across
+// the wave.
+__device__ __forceinline__ int rs_lds(int idx) { return idx + (idx >> 4); }
+
+__global__ __launch_bounds__(RS_T) void run_final_kernel(const uint8_t *__restrict__ flag,
+                                                         const uint2 *__restrict__ pp, uint64_t L0,
+                                                         const int64_t *__restrict__ chead,
+                                                         const int64_t *__restrict__ ctail,
+                                                         const uint32_t *__restrict__ lft,
+                                                         const uint32_t *__restrict__ rgt, int64_t Lb, int k,
+                                                         SeedRec *__restrict__ out)
+{
+    __shared__ int64_t h[RS_T], tt[RS_T];
+    __shared__ uint32_t hs[RS_P + RS_P / 16], ts[RS_P + RS_P / 16];
+    const int t = threadIdx.x;
+    const uint64_t bbase = (uint64_t)blockIdx.x * RS_P;
+    const uint64_t base = bbase + (uint64_t)t * RS_ITEMS;
+    uint8_t fv[RS_ITEMS];
+    int64_t lh = -1, ft = (int64_t)L0;
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const uint64_t p = base + r;
+        fv[r] = p < L0 ? flag[p] : (uint8_t)0;
+        if (rf_head(fv[r])) lh = (int64_t)p;
+        if (rf_tail(fv[r]) && ft == (int64_t)L0) ft = (int64_t)p;
+    }
+    h[t] = lh;
+    tt[t] = ft;
+    __syncthreads();
+    for (int o = 1; o < RS_T; o <<= 1) {
+        const int64_t a = t >= o ? h[t - o] : -1;
+        const int64_t b = t + o < RS_T ? tt[t + o] : (int64_t)L0;
+        __syncthreads();
+        h[t] = max(h[t], a);
+        tt[t] = min(tt[t], b);
+        __syncthreads();
+    }
+    // tails among this thread's items, and the first tail after them
+    uint32_t tmask = 0;
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r)
+        if (rf_tail(fv[r])) tmask |= 1u << r;
+    const int64_t after = min(ctail[blockIdx.x], t + 1 < RS_T ? tt[t + 1] : (int64_t)L0);
+    int64_t cur = max(chead[blockIdx.x], t > 0 ? h[t - 1] : (int64_t)-1);
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        if (rf_head(fv[r])) cur = (int64_t)(base + r);
+        const uint32_t rest = tmask >> r;
+        const int64_t tail = rest ? (int64_t)(base + r) + __ffs(rest) - 1 : after;
+        // every run starts with a head and ends with a tail
+        const bool ok = (fv[r] & RF_SEED) && cur >= 0 && tail < (int64_t)L0;
+        const int li = rs_lds(t * RS_ITEMS + r);
+        hs[li] = ok ? (uint32_t)cur : NO_SEED;
+        ts[li] = (uint32_t)tail;
+    }
+    __syncthreads();
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int idx = r * RS_T + t;
+        const uint64_t pu = bbase + idx;
+        if (pu >= L0) break;
+        const uint32_t p0h = hs[rs_lds(idx)];
+        if (p0h == NO_SEED) continue;
+        const int64_t p0 = (int64_t)pu, p0t = ts[rs_lds(idx)];
+        const uint2 q = pp[pu];
+        const bool rev = (q.x & 0x80000000u) != 0;
+        const int64_t p1f = q.x & 0x7FFFFFFFu;
+        const int64_t lh0 = lft[p0h];
+        const int64_t left = lh0 + (p0 - (int64_t)p0h);      // this seed's own left extension
+        const int64_t L = (p0t + k + (int64_t)rgt[p0t]) - ((int64_t)p0h - lh0);
+        SeedRec rec;
+        rec.s0 = (int32_t)(p0 + 1);
+        rec.e0 = (int32_t)(p0 - left + 1);
+        rec.elen = (uint32_t)L;
+        if (!rev) {
+            rec.s1 = (int32_t)(p1f + 1);
+            rec.e1 = (int32_t)(p1f - left + 1);
+        } else {
+            const int64_t p1 = Lb - p1f - k;
+            rec.s1 = (int32_t)(-(p1f + 1));
+            rec.e1 = (int32_t)(-(Lb - (p1 - left + L) + 1));
+        }
+        out[q.y] = rec;
+    }
+}
 
 // ---- bucket partition of the seeds ----------------------------------------
 // AddHashEntry puts a seed in bucket ((offset mod 40000) + 40000) mod 40000
@@ -1252,7 +1550,23 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const Liv
         if ((cut_a && ca == 0) || (cut_b && cb == 0)) return MH_OK;
     }
     MH_HIP(hipMemsetAsync(cnt, 0, 16, ctx->stream));
-    {
+    const bool tiled = !cut_a && !cut_b;
+    const uint64_t ntj = (n0 + TJ_I - 1) / TJ_I;
+    if (tiled) {
+        // every entry live: the LDS-tiled join (count pass here, write pass below)
+        if ((rc = ctx->mf_scratch[2].ensure(ntj * 4 + 64)) != MH_OK) return rc;
+        if ((rc = ctx->mf_scratch[1].ensure(ntj * 16 + 64)) != MH_OK) return rc;
+        tiles = ctx->mf_scratch[2].as<uint32_t>();
+        TimedLaunch tl(ctx, "join");
+        uint64_t *jb = ctx->mf_scratch[1].as<uint64_t>();
+        hipLaunchKernelGGL(tile_bounds_kernel, dim3((unsigned)((ntj + 255) / 256)), dim3(256), 0, ctx->stream,
+                           a->d_keys, n0, b->d_keys, n1, ntj, jb);
+        hipLaunchKernelGGL(tile_join_kernel, dim3((unsigned)ntj), dim3(TJ_T), 0, ctx->stream, a->d_keys, n0,
+                           b->d_keys, n1, (const uint64_t *)jb, 0, tiles, (const uint32_t *)nullptr,
+                           (uint32_t *)nullptr, (uint32_t *)nullptr);
+        hipLaunchKernelGGL(excl_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, tiles, ntj, cnt + 1);
+        MH_HIP(hipGetLastError());
+    } else {
         TimedLaunch tl(ctx, "join");
         hipLaunchKernelGGL(join_kernel, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, ctx->stream, a->d_keys, n0,
                            b->d_keys, n1, lp, flag, partner);
@@ -1271,8 +1585,16 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const Liv
     uint32_t *oi = ctx->mf_scratch[4].as<uint32_t>();
     uint32_t *oj = ctx->mf_scratch[5].as<uint32_t>();
     SeedRec *d_seeds = ctx->mf_scratch[6].as<SeedRec>();
-    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)ntiles), dim3(JT), 0, ctx->stream, flag, partner, n0, tiles, oi,
-                       oj);
+    if (tiled) {
+        TimedLaunch tl(ctx, "join_write");
+        hipLaunchKernelGGL(tile_join_kernel, dim3((unsigned)ntj), dim3(TJ_T), 0, ctx->stream, a->d_keys, n0,
+                           b->d_keys, n1, (const uint64_t *)ctx->mf_scratch[1].as<uint64_t>(), 1,
+                           (uint32_t *)nullptr, (const uint32_t *)tiles, oi, oj);
+        MH_HIP(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(compact_kernel, dim3((unsigned)ntiles), dim3(JT), 0, ctx->stream, flag, partner, n0, tiles,
+                           oi, oj);
+    }
     ExtEnv env;
     env.kp0 = a->d_keys_pos;
     env.kp1 = b->d_keys_pos;
@@ -1286,12 +1608,31 @@ int join_and_extend(mh_context *ctx, const mh_sml *a, const mh_sml *b, const Liv
     if (solid_odd) {
         if ((rc = ctx->mf_scratch[7].ensure(b->n_words * 4 + 64)) != MH_OK) return rc;
         uint32_t *wbrc = ctx->mf_scratch[7].as<uint32_t>();
+        // run scratch, indexed by genome-0 position: partner with strand, left
+        // and right ends of runs, flags, extensions; block carries
+        const uint64_t L0 = a->len, nb = (L0 + RS_P - 1) / RS_P;
+        DevBuf *rs = ctx->run_scratch;
+        if ((rc = rs[0].ensure(L0 * 4 * 4 + 64)) != MH_OK) return rc;
+        if ((rc = rs[1].ensure(L0 + 64)) != MH_OK) return rc;
+        if ((rc = rs[2].ensure(nb * 8 * 4 + 64)) != MH_OK) return rc;
+        uint2 *pp = rs[0].as<uint2>();
+        uint32_t *lft = (uint32_t *)(pp + L0), *rgt = lft + L0;
+        uint8_t *fl = rs[1].as<uint8_t>();
+        int64_t *bhead = rs[2].as<int64_t>(), *btail = bhead + nb, *chead = btail + nb, *ctail = chead + nb;
+        const unsigned gs = (unsigned)((ns + 255) / 256), gp = (unsigned)((L0 + 255) / 256);
         TimedLaunch tl(ctx, "extend");
         hipLaunchKernelGGL(rc_pack_kernel, dim3((unsigned)((b->n_words + 255) / 256)), dim3(256), 0, ctx->stream,
                            b->d_words, b->len, b->n_words, wbrc);
-        hipLaunchKernelGGL(seed_kernel_solid, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, ctx->stream, oi, oj,
-                           ns, a->d_keys, a->d_pos, b->d_keys, b->d_pos, a->d_words, b->d_words, wbrc,
-                           (int64_t)a->len, (int64_t)b->len, a->seed_length, d_seeds);
+        MH_HIP(hipMemsetAsync(pp, 0xFF, L0 * 8, ctx->stream));
+        hipLaunchKernelGGL(run_scatter_kernel, dim3(gs), dim3(256), 0, ctx->stream, oi, oj, ns, a->d_keys, a->d_pos,
+                           b->d_keys, b->d_pos, pp);
+        hipLaunchKernelGGL(run_flags_kernel, dim3(gp), dim3(256), 0, ctx->stream, pp, L0, fl);
+        hipLaunchKernelGGL(run_block_kernel, dim3((unsigned)nb), dim3(RS_T), 0, ctx->stream, fl, L0, bhead, btail);
+        hipLaunchKernelGGL(run_carry_kernel, dim3(1), dim3(1024), 0, ctx->stream, bhead, btail, nb, L0, chead, ctail);
+        hipLaunchKernelGGL(run_walk_kernel, dim3(gp), dim3(256), 0, ctx->stream, fl, pp, L0, a->d_words, b->d_words,
+                           wbrc, (int64_t)a->len, (int64_t)b->len, a->seed_length, lft, rgt);
+        hipLaunchKernelGGL(run_final_kernel, dim3((unsigned)nb), dim3(RS_T), 0, ctx->stream, fl, pp, L0, chead,
+                           ctail, lft, rgt, (int64_t)b->len, a->seed_length, d_seeds);
         MH_HIP(hipGetLastError());
         ctx->last_walks = ns;
     } else if (a->len < (1ull << 30) && b->len < (1ull << 30)) {

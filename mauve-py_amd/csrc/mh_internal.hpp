@@ -110,6 +110,7 @@ struct mh_context {
     mh::DevBuf diag_scratch[8];    // long extensions resolved per diagonal
     mh::DevBuf multi_scratch[11];  // the seeds of every genome pair, merged and filtered
     mh::DevBuf heavy_buf, live_buf; // masked mers above the repeat limit; live ranges of a join
+    mh::DevBuf run_scratch[4];      // odd solid seeds extended by runs
     uint64_t last_walks = 0, last_long_walks = 0;   // of the last seed extension
     // contexts (own streams and scratch) that run independent searches
     // concurrently for this one, created on first use
