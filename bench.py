@@ -149,8 +149,8 @@ def main():
         w = mauvehip.get_seed_weight(seed)
         passes = (2 * w + 1 + 7) // 8
         kern = {}
-        for name in ("sml_pack", "sml_keys", "sml_sort", "join", "extend", "partition", "dp_forward", "dp_traceback",
-                     "host_wait_seeds", "host_partition", "host_replay", "host_list", "host_dp_plan", "host_dp_sort"):
+        for name in ("sml_pack", "sml_keys", "sml_sort", "join", "join_write", "extend", "partition", "dp_forward",
+                     "dp_traceback", "host_wait_seeds", "host_partition", "host_replay", "host_list", "host_dp_plan", "host_dp_sort"):
             ms, n = ctx.kernel_time(name)
             kern[name] = {"avg_ms": ms, "launches": n, "ms_per_step": ms * n / args.steps}
         nseeds = int(state["stats"].seed_tuples)
@@ -160,7 +160,8 @@ def main():
         lines = {
             "sml_sort": ("hbm", sort_bytes / (kern["sml_sort"]["avg_ms"] * 1e-3) / 1e9 if kern["sml_sort"]["avg_ms"] else 0,
                          HBM_PEAK_GBS, "GB/s", sort_bytes),
-            "extend": ("hbm", k3_bytes / ((kern["join"]["avg_ms"] + kern["extend"]["avg_ms"]) * 1e-3) / 1e9
+            "extend": ("hbm", k3_bytes / ((kern["join"]["avg_ms"] + kern["join_write"]["avg_ms"]
+                                           + kern["extend"]["avg_ms"]) * 1e-3) / 1e9
                        if kern["extend"]["avg_ms"] else 0, HBM_PEAK_GBS, "GB/s", k3_bytes),
             "dp_forward": ("valu", cells / (kern["dp_forward"]["ms_per_step"] * 1e-3) / 1e9
                            if kern["dp_forward"]["ms_per_step"] else 0, DP_PEAK_GCUPS, "GCUPS", cells),
@@ -184,6 +185,9 @@ def main():
                        "matches": int(state["stats"].matches), "chain_anchors": int(len(state["chain"])),
                        "dp_jobs": int(len(jobs)), "dp_cells": cells, "max_gap": MAX_GAP},
             "roofline": roofline,
+            "rooflines": {k: {"bound": v[0], "achieved": round(v[1], 2), "peak": round(v[2], 1), "unit": v[3],
+                              "frac": round(v[1] / v[2], 4) if v[2] else None, "work_per_launch": v[4]}
+                          for k, v in lines.items()},
             "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "launches_per_step": v["launches"] / args.steps}
                         for k, v in kern.items()},
             "stages_ms_per_step": {k: round(1e3 * v / args.steps, 3) for k, v in stage_s.items()},
@@ -220,6 +224,11 @@ def main():
                                         "frac": round(gk / DP_PEAK_GCUPS, 4)}}
         log(f"[bench] DP batch {len(al)} pairs {cells3 / 1e12:.2f} Tcells: {gc:.1f} GCUPS end-to-end, "
             f"{gk:.1f} GCUPS forward kernel")
+        if not args.no_cpu_baseline:
+            res3 = np.zeros(len(al), dtype=mauvehip.DP_RESULT_DTYPE)
+            ctx.d2h(res3, dR3)
+            out["dp_batch"]["cpu_baseline"] = cpu_dp_batch(dp, res3["score"])
+            log(f"[bench] CPU NWSmall {json.dumps(out['dp_batch']['cpu_baseline'])}")
         for p in (dA3, dB3, dP3, dR3):
             ctx.free(p)
 
@@ -235,6 +244,7 @@ def main():
     if not args.whole_only:
         out["core"] = {"value": out["value"], "ms_per_step": out["ms_per_step"],
                        "workload": out["config"]["workload"], "roofline": out["roofline"],
+                       "rooflines": out.pop("rooflines"),
                        "stages_ms_per_step": out.pop("stages_ms_per_step"), "dp_jobs": out["config"]["dp_jobs"],
                        "dp_cells": out["config"]["dp_cells"]}
     out["value"] = full["value"]
@@ -250,7 +260,7 @@ def main():
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_full(mut, ref, seed, full)
-        log(f"[bench] CPU oracle (whole path, sampled) {out['cpu_baseline']['seconds']:.2f}s per step")
+        log(f"[bench] CPU oracle (whole path, one thread) {out['cpu_baseline']['seconds']:.2f}s per step")
     if not args.no_cpu_baseline and not args.whole_only:
         from oracle import pyoracle as po
         t0 = time.perf_counter()
@@ -363,8 +373,8 @@ def full_leg(ctx, g, r, seed, steps, warmup):
     el = (time.perf_counter() - t0) / steps
     cells = int(rs.profile_cells + rs.window_cells)
     kern = {}
-    for name in ("sml_pack", "sml_keys", "sml_sort", "join", "extend", "partition", "dp_forward", "dp_traceback",
-                 "host_replay", "host_dp_plan", "host_refine_prefix", "host_refine_pieces", "host_refine_plan",
+    for name in ("sml_pack", "sml_keys", "sml_sort", "join", "join_write", "extend", "partition", "dp_forward",
+                 "dp_traceback", "host_replay", "host_dp_plan", "host_refine_prefix", "host_refine_pieces", "host_refine_plan",
                  "host_refine_dp", "host_refine_splice"):
         ms, n = ctx.kernel_time(name)
         kern[name] = {"avg_ms": round(ms, 4), "launches_per_step": n / steps, "ms_per_step": round(ms * n / steps, 3)}
@@ -380,7 +390,7 @@ def full_leg(ctx, g, r, seed, steps, warmup):
          "stages_ms_per_step": {k: round(1e3 * v / steps, 3) for k, v in stages.items()},
          "config": {**info, "profile_dp_cells": int(rs.profile_cells), "window_dp_cells": int(rs.window_cells),
                     "lut_mapped": round(float((lut >= 0).mean()), 4), "xmfa_bytes": os.path.getsize(xmfa)},
-         "rows": (ra, rb)}
+         "rows": (ra, rb), "lut": lut}
     for p in d:
         ctx.free(p)
     os.remove(xmfa)
@@ -388,19 +398,17 @@ def full_leg(ctx, g, r, seed, steps, warmup):
     return t
 
 
-CPU_SAMPLE_COLS = 150_000
-
-
 def cpu_full(g, r, seed, full):
-    """The whole path on this host's CPU, one core, with the oracle's
-    restatement of every stage.  Sorted mer lists and matching, the chain and
-    the recursive anchor search run whole; the refinement passes, the XMFA
-    text and the LUT run on the first CPU_SAMPLE_COLS columns of the chain's
-    rows and are scaled by the column count."""
+    """The whole path on this host's CPU, one thread, every stage over the
+    whole pair (no sampling, no scaling): the oracle's sorted mer lists and
+    match finding, the library's host chain (the stand-in for the LCB pass,
+    host code in both paths), the oracle's recursive anchor search, then the
+    compiled restatement of the profile and refinement passes, the XMFA file
+    and the buildIndex LUT (oracle/cpu_path.cpp, -O3).  The rows and the LUT
+    are compared with the GPU path's."""
+    import tempfile
     import mauvehip
-    from oracle import index_oracle as io
     from oracle import pyoracle as po
-    from oracle import refine_oracle as ro
     g, r = np.asarray(g).tobytes(), np.asarray(r).tobytes()
     t = {}
     t0 = time.perf_counter()
@@ -414,31 +422,91 @@ def cpu_full(g, r, seed, full):
     ra, rb = mauvehip.chain_rows(m, chain, g, r)
     t["chain"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    A, B = ra.decode(), rb.decode()
-    for lend, rend in po.recursion_gaps(A, B, 1, 1, 200):
-        po.anchor_gap(g, r, lend, rend)
+    extra = []
+    for lend, rend in po.c_recursion_gaps(ra, rb, 1, 1, 200):
+        gs, gl = po.anchor_gap(g, r, lend, rend)
+        if len(gl):
+            e = np.zeros(len(gl), dtype=mauvehip.MATCH_DTYPE)
+            e["start"][:, :2] = gs
+            e["length"] = gl
+            extra.append(e)
+    if extra:
+        m = np.concatenate([m] + extra)
+        chain, _ = mauvehip.chain_gap_jobs_raw(m, len(g), len(r), 0)
+        ra, rb = mauvehip.chain_rows(m, chain, g, r)
     t["recurse"] = time.perf_counter() - t0
-    n = len(A)
-    s = min(n, CPU_SAMPLE_COLS)
-    scale = n / s
     t0 = time.perf_counter()
-    a2, b2 = ro.gapped_alignment(A[:s], B[:s])
-    t["refine"] = (time.perf_counter() - t0) * scale
+    a2, b2, cells = po.c_refine_gapped(ra, rb)
+    t["refine"] = time.perf_counter() - t0
+    tmpd = tempfile.mkdtemp(prefix="mauvehip_cpu_")
+    xmfa = os.path.join(tmpd, "cpu.xmfa")
     t0 = time.perf_counter()
-    body = lambda x: "\n".join(x[k:k + 80] for k in range(0, len(x), 80))
-    text = f"#FormatVersion Mauve1\n> 1:1-{s} + g\n{body(a2)}\n> 2:1-{s} + r\n{body(b2)}\n=\n"
-    t["xmfa"] = (time.perf_counter() - t0) * scale
-    t0 = time.perf_counter()
-    lut = io.lut_from_groups(io.parse_xmfa(text), s)
-    io.fill_gaps(lut, 300)
-    io.smooth_edges(lut, 20)
-    t["lut"] = (time.perf_counter() - t0) * scale
+    lut, nbytes = po.c_xmfa_lut(xmfa, ("genome1.fas", "genome2.fas"), a2, b2, len(g), len(r), g, r)
+    t["xmfa_lut"] = time.perf_counter() - t0
+    os.remove(xmfa)
+    os.rmdir(tmpd)
     total = sum(t.values())
+    gr = full.get("rows")
+    glut = full.get("lut")
     return {"value": round((len(g) + len(r)) / 1e6 / total, 4), "unit": "Mb/s", "cores": 1, "kind": "port",
             "seconds": round(total, 2), "stages_seconds": {k: round(v, 3) for k, v in t.items()},
-            "sample": f"oracle restatement of every stage on one core: sorted mer lists + matching, chain and "
-                      f"recursive anchor search over the whole pair; refinement passes, XMFA text and LUT on "
-                      f"the first {s} of {n} alignment columns, scaled by {scale:.1f}"}
+            "dp_cells": int(cells), "xmfa_bytes": int(nbytes),
+            "rows_identical": bool(gr is not None and bytes(gr[0]) == a2.encode() and bytes(gr[1]) == b2.encode()),
+            "lut_identical": bool(glut is not None and np.array_equal(np.asarray(glut), lut)),
+            "sample": "the whole step on one thread, every stage over the whole pair: oracle sorted mer lists + "
+                      "matching, host chain, oracle recursive anchor search, compiled restatement of the "
+                      "profile and refinement passes, XMFA file and LUT over every column"}
+
+
+CPU_DP_SECONDS = 12.0
+
+
+def cpu_dp_batch(dp, gpu_scores):
+    """The oracle's NWSmall (-O3, one job per call of or_nwsmall) on this
+    host's CPU over a sample of the batch: jobs in a seeded random order,
+    taken until about CPU_DP_SECONDS of work, on one thread and then on up to
+    16 threads (ctypes releases the GIL; one job per task, largest first).
+    The sample's scores are compared with the GPU's."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import pyoracle as po
+    A, ao, al, B, bo, bl = dp
+    order = np.random.default_rng(5).permutation(len(al))
+    pa, pl = np.ascontiguousarray(ao[order], dtype=np.uint64), np.ascontiguousarray(al[order], dtype=np.uint32)
+    pb, ql = np.ascontiguousarray(bo[order], dtype=np.uint64), np.ascontiguousarray(bl[order], dtype=np.uint32)
+    cum = np.cumsum(pl.astype(np.int64) * ql)
+    A = np.ascontiguousarray(A, dtype=np.uint8)
+    B = np.ascontiguousarray(B, dtype=np.uint8)
+    scores = np.zeros(len(al), dtype=np.float32)
+
+    def run(lo, hi):
+        return po.c_nwsmall_batch(A, B, pa, pl, pb, ql, lo, hi, scores)
+
+    # rate probe on the first jobs, then one thread for about CPU_DP_SECONDS
+    t0 = time.perf_counter()
+    n0 = int(np.searchsorted(cum, 2e8)) + 1
+    run(0, n0)
+    rate = cum[n0 - 1] / max(time.perf_counter() - t0, 1e-6)
+    n1 = int(min(len(al), np.searchsorted(cum, rate * CPU_DP_SECONDS) + 1))
+    t0 = time.perf_counter()
+    c1 = run(0, n1)
+    s1 = time.perf_counter() - t0
+    threads = max(1, min(16, os.cpu_count() or 1))
+    nt = int(min(len(al), np.searchsorted(cum, c1 / s1 * threads * CPU_DP_SECONDS) + 1))
+    big_first = np.argsort(-(pl[:nt].astype(np.int64) * ql[:nt]), kind="stable")
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        ct = sum(ex.map(lambda k: run(int(k), int(k) + 1), big_first))
+    st = time.perf_counter() - t0
+    m = max(n1, nt)
+    same = bool(np.array_equal(np.rint(scores[:m]).astype(np.int64), np.asarray(gpu_scores)[order[:m]]))
+    return {"value": round(c1 / s1 / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
+            "seconds": round(s1, 2), "cells": int(c1), "jobs": int(n1),
+            "threads": {"value": round(ct / st / 1e9, 4), "cores": threads, "seconds": round(st, 2),
+                        "cells": int(ct), "jobs": int(nt)},
+            "scores_identical": same,
+            "sample": f"oracle NWSmall (compiled, -O3) over the first jobs of a seeded random order of the "
+                      f"{len(al)}-pair batch: {n1} jobs on one thread, {nt} jobs on {threads} threads; "
+                      f"their scores compared with the GPU's"}
 
 
 def learned_band_leg(ctx, bandset):

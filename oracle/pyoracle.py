@@ -83,6 +83,25 @@ def lib():
         L.or_path_dev.argtypes = [C.c_uint32, C.c_uint32, C.c_char_p, C.c_int64]
         L.or_path_score.restype = C.c_float
         L.or_path_score.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_char_p, C.c_int64]
+        L.or_refine_gapped.restype = C.c_int64
+        L.or_refine_gapped.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                       C.POINTER(C.c_uint64)]
+        L.or_refine_pass.restype = C.c_int64
+        L.or_refine_pass.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_int64]
+        L.or_column_scores.restype = C.c_int64
+        L.or_column_scores.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_void_p]
+        L.or_anchor_cols.restype = C.c_int64
+        L.or_anchor_cols.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_void_p, C.c_int64]
+        L.or_xmfa_lut.restype = C.c_int64
+        L.or_xmfa_lut.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int64,
+                                  C.c_int64, C.c_int64, C.c_char_p, C.c_char_p, C.c_void_p, C.c_int64]
+        L.or_recursion_gaps.restype = C.c_int64
+        L.or_recursion_gaps.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                        C.c_void_p, C.c_int64]
+        L.or_nwsmall_batch.restype = C.c_int64
+        L.or_nwsmall_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_int64, C.c_int64, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -387,3 +406,72 @@ def rows_from_path(a, b, path):
         else:
             ra.append("-"); rb.append(b[j]); j += 1
     return "".join(ra), "".join(rb)
+
+
+# ---- compiled host stages of the whole path (oracle/cpu_path.cpp) ----
+
+def c_refine_gapped(r1, r2):
+    """Profile pass then refinement pass of two aligned rows, compiled
+    (refine_oracle.gapped_alignment restated in C++); returns (row1, row2,
+    DP cells)."""
+    r1, r2 = _b(r1), _b(r2)
+    cap = 2 * len(r1) + 64
+    o1, o2 = C.create_string_buffer(cap), C.create_string_buffer(cap)
+    cells = C.c_uint64()
+    n = lib().or_refine_gapped(r1, r2, len(r1), o1, o2, cap, C.byref(cells))
+    if n < 0:
+        raise ValueError(f"or_refine_gapped: {n}")
+    return o1.raw[:n].decode(), o2.raw[:n].decode(), cells.value
+
+
+def c_refine_pass(r1, r2, profile):
+    r1, r2 = _b(r1), _b(r2)
+    cap = 2 * len(r1) + 64
+    o1, o2 = C.create_string_buffer(cap), C.create_string_buffer(cap)
+    n = lib().or_refine_pass(r1, r2, len(r1), 1 if profile else 0, o1, o2, cap)
+    if n < 0:
+        raise ValueError(f"or_refine_pass: {n}")
+    return o1.raw[:n].decode(), o2.raw[:n].decode()
+
+
+def c_column_scores(r1, r2):
+    out = np.zeros(len(r1), dtype=np.float32)
+    lib().or_column_scores(_b(r1), _b(r2), len(r1), out.ctypes.data)
+    return out
+
+
+def c_anchor_cols(r1, r2):
+    out = np.zeros(len(r1) + 1, dtype=np.int64)
+    n = lib().or_anchor_cols(_b(r1), _b(r2), len(r1), out.ctypes.data, len(out))
+    return [int(x) for x in out[:n]]
+
+
+def c_xmfa_lut(path, names, r1, r2, len1, len2, seq1, seq2):
+    """XMFA of one interval written to path, read back, LUT over genome 1
+    with the three post-passes; returns (lut int32, xmfa bytes)."""
+    lut = np.zeros(len1, dtype=np.int32)
+    r1, r2 = _b(r1), _b(r2)
+    n = lib().or_xmfa_lut(_b(path), _b(names[0]), _b(names[1]), r1, r2, len(r1), len1, len2, _b(seq1), _b(seq2),
+                          lut.ctypes.data, len1)
+    if n < 0:
+        raise ValueError(f"or_xmfa_lut: {n}")
+    return lut, n
+
+
+def c_nwsmall_batch(A, B, a_off, a_len, b_off, b_len, lo, hi, scores=None):
+    """NWSmall over jobs [lo, hi) of a batch on one thread; returns the cells."""
+    n = lib().or_nwsmall_batch(A.ctypes.data, B.ctypes.data, a_off.ctypes.data, a_len.ctypes.data,
+                               b_off.ctypes.data, b_len.ctypes.data, lo, hi,
+                               scores.ctypes.data if scores is not None else None)
+    if n < 0:
+        raise ValueError("NWSmall: length=0")
+    return n
+
+
+def c_recursion_gaps(row_a, row_b, left_a=1, left_b=1, min_gap=200):
+    """recursion_gaps compiled: the same list of ((lend_a, lend_b), (rend_a, rend_b))."""
+    ra, rb = _b(row_a), _b(row_b)
+    n = lib().or_recursion_gaps(ra, rb, len(ra), left_a, left_b, min_gap, None, 0)
+    out = np.zeros((max(n, 1), 4), dtype=np.int64)
+    lib().or_recursion_gaps(ra, rb, len(ra), left_a, left_b, min_gap, out.ctypes.data, n)
+    return [((int(o[0]), int(o[1])), (int(o[2]), int(o[3]))) for o in out[:n]]
