@@ -37,10 +37,13 @@ DP_OPS_PER_CELL = 10                         # cost model of a DP cell, int32 VA
 DP_PEAK_GCUPS = VALU_LANE_OPS / DP_OPS_PER_CELL / 1e9
 MAX_GAP = 5000                               # max_gap_length, LM/ProgressiveAligner.cpp:62
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
-# --pmc WRITE_SIZE on this workload's sorted-mer-list sort (hist, digit scan
-# and scatter over 5 passes of 4,599,984 keys), scaled by the counters' reading
-# on a plain 1 GiB copy kernel: FETCH_SIZE x2, WRITE_SIZE x1
-MEASURED_TRAFFIC = {"sml_sort": 894084512}
+# --pmc WRITE_SIZE on the recoded 4.6 Mb pair, scaled by the counters' reading
+# on a plain 1 GiB copy kernel (FETCH_SIZE x2, WRITE_SIZE x1): the
+# sorted-mer-list sort (hist, digit scan and scatter over 5 passes of
+# 4,599,984 keys), the join + solid-seed extension of the pair, and the
+# whole path's NWSmall forward kernel (mean of the profile-pass and the
+# refinement-pass launches)
+MEASURED_TRAFFIC = {"sml_sort": 894084512, "extend": 782747872, "dp_forward_whole": 453269683}
 
 
 def log(*a):
@@ -186,7 +189,8 @@ def main():
                        "dp_jobs": int(len(jobs)), "dp_cells": cells, "max_gap": MAX_GAP},
             "roofline": roofline,
             "rooflines": {k: {"bound": v[0], "achieved": round(v[1], 2), "peak": round(v[2], 1), "unit": v[3],
-                              "frac": round(v[1] / v[2], 4) if v[2] else None, "work_per_launch": v[4]}
+                              "frac": round(v[1] / v[2], 4) if v[2] else None, "work_per_launch": v[4],
+                              "traffic": MEASURED_TRAFFIC.get(k) if (La, Lb) == (4600000, 4600339) else None}
                           for k, v in lines.items()},
             "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "launches_per_step": v["launches"] / args.steps}
                         for k, v in kern.items()},
@@ -381,7 +385,9 @@ def full_leg(ctx, g, r, seed, steps, warmup):
     fwd = kern["dp_forward"]["ms_per_step"]
     gk = cells / (fwd * 1e-3) / 1e9 if fwd else 0.0
     roofline = {"kernel": "dp_forward", "bound": "valu", "achieved": round(gk, 2), "peak": round(DP_PEAK_GCUPS, 1),
-                "unit": "GCUPS", "frac": round(gk / DP_PEAK_GCUPS, 4), "traffic": None, "work_per_launch": cells}
+                "unit": "GCUPS", "frac": round(gk / DP_PEAK_GCUPS, 4),
+                "traffic": MEASURED_TRAFFIC["dp_forward_whole"] if (len(g), len(r)) == (4600339, 4600000) else None,
+                "work_per_launch": cells // max(1, int(kern["dp_forward"]["launches_per_step"]))}
     value = (len(g) + len(r)) / 1e6 / el
     log(f"[bench] whole path {value:.2f} Mb/s, {1e3 * el:.1f} ms/step; stages "
         f"{ {k: round(1e3 * v / steps, 2) for k, v in stages.items()} }")
