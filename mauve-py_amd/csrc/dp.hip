@@ -28,6 +28,7 @@
 // per step; the word of cell (i, j) is at
 //   stripe*(LA+63)*64 + (i-1 + lane)*64 + lane,  lane = ((j-1) mod 512) / 8.
 #include "mh_internal.hpp"
+#include "dp_tune_table.h"
 #include <algorithm>
 #include <numeric>
 #include <cstdlib>
@@ -375,53 +376,48 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
 constexpr uint32_t GROUP_TAG = 32;
 constexpr uint32_t ROWS_MAX_LB = 256;   // widest job run four to a wavefront
 
-// Every wavefront-stripe job and every row-packed job of a batch in one launch
-// (one launch per shape would pay a job's duration of drain per shape).  Waves
-// [0, n_stripe) take stripe job w (64 lanes); after them, row shape q (16 - 2q
-// columns a lane, widest first) holds jobs [start[q], start[q+1]) in waves
-// n_stripe + [wave[q], wave[q+1]).
+// ---- one round of jobs: a length-binned work queue ----------------------
+//
+// A round's jobs sit in one table in queue order: the 64-lane stripe jobs
+// (those with long trace-back walks first, each part largest first), the
+// row-packed jobs by shape (16, 14, ..., 4 columns a lane), the sliding-window
+// jobs, the one-lane jobs.  The queue cuts that table into bins: stripe jobs
+// by trace size (a new bin where it falls below the next power of 4), one bin
+// per row shape, one of window jobs, one of one-lane jobs.  A bin hands out
+// items (1 stripe or window job, 4 row-packed jobs, 64 one-lane jobs) through
+// its own atomic cursor.  One launch drains the bins in order: a wavefront
+// takes the next item of the first bin that is not empty (a persistent
+// wavefront keeps taking items until the last bin is empty).  Everything a
+// launch needs is in the round's descriptor in device memory, so the
+// launches of a round take one pointer and are captured in a graph once per
+// context; a round only sets the grid sizes of the graph's nodes.
 constexpr int ROW_SHAPES = 7;
-struct RowShapes {
-    uint32_t start[ROW_SHAPES + 1];
-    uint32_t wave[ROW_SHAPES + 1];
+enum : uint32_t { QK_STRIPE = 0, QK_ROWS = 1, QK_WINDOW = QK_ROWS + ROW_SHAPES, QK_TINY = QK_WINDOW + 1 };
+constexpr int DPQ_MAXB = 48;
+constexpr int DPQ_CURSORS = DPQ_MAXB;
+
+struct DpRound {
+    const DevJob *jobs;
+    DpOut *outs;
+    const uint8_t *A, *B;
+    uint8_t *tb;
+    int *bnd;
+    uint8_t *paths;
+    mh_dp_result *res;
+    uint32_t *dev;
+    uint32_t *cursor;          // DPQ_CURSORS counters, zeroed at the start of every round
+    uint32_t sr[4];
+    int32_t goh;
+    uint32_t nbins;
+    uint32_t nwave;            // jobs [0, nwave) walk back on a wavefront each
+    uint32_t n;                // jobs in the round
+    uint32_t kind[DPQ_MAXB], first[DPQ_MAXB], count[DPQ_MAXB];
+    uint32_t item0[DPQ_MAXB + 1];   // the bins' first items in queue order (prefix sums)
 };
 
-// At most 168 VGPRs (3 waves a SIMD; the 16-column shape alone would take 182
-// and 2 waves): measured 3% faster on the whole-path batches.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void dp_forward_all(const DevJob *__restrict__ jobs, uint32_t n_stripe, RowShapes sh,
-                                                          const uint8_t *__restrict__ A,
-                                                          const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
-                                                          int *__restrict__ bnd, DpOut *__restrict__ outs,
-                                                          uint32_t sr0, uint32_t sr1, uint32_t sr2, uint32_t sr3,
-                                                          int goh)
+__device__ __forceinline__ uint32_t qk_per_item(uint32_t kind)
 {
-    __shared__ int lds_in[4][64];
-    __shared__ int lds_out[3][64];
-    if (blockIdx.x < n_stripe) {
-        const DevJob jb = jobs[blockIdx.x];
-        DpOut *o = &outs[blockIdx.x];
-        if (jb.cols == 16)
-            dp_job<16, 64>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
-        else if (jb.cols == 8)
-            dp_job<8, 64>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
-        else
-            dp_job<4, 64>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
-        return;
-    }
-    const uint32_t w = blockIdx.x - n_stripe;
-    int q = 0;
-    while (q < ROW_SHAPES - 1 && w >= sh.wave[q + 1]) ++q;
-    const uint32_t x = sh.start[q] + (w - sh.wave[q]) * 4 + (threadIdx.x >> 4);
-    if (x >= sh.start[q + 1]) return;
-    switch (q) {
-    case 0: dp_job<16, 16>(jobs[x], A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
-    case 1: dp_job<14, 16>(jobs[x], A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
-    case 2: dp_job<12, 16>(jobs[x], A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
-    case 3: dp_job<10, 16>(jobs[x], A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
-    case 4: dp_job<8, 16>(jobs[x], A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
-    case 5: dp_job<6, 16>(jobs[x], A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
-    default: dp_job<4, 16>(jobs[x], A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
-    }
+    return kind == QK_STRIPE || kind == QK_WINDOW ? 1u : (kind == QK_TINY ? 64u : 4u);
 }
 
 // ---- sliding-window banded DP ---------------------------------------------
@@ -663,20 +659,6 @@ __device__ __forceinline__ void dp_window_job(const DevJob &jb, const uint8_t *_
     }
 }
 
-__global__ __launch_bounds__(64) void dp_window(const DevJob *__restrict__ jobs, const uint8_t *__restrict__ A,
-                                                const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
-                                                DpOut *__restrict__ outs, uint32_t sr0, uint32_t sr1, uint32_t sr2,
-                                                uint32_t sr3, int goh)
-{
-    const DevJob jb = jobs[blockIdx.x];
-    switch (jb.cols - WINDOW_TAG) {
-    case 1: dp_window_job<1>(jb, A, B, tb, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh); break;
-    case 2: dp_window_job<2>(jb, A, B, tb, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh); break;
-    case 4: dp_window_job<4>(jb, A, B, tb, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh); break;
-    default: dp_window_job<8>(jb, A, B, tb, &outs[blockIdx.x], sr0, sr1, sr2, sr3, goh); break;
-    }
-}
-
 // Narrow jobs (lb <= 16): one lane per job.  The lane keeps a whole row of
 // M, D' and I' in registers and walks the rows of A, with the same two
 // register passes per row as dp_job; columns past lb are computed and
@@ -686,14 +668,11 @@ __global__ __launch_bounds__(64) void dp_window(const DevJob *__restrict__ jobs,
 constexpr int TINY_LB = 16;
 constexpr uint32_t TINY_COLS_TAG = 1;   // DevJob::cols of a one-lane job
 
-__global__ __launch_bounds__(64) void dp_tiny(const DevJob *__restrict__ jobs, int n, const uint8_t *__restrict__ A,
-                                              const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
-                                              DpOut *__restrict__ outs, uint32_t sr0, uint32_t sr1, uint32_t sr2,
-                                              uint32_t sr3, int goh)
+__device__ __forceinline__ void tiny_job(const DevJob &jb, const uint8_t *__restrict__ A,
+                                         const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
+                                         DpOut *__restrict__ out, uint32_t sr0, uint32_t sr1, uint32_t sr2,
+                                         uint32_t sr3, int goh)
 {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    if (x >= n) return;
-    const DevJob jb = jobs[x];
     const int la = (int)jb.la, lb = (int)jb.lb;
     const uint8_t *a = A + jb.a_off;
     const uint8_t *b = B + jb.b_off;
@@ -763,8 +742,8 @@ __global__ __launch_bounds__(64) void dp_tiny(const DevJob *__restrict__ jobs, i
     int score = finM, edge = 'M';
     if (finD > score) { score = finD; edge = 'D'; }
     if (finI > score) { score = finI; edge = 'I'; }
-    outs[x].score = score;
-    outs[x].edge = edge;
+    out->score = score;
+    out->edge = edge;
 }
 
 __device__ __forceinline__ int xchar(uint32_t bits, int t)
@@ -779,12 +758,12 @@ __device__ __forceinline__ int xchar(uint32_t bits, int t)
 // path slot.  It also records the path's largest distance from the
 // corner-to-corner line, ceil(max |i*LB - j*LA| / LA) over the visited
 // positions, which is the band label and the edge test of a banded job.
-__global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__restrict__ outs,
-                             const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
-                             mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev, int njobs)
+__device__ __forceinline__ void traceback_thread_job(const DevJob *__restrict__ jobs,
+                                                     const DpOut *__restrict__ outs,
+                                                     const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
+                                                     mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev,
+                                                     uint32_t x)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= njobs) return;
     const DevJob jb = jobs[x];
     const int la = (int)jb.la, lb = (int)jb.lb;
     // stripe layouts: G lanes per job (16 for the row-packed jobs) of cols columns
@@ -867,14 +846,13 @@ __global__ void dp_traceback(const DevJob *__restrict__ jobs, const DpOut *__res
 constexpr int TBC_STEPS = 128, TBC_SEG = 64;
 constexpr uint32_t WAVE_WALK_MIN = 1024;   // a + b above which a job's walk gets a wavefront
 
-__global__ __launch_bounds__(64) void dp_traceback_wave(const DevJob *__restrict__ jobs,
-                                                        const DpOut *__restrict__ outs,
-                                                        const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
-                                                        mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev)
+__device__ __forceinline__ void traceback_wave_job(const DevJob *__restrict__ jobs, const DpOut *__restrict__ outs,
+                                                   const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
+                                                   mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev,
+                                                   uint32_t x, uint8_t *__restrict__ cache)
 {
-    __shared__ __align__(16) uint8_t cache[TBC_STEPS * TBC_SEG];
     const int lane = threadIdx.x;
-    const DevJob jb = jobs[blockIdx.x];
+    const DevJob jb = jobs[x];
     const int la = (int)jb.la, lb = (int)jb.lb;
     const int cols = (int)jb.cols, sw = 64 * cols, lane_bytes = cols / 2;
     const int K = TBC_SEG / lane_bytes;      // lanes per segment: 128 columns
@@ -897,7 +875,7 @@ __global__ __launch_bounds__(64) void dp_traceback_wave(const DevJob *__restrict
     int64_t dnum = 0;
     int pos = la + lb;
     int i = la, j = lb;
-    int type = outs[blockIdx.x].edge;
+    int type = outs[x].edge;
     bool ok = true;
     int64_t maxdev = 0;
     for (;;) {
@@ -949,12 +927,139 @@ __global__ __launch_bounds__(64) void dp_traceback_wave(const DevJob *__restrict
     }
     if (lane == 0) {
         mh_dp_result r;
-        r.score = outs[blockIdx.x].score;
+        r.score = outs[x].score;
         r.path_len = ok ? (uint32_t)(la + lb - pos) : 0u;
         r.path_start = jb.path_off + (uint64_t)pos;
         res[jb.user] = r;
         if (dev) dev[jb.user] = ok ? (uint32_t)((maxdev + la - 1) / la) : 0xFFFFFFFFu;
     }
+}
+
+// ---- the round's launches ------------------------------------------------
+
+__global__ __launch_bounds__(64) void dp_queue_reset(uint32_t *__restrict__ cursor)
+{
+    for (int k = threadIdx.x; k < DPQ_CURSORS; k += 64) cursor[k] = 0u;
+}
+
+// The forward kernels' arguments: every pointer restrict-qualified (so that
+// job records and the bin table are read with scalar loads), the bin table in
+// the round's descriptor.
+#define DP_FWD_PARAMS                                                                                            \
+    const DevJob *__restrict__ jobs, DpOut *__restrict__ outs, const uint8_t *__restrict__ A,                      \
+        const uint8_t *__restrict__ B, uint8_t *__restrict__ tb, int *__restrict__ bnd,                          \
+        const DpRound *__restrict__ rd, uint32_t *__restrict__ cursor, uint32_t sr0, uint32_t sr1, uint32_t sr2, \
+        uint32_t sr3, int goh
+#define DP_FWD_ARGS jobs, outs, A, B, tb, bnd, rd, cursor, sr0, sr1, sr2, sr3, goh
+
+template <int MAXC>
+__device__ __forceinline__ void run_item(DP_FWD_PARAMS, uint32_t kind, uint32_t first, uint32_t count,
+                                         uint32_t item, int (*lds_in)[64], int (*lds_out)[64])
+{
+    if (kind == QK_STRIPE) {
+        const uint32_t x = first + item;
+        const DevJob jb = jobs[x];
+        if constexpr (MAXC >= 16) {
+            if (jb.cols == 16) {
+                dp_job<16, 64>(jb, A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+                return;
+            }
+        }
+        if (jb.cols == 8)
+            dp_job<8, 64>(jb, A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+        else
+            dp_job<4, 64>(jb, A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out);
+    } else if (kind == QK_WINDOW) {
+        const uint32_t x = first + item;
+        const DevJob jb = jobs[x];
+        switch (jb.cols - WINDOW_TAG) {
+        case 1: dp_window_job<1>(jb, A, B, tb, &outs[x], sr0, sr1, sr2, sr3, goh); break;
+        case 2: dp_window_job<2>(jb, A, B, tb, &outs[x], sr0, sr1, sr2, sr3, goh); break;
+        case 4: dp_window_job<4>(jb, A, B, tb, &outs[x], sr0, sr1, sr2, sr3, goh); break;
+        default: dp_window_job<8>(jb, A, B, tb, &outs[x], sr0, sr1, sr2, sr3, goh); break;
+        }
+    } else if (kind == QK_TINY) {
+        const uint32_t x = first + item * 64 + threadIdx.x;
+        if (x < first + count) tiny_job(jobs[x], A, B, tb, &outs[x], sr0, sr1, sr2, sr3, goh);
+    } else {
+        // four jobs of one shape, one per 16-lane row; a row past the bin's end sits out
+        const uint32_t x = first + item * 4 + (threadIdx.x >> 4);
+        if (x >= first + count) return;
+        const DevJob jb = jobs[x];
+        DpOut *o = &outs[x];
+        switch (kind - QK_ROWS) {
+        case 0: if constexpr (MAXC >= 16) dp_job<16, 16>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
+        case 1: if constexpr (MAXC >= 14) dp_job<14, 16>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
+        case 2: if constexpr (MAXC >= 12) dp_job<12, 16>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
+        case 3: if constexpr (MAXC >= 10) dp_job<10, 16>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
+        case 4: dp_job<8, 16>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
+        case 5: dp_job<6, 16>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
+        default: dp_job<4, 16>(jb, A, B, tb, bnd, o, sr0, sr1, sr2, sr3, goh, lds_in, lds_out); break;
+        }
+    }
+}
+
+// The forward launch.  MAXC is the widest shape it holds (16: every shape;
+// 8: stripes and row shapes of at most 8 columns a lane, which need fewer
+// registers and so fit more wavefronts a SIMD).  PERSIST: a grid of resident
+// wavefronts, each taking the next item from the bins' atomic cursors until
+// the last bin is empty.  Otherwise one workgroup per item in queue order
+// (workgroup w takes item w; workgroups start in that order), which needs no
+// cursor.  The host picks the variant a round's shapes allow, by the tuning
+// table.
+template <int MAXC, int WPE, bool PERSIST>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void dp_fwd(DP_FWD_PARAMS)
+{
+    __shared__ int lds_in[4][64];
+    __shared__ int lds_out[3][64];
+    const uint32_t nb = rd->nbins;
+    if constexpr (!PERSIST) {
+        const uint32_t w = blockIdx.x;
+        uint32_t b = 0;
+        while (b + 1 < nb && w >= rd->item0[b + 1]) ++b;
+        if (b >= nb || w >= rd->item0[nb]) return;
+        run_item<MAXC>(DP_FWD_ARGS, rd->kind[b], rd->first[b], rd->count[b], w - rd->item0[b], lds_in, lds_out);
+    } else {
+        uint32_t b = 0;
+        for (;;) {
+            // the next item: the first bin whose cursor is still inside it
+            uint32_t item = 0;
+            for (; b < nb; ++b) {
+                uint32_t v = 0;
+                if (threadIdx.x == 0) v = atomicAdd(&cursor[b], 1u);
+                item = (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+                if ((uint64_t)item * qk_per_item(rd->kind[b]) < rd->count[b]) break;
+            }
+            if (b >= nb) return;
+            run_item<MAXC>(DP_FWD_ARGS, rd->kind[b], rd->first[b], rd->count[b], item, lds_in, lds_out);
+        }
+    }
+}
+
+// Trace-backs: the jobs with long walks a wavefront each (workgroup x walks
+// job x), the others a thread each.  Their pointers come as arguments (not
+// from the round's descriptor): with every pointer restrict-qualified the
+// walk's job record is read with scalar loads and its bookkeeping stays on the
+// scalar unit.
+__global__ __launch_bounds__(64) void dp_tb_wave(const DevJob *__restrict__ jobs, const DpOut *__restrict__ outs,
+                                                 const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
+                                                 mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev,
+                                                 uint32_t nwave)
+{
+    __shared__ __align__(16) uint8_t cache[TBC_STEPS * TBC_SEG];
+    const uint32_t x = blockIdx.x;
+    if (x >= nwave) return;
+    traceback_wave_job(jobs, outs, tb, paths, res, dev, x, cache);
+}
+
+__global__ __launch_bounds__(64) void dp_tb_thread(const DevJob *__restrict__ jobs, const DpOut *__restrict__ outs,
+                                                   const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
+                                                   mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev,
+                                                   uint32_t first, uint32_t n)
+{
+    const uint32_t x = first + blockIdx.x * 64 + threadIdx.x;
+    if (x >= n) return;
+    traceback_thread_job(jobs, outs, tb, paths, res, dev, x);
 }
 
 // NUC_SP (MU/nucmx.cpp:8-25) rows packed as biased bytes: row a, column b at
@@ -1029,14 +1134,51 @@ int dp_force_cols()
     return 0;
 }
 
-// Stripe shape of a job: the full DP uses 512-column stripes for narrow jobs
-// and 1024 for wide ones; a banded job takes the shape with the lowest
-// estimated cost, ceil(lb/w) stripes of min(la, la*(w+2W)/lb + 1) + 63 steps
-// of w cells.  force (4, 8 or 16) overrides.
+// class of B's width in the tuning table
+static int dp_tune_class(uint32_t lb)
+{
+    int c = 0;
+    while (c < DP_TUNE_NCLASS - 1 && lb > DP_TUNE_LB_CLASS[c]) ++c;
+    return c;
+}
+
+// The full DP's shape per class: the tuning table, or MH_DP_SHAPES (a comma
+// list of DP_TUNE_NCLASS values, each 0, 4, 8 or 16; read once per process,
+// for the tuner's sweeps)
+static const uint8_t *shape_table()
+{
+    static uint8_t t[DP_TUNE_NCLASS];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (int c = 0; c < DP_TUNE_NCLASS; ++c) t[c] = DP_TUNE_SHAPE[c];
+        const char *e = getenv("MH_DP_SHAPES");
+        if (!e) return;
+        uint8_t v[DP_TUNE_NCLASS];
+        int c = 0;
+        for (const char *p = e; *p && c < DP_TUNE_NCLASS;) {
+            char *end = nullptr;
+            const long x = strtol(p, &end, 10);
+            if (end == p || !(x == 0 || x == 4 || x == 8 || x == 16)) return;
+            v[c++] = (uint8_t)x;
+            p = *end == ',' ? end + 1 : end;
+        }
+        if (c == DP_TUNE_NCLASS) std::memcpy(t, v, sizeof(v));
+    });
+    return t;
+}
+
+// Stripe shape of a job: the full DP takes the shape the tuning table names
+// for B's width (stripes of 8 columns a lane where it names the row-packed
+// shape and the job cannot take it); a banded job takes the shape with the
+// lowest estimated cost, ceil(lb/w) stripes of min(la, la*(w+2W)/lb + 1) + 63
+// steps of w cells.  force (4, 8 or 16) overrides.
 uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force)
 {
     if (force) return (uint32_t)force;
-    if (band == 0) return lb > 512 ? 16u : 8u;
+    if (band == 0) {
+        const uint32_t t = shape_table()[dp_tune_class(lb)];
+        return t ? t : 8u;
+    }
     uint32_t best = 16;
     double best_cost = 0;
     for (int q = 0; q < 3; ++q) {
@@ -1095,6 +1237,257 @@ uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t col
 
 static inline uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
 
+// device bytes of a round's descriptor and its queue cursors
+constexpr uint64_t DP_ROUND_BYTES = (sizeof(DpRound) + DPQ_CURSORS * 4 + 255) & ~255ull;
+
+// The forward kernel's variants: the widest shape each holds, its
+// wavefronts a SIMD, and whether its wavefronts persist.  Rounds with a shape
+// wider than 8 columns a lane run the tuning table's wide variant, the others
+// its narrow one; MH_DP_VARIANT forces one (for sweeps), where the round's
+// shapes allow it.
+struct FwdVariant {
+    const void *fn;
+    int maxc, wpe;
+    bool persist;
+};
+const FwdVariant FWD_VARIANTS[] = {
+    {reinterpret_cast<const void *>(&dp_fwd<16, 3, false>), 16, 3, false},
+    {reinterpret_cast<const void *>(&dp_fwd<8, 4, false>), 8, 4, false},
+    {reinterpret_cast<const void *>(&dp_fwd<16, 2, false>), 16, 2, false},
+    {reinterpret_cast<const void *>(&dp_fwd<16, 3, true>), 16, 3, true},
+    {reinterpret_cast<const void *>(&dp_fwd<16, 2, true>), 16, 2, true},
+    {reinterpret_cast<const void *>(&dp_fwd<8, 4, true>), 8, 4, true},
+};
+constexpr int N_FWD_VARIANTS = (int)(sizeof(FWD_VARIANTS) / sizeof(FWD_VARIANTS[0]));
+static_assert(N_FWD_VARIANTS <= MH_DP_MAX_VARIANTS, "context graph slots");
+static_assert(DP_TUNE_WIDE_VARIANT < N_FWD_VARIANTS && DP_TUNE_NARROW_VARIANT < N_FWD_VARIANTS, "tuning table");
+
+static int dp_variant_for(bool wide)
+{
+    for (const char *name : {"MH_DP_VARIANT", wide ? "MH_DP_VARIANT_WIDE" : "MH_DP_VARIANT_NARROW"})
+        if (const char *e = getenv(name)) {
+            const int v = atoi(e);
+            if (v >= 0 && v < N_FWD_VARIANTS && (!wide || FWD_VARIANTS[v].maxc >= 16)) return v;
+        }
+    return wide ? DP_TUNE_WIDE_VARIANT : DP_TUNE_NARROW_VARIANT;
+}
+
+// Graph replay of the round's launches (MH_DP_GRAPH=0 launches them directly)
+static bool dp_use_graph()
+{
+    const char *e = getenv("MH_DP_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+static int dp_round_staging(mh_context *ctx)
+{
+    if (ctx->dp_round_host) return MH_OK;
+    if (hipHostMalloc(&ctx->dp_round_host, sizeof(DpRound), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->dp_round_host = nullptr;
+        set_error("pinned host allocation failed");
+        return MH_ERR_NOMEM;
+    }
+    return MH_OK;
+}
+
+// resident workgroups of a 64-thread kernel over the whole device
+static int resident_grid(mh_context *ctx, const void *fn, unsigned *grid)
+{
+    int cus = 0, per = 0;
+    MH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    MH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64, 0));
+    *grid = (unsigned)std::max(1, cus * std::max(1, per));
+    return MH_OK;
+}
+
+// grid sizes of a round: forward (items, or the resident grid of a
+// persistent variant), wavefront trace-backs, thread trace-backs
+struct RoundGrids {
+    unsigned fwd, tbw, tbt;
+};
+
+static int round_grids(mh_context *ctx, int v, uint64_t items, uint32_t nwave, uint32_t n, RoundGrids *g)
+{
+    if (FWD_VARIANTS[v].persist) {
+        if (!ctx->dp_fwd_grid[v]) {
+            unsigned r = 0;
+            int rc;
+            if ((rc = resident_grid(ctx, FWD_VARIANTS[v].fn, &r)) != MH_OK) return rc;
+            ctx->dp_fwd_grid[v] = r;
+        }
+        g->fwd = ctx->dp_fwd_grid[v];
+    } else {
+        MH_CHECK(items < (1ull << 31), MH_ERR_INVALID, "too many DP work items in one round");
+        g->fwd = (unsigned)std::max<uint64_t>(1, items);
+    }
+    g->tbw = std::max(1u, nwave);
+    g->tbt = std::max(1u, (n - nwave + 63) / 64);
+    return MH_OK;
+}
+
+// the forward kernels' arguments of a round (DP_FWD_PARAMS)
+struct FwdArgs {
+    const DevJob *jobs;
+    DpOut *outs;
+    const uint8_t *A, *B;
+    uint8_t *tb;
+    int *bnd;
+    const DpRound *rd;
+    uint32_t *cursor;
+    uint32_t sr0, sr1, sr2, sr3;
+    int goh;
+    void *args[13];
+    void set()
+    {
+        void *a[13] = {&jobs, &outs, &A, &B, &tb, &bnd, &rd, &cursor, &sr0, &sr1, &sr2, &sr3, &goh};
+        std::memcpy(args, a, sizeof(a));
+    }
+};
+
+static int launch_fwd(mh_context *ctx, FwdArgs &a, int v, unsigned grid)
+{
+    a.set();
+    hipLaunchKernelGGL(dp_queue_reset, dim3(1), dim3(64), 0, ctx->stream, a.cursor);
+    MH_HIP(hipLaunchKernel(FWD_VARIANTS[v].fn, dim3(grid), dim3(64), a.args, 0, ctx->stream));
+    MH_HIP(hipGetLastError());
+    return MH_OK;
+}
+
+// the trace-back kernels' arguments of a round
+struct TbArgs {
+    const DevJob *jobs;
+    const DpOut *outs;
+    const uint8_t *tb;
+    uint8_t *paths;
+    mh_dp_result *res;
+    uint32_t *dev;
+    uint32_t nwave, first, n;
+    void *wave[7];
+    void *thread[8];
+    void set()
+    {
+        void *w[7] = {&jobs, &outs, &tb, &paths, &res, &dev, &nwave};
+        void *t[8] = {&jobs, &outs, &tb, &paths, &res, &dev, &first, &n};
+        std::memcpy(wave, w, sizeof(w));
+        std::memcpy(thread, t, sizeof(t));
+    }
+};
+
+static int launch_tb(mh_context *ctx, TbArgs &a, const RoundGrids &g)
+{
+    a.set();
+    MH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&dp_tb_wave), dim3(g.tbw), dim3(64), a.wave, 0,
+                           ctx->stream));
+    MH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(&dp_tb_thread), dim3(g.tbt), dim3(64), a.thread, 0,
+                           ctx->stream));
+    MH_HIP(hipGetLastError());
+    return MH_OK;
+}
+
+// Captures f's launches on the context stream: the graph (kept, for its
+// nodes), its executable form, and the kernel nodes of the given functions.
+template <class F>
+static int capture(mh_context *ctx, hipGraph_t *graph, hipGraphExec_t *exec, const void *const *fns,
+                   hipGraphNode_t *nodes, int nfns, F f)
+{
+    MH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = f();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != MH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    MH_HIP(e);
+    size_t nn = 0;
+    MH_HIP(hipGraphGetNodes(g, nullptr, &nn));
+    std::vector<hipGraphNode_t> all(nn);
+    MH_HIP(hipGraphGetNodes(g, all.data(), &nn));
+    for (int k = 0; k < nfns; ++k) nodes[k] = nullptr;
+    for (hipGraphNode_t nd : all) {
+        hipGraphNodeType t;
+        MH_HIP(hipGraphNodeGetType(nd, &t));
+        if (t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams kp;
+        MH_HIP(hipGraphKernelNodeGetParams(nd, &kp));
+        for (int k = 0; k < nfns; ++k)
+            if (kp.func == fns[k]) nodes[k] = nd;
+    }
+    for (int k = 0; k < nfns; ++k)
+        if (!nodes[k]) {
+            (void)hipGraphDestroy(g);
+            MH_CHECK(false, MH_ERR_DEVICE, "a captured DP launch has no kernel node");
+        }
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        MH_HIP(ei);
+    }
+    *graph = g;
+    return MH_OK;
+}
+
+// sets a captured kernel node's grid and arguments in the executable graph
+static int set_node(hipGraphExec_t exec, hipGraphNode_t node, unsigned grid, void **args)
+{
+    hipKernelNodeParams kp;
+    MH_HIP(hipGraphKernelNodeGetParams(node, &kp));
+    kp.gridDim = dim3(grid);
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    MH_HIP(hipGraphExecKernelNodeSetParams(exec, node, &kp));
+    return MH_OK;
+}
+
+// The round's forward launch (queue reset + forward kernel) and its
+// trace-backs, timed apart; from graphs captured once per context (the
+// descriptor's address is fixed; a round sets the nodes' grids) or launched
+// directly.
+static int dp_launch_round(mh_context *ctx, const DpRound *d_round, int v, bool graph, const RoundGrids &g,
+                           FwdArgs &fwa, TbArgs &tba)
+{
+    int rc;
+    if (graph && d_round != ctx->dp_round.as<DpRound>()) graph = false;
+    if (graph && !ctx->dp_graph_fwd[v]) {
+        const void *fns[1] = {FWD_VARIANTS[v].fn};
+        if ((rc = capture(ctx, &ctx->dp_graph_fwd_src[v], &ctx->dp_graph_fwd[v], fns, &ctx->dp_graph_fwd_node[v], 1,
+                          [&] { return launch_fwd(ctx, fwa, v, g.fwd); })) != MH_OK)
+            return rc;
+    }
+    if (graph && !ctx->dp_graph_tb) {
+        const void *fns[2] = {reinterpret_cast<const void *>(&dp_tb_wave),
+                              reinterpret_cast<const void *>(&dp_tb_thread)};
+        if ((rc = capture(ctx, &ctx->dp_graph_tb_src, &ctx->dp_graph_tb, fns, ctx->dp_graph_tb_node, 2,
+                          [&] { return launch_tb(ctx, tba, g); })) != MH_OK)
+            return rc;
+    }
+    if (graph) {
+        fwa.set();
+        tba.set();
+        if ((rc = set_node(ctx->dp_graph_fwd[v], ctx->dp_graph_fwd_node[v], g.fwd, fwa.args)) != MH_OK) return rc;
+        if ((rc = set_node(ctx->dp_graph_tb, ctx->dp_graph_tb_node[0], g.tbw, tba.wave)) != MH_OK) return rc;
+        if ((rc = set_node(ctx->dp_graph_tb, ctx->dp_graph_tb_node[1], g.tbt, tba.thread)) != MH_OK) return rc;
+    }
+    {
+        TimedLaunch tl(ctx, "dp_forward");
+        if (graph) {
+            MH_HIP(hipGraphLaunch(ctx->dp_graph_fwd[v], ctx->stream));
+        } else if ((rc = launch_fwd(ctx, fwa, v, g.fwd)) != MH_OK) {
+            return rc;
+        }
+    }
+    {
+        TimedLaunch tl(ctx, "dp_traceback");
+        if (graph) {
+            MH_HIP(hipGraphLaunch(ctx->dp_graph_tb, ctx->stream));
+        } else if ((rc = launch_tb(ctx, tba, g)) != MH_OK) {
+            return rc;
+        }
+    }
+    return MH_OK;
+}
+
 static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const mh_dp_job *jobs,
                         const std::vector<uint64_t> &slot, std::vector<DpItem> &items, uint8_t *d_paths,
                         mh_dp_result *d_results, uint32_t *d_dev, uint64_t *cells,
@@ -1134,7 +1527,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
     // footprint in a fixed scratch of a round of `cnt` jobs with trace bytes
     // tbb (each rounded up to 16) and bi boundary ints
     auto fixed_need = [&](uint64_t cnt, uint64_t tbb, uint64_t bi) {
-        return align256(cnt * (sizeof(DevJob) + sizeof(DpOut))) + align256(bi * 4) + tbb + 256;
+        return DP_ROUND_BYTES + align256(cnt * (sizeof(DevJob) + sizeof(DpOut))) + align256(bi * 4) + tbb + 256;
     };
     if (fx) {
         for (const DpItem &it : items)
@@ -1254,9 +1647,12 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         DpOut *d_outs;
         uint8_t *d_tb;
         int *d_bnd;
+        DpRound *d_round;
         if (fx) {
-            // [jobs | results] [boundary ints] [trace]
+            // [round descriptor + cursors] [jobs | results] [boundary ints] [trace]
             uint8_t *q = fx->ptr;
+            d_round = reinterpret_cast<DpRound *>(q);
+            q += DP_ROUND_BYTES;
             d_jobs = reinterpret_cast<DevJob *>(q);
             d_outs = reinterpret_cast<DpOut *>(d_jobs + n);
             q += align256(n * (sizeof(DevJob) + sizeof(DpOut)));
@@ -1269,49 +1665,108 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             if ((rc = ctx->dp_tb.ensure(tw + 256)) != MH_OK) return rc;
             if ((rc = ctx->dp_bnd.ensure(bo * 4 + 256)) != MH_OK) return rc;
             if ((rc = ctx->dp_jobs.ensure(n * sizeof(DevJob) + n * sizeof(DpOut) + 256)) != MH_OK) return rc;
+            if ((rc = ctx->dp_round.ensure(DP_ROUND_BYTES)) != MH_OK) return rc;
             d_jobs = ctx->dp_jobs.as<DevJob>();
             d_outs = reinterpret_cast<DpOut *>(d_jobs + n);
             d_tb = ctx->dp_tb.as<uint8_t>();
             d_bnd = ctx->dp_bnd.as<int>();
+            d_round = ctx->dp_round.as<DpRound>();
         }
         MH_HIP(hipMemcpyAsync(d_jobs, dj, n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
+        // the round's descriptor: pointers, the trace-back split and the queue's bins
+        DpRound hr;
+        std::memset(&hr, 0, sizeof(hr));
+        hr.jobs = d_jobs;
+        hr.outs = d_outs;
+        hr.A = d_a;
+        hr.B = d_b;
+        hr.tb = d_tb;
+        hr.bnd = d_bnd;
+        hr.paths = d_paths;
+        hr.res = d_results;
+        hr.dev = d_dev;
+        hr.cursor = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(d_round) + sizeof(DpRound));
+        for (int k = 0; k < 4; ++k) hr.sr[k] = sr[k];
+        hr.goh = -200;
+        hr.nwave = (uint32_t)nwave;
+        hr.n = (uint32_t)n;
+        bool wide = false;
+        auto add_bin = [&](uint32_t kind, uint64_t b0, uint64_t cnt) {
+            if (cnt == 0) return;
+            hr.kind[hr.nbins] = kind;
+            hr.first[hr.nbins] = (uint32_t)b0;
+            hr.count[hr.nbins] = (uint32_t)cnt;
+            ++hr.nbins;
+        };
         {
-            TimedLaunch tl(ctx, "dp_forward");
-            RowShapes sh;
-            uint32_t waves = 0;
-            for (int q = 0; q < ROW_SHAPES; ++q) {
-                sh.start[q] = (uint32_t)rows_at[q + 1];
-                sh.wave[q] = waves;
-                waves += (uint32_t)((rows_at[q + 2] - rows_at[q + 1] + 3) / 4);
+            // stripe jobs: a new bin where the trace size falls below the next power of 4
+            auto cls = [](uint64_t v) { return v ? (63 - __builtin_clzll(v)) / 2 : 0; };
+            uint64_t x = 0;
+            while (x < n64) {
+                const int c0 = cls(items[first + x].tb_bytes);
+                uint64_t y = x + 1;
+                while (y < n64 && cls(items[first + y].tb_bytes) == c0) ++y;
+                // keep ROW_SHAPES + 2 bins for the row, window and one-lane jobs
+                if (hr.nbins + 1 >= (uint32_t)(DPQ_MAXB - ROW_SHAPES - 2)) y = n64;
+                add_bin(QK_STRIPE, x, y - x);
+                x = y;
             }
-            sh.start[ROW_SHAPES] = (uint32_t)rows_at[ROW_SHAPES + 1];
-            sh.wave[ROW_SHAPES] = waves;
-            if (n64 + waves)
-                hipLaunchKernelGGL(dp_forward_all, dim3((unsigned)(n64 + waves)), dim3(64), 0, ctx->stream, d_jobs,
-                                   (uint32_t)n64, sh, d_a, d_b, d_tb, d_bnd, d_outs,
-                                   sr[0], sr[1], sr[2], sr[3], -200);
-            if (nbig > nstripe)
-                hipLaunchKernelGGL(dp_window, dim3((unsigned)(nbig - nstripe)), dim3(64), 0, ctx->stream,
-                                   d_jobs + nstripe, d_a, d_b, d_tb, d_outs + nstripe, sr[0],
-                                   sr[1], sr[2], sr[3], -200);
-            if (n > nbig)
-                hipLaunchKernelGGL(dp_tiny, dim3((unsigned)((n - nbig + 63) / 64)), dim3(64), 0, ctx->stream,
-                                   d_jobs + nbig, (int)(n - nbig), d_a, d_b, d_tb,
-                                   d_outs + nbig, sr[0], sr[1], sr[2], sr[3], -200);
-            MH_HIP(hipGetLastError());
+            for (uint64_t z = 0; z < n64; ++z)
+                if (items[first + z].cols == 16) wide = true;
+            for (int q = 0; q < ROW_SHAPES; ++q) {
+                add_bin(QK_ROWS + q, rows_at[q + 1], rows_at[q + 2] - rows_at[q + 1]);
+                if (q < 4 && rows_at[q + 2] > rows_at[q + 1]) wide = true;
+            }
+            add_bin(QK_WINDOW, nstripe, nbig - nstripe);
+            add_bin(QK_TINY, nbig, n - nbig);
         }
+        const int variant = dp_variant_for(wide);
+        uint64_t nitems_q = 0;
+        for (uint32_t q = 0; q < hr.nbins; ++q) {
+            const uint32_t per = hr.kind[q] == QK_STRIPE || hr.kind[q] == QK_WINDOW ? 1u : (hr.kind[q] == QK_TINY ? 64u : 4u);
+            hr.item0[q] = (uint32_t)nitems_q;
+            nitems_q += (hr.count[q] + per - 1) / per;
+        }
+        hr.item0[hr.nbins] = (uint32_t)nitems_q;
+        RoundGrids grids;
         {
-            // stripe-layout jobs with long walks: one wavefront each; the rest:
-            // one thread each
-            TimedLaunch tl(ctx, "dp_traceback");
-            if (nwave)
-                hipLaunchKernelGGL(dp_traceback_wave, dim3((unsigned)nwave), dim3(64), 0, ctx->stream, d_jobs,
-                                   d_outs, d_tb, d_paths, d_results, d_dev);
-            if (n > nwave)
-                hipLaunchKernelGGL(dp_traceback, dim3((unsigned)((n - nwave + 63) / 64)), dim3(64), 0, ctx->stream,
-                                   d_jobs + nwave, d_outs + nwave, d_tb, d_paths, d_results,
-                                   d_dev, (int)(n - nwave));
-            MH_HIP(hipGetLastError());
+            int rc;
+            if ((rc = round_grids(ctx, variant, nitems_q, hr.nwave, hr.n, &grids)) != MH_OK) return rc;
+        }
+        FwdArgs fwa;
+        fwa.jobs = d_jobs;
+        fwa.outs = d_outs;
+        fwa.A = d_a;
+        fwa.B = d_b;
+        fwa.tb = d_tb;
+        fwa.bnd = d_bnd;
+        fwa.rd = d_round;
+        fwa.cursor = hr.cursor;
+        fwa.sr0 = sr[0];
+        fwa.sr1 = sr[1];
+        fwa.sr2 = sr[2];
+        fwa.sr3 = sr[3];
+        fwa.goh = -200;
+        TbArgs tba;
+        tba.jobs = d_jobs;
+        tba.outs = d_outs;
+        tba.tb = d_tb;
+        tba.paths = d_paths;
+        tba.res = d_results;
+        tba.dev = d_dev;
+        tba.nwave = hr.nwave;
+        tba.first = hr.nwave;
+        tba.n = hr.n;
+        if (fx) {
+            MH_HIP(hipMemcpyAsync(d_round, &hr, sizeof(hr), hipMemcpyHostToDevice, ctx->stream));
+            int rc;
+            if ((rc = dp_launch_round(ctx, d_round, variant, false, grids, fwa, tba)) != MH_OK) return rc;
+        } else {
+            int rc;
+            if ((rc = dp_round_staging(ctx)) != MH_OK) return rc;
+            std::memcpy(ctx->dp_round_host, &hr, sizeof(hr));
+            MH_HIP(hipMemcpyAsync(d_round, ctx->dp_round_host, sizeof(hr), hipMemcpyHostToDevice, ctx->stream));
+            if ((rc = dp_launch_round(ctx, d_round, variant, dp_use_graph(), grids, fwa, tba)) != MH_OK) return rc;
         }
         // the host job table is reused by the next round
         MH_HIP(hipStreamSynchronize(ctx->stream));
@@ -1372,7 +1827,8 @@ static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band, int fo
         it.tb_bytes = (uint64_t)jb.a_len * 8;
         return it;
     }
-    if (it.band == 0 && !force && jb.b_len <= ROWS_MAX_LB && jb.a_len + jb.b_len < WAVE_WALK_MIN) {
+    if (it.band == 0 && !force && jb.b_len <= ROWS_MAX_LB && jb.a_len + jb.b_len < WAVE_WALK_MIN &&
+        shape_table()[dp_tune_class(jb.b_len)] == 0) {
         // four jobs per wavefront, one stripe each of the fewest columns a lane
         // (an even number from 4 to 16) that holds the job
         const uint32_t c = std::max(4u, 2u * ((jb.b_len + 31) / 32));

@@ -93,6 +93,8 @@ struct KernelTimer {
 
 }  // namespace mh
 
+#define MH_DP_MAX_VARIANTS 8   // forward kernel variants of the DP (dp.hip)
+
 struct mh_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -103,6 +105,15 @@ struct mh_context {
     // DP scratch
     mh::DevBuf dp_tb, dp_bnd, dp_jobs, dp_a, dp_b, dp_paths, dp_res, dp_dev;
     mh::DevBuf dp_wild, dp_fl;   // operand scan; the float DP's job table and rows
+    // DP rounds: the descriptor and queue cursors (fixed address), its pinned
+    // host copy, the launches captured as graphs per forward variant, and the
+    // persistent grids
+    mh::DevBuf dp_round;
+    void *dp_round_host = nullptr;
+    hipGraph_t dp_graph_fwd_src[MH_DP_MAX_VARIANTS] = {}, dp_graph_tb_src = nullptr;
+    hipGraphExec_t dp_graph_fwd[MH_DP_MAX_VARIANTS] = {}, dp_graph_tb = nullptr;
+    hipGraphNode_t dp_graph_fwd_node[MH_DP_MAX_VARIANTS] = {}, dp_graph_tb_node[2] = {};
+    unsigned dp_fwd_grid[MH_DP_MAX_VARIANTS] = {};
     mh::DevBuf ml_jobs, ml_y;   // learned band: job table, network outputs + bands
     // seed-matching scratch
     mh::DevBuf mf_scratch[16];

@@ -342,6 +342,14 @@ void mh_destroy(mh_context *ctx)
     for (mh_context *h : ctx->helpers) mh_destroy(h);
     collect_timers(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (hipGraphExec_t &g : ctx->dp_graph_fwd)
+        if (g) (void)hipGraphExecDestroy(g);
+    for (hipGraph_t &g : ctx->dp_graph_fwd_src)
+        if (g) (void)hipGraphDestroy(g);
+    if (ctx->dp_graph_tb) (void)hipGraphExecDestroy(ctx->dp_graph_tb);
+    if (ctx->dp_graph_tb_src) (void)hipGraphDestroy(ctx->dp_graph_tb_src);
+    if (ctx->dp_round_host) (void)hipHostFree(ctx->dp_round_host);
+    ctx->dp_round.release();
     ctx->dp_tb.release();
     ctx->dp_bnd.release();
     ctx->dp_jobs.release();
