@@ -560,6 +560,8 @@ typedef struct {
     mhe_t **pool;          /* every stored entry, for freeing */
     size_t npool, cappool;
     or_match_stats st;
+    /* UniqueMatchFinder's EnumerateMatches instead of PairwiseMatchFinder's */
+    int unique;
     /* seed-tuple capture mode: record HashMatch calls instead of inserting */
     int tup_mode;
     or_match *tup_out;
@@ -680,6 +682,35 @@ static void enumerate_pairwise(memhash_t *h, idmer_t *run, size_t cnt)
         }
 }
 
+/* UniqueMatchFinder::EnumerateMatches, MA/UniqueMatchFinder.cpp:36-60: the
+ * run sorted by genome (stable), every genome with exactly one entry kept,
+ * one HashMatch of all of them when there are at least two. */
+static void enumerate_unique(memhash_t *h, idmer_t *run, size_t cnt)
+{
+    for (size_t i = 1; i < cnt; ++i) {
+        idmer_t t = run[i];
+        size_t j = i;
+        while (j > 0 && run[j - 1].id > t.id) { run[j] = run[j - 1]; --j; }
+        run[j] = t;
+    }
+    idmer_t uniq[OR_MAX_SEQ];
+    int nu = 0;
+    size_t i = 0;
+    while (i < cnt) {
+        size_t j = i;
+        while (j < cnt && run[j].id == run[i].id) ++j;
+        if (j - i == 1) uniq[nu++] = run[i];
+        i = j;
+    }
+    if (nu >= 2) hash_match(h, uniq, nu);
+}
+
+static void enumerate(memhash_t *h, idmer_t *run, size_t cnt)
+{
+    if (h->unique) enumerate_unique(h, run, cnt);
+    else enumerate_pairwise(h, run, cnt);
+}
+
 /* Read(), LM/FileSML.cpp:470-500: entries [off, off+size) clipped to SMLLength */
 typedef struct { uint64_t pos, mer; } bm_t;
 
@@ -761,7 +792,7 @@ static int search_range(memhash_t *h, uint64_t *start_points)
         int cid = it.id;
         if (mcnt > 0) {
             if (it.mer > match[0].mer) {
-                if (mcnt > 1) enumerate_pairwise(h, match, mcnt);
+                if (mcnt > 1) enumerate(h, match, mcnt);
                 mcnt = 0;
             }
         }
@@ -817,7 +848,7 @@ static int search_range(memhash_t *h, uint64_t *start_points)
             ++ncur;
         }
     }
-    if (mcnt > 1) enumerate_pairwise(h, match, mcnt);
+    if (mcnt > 1) enumerate(h, match, mcnt);
 done:
     free(match);
     for (int n = 0; n < k; ++n) free(buf[n]);
@@ -844,9 +875,8 @@ static void run_search(memhash_t *h)
     while (search_range(h, sp) == 0) { }
 }
 
-int64_t or_find_matches_pairwise(int k, const char *const *seqs, const uint64_t *lens,
-                                 uint64_t seed, or_match *out, int64_t max_out,
-                                 or_match_stats *stats)
+static int64_t find_matches(int k, const char *const *seqs, const uint64_t *lens, uint64_t seed, or_match *out,
+                            int64_t max_out, or_match_stats *stats, int unique)
 {
     sml_t sm[OR_MAX_SEQ];
     if (build_smls(k, seqs, lens, seed, sm) != 0) return -1;
@@ -854,6 +884,7 @@ int64_t or_find_matches_pairwise(int k, const char *const *seqs, const uint64_t 
     memset(&h, 0, sizeof(h));
     h.k = k;
     h.sm = sm;
+    h.unique = unique;
     h.tab = (bucket_t *)calloc(MEM_TABLE_SIZE, sizeof(bucket_t));
     run_search(&h);
     /* GetMatchList, LM/MemHash.h:182-203: bucket order, then vector order */
@@ -878,9 +909,23 @@ int64_t or_find_matches_pairwise(int k, const char *const *seqs, const uint64_t 
     return n < max_out ? n : max_out;
 }
 
-int64_t or_seed_tuples_pairwise(int k, const char *const *seqs, const uint64_t *lens,
-                                uint64_t seed, or_match *out, int64_t *offsets,
-                                int64_t max_out)
+int64_t or_find_matches_pairwise(int k, const char *const *seqs, const uint64_t *lens,
+                                 uint64_t seed, or_match *out, int64_t max_out,
+                                 or_match_stats *stats)
+{
+    return find_matches(k, seqs, lens, seed, out, max_out, stats, 0);
+}
+
+int64_t or_find_matches_unique(int k, const char *const *seqs, const uint64_t *lens,
+                               uint64_t seed, or_match *out, int64_t max_out,
+                               or_match_stats *stats)
+{
+    if (k < 2) return -1;
+    return find_matches(k, seqs, lens, seed, out, max_out, stats, 1);
+}
+
+static int64_t seed_tuples(int k, const char *const *seqs, const uint64_t *lens, uint64_t seed, or_match *out,
+                           int64_t *offsets, int64_t max_out, int unique)
 {
     sml_t sm[OR_MAX_SEQ];
     if (build_smls(k, seqs, lens, seed, sm) != 0) return -1;
@@ -888,6 +933,7 @@ int64_t or_seed_tuples_pairwise(int k, const char *const *seqs, const uint64_t *
     memset(&h, 0, sizeof(h));
     h.k = k;
     h.sm = sm;
+    h.unique = unique;
     h.tup_mode = 1;
     h.tup_out = out;
     h.tup_off = offsets;
@@ -895,6 +941,21 @@ int64_t or_seed_tuples_pairwise(int k, const char *const *seqs, const uint64_t *
     run_search(&h);
     for (int i = 0; i < k; ++i) sml_free(&sm[i]);
     return h.tup_n;
+}
+
+int64_t or_seed_tuples_pairwise(int k, const char *const *seqs, const uint64_t *lens,
+                                uint64_t seed, or_match *out, int64_t *offsets,
+                                int64_t max_out)
+{
+    return seed_tuples(k, seqs, lens, seed, out, offsets, max_out, 0);
+}
+
+int64_t or_seed_tuples_unique(int k, const char *const *seqs, const uint64_t *lens,
+                              uint64_t seed, or_match *out, int64_t *offsets,
+                              int64_t max_out)
+{
+    if (k < 2) return -1;
+    return seed_tuples(k, seqs, lens, seed, out, offsets, max_out, 1);
 }
 
 int or_extend_seed(int k, const char *const *seqs, const uint64_t *lens, uint64_t seed,

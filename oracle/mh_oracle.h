@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define OR_MAX_SEQ 4
+#define OR_MAX_SEQ 16
 
 /* ---- seed patterns (LM/SeedMasks.h) ---- */
 int64_t  or_get_seed(int weight, int seed_rank);
@@ -91,6 +91,17 @@ int64_t or_find_matches_pairwise(int k, const char *const *seqs, const uint64_t 
 
 /* The seed tuples in HashMatch call order (after SetDirection), with their
  * generalized offsets.  Returns count or -1. */
+/* UniqueMatchFinder (MA/UniqueMatchFinder.cpp:36-60) over k genomes (2 <= k
+ * <= OR_MAX_SEQ): SearchRange's runs as for the pairwise finder, each run
+ * hashed once with every genome that holds its mer exactly once (when there
+ * are at least two).  Same outputs and conventions as
+ * or_find_matches_pairwise. */
+int64_t or_find_matches_unique(int k, const char *const *seqs, const uint64_t *lens,
+                               uint64_t seed, or_match *out, int64_t max_out,
+                               or_match_stats *stats);
+int64_t or_seed_tuples_unique(int k, const char *const *seqs, const uint64_t *lens,
+                              uint64_t seed, or_match *out, int64_t *offsets,
+                              int64_t max_out);
 int64_t or_seed_tuples_pairwise(int k, const char *const *seqs, const uint64_t *lens,
                                 uint64_t seed, or_match *out, int64_t *offsets,
                                 int64_t max_out);

@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-OR_MAX_SEQ = 4
+OR_MAX_SEQ = 16
 CODE = {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}
 
 
@@ -61,6 +61,12 @@ def lib():
         L.or_find_matches_pairwise.restype = C.c_int64
         L.or_find_matches_pairwise.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
                                                C.c_uint64, C.POINTER(OrMatch), C.c_int64, C.POINTER(OrStats)]
+        L.or_find_matches_unique.restype = C.c_int64
+        L.or_find_matches_unique.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
+                                             C.c_uint64, C.POINTER(OrMatch), C.c_int64, C.POINTER(OrStats)]
+        L.or_seed_tuples_unique.restype = C.c_int64
+        L.or_seed_tuples_unique.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
+                                            C.c_uint64, C.POINTER(OrMatch), C.c_void_p, C.c_int64]
         L.or_seed_tuples_pairwise.restype = C.c_int64
         L.or_seed_tuples_pairwise.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
                                               C.c_uint64, C.POINTER(OrMatch), C.c_void_p, C.c_int64]
@@ -207,6 +213,34 @@ def find_matches(seqs, seed, max_out=None):
     starts = np.array([[out[i].start[s] for s in range(k)] for i in range(n)], dtype=np.int64).reshape(n, k)
     lengths = np.array([out[i].length for i in range(n)], dtype=np.uint64)
     return starts, lengths, st
+
+
+def find_matches_unique(seqs, seed, max_out=None):
+    """UniqueMatchFinder's match list in reference order: (starts (n, k) int64,
+    lengths, stats)."""
+    k = len(seqs)
+    bs, arr, lens = _seq_args(seqs)
+    cap = max_out or (max(len(s) for s in bs) + 16)
+    out = (OrMatch * cap)()
+    st = OrStats()
+    n = lib().or_find_matches_unique(k, arr, lens, seed, out, cap, C.byref(st))
+    if n < 0:
+        raise ValueError("match finding failed")
+    starts = np.array([[out[i].start[s] for s in range(k)] for i in range(n)], dtype=np.int64).reshape(n, k)
+    lengths = np.array([out[i].length for i in range(n)], dtype=np.uint64)
+    return starts, lengths, st
+
+
+def seed_tuples_unique(seqs, seed):
+    """UniqueMatchFinder's HashMatch calls in order: (starts (n, k), offsets)."""
+    k = len(seqs)
+    bs, arr, lens = _seq_args(seqs)
+    n = lib().or_seed_tuples_unique(k, arr, lens, seed, None, None, 0)
+    out = (OrMatch * max(n, 1))()
+    off = np.zeros(max(n, 1), dtype=np.int64)
+    n2 = lib().or_seed_tuples_unique(k, arr, lens, seed, out, off.ctypes.data, n)
+    starts = np.array([[out[i].start[s] for s in range(k)] for i in range(n2)], dtype=np.int64).reshape(n2, k)
+    return starts, off[:n2]
 
 
 def eliminate_overlaps(starts, lengths, min_length=0):
