@@ -168,6 +168,26 @@ int mh_find_matches_pairwise_list(mh_context *ctx, const mh_sml *const *smls, in
 uint64_t        mh_match_list_size(const mh_match_list *list);
 const mh_match *mh_match_list_data(const mh_match_list *list);
 void            mh_match_list_free(mh_match_list *list);
+/* Replaces UniqueMatchFinder::FindMatches (MA/UniqueMatchFinder.cpp:36-60 over
+ * MemHash::FindMatches, LM/MemHash.cpp:109-127, LM/MatchFinder.cpp:172-340,
+ * LM/MemHash.cpp:167-251, LM/MatchFinder.h:218-374), the finder
+ * progressiveMauve runs for more than four genomes (MA/progressiveMauve.cpp:
+ * 493-498): each run of one masked mer is hashed once, with every genome that
+ * holds the mer exactly once, when there are at least two.  k is 2 to 16.
+ * The list (GetMatchList order) is library-owned: size() matches of k starts
+ * each (starts row-major, size() x k; 1-based, negative = reverse strand,
+ * 0 = absent) and a length each.  MH_ERR_UNSUPPORTED when a seed's extension
+ * would leave a sorted list, or the seed weight leaves no room for the merged
+ * sort key (2 weight + 4 + restart bits above 64). */
+typedef struct mh_match_k_list mh_match_k_list;
+int mh_find_matches_unique(mh_context *ctx, const mh_sml *const *smls, int k,
+                           mh_match_k_list **out, mh_match_stats *stats);
+uint64_t        mh_match_k_list_size(const mh_match_k_list *list);
+int             mh_match_k_list_genomes(const mh_match_k_list *list);
+const int64_t  *mh_match_k_list_starts(const mh_match_k_list *list);
+const uint64_t *mh_match_k_list_lengths(const mh_match_k_list *list);
+void            mh_match_k_list_free(mh_match_k_list *list);
+
 /* Sequence count, file names and lengths a list carries (the genomes it was
  * found in, or the header of a file it was read from; names "" when unknown). */
 int             mh_match_list_seq_count(const mh_match_list *list);

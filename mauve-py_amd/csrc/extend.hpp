@@ -159,4 +159,106 @@ MH_PRAGMA_UNROLL
     return EXT_OK;
 }
 
+// The k-genome form of the same walk (the genomes a seed holds; start 0
+// absent): ExtendMatch compares every present genome's window with the first
+// present genome's, and a step matches when all of them do
+// (LM/MatchFinder.h:218-374).  EnvK holds each genome's GetDnaSeedMer by
+// position, list length and sequence length.
+constexpr int EXT_MAXK = 16;
+
+struct ExtEnvK {
+    const uint64_t *kp[EXT_MAXK];
+    uint64_t n[EXT_MAXK];
+    int64_t L[EXT_MAXK];
+    int seed_len;
+    uint64_t mer_mask;
+};
+
+MH_HD inline bool lookup_k(const ExtEnvK &e, int g, int64_t pos1, int64_t len, uint64_t &mer, int &parity)
+{
+    int64_t p = pos1;
+    if (p < 0) p = -p + len - e.seed_len;
+    const int64_t idx = p - 1;
+    if (idx < 0 || (uint64_t)idx >= e.n[g]) return false;
+    const uint64_t k = e.kp[g][idx];
+    parity = pos1 < 0 ? (int)(k & 1ull) : (int)!(k & 1ull);
+    mer = k & e.mer_mask;
+    return true;
+}
+
+// st[x] is the start in genome gs[x], x < used (the present genomes in genome
+// order).  EXT_ERR if a lookup would leave a list or the walk runs more than
+// 8 (sum of the lengths) steps.
+MH_HD inline int extend_k(const ExtEnvK &e, const int *gs, int used, int64_t *st, int64_t &len)
+{
+    int jump = e.seed_len;
+    uint32_t extend_limit = 0, extend_attempts = 0;
+    bool extend_again = false;
+    int64_t guard = 0, guard_max = 64;
+    for (int x = 0; x < used; ++x) guard_max += 8 * e.L[gs[x]];
+    for (int dir = 0; dir < 4; ++dir) {
+        int64_t maxlen = dir >= 2 ? (int64_t)e.seed_len : (int64_t)4294967295ll;
+        for (int x = 0; x < used; ++x) {
+            if (st[x] < 0) {
+                const int64_t rc_len = e.L[gs[x]] - len + st[x] + 1;
+                if (rc_len < maxlen) maxlen = rc_len;
+            } else if (st[x] - 1 < maxlen) {
+                maxlen = st[x] - 1;
+            }
+        }
+        int j = 0, i = used;
+        extend_limit = 0;
+        extend_attempts = 0;
+        while (maxlen - jump >= 0) {
+            if (++guard > guard_max) return EXT_ERR;
+            len += jump;
+            maxlen -= jump;
+            for (j = 0; j < used; ++j)
+                if (st[j] > 0) {
+                    st[j] -= jump;
+                    if (st[j] <= 0) st[j] += e.L[gs[j]];
+                }
+            uint64_t cm;
+            int cp;
+            if (!lookup_k(e, gs[0], st[0], len, cm, cp)) return EXT_ERR;
+            for (i = 1; i < used; ++i) {
+                uint64_t om;
+                int op;
+                if (!lookup_k(e, gs[i], st[i], len, om, op)) return EXT_ERR;
+                if (cm != om || cp != op) {
+                    if (dir < 2) maxlen = 0;
+                    break;
+                }
+            }
+            extend_attempts += jump;
+            if (i == used) extend_limit = extend_attempts;
+            if (dir > 1 && extend_attempts == (uint32_t)e.seed_len) break;
+        }
+        if (i < used) {
+            len -= jump;
+            for (; j > 0; j--)
+                if (st[j - 1] >= 0) st[j - 1] += jump;
+        }
+        if (dir > 1 && extend_attempts > 0) {
+            if (extend_limit > 0) extend_again = true;
+            int unmatched = (int)(extend_attempts - extend_limit);
+            if (i < used) unmatched -= jump;
+            len -= unmatched;
+            for (j = 0; j < used; ++j)
+                if (st[j] > 0) {
+                    st[j] += unmatched;
+                    if (st[j] > e.L[gs[j]]) st[j] -= e.L[gs[j]];
+                }
+        }
+        for (int x = 0; x < used; ++x) st[x] = -st[x];
+        if (dir >= 1) jump = 1;
+        if (dir == 3 && extend_again) {
+            dir = -1;
+            jump = e.seed_len;
+            extend_again = false;
+        }
+    }
+    return EXT_OK;
+}
+
 }  // namespace mh

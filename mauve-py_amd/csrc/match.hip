@@ -1450,6 +1450,7 @@ int search_live_ranges(mh_context *ctx, const mh_sml *const *smls, int k, LiveRa
     for (int g = 0; g < k; ++g) {
         out->lo[g].assign(1, 0);
         out->hi[g].assign(1, smls[g]->n);
+        out->ep[g].assign(1, 0);
     }
     KLists L;
     std::memset(&L, 0, sizeof(L));
@@ -1755,6 +1756,239 @@ int partition_seeds(mh_context *ctx, const SeedRec *d_seeds, uint64_t ns, int k,
         }
     }
     host_time(ctx, "host_partition", now_ms() - t0);
+    return MH_OK;
+}
+
+// ---- UniqueMatchFinder seeds (k genomes) ----------------------------------
+// UniqueMatchFinder::EnumerateMatches (MA/UniqueMatchFinder.cpp:36-60) hashes
+// each run of one masked mer once, with every genome that holds the mer
+// exactly once, when there are at least two.  The live entries of all k
+// lists are merged into one order by a radix sort on (search, masked mer,
+// genome) -- the search being the number of repeat-limit restarts before the
+// range that holds the entry, since SearchRange's runs never span a restart
+// -- so that a run's entries are adjacent and grouped by genome.  An entry is
+// its genome's only one in the run when its neighbours hold other keys; the
+// runs with two or more such entries become seeds, in mer (HashMatch) order.
+// Each seed is then set up as HashMatch does (SetDirection, CalculateOffset
+// with the seed length) and extended over its genomes (ExtendMatch).
+
+struct UqLists {
+    const uint64_t *ck[EXT_MAXK];
+    const uint32_t *pos[EXT_MAXK];
+    const uint32_t *w[EXT_MAXK];      // packed sequence
+    const uint32_t *wrc[EXT_MAXK];    // packed reverse complement
+    uint64_t off[EXT_MAXK];           // each list's first entry in the merged numbering
+};
+
+__global__ void uq_gather_kernel(const uint64_t *__restrict__ ck, uint64_t lo, uint64_t cnt, uint64_t tag,
+                                 uint64_t gbase, uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const uint64_t i = lo + t;
+    key[t] = tag | ((ck[i] >> 1) << 4);
+    val[t] = (uint32_t)(gbase + i);
+}
+
+// flag[e]: entry e is the only one of its genome in its run
+__global__ void uq_mark_kernel(const uint64_t *__restrict__ key, uint64_t n, uint32_t *__restrict__ flag)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t k = key[e];
+    flag[e] = (e == 0 || key[e - 1] != k) && (e + 1 == n || key[e + 1] != k);
+}
+
+// over the genome-only entries U (in order): flag the first entry of each run
+// that holds at least two of them
+__global__ void uq_leader_kernel(const uint64_t *__restrict__ key, const uint32_t *__restrict__ U, uint64_t m,
+                                 uint32_t *__restrict__ flag)
+{
+    const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= m) return;
+    const uint64_t r = key[U[u]] >> 4;
+    const bool first = u == 0 || (key[U[u - 1]] >> 4) != r;
+    const bool more = u + 1 < m && (key[U[u + 1]] >> 4) == r;
+    flag[u] = first && more;
+}
+
+__global__ void uq_seed_kernel(const uint64_t *__restrict__ key, const uint32_t *__restrict__ val,
+                               const uint32_t *__restrict__ U, uint64_t m, const uint32_t *__restrict__ lead,
+                               uint64_t ns, UqLists ls, ExtEnvK env, int k, int solid_odd,
+                               int32_t *__restrict__ seed_st, int32_t *__restrict__ ext_st,
+                               uint32_t *__restrict__ ext_len, int64_t *__restrict__ offset,
+                               uint32_t *__restrict__ err)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int sl = env.seed_len;
+    int gs[EXT_MAXK];
+    int64_t st[EXT_MAXK], p[EXT_MAXK];
+    int used = 0, fstrand = 0;
+    const uint64_t r = key[U[lead[s]]] >> 4;
+    for (uint64_t u = lead[s]; u < m && used < EXT_MAXK; ++u) {
+        const uint32_t e = U[u];
+        const uint64_t kk = key[e];
+        if ((kk >> 4) != r) break;
+        const int g = (int)(kk & 15u);
+        const uint64_t i = val[e] - ls.off[g];
+        const int64_t q = ls.pos[g][i];
+        const int strand = (int)(ls.ck[g][i] & 1ull);
+        if (used == 0) fstrand = strand;
+        gs[used] = g;
+        p[used] = q;
+        // SetDirection, LM/MemHash.cpp:167-203: a genome whose window is on the
+        // other strand than the first genome's is reversed
+        st[used] = strand == fstrand ? q + 1 : -(q + 1);
+        ++used;
+    }
+    int64_t off = 0;   // CalculateOffset, LM/MatchHashEntry.cpp:141-160, with the seed length
+    for (int x = 1; x < used; ++x) off += st[x] - st[0] - (st[x] < 0 ? (int64_t)sl : 0);
+    for (int g = 0; g < k; ++g) seed_st[s * k + g] = 0;
+    for (int x = 0; x < used; ++x) seed_st[s * k + gs[x]] = (int32_t)st[x];
+    offset[s] = off;
+    int64_t len = sl;
+    int64_t ex[EXT_MAXK];
+    if (solid_odd) {
+        // the shortest of the pairwise maximal exact matches with the first genome
+        const int f = gs[0];
+        const int64_t pf = p[0], Lf = env.L[f];
+        int64_t left = pf, right = Lf - (pf + sl);
+        int64_t q1[EXT_MAXK];
+        for (int x = 1; x < used; ++x) {
+            const int g = gs[x];
+            const bool rev = st[x] < 0;
+            const int64_t Lg = env.L[g];
+            q1[x] = rev ? Lg - p[x] - sl : p[x];
+            const uint32_t *wg = rev ? ls.wrc[g] : ls.w[g];
+            const int64_t lim_l = left < q1[x] ? left : q1[x];
+            const int64_t lg = match_left(ls.w[f], pf, wg, q1[x], lim_l);
+            if (lg < left) left = lg;
+            const int64_t rr = Lg - (q1[x] + sl);
+            const int64_t lim_r = right < rr ? right : rr;
+            const int64_t rg = match_right(ls.w[f], pf + sl, wg, q1[x] + sl, lim_r);
+            if (rg < right) right = rg;
+        }
+        len = sl + left + right;
+        ex[0] = pf - left + 1;
+        for (int x = 1; x < used; ++x)
+            ex[x] = st[x] > 0 ? q1[x] - left + 1 : -(env.L[gs[x]] - (q1[x] - left + len) + 1);
+    } else {
+        for (int x = 0; x < used; ++x) ex[x] = st[x];
+        if (extend_k(env, gs, used, ex, len) != EXT_OK) atomicOr(err, 1u);
+    }
+    for (int g = 0; g < k; ++g) ext_st[s * k + g] = 0;
+    for (int x = 0; x < used; ++x) ext_st[s * k + gs[x]] = (int32_t)ex[x];
+    ext_len[s] = (uint32_t)len;
+}
+
+int unique_seeds(mh_context *ctx, const mh_sml *const *smls, int k, const LiveRanges &live, UniqueSeeds *out)
+{
+    MH_CHECK(k >= 2 && k <= EXT_MAXK && k <= PLAN_MAXK, MH_ERR_UNSUPPORTED, "UniqueMatchFinder takes 2 to 16 genomes");
+    const int w = smls[0]->seed_weight, sl = smls[0]->seed_length;
+    int ebits = 0;
+    while (ebits < 32 && (1ull << ebits) <= (uint64_t)live.restarts) ++ebits;
+    const int bits = 2 * w + 4 + ebits;
+    MH_CHECK(bits <= 64, MH_ERR_UNSUPPORTED, "seed weight too large for the merged sort key");
+    out->k = k;
+    out->n = 0;
+    uint64_t N = 0;
+    for (int g = 0; g < k; ++g)
+        for (size_t r = 0; r < live.lo[g].size(); ++r) N += live.hi[g][r] - live.lo[g][r];
+    MH_CHECK(N < (1ull << 32), MH_ERR_UNSUPPORTED, "more than 2^32 sorted-list entries");
+    DevBuf *b = ctx->uq_scratch;
+    int rc;
+    if ((rc = b[0].ensure(N * 8 + 64)) != MH_OK) return rc;
+    if ((rc = b[1].ensure(N * 8 + 64)) != MH_OK) return rc;
+    if ((rc = b[2].ensure(N * 4 + 64)) != MH_OK) return rc;
+    if ((rc = b[3].ensure(N * 4 + 64)) != MH_OK) return rc;
+    uint64_t *k0 = b[0].as<uint64_t>(), *k1 = b[1].as<uint64_t>();
+    uint32_t *v0 = b[2].as<uint32_t>(), *v1 = b[3].as<uint32_t>();
+    UqLists ls;
+    std::memset(&ls, 0, sizeof(ls));
+    uint64_t at = 0, gbase = 0;
+    for (int g = 0; g < k; ++g) {
+        ls.ck[g] = smls[g]->d_keys;
+        ls.pos[g] = smls[g]->d_pos;
+        ls.w[g] = smls[g]->d_words;
+        ls.off[g] = gbase;
+        for (size_t r = 0; r < live.lo[g].size(); ++r) {
+            const uint64_t lo = live.lo[g][r], cnt = live.hi[g][r] - lo;
+            const uint64_t tag = ((uint64_t)live.ep[g][r] << (2 * w + 4)) | (uint64_t)g;
+            if (cnt)
+                hipLaunchKernelGGL(uq_gather_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   smls[g]->d_keys, lo, cnt, tag, gbase, k0 + at, v0 + at);
+            at += cnt;
+        }
+        gbase += smls[g]->n;
+    }
+    MH_HIP(hipGetLastError());
+    if (N == 0) return MH_OK;
+    uint64_t *sk;
+    uint32_t *sv;
+    if ((rc = radix_sort_kv(ctx, k0, v0, k1, v1, N, bits, &sk, &sv)) != MH_OK) return rc;
+    uint32_t *flag = sk == k0 ? reinterpret_cast<uint32_t *>(k1) : reinterpret_cast<uint32_t *>(k0);
+    uint32_t *U = sv == v0 ? v1 : v0;
+    if ((rc = b[4].ensure(((N + JTILE - 1) / JTILE) * 4 + 64)) != MH_OK) return rc;
+    if ((rc = b[5].ensure(64)) != MH_OK) return rc;
+    uint32_t *tiles = b[4].as<uint32_t>(), *d_cnt = b[5].as<uint32_t>();
+    hipLaunchKernelGGL(uq_mark_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, sk, N, flag);
+    uint64_t M = 0;
+    if ((rc = compact_flags(ctx, flag, N, U, tiles, d_cnt, &M)) != MH_OK) return rc;
+    if (M < 2) return MH_OK;
+    // leaders: flags and indices after U, in the other key buffer's space
+    uint32_t *flag2 = flag;
+    if ((rc = b[6].ensure(M * 4 + 64)) != MH_OK) return rc;
+    uint32_t *lead = b[6].as<uint32_t>();
+    hipLaunchKernelGGL(uq_leader_kernel, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ctx->stream, sk, U, M,
+                       flag2);
+    uint64_t S = 0;
+    if ((rc = compact_flags(ctx, flag2, M, lead, tiles, d_cnt, &S)) != MH_OK) return rc;
+    out->n = S;
+    if (S == 0) return MH_OK;
+    const bool solid_odd = smls[0]->seed_length == smls[0]->seed_weight && (smls[0]->seed_length & 1);
+    ExtEnvK env;
+    std::memset(&env, 0, sizeof(env));
+    env.seed_len = sl;
+    env.mer_mask = w >= 32 ? ~0ull : (~0ull << (64 - 2 * w));
+    for (int g = 0; g < k; ++g) {
+        env.kp[g] = smls[g]->d_keys_pos;
+        env.n[g] = smls[g]->n;
+        env.L[g] = (int64_t)smls[g]->len;
+        MH_CHECK(smls[g]->len < (1ull << 31), MH_ERR_UNSUPPORTED, "genomes of 2^31 bases or more");
+    }
+    if (solid_odd) {
+        for (int g = 0; g < k; ++g) {
+            if ((rc = ctx->uq_rc[g].ensure(smls[g]->n_words * 4 + 64)) != MH_OK) return rc;
+            ls.wrc[g] = ctx->uq_rc[g].as<uint32_t>();
+            hipLaunchKernelGGL(rc_pack_kernel, dim3((unsigned)((smls[g]->n_words + 255) / 256)), dim3(256), 0,
+                               ctx->stream, smls[g]->d_words, smls[g]->len, smls[g]->n_words, ctx->uq_rc[g].as<uint32_t>());
+        }
+    }
+    if ((rc = b[7].ensure(S * (uint64_t)k * 8 + S * 12 + 256)) != MH_OK) return rc;
+    int32_t *d_sst = b[7].as<int32_t>();
+    int32_t *d_est = d_sst + S * k;
+    uint32_t *d_len = reinterpret_cast<uint32_t *>(d_est + S * k);
+    int64_t *d_off = reinterpret_cast<int64_t *>((reinterpret_cast<uintptr_t>(d_len + S) + 7) & ~(uintptr_t)7);
+    MH_HIP(hipMemsetAsync(d_cnt, 0, 4, ctx->stream));
+    {
+        TimedLaunch tl(ctx, "extend");
+        hipLaunchKernelGGL(uq_seed_kernel, dim3((unsigned)((S + 127) / 128)), dim3(128), 0, ctx->stream, sk, sv, U, M,
+                           lead, S, ls, env, k, solid_odd ? 1 : 0, d_sst, d_est, d_len, d_off, d_cnt);
+        MH_HIP(hipGetLastError());
+    }
+    out->seed_st.resize(S * k);
+    out->ext_st.resize(S * k);
+    out->ext_len.resize(S);
+    out->offset.resize(S);
+    uint32_t err = 0;
+    MH_HIP(hipMemcpyAsync(out->seed_st.data(), d_sst, S * k * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(out->ext_st.data(), d_est, S * k * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(out->ext_len.data(), d_len, S * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(out->offset.data(), d_off, S * 8, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(&err, d_cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    MH_CHECK(err == 0, MH_ERR_UNSUPPORTED, "seed extension left the sorted mer lists");
     return MH_OK;
 }
 

@@ -82,27 +82,29 @@ inline size_t lower_bound_ent(const std::vector<Ent> &v, const Ent &q)
     return first;
 }
 
-// A MatchHashEntry of up to four genomes (start 0: absent), first = the
-// index of its first present genome (FirstStart).
-struct EntK {
-    int64_t s[4];
+// A MatchHashEntry of up to MK genomes (start 0: absent), first = the index
+// of its first present genome (FirstStart); starts of type T.
+template <int MK, class T> struct EntT {
+    T s[MK];
     uint64_t len;
     int64_t off;
     uint64_t mersize;
     int first;
 };
+using EntK = EntT<4, int64_t>;     // PairwiseMatchFinder, up to four genomes
+using EntU = EntT<16, int32_t>;    // UniqueMatchFinder, up to sixteen
 
 // MatchHashEntry::Contains (a contains b), LM/MatchHashEntry.cpp:164-200
-inline bool contains_k(const EntK &a, const EntK &b, int k)
+template <class E> inline bool contains_k(const E &a, const E &b, int k)
 {
     if (a.off != b.off) return false;
     int i = b.first;
-    const int64_t diff = b.s[i] - a.s[i];
+    const int64_t diff = (int64_t)b.s[i] - (int64_t)a.s[i];
     if (a.s[i] == 0) return false;
     if (diff < 0 || a.len < b.len + (uint64_t)diff) return false;
     const int64_t diff_rc = (int64_t)b.len - (int64_t)a.len + diff;
     for (++i; i < k; ++i) {
-        const int64_t di = b.s[i] - a.s[i];
+        const int64_t di = (int64_t)b.s[i] - (int64_t)a.s[i];
         if (b.s[i] == 0 && a.s[i] == 0) continue;
         if (b.s[i] < 0 && diff_rc == di) continue;
         if (diff != di) return false;
@@ -111,7 +113,7 @@ inline bool contains_k(const EntK &a, const EntK &b, int k)
 }
 
 // strict_start_lessthan_ptr, LM/MatchHashEntry.cpp:48-67
-inline bool strict_less_k(const EntK &a, const EntK &b, int k)
+template <class E> inline bool strict_less_k(const E &a, const E &b, int k)
 {
     if (a.first == b.first)
         for (int i = 0; i < k; ++i) {
@@ -124,7 +126,7 @@ inline bool strict_less_k(const EntK &a, const EntK &b, int k)
 }
 
 // MheCompare, LM/MatchHashEntry.h:121-143
-inline bool mhe_less_k(const EntK &a, const EntK &b, int k)
+template <class E> inline bool mhe_less_k(const E &a, const E &b, int k)
 {
     if (a.first > b.first) return true;
     if (a.first != b.first) return false;
@@ -136,7 +138,7 @@ inline bool mhe_less_k(const EntK &a, const EntK &b, int k)
     return strict_less_k(a, b, k);
 }
 
-inline size_t lower_bound_k(const std::vector<EntK> &v, const EntK &q, int k)
+template <class E> inline size_t lower_bound_k(const std::vector<E> &v, const E &q, int k)
 {
     size_t first = 0, len = v.size();
     while (len > 0) {
@@ -153,12 +155,12 @@ inline size_t lower_bound_k(const std::vector<EntK> &v, const EntK &q, int k)
 
 // CalculateOffset, LM/MatchHashEntry.cpp:141-160 (Length is the seed length
 // when HashMatch calls it)
-inline int64_t offset_k(const EntK &e, int k)
+template <class E> inline int64_t offset_k(const E &e, int k)
 {
     int64_t off = 0;
     const int64_t ref = e.s[e.first];
     for (int i = e.first + 1; i < k; ++i)
-        if (e.s[i] != 0) off += e.s[i] - ref - (e.s[i] < 0 ? (int64_t)e.len : 0);
+        if (e.s[i] != 0) off += (int64_t)e.s[i] - ref - (e.s[i] < 0 ? (int64_t)e.len : 0);
     return off;
 }
 
@@ -463,6 +465,91 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
     return MH_OK;
 }
 
+// UniqueMatchFinder over 2 to 16 genomes (MA/UniqueMatchFinder.cpp:36-60 with
+// LM/MatchFinder.cpp:172-340 and LM/MemHash.cpp:167-251): the seeds and their
+// extensions from the device (unique_seeds), the insertion into the 40000
+// buckets replayed on the host, buckets in parallel (each bucket's seeds in
+// HashMatch order), the list bucket by bucket (LM/MemHash.h:182-203).
+int find_matches_unique(mh_context *ctx, const mh_sml *const *smls, int k, std::vector<int64_t> &starts,
+                        std::vector<uint64_t> &lengths, mh_match_stats *st)
+{
+    MH_CHECK(k >= 2 && k <= 16, MH_ERR_UNSUPPORTED, "UniqueMatchFinder takes 2 to 16 genomes");
+    for (int i = 0; i < k; ++i) {
+        MH_CHECK(smls[i], MH_ERR_INVALID, "null sorted mer list");
+        MH_CHECK(smls[i]->seed == smls[0]->seed, MH_ERR_INVALID, "Different seed patterns.");
+    }
+    int rc;
+    LiveRanges live;
+    if ((rc = search_live_ranges(ctx, smls, k, &live)) != MH_OK) return rc;
+    UniqueSeeds us;
+    if ((rc = unique_seeds(ctx, smls, k, live, &us)) != MH_OK) return rc;
+    const double t0 = now_ms();
+    const uint64_t S = us.n, sl = (uint64_t)smls[0]->seed_length;
+    std::vector<uint32_t> cnt(TABLE_SIZE + 1, 0), bucket(S);
+    for (uint64_t x = 0; x < S; ++x) {
+        const int64_t m = us.offset[x] % (int64_t)TABLE_SIZE;
+        bucket[x] = (uint32_t)(m < 0 ? m + TABLE_SIZE : m);
+        ++cnt[bucket[x] + 1];
+    }
+    for (uint32_t q = 0; q < TABLE_SIZE; ++q) cnt[q + 1] += cnt[q];
+    std::vector<uint32_t> order(S);
+    {
+        std::vector<uint32_t> at(cnt.begin(), cnt.end() - 1);
+        for (uint64_t x = 0; x < S; ++x) order[at[bucket[x]]++] = (uint32_t)x;
+    }
+    std::vector<std::vector<EntU>> table(TABLE_SIZE);
+    std::atomic<int64_t> collisions{0};
+    const uint32_t per = 64;
+    host_parallel((TABLE_SIZE + per - 1) / per, [&](size_t t) {
+        int64_t coll = 0;
+        for (uint32_t q = (uint32_t)t * per; q < std::min<uint32_t>(TABLE_SIZE, (uint32_t)(t + 1) * per); ++q) {
+            std::vector<EntU> &v = table[q];
+            for (uint32_t y = cnt[q]; y < cnt[q + 1]; ++y) {
+                const uint32_t x = order[y];
+                EntU c;
+                std::memset(&c, 0, sizeof(c));
+                c.first = -1;
+                for (int g = 0; g < k; ++g) {
+                    c.s[g] = us.seed_st[(uint64_t)x * k + g];
+                    if (c.first < 0 && c.s[g] != 0) c.first = g;
+                }
+                c.len = sl;
+                c.mersize = sl;
+                c.off = us.offset[x];
+                size_t it = lower_bound_k(v, c, k);
+                if (it != v.size() && !mhe_less_k(c, v[it], k)) {   // !comp(*it, c) holds at lower_bound
+                    ++coll;
+                    continue;
+                }
+                EntU e = c;
+                for (int g = 0; g < k; ++g) e.s[g] = us.ext_st[(uint64_t)x * k + g];
+                e.len = us.ext_len[x];
+                e.mersize = 0;       // MatchHashEntry::operator= zeroes m_mersize; the offset is kept
+                it = lower_bound_k(v, e, k);
+                v.insert(v.begin() + (ptrdiff_t)it, e);
+            }
+        }
+        collisions += coll;
+    });
+    host_time(ctx, "host_replay", now_ms() - t0);
+    starts.clear();
+    lengths.clear();
+    for (uint32_t q = 0; q < TABLE_SIZE; ++q)
+        for (const EntU &e : table[q]) {
+            for (int g = 0; g < k; ++g) starts.push_back(e.s[g]);
+            lengths.push_back(e.len);
+        }
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->seed_tuples = (int64_t)S;
+        st->collisions = collisions.load();
+        st->matches = (int64_t)lengths.size();
+        st->repeat_runs = live.restarts;
+        st->walks = (int64_t)S;
+    }
+    return MH_OK;
+}
+
 }  // namespace mh
 
 using namespace mh;
@@ -646,6 +733,40 @@ int mh_find_matches_pairwise_list(mh_context *ctx, const mh_sml *const *smls, in
     *out = l;
     return MH_OK;
 }
+
+struct mh_match_k_list {
+    int k = 0;
+    std::vector<int64_t> starts;
+    std::vector<uint64_t> lengths;
+};
+
+int mh_find_matches_unique(mh_context *ctx, const mh_sml *const *smls, int k, mh_match_k_list **out,
+                           mh_match_stats *stats)
+{
+    MH_CHECK(ctx && smls && out, MH_ERR_INVALID, "null argument");
+    MH_HIP(hipSetDevice(ctx->device));
+    mh_match_k_list *l = new mh_match_k_list();
+    int rc = find_matches_unique(ctx, smls, k, l->starts, l->lengths, stats);
+    if (rc != MH_OK) {
+        delete l;
+        return rc;
+    }
+    l->k = k;
+    *out = l;
+    return MH_OK;
+}
+
+uint64_t mh_match_k_list_size(const mh_match_k_list *l) { return l ? l->lengths.size() : 0; }
+int mh_match_k_list_genomes(const mh_match_k_list *l) { return l ? l->k : 0; }
+const int64_t *mh_match_k_list_starts(const mh_match_k_list *l)
+{
+    return l && !l->starts.empty() ? l->starts.data() : nullptr;
+}
+const uint64_t *mh_match_k_list_lengths(const mh_match_k_list *l)
+{
+    return l && !l->lengths.empty() ? l->lengths.data() : nullptr;
+}
+void mh_match_k_list_free(mh_match_k_list *l) { delete l; }
 
 uint64_t mh_match_list_size(const mh_match_list *l) { return l ? l->v.size() : 0; }
 const mh_match *mh_match_list_data(const mh_match_list *l) { return l && !l->v.empty() ? l->v.data() : nullptr; }

@@ -122,6 +122,8 @@ struct mh_context {
     mh::DevBuf multi_scratch[11];  // the seeds of every genome pair, merged and filtered
     mh::DevBuf heavy_buf, live_buf; // masked mers above the repeat limit; live ranges of a join
     mh::DevBuf run_scratch[4];      // odd solid seeds extended by runs
+    mh::DevBuf uq_scratch[8];       // UniqueMatchFinder: merged lists, runs, seeds
+    mh::DevBuf uq_rc[16];           // UniqueMatchFinder: packed reverse complements
     uint64_t last_walks = 0, last_long_walks = 0;   // of the last seed extension
     // contexts (own streams and scratch) that run independent searches
     // concurrently for this one, created on first use
@@ -304,6 +306,7 @@ struct PlanLists {
 // (ascending, disjoint), and the number of restarts.
 struct LiveRanges {
     std::vector<uint64_t> lo[PLAN_MAXK], hi[PLAN_MAXK];
+    std::vector<uint32_t> ep[PLAN_MAXK];   // the search (restarts before it) that reads each range
     int64_t restarts = 0;
 };
 // Plans SearchRange's repeat-limit restarts (LM/MatchFinder.cpp:166-340) given
@@ -315,6 +318,19 @@ std::vector<uint64_t> heavy_mers_host(const PlanLists &L);
 // are any, the keys come to the host and the restarts are planned.  A list
 // with no restart reads [0, n).
 int search_live_ranges(mh_context *ctx, const mh_sml *const *smls, int k, LiveRanges *out);
+
+// UniqueMatchFinder's seeds of k genomes (2..16), in HashMatch order: per
+// seed the starts HashMatch sets (genome g at [s * k + g], 0 = absent), the
+// extension's starts and length, and the offset CalculateOffset gives with
+// the seed length.  match.hip.
+struct UniqueSeeds {
+    int k = 0;
+    uint64_t n = 0;
+    std::vector<int32_t> seed_st, ext_st;
+    std::vector<uint32_t> ext_len;
+    std::vector<int64_t> offset;
+};
+int unique_seeds(mh_context *ctx, const mh_sml *const *smls, int k, const LiveRanges &live, UniqueSeeds *out);
 
 // host-side seed helpers (LM/SeedMasks.h)
 int64_t get_seed(int weight, int rank);

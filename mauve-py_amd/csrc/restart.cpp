@@ -210,6 +210,7 @@ int plan_restarts(const PlanLists &L, const std::vector<uint64_t> &heavy, LiveRa
     for (int g = 0; g < k; ++g) {
         out->lo[g].clear();
         out->hi[g].clear();
+        out->ep[g].clear();
     }
     out->restarts = 0;
     Planner P(L);
@@ -272,6 +273,7 @@ int plan_restarts(const PlanLists &L, const std::vector<uint64_t> &heavy, LiveRa
             if (P.sp[g] < E) {
                 out->lo[g].push_back(P.sp[g]);
                 out->hi[g].push_back(E);
+                out->ep[g].push_back((uint32_t)out->restarts);
             }
         }
         // LM/MatchFinder.cpp:253-276
@@ -300,6 +302,7 @@ int plan_restarts(const PlanLists &L, const std::vector<uint64_t> &heavy, LiveRa
         if (P.sp[g] < L.n[g]) {
             out->lo[g].push_back(P.sp[g]);
             out->hi[g].push_back(L.n[g]);
+            out->ep[g].push_back((uint32_t)out->restarts);
         }
     return MH_OK;
 }

@@ -43,6 +43,8 @@ EXPORTS = [
     "mh_dp_align", "mh_dp_align_device", "mh_dp_align_device_noalloc", "mh_dp_noalloc_scratch_bytes",
     "mh_device_alloc", "mh_device_free", "mh_memcpy_h2d", "mh_memcpy_d2h",
     "mh_ascii_to_codes_device", "mh_chain_gap_jobs", "mh_find_matches_pairwise_list",
+    "mh_find_matches_unique", "mh_match_k_list_size", "mh_match_k_list_genomes", "mh_match_k_list_starts",
+    "mh_match_k_list_lengths", "mh_match_k_list_free",
     "mh_match_list_size", "mh_match_list_data", "mh_match_list_free", "mh_match_list_seq_count",
     "mh_match_list_seq_file", "mh_match_list_seq_length", "mh_match_list_write", "mh_match_list_read",
     "mh_index_from_xmfa", "mh_index_from_xmfa_file", "mh_index_fix_zero", "mh_index_fill_gaps", "mh_index_smooth_edges", "mh_xmfa_write",
@@ -169,6 +171,12 @@ def load_library(path=LIB_PATH):
         "mh_ascii_to_codes_device": (C.c_int, [vp, vp, u64, vp]),
         "mh_find_matches_pairwise_list": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(vp),
                                                     C.POINTER(MatchStats)]),
+        "mh_find_matches_unique": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(vp), C.POINTER(MatchStats)]),
+        "mh_match_k_list_size": (u64, [vp]),
+        "mh_match_k_list_genomes": (C.c_int, [vp]),
+        "mh_match_k_list_starts": (vp, [vp]),
+        "mh_match_k_list_lengths": (vp, [vp]),
+        "mh_match_k_list_free": (None, [vp]),
         "mh_match_list_size": (u64, [vp]),
         "mh_match_list_data": (vp, [vp]),
         "mh_match_list_free": (None, [vp]),
@@ -618,6 +626,26 @@ def find_matches_raw(ctx, smls):
     finally:
         ctx.L.mh_match_list_free(h)
     return out, st
+
+
+def find_matches_unique(ctx, smls):
+    """UniqueMatchFinder::FindMatches over 2 to 16 sorted mer lists:
+    (starts (n, k) int64, lengths uint64, MatchStats)."""
+    k = len(smls)
+    arr = (C.c_void_p * k)(*[s.h.value for s in smls])
+    st = MatchStats()
+    h = C.c_void_p()
+    _check(ctx.L.mh_find_matches_unique(ctx.h, arr, k, C.byref(h), C.byref(st)))
+    try:
+        n = ctx.L.mh_match_k_list_size(h)
+        starts = np.zeros((n, k), dtype=np.int64)
+        lengths = np.zeros(n, dtype=np.uint64)
+        if n:
+            C.memmove(starts.ctypes.data, ctx.L.mh_match_k_list_starts(h), n * k * 8)
+            C.memmove(lengths.ctypes.data, ctx.L.mh_match_k_list_lengths(h), n * 8)
+    finally:
+        ctx.L.mh_match_k_list_free(h)
+    return starts, lengths, st
 
 
 def search_live_ranges(keys, lens, seed):
