@@ -241,6 +241,7 @@ def main():
 
     if not args.whole_only:
         out["multi_genome"] = multi_leg(ctx, ref, seed, args.steps, args.warmup, not args.no_cpu_baseline)
+        out["multi_genome_unique"] = unique_leg(ctx, args.steps, args.warmup, not args.no_cpu_baseline)
 
     # the whole path (stand-in LCB chain, recursive anchoring, refinement
     # passes, XMFA and index LUT) is the headline
@@ -345,6 +346,78 @@ def multi_leg(ctx, ref, seed, steps, warmup, cpu):
         log(f"[bench] CPU oracle four-genome matching {cs:.2f}s")
     for d in dev:
         ctx.free(d)
+    return res
+
+
+UNIQUE_CPU_LENGTH = 2_000_000
+
+
+def unique_leg(ctx, steps, warmup, cpu):
+    """Seed matching over eight 5 Mb genomes (45% GC) derived from one ancestor
+    along a random binary tree (1-5% SNPs, 0.2% indels, two inversions and a
+    2-20 kb deletion per branch): UniqueMatchFinder, the finder
+    progressiveMauve runs for more than four genomes, with the default seed of
+    weight 19 (rank 0, spaced).  Genomes resident in HBM; a step builds the
+    eight sorted mer lists and finds the matches.  The CPU baseline is the
+    oracle on the same generator at UNIQUE_CPU_LENGTH bases, whose list is
+    compared with the GPU's."""
+    import mauvehip
+    from mauvehip import synth
+    fam = synth.genome_family()
+    seed = mauvehip.get_seed(19, 0)
+    total = sum(len(q) for q in fam)
+
+    def run(seqs, dev):
+        smls = [mauvehip.SortedMerList.from_device(ctx, d, len(q), seed) for d, q in zip(dev, seqs)]
+        try:
+            return mauvehip.find_matches_unique(ctx, smls)
+        finally:
+            for x in smls:
+                x.free()
+
+    def upload(seqs):
+        dev = []
+        for q in seqs:
+            d = ctx.alloc(len(q) + 16)
+            ctx.h2d(d, q)
+            dev.append(d)
+        return dev
+
+    dev = upload(fam)
+    for _ in range(max(1, warmup)):
+        st_, ln_, stats = run(fam, dev)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st_, ln_, stats = run(fam, dev)
+        log(f"[bench] unique-matching step {k + 1}/{steps} done")
+    ctx.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    for d in dev:
+        ctx.free(d)
+    res = {"genomes": len(fam), "genome_bp": [len(q) for q in fam], "seed": hex(seed),
+           "value": round(total / 1e6 / el, 3), "unit": "Mb/s", "ms_per_step": round(1e3 * el, 3),
+           "seed_tuples": int(stats.seed_tuples), "matches": int(stats.matches),
+           "collisions": int(stats.collisions), "repeat_runs": int(stats.repeat_runs)}
+    log(f"[bench] eight-genome unique matching {res['value']:.1f} Mb/s, {res['ms_per_step']:.1f} ms/step")
+    if cpu:
+        from oracle import pyoracle as po
+        small = synth.genome_family(length=UNIQUE_CPU_LENGTH)
+        dev = upload(small)
+        gs, gl, _ = run(small, dev)
+        for d in dev:
+            ctx.free(d)
+        t0 = time.perf_counter()
+        os_, ol, _ = po.find_matches_unique([q.tobytes() for q in small], seed)
+        cs = time.perf_counter() - t0
+        stot = sum(len(q) for q in small)
+        res["cpu_baseline"] = {"value": round(stot / 1e6 / cs, 4), "unit": "Mb/s", "cores": 1, "kind": "port",
+                               "seconds": round(cs, 2),
+                               "sample": f"the same generator with a {UNIQUE_CPU_LENGTH // 1_000_000} Mb ancestor "
+                                         f"({stot / 1e6:.1f} Mb in all), oracle sorted mer lists and "
+                                         f"UniqueMatchFinder, single thread",
+                               "matches_identical": bool(np.array_equal(gs, os_) and np.array_equal(gl, ol))}
+        log(f"[bench] CPU oracle unique matching {cs:.2f}s")
     return res
 
 

@@ -23,6 +23,7 @@
 // (restart.cpp) and the join reads only the index ranges the reference's
 // search reads (live ranges).
 #include "extend.hpp"
+#include "mhe.hpp"
 #include "mh_internal.hpp"
 
 #include <algorithm>
@@ -1882,6 +1883,119 @@ __global__ void uq_seed_kernel(const uint64_t *__restrict__ key, const uint32_t 
     ext_len[s] = (uint32_t)len;
 }
 
+// ---- AddHashEntry of the unique seeds, one bucket per thread --------------
+// AddHashEntry (LM/MemHash.cpp:209-251) keeps each of the 40000 buckets a
+// vector sorted by MheCompare and inserts with std::lower_bound; buckets never
+// interact, so each is replayed literally by one thread, its seeds in HashMatch
+// order (a stable sort by bucket): the seed is looked up, a collision when the
+// entry at the lower bound compares equal, else the seed's extension (offset
+// kept, mersize 0) goes in at its own lower bound.  The bucket's vector holds
+// seed indices (an entry is its seed's extension).
+
+struct UqSeeds {
+    const int32_t *sst, *est;   // seed starts, extension starts (k a seed)
+    const uint32_t *elen;
+    const int64_t *off;
+    int k, sl;
+};
+
+__device__ __forceinline__ void uq_entry(const UqSeeds &q, uint32_t x, bool ext, EntU &e)
+{
+    const int32_t *s = (ext ? q.est : q.sst) + (uint64_t)x * q.k;
+    e.first = -1;
+    for (int g = 0; g < EXT_MAXK; ++g) {
+        e.s[g] = g < q.k ? s[g] : 0;
+        if (e.first < 0 && e.s[g] != 0) e.first = g;
+    }
+    e.len = ext ? q.elen[x] : (uint64_t)q.sl;
+    e.mersize = ext ? 0 : (uint64_t)q.sl;
+    e.off = q.off[x];
+}
+
+__global__ void uq_bucket_kernel(const int64_t *__restrict__ off, uint64_t ns, uint64_t *__restrict__ key,
+                                 uint32_t *__restrict__ val, uint32_t *__restrict__ cnt)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ns) return;
+    const int64_t m = off[x] % 40000;
+    const uint32_t b = (uint32_t)(m < 0 ? m + 40000 : m);
+    key[x] = b;
+    val[x] = (uint32_t)x;
+    atomicAdd(&cnt[b], 1u);
+}
+
+// One wavefront per bucket: the first lane walks std::lower_bound's probes
+// (the comparisons are the reference's, one at a time); the insertion's shift
+// of the vector's tail runs across the wave, 64 entries a step from the end
+// (every lane reads its entry before any lane writes).
+__global__ __launch_bounds__(64) void uq_replay_kernel(UqSeeds q, const uint32_t *__restrict__ order,
+                                                       const uint32_t *__restrict__ bstart, uint32_t *__restrict__ vec,
+                                                       uint32_t *__restrict__ nent,
+                                                       unsigned long long *__restrict__ coll)
+{
+    const uint32_t b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint32_t y0 = bstart[b], y1 = bstart[b + 1];
+    uint32_t *v = vec + y0;
+    uint32_t ne = 0;
+    unsigned long long c = 0;
+    for (uint32_t y = y0; y < y1; ++y) {
+        const uint32_t x = order[y];
+        uint32_t pos = 0xFFFFFFFFu;     // insertion point, or none (a collision)
+        if (lane == 0) {
+            EntU cand;
+            uq_entry(q, x, false, cand);
+            auto at = [&](size_t i) {
+                EntU e;
+                uq_entry(q, v[i], true, e);
+                return e;
+            };
+            size_t it = lower_bound_at(ne, cand, q.k, at);
+            if (it != ne && !mhe_less_k(cand, at(it), q.k)) {   // !comp(*it, cand) holds at lower_bound
+                ++c;
+            } else {
+                EntU e;
+                uq_entry(q, x, true, e);
+                pos = (uint32_t)lower_bound_at(ne, e, q.k, at);
+            }
+        }
+        pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)pos);
+        if (pos == 0xFFFFFFFFu) continue;
+        // shift [pos, ne) up by one, 64 entries a step from the end
+        for (int64_t j = (int64_t)ne; j > (int64_t)pos; j -= 64) {
+            const int64_t i = j - 64 + lane;
+            uint32_t t = 0;
+            const bool in = i >= (int64_t)pos;
+            if (in) t = v[i];
+            __syncthreads();
+            if (in) v[i + 1] = t;
+            __syncthreads();
+        }
+        if (lane == 0) v[pos] = x;
+        __syncthreads();
+        ++ne;
+    }
+    if (lane == 0) {
+        nent[b] = ne;
+        if (c) atomicAdd(coll, c);
+    }
+}
+
+// the list in bucket order: bucket b's entries at outpos[b] ...
+__global__ void uq_emit_kernel(UqSeeds q, const uint32_t *__restrict__ bstart, const uint32_t *__restrict__ vec,
+                               const uint32_t *__restrict__ nent, const uint64_t *__restrict__ outpos,
+                               int64_t *__restrict__ starts, uint64_t *__restrict__ lengths)
+{
+    const uint32_t b = blockIdx.x;
+    const uint32_t ne = nent[b];
+    for (uint32_t i = threadIdx.x; i < ne; i += blockDim.x) {
+        const uint32_t x = vec[bstart[b] + i];
+        const uint64_t o = outpos[b] + i;
+        for (int g = 0; g < q.k; ++g) starts[o * q.k + g] = q.est[(uint64_t)x * q.k + g];
+        lengths[o] = q.elen[x];
+    }
+}
+
 int unique_seeds(mh_context *ctx, const mh_sml *const *smls, int k, const LiveRanges &live, UniqueSeeds *out)
 {
     MH_CHECK(k >= 2 && k <= EXT_MAXK && k <= PLAN_MAXK, MH_ERR_UNSUPPORTED, "UniqueMatchFinder takes 2 to 16 genomes");
@@ -1977,18 +2091,69 @@ int unique_seeds(mh_context *ctx, const mh_sml *const *smls, int k, const LiveRa
                            lead, S, ls, env, k, solid_odd ? 1 : 0, d_sst, d_est, d_len, d_off, d_cnt);
         MH_HIP(hipGetLastError());
     }
-    out->seed_st.resize(S * k);
-    out->ext_st.resize(S * k);
-    out->ext_len.resize(S);
-    out->offset.resize(S);
     uint32_t err = 0;
-    MH_HIP(hipMemcpyAsync(out->seed_st.data(), d_sst, S * k * 4, hipMemcpyDeviceToHost, ctx->stream));
-    MH_HIP(hipMemcpyAsync(out->ext_st.data(), d_est, S * k * 4, hipMemcpyDeviceToHost, ctx->stream));
-    MH_HIP(hipMemcpyAsync(out->ext_len.data(), d_len, S * 4, hipMemcpyDeviceToHost, ctx->stream));
-    MH_HIP(hipMemcpyAsync(out->offset.data(), d_off, S * 8, hipMemcpyDeviceToHost, ctx->stream));
     MH_HIP(hipMemcpyAsync(&err, d_cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
     MH_HIP(hipStreamSynchronize(ctx->stream));
     MH_CHECK(err == 0, MH_ERR_UNSUPPORTED, "seed extension left the sorted mer lists");
+    // AddHashEntry, bucket by bucket: seeds ordered by bucket (stable), bucket
+    // starts, one thread per bucket, the list in bucket order
+    const double t0 = now_ms();
+    constexpr uint32_t NB = 40000;
+    if ((rc = b[0].ensure(S * 8 * 2 + 64)) != MH_OK) return rc;
+    if ((rc = b[2].ensure(S * 4 * 2 + (NB + 1) * 4 * 3 + NB * 8 + 256)) != MH_OK) return rc;
+    uint64_t *bk0 = b[0].as<uint64_t>(), *bk1 = bk0 + S;
+    uint32_t *bv0 = b[2].as<uint32_t>(), *bv1 = bv0 + S;
+    uint32_t *bcnt = bv1 + S, *bst = bcnt + (NB + 1), *nent = bst + (NB + 1);
+    uint64_t *outpos = reinterpret_cast<uint64_t *>((reinterpret_cast<uintptr_t>(nent + NB + 1) + 7) & ~(uintptr_t)7);
+    MH_HIP(hipMemsetAsync(bcnt, 0, (NB + 1) * 4, ctx->stream));
+    hipLaunchKernelGGL(uq_bucket_kernel, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, ctx->stream, d_off, S,
+                       bk0, bv0, bcnt);
+    uint64_t *ok;
+    uint32_t *order;
+    if ((rc = radix_sort_kv(ctx, bk0, bv0, bk1, bv1, S, 16, &ok, &order)) != MH_OK) return rc;
+    std::vector<uint32_t> hc(NB + 1, 0), hs(NB + 1, 0);
+    MH_HIP(hipMemcpyAsync(hc.data(), bcnt, NB * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint32_t q = 0; q < NB; ++q) hs[q + 1] = hs[q] + hc[q];
+    MH_HIP(hipMemcpyAsync(bst, hs.data(), (NB + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    uint32_t *vec = order == bv0 ? bv1 : bv0;     // the other value buffer: S slots
+    unsigned long long *d_coll = reinterpret_cast<unsigned long long *>(ok == bk0 ? bk1 : bk0);
+    MH_HIP(hipMemsetAsync(d_coll, 0, 8, ctx->stream));
+    UqSeeds qs;
+    qs.sst = d_sst;
+    qs.est = d_est;
+    qs.elen = d_len;
+    qs.off = d_off;
+    qs.k = k;
+    qs.sl = sl;
+    {
+        TimedLaunch tl(ctx, "partition");
+        hipLaunchKernelGGL(uq_replay_kernel, dim3(NB), dim3(64), 0, ctx->stream, qs, order, bst, vec, nent, d_coll);
+        MH_HIP(hipGetLastError());
+    }
+    std::vector<uint32_t> hn(NB);
+    unsigned long long coll = 0;
+    MH_HIP(hipMemcpyAsync(hn.data(), nent, NB * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipMemcpyAsync(&coll, d_coll, 8, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<uint64_t> hp(NB + 1, 0);
+    for (uint32_t q = 0; q < NB; ++q) hp[q + 1] = hp[q] + hn[q];
+    const uint64_t nm = hp[NB];
+    MH_HIP(hipMemcpyAsync(outpos, hp.data(), NB * 8, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = b[1].ensure(nm * (uint64_t)k * 8 + nm * 8 + 64)) != MH_OK) return rc;
+    int64_t *d_ost = b[1].as<int64_t>();
+    uint64_t *d_olen = reinterpret_cast<uint64_t *>(d_ost + nm * k);
+    hipLaunchKernelGGL(uq_emit_kernel, dim3(NB), dim3(64), 0, ctx->stream, qs, bst, vec, nent, outpos, d_ost, d_olen);
+    MH_HIP(hipGetLastError());
+    out->starts.resize(nm * k);
+    out->lengths.resize(nm);
+    if (nm) {
+        MH_HIP(hipMemcpyAsync(out->starts.data(), d_ost, nm * k * 8, hipMemcpyDeviceToHost, ctx->stream));
+        MH_HIP(hipMemcpyAsync(out->lengths.data(), d_olen, nm * 8, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    out->collisions = (int64_t)coll;
+    host_time(ctx, "host_replay", now_ms() - t0);
     return MH_OK;
 }
 

@@ -12,6 +12,7 @@
 // they are replayed in parallel host threads, each in the seeds' HashMatch
 // order.  Extensions were computed on the GPU.
 #include "mh_internal.hpp"
+#include "mhe.hpp"
 #include <algorithm>
 #include <atomic>
 #include <thread>
@@ -82,86 +83,9 @@ inline size_t lower_bound_ent(const std::vector<Ent> &v, const Ent &q)
     return first;
 }
 
-// A MatchHashEntry of up to MK genomes (start 0: absent), first = the index
-// of its first present genome (FirstStart); starts of type T.
-template <int MK, class T> struct EntT {
-    T s[MK];
-    uint64_t len;
-    int64_t off;
-    uint64_t mersize;
-    int first;
-};
-using EntK = EntT<4, int64_t>;     // PairwiseMatchFinder, up to four genomes
-using EntU = EntT<16, int32_t>;    // UniqueMatchFinder, up to sixteen
-
-// MatchHashEntry::Contains (a contains b), LM/MatchHashEntry.cpp:164-200
-template <class E> inline bool contains_k(const E &a, const E &b, int k)
-{
-    if (a.off != b.off) return false;
-    int i = b.first;
-    const int64_t diff = (int64_t)b.s[i] - (int64_t)a.s[i];
-    if (a.s[i] == 0) return false;
-    if (diff < 0 || a.len < b.len + (uint64_t)diff) return false;
-    const int64_t diff_rc = (int64_t)b.len - (int64_t)a.len + diff;
-    for (++i; i < k; ++i) {
-        const int64_t di = (int64_t)b.s[i] - (int64_t)a.s[i];
-        if (b.s[i] == 0 && a.s[i] == 0) continue;
-        if (b.s[i] < 0 && diff_rc == di) continue;
-        if (diff != di) return false;
-    }
-    return true;
-}
-
-// strict_start_lessthan_ptr, LM/MatchHashEntry.cpp:48-67
-template <class E> inline bool strict_less_k(const E &a, const E &b, int k)
-{
-    if (a.first == b.first)
-        for (int i = 0; i < k; ++i) {
-            int64_t as = a.s[i], bs = b.s[i];
-            if (as < 0) as = (int64_t)((uint64_t)(-as) + a.len - a.mersize);
-            if (bs < 0) bs = (int64_t)((uint64_t)(-bs) + b.len - b.mersize);
-            if (as != bs) return as < bs;
-        }
-    return a.first < b.first;
-}
-
-// MheCompare, LM/MatchHashEntry.h:121-143
-template <class E> inline bool mhe_less_k(const E &a, const E &b, int k)
-{
-    if (a.first > b.first) return true;
-    if (a.first != b.first) return false;
-    for (int i = a.first; i < k; ++i) {
-        if (a.s[i] == 0 && b.s[i] != 0) return true;
-        if (a.s[i] != 0 && b.s[i] == 0) return false;
-    }
-    if (contains_k(a, b, k) || contains_k(b, a, k)) return false;
-    return strict_less_k(a, b, k);
-}
-
 template <class E> inline size_t lower_bound_k(const std::vector<E> &v, const E &q, int k)
 {
-    size_t first = 0, len = v.size();
-    while (len > 0) {
-        const size_t half = len >> 1, mid = first + half;
-        if (mhe_less_k(v[mid], q, k)) {
-            first = mid + 1;
-            len = len - half - 1;
-        } else {
-            len = half;
-        }
-    }
-    return first;
-}
-
-// CalculateOffset, LM/MatchHashEntry.cpp:141-160 (Length is the seed length
-// when HashMatch calls it)
-template <class E> inline int64_t offset_k(const E &e, int k)
-{
-    int64_t off = 0;
-    const int64_t ref = e.s[e.first];
-    for (int i = e.first + 1; i < k; ++i)
-        if (e.s[i] != 0) off += (int64_t)e.s[i] - ref - (e.s[i] < 0 ? (int64_t)e.len : 0);
-    return off;
+    return lower_bound_at(v.size(), q, k, [&](size_t i) -> const E & { return v[i]; });
 }
 
 // b grown to at least need bytes, its first keep bytes kept.
@@ -466,10 +390,10 @@ int find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, std
 }
 
 // UniqueMatchFinder over 2 to 16 genomes (MA/UniqueMatchFinder.cpp:36-60 with
-// LM/MatchFinder.cpp:172-340 and LM/MemHash.cpp:167-251): the seeds and their
-// extensions from the device (unique_seeds), the insertion into the 40000
-// buckets replayed on the host, buckets in parallel (each bucket's seeds in
-// HashMatch order), the list bucket by bucket (LM/MemHash.h:182-203).
+// LM/MatchFinder.cpp:172-340 and LM/MemHash.cpp:167-251): SearchRange's live
+// ranges, then the seeds, their extensions and the bucket insertion on the
+// device (unique_seeds); the list comes out bucket by bucket
+// (LM/MemHash.h:182-203).
 int find_matches_unique(mh_context *ctx, const mh_sml *const *smls, int k, std::vector<int64_t> &starts,
                         std::vector<uint64_t> &lengths, mh_match_stats *st)
 {
@@ -483,69 +407,15 @@ int find_matches_unique(mh_context *ctx, const mh_sml *const *smls, int k, std::
     if ((rc = search_live_ranges(ctx, smls, k, &live)) != MH_OK) return rc;
     UniqueSeeds us;
     if ((rc = unique_seeds(ctx, smls, k, live, &us)) != MH_OK) return rc;
-    const double t0 = now_ms();
-    const uint64_t S = us.n, sl = (uint64_t)smls[0]->seed_length;
-    std::vector<uint32_t> cnt(TABLE_SIZE + 1, 0), bucket(S);
-    for (uint64_t x = 0; x < S; ++x) {
-        const int64_t m = us.offset[x] % (int64_t)TABLE_SIZE;
-        bucket[x] = (uint32_t)(m < 0 ? m + TABLE_SIZE : m);
-        ++cnt[bucket[x] + 1];
-    }
-    for (uint32_t q = 0; q < TABLE_SIZE; ++q) cnt[q + 1] += cnt[q];
-    std::vector<uint32_t> order(S);
-    {
-        std::vector<uint32_t> at(cnt.begin(), cnt.end() - 1);
-        for (uint64_t x = 0; x < S; ++x) order[at[bucket[x]]++] = (uint32_t)x;
-    }
-    std::vector<std::vector<EntU>> table(TABLE_SIZE);
-    std::atomic<int64_t> collisions{0};
-    const uint32_t per = 64;
-    host_parallel((TABLE_SIZE + per - 1) / per, [&](size_t t) {
-        int64_t coll = 0;
-        for (uint32_t q = (uint32_t)t * per; q < std::min<uint32_t>(TABLE_SIZE, (uint32_t)(t + 1) * per); ++q) {
-            std::vector<EntU> &v = table[q];
-            for (uint32_t y = cnt[q]; y < cnt[q + 1]; ++y) {
-                const uint32_t x = order[y];
-                EntU c;
-                std::memset(&c, 0, sizeof(c));
-                c.first = -1;
-                for (int g = 0; g < k; ++g) {
-                    c.s[g] = us.seed_st[(uint64_t)x * k + g];
-                    if (c.first < 0 && c.s[g] != 0) c.first = g;
-                }
-                c.len = sl;
-                c.mersize = sl;
-                c.off = us.offset[x];
-                size_t it = lower_bound_k(v, c, k);
-                if (it != v.size() && !mhe_less_k(c, v[it], k)) {   // !comp(*it, c) holds at lower_bound
-                    ++coll;
-                    continue;
-                }
-                EntU e = c;
-                for (int g = 0; g < k; ++g) e.s[g] = us.ext_st[(uint64_t)x * k + g];
-                e.len = us.ext_len[x];
-                e.mersize = 0;       // MatchHashEntry::operator= zeroes m_mersize; the offset is kept
-                it = lower_bound_k(v, e, k);
-                v.insert(v.begin() + (ptrdiff_t)it, e);
-            }
-        }
-        collisions += coll;
-    });
-    host_time(ctx, "host_replay", now_ms() - t0);
-    starts.clear();
-    lengths.clear();
-    for (uint32_t q = 0; q < TABLE_SIZE; ++q)
-        for (const EntU &e : table[q]) {
-            for (int g = 0; g < k; ++g) starts.push_back(e.s[g]);
-            lengths.push_back(e.len);
-        }
+    starts.swap(us.starts);
+    lengths.swap(us.lengths);
     if (st) {
         std::memset(st, 0, sizeof(*st));
-        st->seed_tuples = (int64_t)S;
-        st->collisions = collisions.load();
+        st->seed_tuples = (int64_t)us.n;
+        st->collisions = us.collisions;
         st->matches = (int64_t)lengths.size();
         st->repeat_runs = live.restarts;
-        st->walks = (int64_t)S;
+        st->walks = (int64_t)us.n;
     }
     return MH_OK;
 }

@@ -319,16 +319,16 @@ std::vector<uint64_t> heavy_mers_host(const PlanLists &L);
 // with no restart reads [0, n).
 int search_live_ranges(mh_context *ctx, const mh_sml *const *smls, int k, LiveRanges *out);
 
-// UniqueMatchFinder's seeds of k genomes (2..16), in HashMatch order: per
-// seed the starts HashMatch sets (genome g at [s * k + g], 0 = absent), the
-// extension's starts and length, and the offset CalculateOffset gives with
-// the seed length.  match.hip.
+// UniqueMatchFinder over k genomes (2..16) on the device: its seeds (n, in
+// HashMatch order), the collisions AddHashEntry counts, and the match list in
+// GetMatchList order (starts k a match, 1-based, negative = reverse, 0 =
+// absent; lengths).  match.hip.
 struct UniqueSeeds {
     int k = 0;
     uint64_t n = 0;
-    std::vector<int32_t> seed_st, ext_st;
-    std::vector<uint32_t> ext_len;
-    std::vector<int64_t> offset;
+    int64_t collisions = 0;
+    std::vector<int64_t> starts;
+    std::vector<uint64_t> lengths;
 };
 int unique_seeds(mh_context *ctx, const mh_sml *const *smls, int k, const LiveRanges &live, UniqueSeeds *out);
 

@@ -92,6 +92,34 @@ def pair_iid(length=200_000, seed_ref=101, seed_mut=102):
     return ref, mut
 
 
+def _branch(rng, seq):
+    """One branch of the multi-genome tree: 1-5% SNPs, indels at 0.2% per base
+    (lengths 1-20, half insertions), two inversions of 10-50 kb and one
+    deletion of 2-20 kb."""
+    out = _mutate_snps(rng, seq, rng.uniform(0.01, 0.05))
+    ev = np.nonzero(rng.random(len(out)) < 0.002)[0]
+    out = _apply_indels(rng, out, ev, rng.integers(1, 21, size=len(ev)), rng.random(len(ev)) < 0.5)
+    for _ in range(2):
+        ln = int(rng.integers(10_000, 50_001))
+        out = _invert(out, int(rng.integers(0, max(1, len(out) - ln))), ln)
+    ln = int(rng.integers(2_000, 20_001))
+    at = int(rng.integers(0, max(1, len(out) - ln)))
+    return np.concatenate([out[:at], out[at + ln:]])
+
+
+def genome_family(length=5_000_000, n=8, seed=501, gc=0.45):
+    """n genomes derived from one random ancestor (length bases, GC fraction
+    gc) along a random binary tree: a random leaf is split into two children
+    until there are n leaves, each child one branch of mutations (_branch).
+    Returns the leaves (ASCII uint8 arrays) in the order they were made."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    leaves = [random_bases(rng, length, gc)]
+    while len(leaves) < n:
+        parent = leaves.pop(int(rng.integers(0, len(leaves))))
+        leaves += [_branch(rng, parent), _branch(rng, parent)]
+    return leaves
+
+
 def _orf(rng, n_codons):
     body = rng.integers(0, len(SENSE), size=n_codons - 2)
     cods = ["ATG"] + [SENSE[i] for i in body] + [STOPS[rng.integers(0, 3)]]

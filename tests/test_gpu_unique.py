@@ -102,3 +102,13 @@ def test_unique_sixteen_genomes(ctx):
     seqs = _family(rng, 12000, 16, snp=0.01)
     st = _check(ctx, seqs, (1 << 15) - 1)
     assert st.matches > 100
+
+
+def test_unique_eight_genome_tree_family(ctx):
+    """The multi-genome workload's generator (eight genomes along a random
+    binary tree: SNPs, indels, inversions, deletions) with a 300 kb ancestor
+    and the default seed of weight 19 (rank 0, spaced)."""
+    from mauvehip import synth
+    fam = [q.tobytes() for q in synth.genome_family(length=300_000)]
+    st = _check(ctx, fam, mauvehip.get_seed(19, 0))
+    assert st.matches > 10000
