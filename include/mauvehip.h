@@ -311,6 +311,12 @@ int mh_dp_align_banded_device(mh_context *ctx, const uint8_t *d_a, const uint8_t
 #define MH_REGION_STRIPES 1u
 #define MH_REGION_WINDOW 2u
 uint32_t mh_dp_band_region(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *kind, uint32_t *width);
+/* The same with flags: MH_REGION_RERUN gives the region of the fallback's
+ * second pass (band 4W), which takes 64-lane stripes only: a rerun pass holds
+ * few jobs and its time is its longest job's. */
+#define MH_REGION_RERUN 1u
+uint32_t mh_dp_band_region_ex(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t flags, uint32_t *kind,
+                              uint32_t *width);
 /* DP cells the scheduler evaluates for each (a_len, b_len, band) -- the
  * region's cells, 64C per row for a window -- for cost models (host only). */
 int mh_dp_band_cells(const uint32_t *a_len, const uint32_t *b_len, const uint32_t *band, uint64_t n,

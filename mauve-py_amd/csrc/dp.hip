@@ -854,8 +854,12 @@ __device__ __forceinline__ void traceback_wave_job(const DevJob *__restrict__ jo
     const int lane = threadIdx.x;
     const DevJob jb = jobs[x];
     const int la = (int)jb.la, lb = (int)jb.lb;
-    const int cols = (int)jb.cols, sw = 64 * cols, lane_bytes = cols / 2;
-    const int K = TBC_SEG / lane_bytes;      // lanes per segment: 128 columns
+    // G lanes a step: 64, or 16 for the row-packed layout
+    const bool rows16 = jb.cols > GROUP_TAG;
+    const int G = rows16 ? 16 : 64;
+    const int cols = rows16 ? (int)(jb.cols - GROUP_TAG) : (int)jb.cols, sw = G * cols, lane_bytes = cols / 2;
+    const int K = TBC_SEG / lane_bytes < G ? TBC_SEG / lane_bytes : G;   // lanes a cached step holds
+    const int P = K * lane_bytes / 16;                                   // 16-byte parts a cached step
     const int kalign = 16 / lane_bytes;      // segment starts on a 16-byte boundary
     const uint8_t *tj = tb + jb.tb_off;
     uint8_t *slot = paths + jb.path_off;
@@ -864,7 +868,7 @@ __device__ __forceinline__ void traceback_wave_job(const DevJob *__restrict__ jo
     for (int s = 0; s < cur; ++s) {
         int a0, a1;
         stripe_rows(jb.la, jb.lb, jb.band, sw, s, a0, a1);
-        if (a1 >= a0) base += (uint64_t)(a1 - a0 + 64) * 64 * lane_bytes;
+        if (a1 >= a0) base += (uint64_t)(a1 - a0 + G) * G * lane_bytes;
     }
     stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
     int c_s = -1, c_tlo = 0, c_thi = -1, c_klo = 0;   // cached stripe, steps and first lane
@@ -878,7 +882,47 @@ __device__ __forceinline__ void traceback_wave_job(const DevJob *__restrict__ jo
     int type = outs[x].edge;
     bool ok = true;
     int64_t maxdev = 0;
+    const int cshift = cols == 4 ? 2 : (cols == 8 ? 3 : 4);
     for (;;) {
+        // A run of cells that keep the walk's state (M from M along the
+        // diagonal, D from D up the column, I from I along the row): lane l
+        // reads the l-th cell of the run from the cached tile, and the walk
+        // moves over all but the last good cell at once (the step below takes
+        // that one, so the run never leaves the stripe or the tile).
+        if (i > 0 && j > 0 && cur == c_s) {
+            const int ci = i - (type == 'I' ? 0 : lane);
+            const int off = k * cols + c - (type == 'D' ? 0 : lane);
+            bool good = ci >= r0 && ci <= r1 && ci >= 1 && off >= 0;
+            if (good) {
+                const int kl = off >> cshift, cl = off & (cols - 1);
+                const int t = ci - r0 + kl;
+                good = t >= c_tlo && t <= c_thi && kl >= c_klo && kl < c_klo + K;
+                if (good) {
+                    const uint32_t nb =
+                        ((uint32_t)cache[(t - c_tlo) * TBC_SEG + (kl - c_klo) * lane_bytes + (cl >> 1)] >>
+                         (4 * (cl & 1))) & 15u;
+                    good = type == 'M' ? (nb & 3u) == 0u : (type == 'D' ? (nb & 4u) == 0u : (nb & 8u) == 0u);
+                }
+            }
+            const uint64_t bad = __ballot(!good);
+            const int n = bad ? (int)__builtin_ctzll(bad) : 64;
+            if (n >= 2) {
+                const int adv = n - 1;
+                if (lane < adv) slot[pos - 1 - lane] = (uint8_t)type;
+                pos -= adv;
+                const int64_t st = type == 'M' ? (int64_t)la - lb : (type == 'D' ? -(int64_t)lb : (int64_t)la);
+                const int64_t d1 = dnum + st, d2 = dnum + (int64_t)adv * st;
+                maxdev = max(maxdev, max(d1 < 0 ? -d1 : d1, d2 < 0 ? -d2 : d2));
+                dnum = d2;
+                if (type != 'I') i -= adv;
+                if (type != 'D') {
+                    j -= adv;
+                    const int o = k * cols + c - adv;
+                    k = o >> cshift;
+                    c = o & (cols - 1);
+                }
+            }
+        }
         --pos;
         if (lane == 0) slot[pos] = (uint8_t)type;
         uint32_t nib = 0;
@@ -893,9 +937,9 @@ __device__ __forceinline__ void traceback_wave_job(const DevJob *__restrict__ jo
                 c_klo = kl < 0 ? 0 : kl;
                 const int nseg = c_thi - c_tlo + 1;
                 __syncthreads();
-                for (int e = lane; e < nseg * 4; e += 64) {
-                    const int seg = e >> 2, part = e & 3;
-                    const uint8_t *src = tj + base + ((uint64_t)(c_tlo + seg) * 64 + c_klo) * lane_bytes + part * 16;
+                for (int e = lane; e < nseg * P; e += 64) {
+                    const int seg = e / P, part = e - seg * P;
+                    const uint8_t *src = tj + base + ((uint64_t)(c_tlo + seg) * G + c_klo) * lane_bytes + part * 16;
                     *reinterpret_cast<uint4 *>(cache + seg * TBC_SEG + part * 16) = *reinterpret_cast<const uint4 *>(src);
                 }
                 __syncthreads();
@@ -914,10 +958,10 @@ __device__ __forceinline__ void traceback_wave_job(const DevJob *__restrict__ jo
         if (left && j > 0 && --c < 0) {
             c = cols - 1;
             if (--k < 0) {   // into the previous stripe: step back over its trace block
-                k = 63;
+                k = G - 1;
                 --cur;
                 stripe_rows(jb.la, jb.lb, jb.band, sw, cur, r0, r1);
-                if (r1 >= r0) base -= (uint64_t)(r1 - r0 + 64) * 64 * lane_bytes;
+                if (r1 >= r0) base -= (uint64_t)(r1 - r0 + G) * G * lane_bytes;
             }
         }
         maxdev = max(maxdev, dnum < 0 ? -dnum : dnum);
@@ -1044,22 +1088,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 __global__ __launch_bounds__(64) void dp_tb_wave(const DevJob *__restrict__ jobs, const DpOut *__restrict__ outs,
                                                  const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
                                                  mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev,
-                                                 uint32_t nwave)
+                                                 const uint32_t *__restrict__ idx, uint32_t nwave)
 {
     __shared__ __align__(16) uint8_t cache[TBC_STEPS * TBC_SEG];
-    const uint32_t x = blockIdx.x;
-    if (x >= nwave) return;
-    traceback_wave_job(jobs, outs, tb, paths, res, dev, x, cache);
+    if (blockIdx.x >= nwave) return;
+    traceback_wave_job(jobs, outs, tb, paths, res, dev, idx[blockIdx.x], cache);
 }
 
 __global__ __launch_bounds__(64) void dp_tb_thread(const DevJob *__restrict__ jobs, const DpOut *__restrict__ outs,
                                                    const uint8_t *__restrict__ tb, uint8_t *__restrict__ paths,
                                                    mh_dp_result *__restrict__ res, uint32_t *__restrict__ dev,
-                                                   uint32_t first, uint32_t n)
+                                                   const uint32_t *__restrict__ idx, uint32_t first, uint32_t n)
 {
-    const uint32_t x = first + blockIdx.x * 64 + threadIdx.x;
-    if (x >= n) return;
-    traceback_thread_job(jobs, outs, tb, paths, res, dev, x);
+    const uint32_t t = first + blockIdx.x * 64 + threadIdx.x;
+    if (t >= n) return;
+    traceback_thread_job(jobs, outs, tb, paths, res, dev, idx[t]);
 }
 
 // NUC_SP (MU/nucmx.cpp:8-25) rows packed as biased bytes: row a, column b at
@@ -1087,10 +1130,17 @@ struct DpItem {
     uint64_t tb_bytes;
 };
 
-// Relative cost per computed cell of the three stripe shapes (narrow stripes
-// spend more of each step on the row hand-off); used to pick a banded job's
-// shape, which decides how much of each stripe lies outside the band.
-constexpr double COST_PER_CELL[3] = {1.30, 1.10, 1.00};   // cols 4, 8, 16
+// Relative cost per computed cell of the stripe shapes, used to pick a banded
+// job's shape (which decides how much of each stripe lies outside the band).
+// Measured on MI355X over a whole 100k-gap first pass, where a round's
+// shapes share one forward kernel: per shape alone (a class of B's width at a
+// time) 64-lane stripes of 16 columns evaluate cells fastest and 16-lane
+// stripes slowest (2-4x), but weights taken from those runs pick more
+// 16-column stripes, whose wide kernel then slows the round's other jobs, and
+// lose 6-15% on the whole pass; these weights gave the fastest pass.
+constexpr double COST_PER_CELL[3] = {1.30, 1.10, 1.00};   // 64-lane stripes of cols 4, 8, 16
+constexpr double COST_PER_CELL_G16[2] = {1.30, 1.10};      // 16-lane stripes of cols 4, 8
+
 
 }  // namespace
 
@@ -1125,11 +1175,14 @@ int dp_scratch_init(int device)
 
 uint64_t dp_scratch_bytes() { return DP_SCRATCH_BYTES; }
 
+// MH_DP_COLS forces one shape: 4, 8 or 16 (64-lane stripes of that many
+// columns a lane) or 36, 40 (16-lane stripes of 4, 8 columns, four jobs a
+// wavefront)
 int dp_force_cols()
 {
     if (const char *e = getenv("MH_DP_COLS")) {
         const int c = atoi(e);
-        if (c == 4 || c == 8 || c == 16) return c;
+        if (c == 4 || c == 8 || c == 16 || c == (int)GROUP_TAG + 4 || c == (int)GROUP_TAG + 8) return c;
     }
     return 0;
 }
@@ -1170,25 +1223,35 @@ static const uint8_t *shape_table()
 // Stripe shape of a job: the full DP takes the shape the tuning table names
 // for B's width (stripes of 8 columns a lane where it names the row-packed
 // shape and the job cannot take it); a banded job takes the shape with the
-// lowest estimated cost, ceil(lb/w) stripes of min(la, la*(w+2W)/lb + 1) + 63
-// steps of w cells.  force (4, 8 or 16) overrides.
-uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force)
+// lowest estimated cost: ceil(lb/w) stripes of min(la, la*(w+2W)/lb + 1) +
+// g - 1 steps of w cells (g lanes), times the shape's cost per cell.  force
+// (any MH_DP_COLS shape) overrides.  A rerun pass holds few jobs, whose time
+// is the longest job's: its banded jobs take 64-lane stripes only
+// (wide_only), since a 16-lane stripe runs one job on a quarter of a
+// wavefront.
+uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force, bool wide_only)
 {
     if (force) return (uint32_t)force;
     if (band == 0) {
         const uint32_t t = shape_table()[dp_tune_class(lb)];
         return t ? t : 8u;
     }
+    // 64-lane stripes of 4, 8 or 16 columns a lane, and (unless
+    // MH_DP_BAND_G16=0) 16-lane stripes of 4 or 8 columns a lane, four jobs a
+    // wavefront: narrower stripes, whose rows follow a narrow band closer
     uint32_t best = 16;
     double best_cost = 0;
-    for (int q = 0; q < 3; ++q) {
-        const uint32_t c = 4u << q, sw = 64 * c;
+    const char *g16e = getenv("MH_DP_BAND_G16");
+    const int nshape = wide_only || (g16e && g16e[0] == '0') ? 3 : 5;
+    for (int q = 0; q < nshape; ++q) {
+        const uint32_t g = q < 3 ? 64u : 16u;
+        const uint32_t c = q < 3 ? 4u << q : 4u << (q - 3), sw = g * c;
         const double ns = (double)((lb + sw - 1) / sw);
         double rows = (double)la * (sw + 2.0 * band) / lb + 1.0;
         if (rows > la) rows = la;
-        const double cost = ns * (rows + 63.0) * sw * COST_PER_CELL[q];
+        const double cost = ns * (rows + (g - 1.0)) * sw * (q < 3 ? COST_PER_CELL[q] : COST_PER_CELL_G16[q - 3]);
         if (q == 0 || cost < best_cost) {
-            best = c;
+            best = g == 64 ? c : GROUP_TAG + c;
             best_cost = cost;
         }
     }
@@ -1223,11 +1286,13 @@ uint32_t dp_window_cols(uint32_t la, uint32_t lb, uint32_t band, int kind)
     return c;
 }
 
+uint32_t dp_stripe_width(uint32_t cols) { return cols > GROUP_TAG ? 16 * (cols - GROUP_TAG) : 64 * cols; }
+
 // A band whose region is the whole matrix is the full DP.
 uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols)
 {
     if (band == 0) return 0;
-    const uint32_t sw = 64 * cols;
+    const uint32_t sw = cols > GROUP_TAG ? 16 * (cols - GROUP_TAG) : 64 * cols;
     const uint32_t ns = (lb + sw - 1) / sw;
     int f0, f1, l0, l1;
     stripe_rows(la, lb, band, sw, 0, f0, f1);
@@ -1362,13 +1427,14 @@ struct TbArgs {
     uint8_t *paths;
     mh_dp_result *res;
     uint32_t *dev;
+    const uint32_t *idx;
     uint32_t nwave, first, n;
-    void *wave[7];
-    void *thread[8];
+    void *wave[8];
+    void *thread[9];
     void set()
     {
-        void *w[7] = {&jobs, &outs, &tb, &paths, &res, &dev, &nwave};
-        void *t[8] = {&jobs, &outs, &tb, &paths, &res, &dev, &first, &n};
+        void *w[8] = {&jobs, &outs, &tb, &paths, &res, &dev, &idx, &nwave};
+        void *t[9] = {&jobs, &outs, &tb, &paths, &res, &dev, &idx, &first, &n};
         std::memcpy(wave, w, sizeof(w));
         std::memcpy(thread, t, sizeof(t));
     }
@@ -1519,15 +1585,17 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
     }
     host_time(ctx, "host_dp_sort", now_ms() - ts0);
     auto bnd_ints = [&](const DpItem &it) {
-        // one-lane, row-packed (a single stripe) and window jobs hand no column over
-        if (it.cols == TINY_COLS_TAG || it.cols > GROUP_TAG) return 0ull;
-        const uint64_t ns = (jobs[it.job].b_len + 64 * it.cols - 1) / (64 * it.cols);
+        // one-lane and window jobs hand no column over; a job of 16-lane
+        // stripes does when it has more than one (banded jobs)
+        if (it.cols == TINY_COLS_TAG || it.cols >= WINDOW_TAG) return 0ull;
+        const uint64_t sw = it.cols > GROUP_TAG ? 16ull * (it.cols - GROUP_TAG) : 64ull * it.cols;
+        const uint64_t ns = (jobs[it.job].b_len + sw - 1) / sw;
         return ns > 1 ? 3ull * ((jobs[it.job].a_len + 63) & ~63ull) : 0ull;
     };
     // footprint in a fixed scratch of a round of `cnt` jobs with trace bytes
     // tbb (each rounded up to 16) and bi boundary ints
     auto fixed_need = [&](uint64_t cnt, uint64_t tbb, uint64_t bi) {
-        return DP_ROUND_BYTES + align256(cnt * (sizeof(DevJob) + sizeof(DpOut))) + align256(bi * 4) + tbb + 256;
+        return DP_ROUND_BYTES + align256(cnt * (sizeof(DevJob) + sizeof(DpOut) + 4)) + align256(bi * 4) + tbb + 256;
     };
     if (fx) {
         for (const DpItem &it : items)
@@ -1604,24 +1672,49 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             for (uint64_t x = 0; x < nstripe; ++x) shaped[put[shape(items[first + x])]++] = items[first + x];
             std::copy(shaped.begin(), shaped.end(), items.begin() + (ptrdiff_t)first);
         }
+        // A wavefront of 16-lane stripes runs four jobs whose stripe loops
+        // diverge: each stripe costs it the longest of the four, so the jobs
+        // of those shapes go in order of stripe count, then rows a stripe
+        // (largest first), and the four of a wavefront step alike.
+        for (int q : {5, 7}) {   // 16-lane stripes of 8 and of 4 columns a lane
+            auto key = [&](const DpItem &it) {
+                const mh_dp_job &jb = jobs[it.job];
+                const uint64_t sw = 16ull * (it.cols - GROUP_TAG);
+                const uint64_t ns = (jb.b_len + sw - 1) / sw;
+                uint64_t rows = jb.a_len;
+                if (it.band) rows = std::min<uint64_t>(jb.a_len, jb.a_len * (sw + 2ull * it.band) / jb.b_len + 1);
+                return (ns << 32) | rows;
+            };
+            std::stable_sort(items.begin() + (ptrdiff_t)(first + rows_at[q]),
+                             items.begin() + (ptrdiff_t)(first + rows_at[q + 1]),
+                             [&](const DpItem &x, const DpItem &y) { return key(x) > key(y); });
+        }
         const uint64_t n64 = rows_at[1];   // rows_at[1..]: the row jobs, 16 columns a lane first
-        // stripe jobs with long trace-back walks first: they get a wavefront each
-        std::stable_partition(items.begin() + (ptrdiff_t)first, items.begin() + (ptrdiff_t)(first + n64),
-                              [&](const DpItem &it) {
-                                  return jobs[it.job].a_len + jobs[it.job].b_len >= WAVE_WALK_MIN;
-                              });
-        uint64_t nwave = 0;
-        while (nwave < n64 &&
-               jobs[items[first + nwave].job].a_len + jobs[items[first + nwave].job].b_len >= WAVE_WALK_MIN)
-            ++nwave;
         DevJob *dj = nullptr;
         std::vector<DevJob> host_jobs;
         if (fx) {
-            host_jobs.resize(n);
+            host_jobs.resize(n + (n * 4 + sizeof(DevJob) - 1) / sizeof(DevJob));
             dj = host_jobs.data();
         } else {
-            const int rc0 = pinned_jobs(ctx, n * sizeof(DevJob), (void **)&dj);
+            const int rc0 = pinned_jobs(ctx, n * sizeof(DevJob) + n * 4 + 64, (void **)&dj);
             if (rc0 != MH_OK) return rc0;
+        }
+        // trace-back order: the jobs whose walks are long and whose layout the
+        // wavefront walk reads (64-lane stripes; 16-lane stripes of 4 or 8
+        // columns a lane) walk on a wavefront each, the others on a thread each
+        uint32_t *tbi = reinterpret_cast<uint32_t *>(dj + n);
+        uint64_t nwave = 0;
+        {
+            auto wave_walk = [&](const DpItem &it) {
+                if (jobs[it.job].a_len + jobs[it.job].b_len < WAVE_WALK_MIN) return false;
+                if (it.cols == TINY_COLS_TAG || it.cols >= WINDOW_TAG) return false;
+                return it.cols < GROUP_TAG || it.cols == GROUP_TAG + 4 || it.cols == GROUP_TAG + 8;
+            };
+            for (uint64_t x = 0; x < n; ++x)
+                if (wave_walk(items[first + x])) tbi[nwave++] = (uint32_t)x;
+            uint64_t t = nwave;
+            for (uint64_t x = 0; x < n; ++x)
+                if (!wave_walk(items[first + x])) tbi[t++] = (uint32_t)x;
         }
         uint64_t tw = 0, bo = 0;
         for (uint64_t x = 0; x < n; ++x) {
@@ -1655,7 +1748,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             q += DP_ROUND_BYTES;
             d_jobs = reinterpret_cast<DevJob *>(q);
             d_outs = reinterpret_cast<DpOut *>(d_jobs + n);
-            q += align256(n * (sizeof(DevJob) + sizeof(DpOut)));
+            q += align256(n * (sizeof(DevJob) + sizeof(DpOut) + 4));
             d_bnd = reinterpret_cast<int *>(q);
             q += align256(bo * 4);
             d_tb = q;
@@ -1664,7 +1757,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             int rc;
             if ((rc = ctx->dp_tb.ensure(tw + 256)) != MH_OK) return rc;
             if ((rc = ctx->dp_bnd.ensure(bo * 4 + 256)) != MH_OK) return rc;
-            if ((rc = ctx->dp_jobs.ensure(n * sizeof(DevJob) + n * sizeof(DpOut) + 256)) != MH_OK) return rc;
+            if ((rc = ctx->dp_jobs.ensure(n * (sizeof(DevJob) + sizeof(DpOut) + 4) + 256)) != MH_OK) return rc;
             if ((rc = ctx->dp_round.ensure(DP_ROUND_BYTES)) != MH_OK) return rc;
             d_jobs = ctx->dp_jobs.as<DevJob>();
             d_outs = reinterpret_cast<DpOut *>(d_jobs + n);
@@ -1672,7 +1765,9 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             d_bnd = ctx->dp_bnd.as<int>();
             d_round = ctx->dp_round.as<DpRound>();
         }
+        uint32_t *d_tbi = reinterpret_cast<uint32_t *>(d_outs + n);
         MH_HIP(hipMemcpyAsync(d_jobs, dj, n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
+        MH_HIP(hipMemcpyAsync(d_tbi, tbi, n * 4, hipMemcpyHostToDevice, ctx->stream));
         // the round's descriptor: pointers, the trace-back split and the queue's bins
         DpRound hr;
         std::memset(&hr, 0, sizeof(hr));
@@ -1754,6 +1849,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         tba.paths = d_paths;
         tba.res = d_results;
         tba.dev = d_dev;
+        tba.idx = d_tbi;
         tba.nwave = hr.nwave;
         tba.first = hr.nwave;
         tba.n = hr.n;
@@ -1815,11 +1911,11 @@ static int dp_check_jobs(const mh_dp_job *jobs, uint64_t njobs, std::vector<uint
     return MH_OK;
 }
 
-static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band, int force)
+static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band, int force, bool rerun = false)
 {
     DpItem it;
     it.job = job;
-    it.cols = dp_cols_for(jb.a_len, jb.b_len, band, force);
+    it.cols = dp_cols_for(jb.a_len, jb.b_len, band, force, rerun);
     it.band = dp_effective_band(jb.a_len, jb.b_len, band, it.cols);
     if (it.band == 0 && band != 0) it.cols = dp_cols_for(jb.a_len, jb.b_len, 0, force);
     if (it.band == 0 && !force && jb.b_len <= (uint32_t)TINY_LB) {
@@ -1836,7 +1932,8 @@ static DpItem make_item(const mh_dp_job &jb, uint32_t job, uint32_t band, int fo
         it.tb_bytes = dp_trace_bytes(jb.a_len, jb.b_len, 0, c, 16);
         return it;
     }
-    it.tb_bytes = dp_trace_bytes(jb.a_len, jb.b_len, it.band, it.cols, 64);
+    it.tb_bytes = it.cols > GROUP_TAG ? dp_trace_bytes(jb.a_len, jb.b_len, it.band, it.cols - GROUP_TAG, 16)
+                                      : dp_trace_bytes(jb.a_len, jb.b_len, it.band, it.cols, 64);
     if (it.band) {
         const uint32_t wc = dp_window_cols(jb.a_len, jb.b_len, it.band, dp_force_kind());
         if (wc) {
@@ -2009,12 +2106,14 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
         std::vector<uint32_t> dev(njobs);
         MH_HIP(hipMemcpyAsync(dev.data(), d_dev, njobs * 4, hipMemcpyDeviceToHost, ctx->stream));
         MH_HIP(hipStreamSynchronize(ctx->stream));
+        double tq = now_ms();
+        host_time(ctx, "band_pass1", tq - tp0);
         std::vector<DpItem> again;
         std::vector<uint32_t> accept2(njobs, 0);
         for (uint64_t j = 0; j < njobs; ++j)
             if (accept1[j] && dev[j] >= accept1[j]) {
                 const uint64_t r2 = 4ull * accept1[j];
-                again.push_back(make_item(jobs[j], (uint32_t)j, r2 > 0xFFFFFFFFull ? 0u : (uint32_t)r2, force));
+                again.push_back(make_item(jobs[j], (uint32_t)j, r2 > 0xFFFFFFFFull ? 0u : (uint32_t)r2, force, true));
                 if (again.back().band) {
                     accept2[j] = 2 * accept1[j];
                     ++st.rerun_doubled;
@@ -2027,15 +2126,20 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
                 return rc;
             MH_HIP(hipMemcpyAsync(dev.data(), d_dev, njobs * 4, hipMemcpyDeviceToHost, ctx->stream));
             MH_HIP(hipStreamSynchronize(ctx->stream));
+            host_time(ctx, "band_pass2", now_ms() - tq);
+            tq = now_ms();
             std::vector<DpItem> full;
             for (uint64_t j = 0; j < njobs; ++j)
                 if (accept2[j] && dev[j] >= accept2[j]) {
                     full.push_back(make_item(jobs[j], (uint32_t)j, 0, force));
                     ++st.rerun_full;
                 }
-            if (!full.empty() &&
-                (rc = dp_run_items(ctx, d_a, d_b, jobs, slot, full, d_paths, d_results, d_dev, &st.cells)) != MH_OK)
-                return rc;
+            if (!full.empty()) {
+                if ((rc = dp_run_items(ctx, d_a, d_b, jobs, slot, full, d_paths, d_results, d_dev, &st.cells)) != MH_OK)
+                    return rc;
+                MH_HIP(hipStreamSynchronize(ctx->stream));
+                host_time(ctx, "band_pass3", now_ms() - tq);
+            }
         }
     }
     if (stats) *stats = st;

@@ -222,8 +222,9 @@ int dp_run_banded(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b, const
                   const uint32_t *bands, uint32_t flags, uint8_t *d_paths, mh_dp_result *d_results,
                   uint32_t *d_dev, mh_band_stats *stats);
 // Stripe shape (columns per lane) and effective band the scheduler uses.
-int dp_force_cols();   // MH_DP_COLS = 4 | 8 | 16 forces one stripe shape (tuning sweeps)
-uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force);
+int dp_force_cols();   // MH_DP_COLS = 4 | 8 | 16 | 36 | 40 forces one stripe shape (tuning sweeps)
+uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force, bool wide_only = false);
+uint32_t dp_stripe_width(uint32_t cols);   // columns of one stripe of a dp_cols_for shape
 uint32_t dp_effective_band(uint32_t la, uint32_t lb, uint32_t band, uint32_t cols);
 int dp_force_kind();   // MH_DP_BAND_KIND = stripes | window forces one banded region (tests, sweeps)
 uint32_t dp_window_cols(uint32_t la, uint32_t lb, uint32_t band, int kind);

@@ -493,22 +493,28 @@ int mh_dp_align_banded_device(mh_context *ctx, const uint8_t *d_a, const uint8_t
     return dp_run_banded(ctx, d_a, d_b, jobs, njobs, bands, flags, d_paths, d_results, d_dev, stats);
 }
 
-uint32_t mh_dp_band_region(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *kind, uint32_t *width)
+uint32_t mh_dp_band_region_ex(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t flags, uint32_t *kind,
+                              uint32_t *width)
 {
     uint32_t k = MH_REGION_FULL, w = 0, eff = 0;
     if (a_len && b_len) {
         const int force = dp_force_cols();
-        uint32_t c = dp_cols_for(a_len, b_len, band, force);
+        uint32_t c = dp_cols_for(a_len, b_len, band, force, (flags & MH_REGION_RERUN) != 0);
         eff = band ? dp_effective_band(a_len, b_len, band, c) : 0u;
         if (eff) {
             const uint32_t wc = dp_window_cols(a_len, b_len, eff, dp_force_kind());
             k = wc ? MH_REGION_WINDOW : MH_REGION_STRIPES;
-            w = wc ? 64 * wc : 64 * c;
+            w = wc ? 64 * wc : dp_stripe_width(c);
         }
     }
     if (kind) *kind = k;
     if (width) *width = w;
     return eff;
+}
+
+uint32_t mh_dp_band_region(uint32_t a_len, uint32_t b_len, uint32_t band, uint32_t *kind, uint32_t *width)
+{
+    return mh_dp_band_region_ex(a_len, b_len, band, 0u, kind, width);
 }
 
 int mh_dp_band_cells(const uint32_t *a_len, const uint32_t *b_len, const uint32_t *band, uint64_t n, uint64_t *cells)

@@ -50,7 +50,7 @@ EXPORTS = [
     "mh_index_from_xmfa", "mh_index_from_xmfa_file", "mh_index_fix_zero", "mh_index_fill_gaps", "mh_index_smooth_edges", "mh_xmfa_write",
     "mh_refine_pairwise", "mh_refine_anchor_cols", "mh_refine_pieces", "mh_anchor_gaps", "mh_chain_rows",
     "mh_recursion_gaps",
-    "mh_dp_align_banded_device", "mh_dp_band_region", "mh_dp_band_cells",
+    "mh_dp_align_banded_device", "mh_dp_band_region", "mh_dp_band_region_ex", "mh_dp_band_cells",
     "mh_gap_features_device", "mh_mlp_create", "mh_mlp_free", "mh_mlp_set_params", "mh_mlp_get_params",
     "mh_mlp_forward", "mh_mlp_grad", "mh_mlp_train", "mh_band_predict",
 ]
@@ -154,6 +154,8 @@ def load_library(path=LIB_PATH):
                                                 C.POINTER(BandStats)]),
         "mh_dp_band_region": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
                                            C.POINTER(C.c_uint32)]),
+        "mh_dp_band_region_ex": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                              C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "mh_dp_band_cells": (C.c_int, [vp, vp, vp, u64, vp]),
         "mh_gap_features_device": (C.c_int, [vp, vp, vp, vp, u64, vp]),
         "mh_mlp_create": (C.c_int, [vp, u64, C.POINTER(vp)]),
@@ -390,10 +392,15 @@ def band_cells(a_len, b_len, bands):
     return out
 
 
-def band_region(a_len, b_len, band):
-    """(effective band, region kind, width) the scheduler uses for a job."""
+REGION_RERUN = 1
+
+
+def band_region(a_len, b_len, band, rerun=False):
+    """(effective band, region kind, width) the scheduler uses for a job; with
+    rerun, the region of the edge fallback's second pass."""
     k, w = C.c_uint32(), C.c_uint32()
-    eff = load_library().mh_dp_band_region(int(a_len), int(b_len), int(band), C.byref(k), C.byref(w))
+    eff = load_library().mh_dp_band_region_ex(int(a_len), int(b_len), int(band), REGION_RERUN if rerun else 0,
+                                              C.byref(k), C.byref(w))
     return int(eff), int(k.value), int(w.value)
 
 

@@ -2,7 +2,8 @@
 
 Bar: bit-exact paths, scores and path deviations.  Without fallback, every
 job equals the oracle's banded region DP (or_nwsmall_region) at the stripe
-width the scheduler reports; the region oracle is pinned by
+width the scheduler reports (64-lane stripes of 4, 8, 16 columns a lane;
+16-lane stripes of 4 or 8, four jobs a wavefront: MH_DP_COLS 36, 40); the region oracle is pinned by
 tests/test_oracle_dp.py (whole matrix = NWSmall = libMUSCLE, brute force over
 the region's paths, containment of the full path).  With fallback, every job
 equals the oracle model of the fallback chain (region of band 2W accepted
@@ -46,9 +47,10 @@ def _pairs(rng, n, la_range, div=0.1):
     return np.concatenate(A), np.concatenate(B), mauvehip.make_jobs(j[:, 0], j[:, 1], j[:, 2], j[:, 3])
 
 
-def _oracle(a, b, band):
-    """The oracle DP over the region the library gives this job and band."""
-    eff, kind, width = mauvehip.band_region(len(a), len(b), band)
+def _oracle(a, b, band, rerun=False):
+    """The oracle DP over the region the library gives this job and band (in
+    the fallback's second pass with rerun)."""
+    eff, kind, width = mauvehip.band_region(len(a), len(b), band, rerun)
     if kind == mauvehip.REGION_WINDOW:
         return po.nwsmall_window(a, b, eff, width), eff
     return po.nwsmall_region(a, b, eff, width if kind == mauvehip.REGION_STRIPES else 0), eff
@@ -80,7 +82,7 @@ def kind_env():
         os.environ["MH_DP_BAND_KIND"] = old
 
 
-@pytest.mark.parametrize("cols", ["4", "8", "16", "auto", "window"])
+@pytest.mark.parametrize("cols", ["4", "8", "16", "36", "40", "auto", "window"])
 def test_banded_equals_region_oracle(ctx, cols, cols_env, kind_env):
     os.environ.pop("MH_DP_COLS", None)
     os.environ.pop("MH_DP_BAND_KIND", None)
@@ -140,7 +142,7 @@ def test_fallback_chain(ctx):
         # region 2W accepted within W, region 4W accepted within 2W, then full
         want, e1 = _oracle(a, b, 2 * W)
         if e1 and (want[0] is None or want[2] >= W):
-            want, e2 = _oracle(a, b, 4 * W)
+            want, e2 = _oracle(a, b, 4 * W, rerun=True)
             n2 += 1
             if e2 and (want[0] is None or want[2] >= 2 * W):
                 want = po.nwsmall_region(a, b, 0, 0)

@@ -4,10 +4,12 @@ full DP, the default -- and, for reference, with one constant band chosen on
 the same training labels by the same cost, with the same edge fallback.
 
 Bar: zero score and path mismatches against the full DP after the edge
-fallback, and at most half of the full DP's cells (these gaps are at most
-4,000 long; the 100k-gap bench set reaches 20,000).  The constant band's
-figures are printed, not asserted: with stripe regions, whose width does not
-shrink with the band, the two come out close on this distribution.
+fallback for both bandings, and at most half of the full DP's cells (these
+gaps are at most 4,000 long; the 100k-gap bench set reaches 20,000).  The
+learned band is not asserted to beat the constant one: on these short gaps
+the constant band chosen by the same cost evaluates about as many cells, and
+on the 100k-gap set the two come out within about 10% of each other in time
+(stripe regions cost more per cell as they narrow).
 """
 import numpy as np
 import pytest
@@ -23,7 +25,7 @@ def _set(ctx, n, seed):
     return band.DeviceJobs(ctx, A, B, mauvehip.make_jobs(ao, al, bo, bl))
 
 
-def test_learned_band_beats_constant_band(ctx):
+def test_learned_band_matches_full_dp(ctx):
     tr = _set(ctx, 8000, 411)
     ev = _set(ctx, 3000, 412)
     try:
@@ -55,6 +57,7 @@ def test_learned_band_beats_constant_band(ctx):
               f"constant W={w_const}: cells {st_c.cells} reruns {st_c.rerun_doubled}/{st_c.rerun_full}; "
               f"full cells {ev.cells}")
         assert sm_l == 0 and pm_l == 0
+        assert sm_c == 0 and pm_c == 0
         assert st_l.cells * 2 < ev.cells
         mlp.free()
     finally:
