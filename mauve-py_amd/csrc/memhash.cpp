@@ -566,6 +566,16 @@ int mh_sml_get_sequence(mh_context *ctx, const mh_sml *s, uint32_t *words, uint6
     return MH_OK;
 }
 
+// the radix sorts' look-back bound of a search's contexts (its own and the
+// helpers that ran its independent searches)
+static int sorts_ok(mh_context *ctx)
+{
+    int rc = sort_check(ctx);
+    for (mh_context *h : ctx->helpers)
+        if (rc == MH_OK) rc = sort_check(h);
+    return rc;
+}
+
 int mh_find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, mh_match *out, uint64_t cap,
                              uint64_t *n_out, mh_match_stats *stats)
 {
@@ -574,6 +584,7 @@ int mh_find_matches_pairwise(mh_context *ctx, const mh_sml *const *smls, int k, 
     MH_HIP(hipSetDevice(ctx->device));
     std::vector<mh_match> v;
     int rc = find_matches_pairwise(ctx, smls, k, v, stats);
+    if (rc == MH_OK) rc = sorts_ok(ctx);
     if (rc != MH_OK) return rc;
     *n_out = v.size();
     const uint64_t m = std::min<uint64_t>(cap, v.size());
@@ -593,6 +604,7 @@ int mh_find_matches_pairwise_list(mh_context *ctx, const mh_sml *const *smls, in
     MH_HIP(hipSetDevice(ctx->device));
     mh_match_list *l = new mh_match_list();
     int rc = find_matches_pairwise(ctx, smls, k, l->v, stats);
+    if (rc == MH_OK) rc = sorts_ok(ctx);
     if (rc != MH_OK) {
         delete l;
         return rc;
@@ -617,6 +629,7 @@ int mh_find_matches_unique(mh_context *ctx, const mh_sml *const *smls, int k, mh
     MH_HIP(hipSetDevice(ctx->device));
     mh_match_k_list *l = new mh_match_k_list();
     int rc = find_matches_unique(ctx, smls, k, l->starts, l->lengths, stats);
+    if (rc == MH_OK) rc = sorts_ok(ctx);
     if (rc != MH_OK) {
         delete l;
         return rc;

@@ -128,8 +128,11 @@ struct mh_context {
     // contexts (own streams and scratch) that run independent searches
     // concurrently for this one, created on first use
     std::vector<mh_context *> helpers;
-    // scratch for sorting
+    // scratch for sorting; the radix sort's look-back status words and the
+    // epoch of its last pass
     mh::DevBuf sort_tmp[4];
+    mh::DevBuf sort_status;
+    uint32_t sort_epoch = 0;
     // pinned host staging for device-to-host transfers; a second one for the
     // DP job tables
     void *pinned = nullptr;
@@ -288,11 +291,15 @@ uint64_t sml_full_key(uint64_t c, int sw);
 int ascii_to_codes(mh_context *ctx, const void *d_ascii, uint64_t len, uint8_t *d_codes);
 uint64_t sml_compact_key(uint64_t k, int sw);
 
-// Stable LSD radix sort of (key, value) pairs on the low `bits` key bits, 8 bits
-// per pass, ping-ponging between (k0,v0) and (k1,v1); *out_k/*out_v get the
-// pair of buffers holding the result.  Runs on ctx->stream.
+// Stable LSD radix sort of (key, value) pairs on the low `bits` key bits, in
+// the fewest passes of digits up to 10 bits, ping-ponging between (k0,v0) and (k1,v1); *out_k/*out_v get the
+// pair of buffers holding the result.  Runs on ctx->stream (onesweep: one
+// histogram kernel, then one kernel per pass with decoupled look-back).
 int radix_sort_kv(mh_context *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, uint64_t n, int bits,
                   uint64_t **out_k, uint32_t **out_v);
+// Synchronises ctx->stream and reports MH_ERR_DEVICE if a radix sort's
+// look-back wait ran past its bound since the last check.
+int sort_check(mh_context *ctx);
 
 // Sorted lists as the restart planner reads them: compact keys
 // ((chars << 1) | strand, ordered as the 64-bit keys), host memory.

@@ -10,13 +10,13 @@
 //             Keys are also written in compact form for sorting:
 //             (selected chars << 1) | flag, 2w+1 significant bits, which
 //             orders exactly like the 64-bit key.
-// K2   sort:  LSD radix sort of (compact key, position) by 8-bit digits.
-//             Each pass: per-tile digit histograms, one exclusive scan over
-//             the digit-major histogram table, then a scatter that ranks the
-//             tile stably in LDS (wave ballots per digit bit) and writes every
-//             digit's run out contiguously.  Positions enter in ascending
-//             order and every pass is stable, so equal keys stay in position
-//             order.
+// K2   sort:  onesweep LSD radix sort of (compact key, position) by digits
+//             of 8 to 10 bits (the fewest passes): one histogram kernel for every pass, then per pass one
+//             kernel whose tiles rank their keys stably in LDS (wave ballots
+//             per digit bit), find their digit offsets by decoupled look-back
+//             over the preceding tiles and write every digit's run out
+//             contiguously.  Positions enter in ascending order and every
+//             pass is stable, so equal keys stay in position order.
 #include "mh_internal.hpp"
 #include <algorithm>
 #include <cstring>
@@ -27,7 +27,6 @@ namespace {
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;   // 4096 entries per tile
-constexpr int RADIX = 256;
 
 struct SeedPattern {
     int sl, sw, spaced;
@@ -114,91 +113,128 @@ __global__ void keys_kernel(const uint32_t *__restrict__ words, uint64_t n, Seed
     pos_out[p] = (uint32_t)p;
 }
 
-__global__ __launch_bounds__(SORT_THREADS) void hist_kernel(const uint64_t *__restrict__ keys, uint64_t n,
-                                                           int shift, uint32_t *__restrict__ counts,
-                                                           uint32_t ntiles)
+// ---- K2: onesweep LSD radix sort ------------------------------------------
+//
+// One histogram kernel counts the digits of every pass in a single read of
+// the keys (per-workgroup LDS histograms, reduced and scanned by a second
+// small kernel into each pass's digit starts).  Then one kernel per pass:
+// each workgroup takes the next tile ticket (tiles in ticket order, so a
+// tile's predecessors are already running), ranks its 4096 keys stably in LDS,
+// publishes its digit counts, and finds each digit's offset among the
+// preceding tiles by decoupled look-back over their published counts: a
+// predecessor's word carries either its own count (aggregate) or the sum over
+// it and every tile before it (inclusive prefix), which ends the walk.  A
+// status word packs (epoch, flag, value) into 64 bits, so one atomic load sees
+// a consistent pair, and words of earlier passes (other epochs) read as not
+// yet published; the array is never cleared between passes.  A wait is
+// bounded: past OS_SPIN_MAX polls the tile records an error (the sort's
+// caller checks it) and moves on.
+
+constexpr int OS_MAX_BITS = 10;                       // widest digit (radix 1024)
+constexpr int OS_MAX_RADIX = 1 << OS_MAX_BITS;
+constexpr int OS_MAX_PASSES = 8;                      // 64-bit keys
+constexpr int OS_HIST_WGS = 256;                      // workgroups of the histogram
+constexpr int OS_HIST_THREADS = 256;
+constexpr int OS_HIST_LDS = 7 * OS_MAX_RADIX;          // passes x radix at most (64-bit keys: 7 x 1024)
+constexpr uint32_t OS_SPIN_MAX = 1u << 22;
+constexpr uint64_t OS_FLAG_A = 1, OS_FLAG_P = 2;
+constexpr uint64_t OS_VALUE_MASK = (1ull << 38) - 1;
+
+__device__ __forceinline__ uint64_t os_word(uint32_t epoch, uint64_t flag, uint64_t v)
 {
-    __shared__ uint32_t h[RADIX];
-    const int t = threadIdx.x;
-    h[t] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
-#pragma unroll
-    for (int r = 0; r < SORT_ITEMS; ++r) {
-        const uint64_t x = base + (uint64_t)r * SORT_THREADS + t;
-        if (x < n) atomicAdd(&h[(uint32_t)(keys[x] >> shift) & 0xFFu], 1u);
-    }
-    __syncthreads();
-    counts[(uint64_t)t * ntiles + blockIdx.x] = h[t];
+    return ((uint64_t)epoch << 40) | (flag << 38) | v;
 }
 
-// Exclusive scan of the digit-major table (RADIX rows of ntiles counts): one
-// workgroup per digit scans its row in place (chunks of 2048 counts, 8 per
-// thread, carried across chunks) and writes the row total; the scatter adds
-// the exclusive scan of the 256 totals itself.
-constexpr int SCAN_T = 256, SCAN_K = 8, SCAN_CHUNK = SCAN_T * SCAN_K;
-
-__global__ __launch_bounds__(SCAN_T) void digit_scan_kernel(uint32_t *__restrict__ counts, uint32_t ntiles,
-                                                            uint32_t *__restrict__ totals)
+// Per-workgroup digit counts of every pass (digits of db bits); workgroup 0
+// also zeroes the passes' tile tickets (the passes run after this kernel on
+// the stream).
+__global__ __launch_bounds__(OS_HIST_THREADS) void os_hist_kernel(const uint64_t *__restrict__ keys, uint64_t n,
+                                                                 int db, int passes, uint32_t *__restrict__ part,
+                                                                 uint32_t *__restrict__ tickets)
 {
-    __shared__ uint32_t part[SCAN_T];
-    const int t = threadIdx.x;
-    uint32_t *row = counts + (uint64_t)blockIdx.x * ntiles;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < ntiles; base += SCAN_CHUNK) {
-        uint32_t v[SCAN_K], s = 0;
+    __shared__ uint32_t h[OS_HIST_LDS];
+    const int t = threadIdx.x, radix = 1 << db;
+    const uint32_t mask = (uint32_t)radix - 1u;
+    for (int i = t; i < passes * radix; i += OS_HIST_THREADS) h[i] = 0;
+    if (blockIdx.x == 0 && t < OS_MAX_PASSES) tickets[t] = 0;
+    __syncthreads();
+    // a contiguous chunk per workgroup, eight keys a thread loaded at once
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = min<uint64_t>(n, lo + chunk);
+    constexpr int U = 8;
+    for (uint64_t x0 = lo + t; x0 < hi; x0 += (uint64_t)U * OS_HIST_THREADS) {
+        uint64_t k[U];
 #pragma unroll
-        for (int r = 0; r < SCAN_K; ++r) {
-            const uint32_t x = base + (uint32_t)t * SCAN_K + r;
-            v[r] = x < ntiles ? row[x] : 0u;
-            s += v[r];
+        for (int u = 0; u < U; ++u) {
+            const uint64_t x = x0 + (uint64_t)u * OS_HIST_THREADS;
+            k[u] = x < hi ? keys[x] : 0ull;
         }
-        part[t] = s;
-        __syncthreads();
-        for (int off = 1; off < SCAN_T; off <<= 1) {
-            const uint32_t w = t >= off ? part[t - off] : 0u;
-            __syncthreads();
-            part[t] += w;
-            __syncthreads();
-        }
-        uint32_t run = carry + part[t] - s;
 #pragma unroll
-        for (int r = 0; r < SCAN_K; ++r) {
-            const uint32_t x = base + (uint32_t)t * SCAN_K + r;
-            if (x < ntiles) row[x] = run;
-            run += v[r];
+        for (int u = 0; u < U; ++u) {
+            if (x0 + (uint64_t)u * OS_HIST_THREADS >= hi) break;
+            for (int q = 0; q < passes; ++q) atomicAdd(&h[q * radix + ((uint32_t)(k[u] >> (db * q)) & mask)], 1u);
         }
-        carry += part[SCAN_T - 1];
-        __syncthreads();
     }
-    if (t == 0) totals[blockIdx.x] = carry;
+    __syncthreads();
+    uint32_t *out = part + (uint64_t)blockIdx.x * passes * radix;
+    for (int i = t; i < passes * radix; i += OS_HIST_THREADS) out[i] = h[i];
 }
 
-// Stable scatter of one tile.  Wave w ranks its own contiguous quarter of the
-// tile (16 rounds of 64 keys): per round, lanes with the same digit find each
-// other with 8 ballots, read the wave's running count of that digit from LDS
-// and the round's leader advances it.  LDS operations of one wave complete in
-// order, so no workgroup barrier is needed between rounds.  At the end the
-// per-wave counts become wave offsets, the tile is reordered by digit in LDS,
-// and each digit run is written to its global offset (digit start from the
-// scanned totals + this tile's offset within the digit).
-__global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *__restrict__ kin,
+// Pass q = blockIdx.x, digit d = threadIdx.x (blockDim = radix): the digit's
+// total over the histogram workgroups, then the exclusive scan of the totals
+// (the first output position of each digit).
+__global__ __launch_bounds__(OS_MAX_RADIX) void os_start_kernel(const uint32_t *__restrict__ part, int nparts,
+                                                               int passes, uint32_t *__restrict__ start)
+{
+    __shared__ uint32_t s[OS_MAX_RADIX];
+    const int d = threadIdx.x, q = blockIdx.x, radix = blockDim.x;
+    uint32_t sum = 0;
+#pragma unroll 16
+    for (int w = 0; w < nparts; ++w) sum += part[((uint64_t)w * passes + q) * radix + d];
+    s[d] = sum;
+    __syncthreads();
+    for (int off = 1; off < radix; off <<= 1) {
+        const uint32_t v = d >= off ? s[d - off] : 0u;
+        __syncthreads();
+        s[d] += v;
+        __syncthreads();
+    }
+    start[q * radix + d] = s[d] - sum;
+}
+
+// One pass over the DB-bit digit (key >> shift) & (2^DB - 1).  Wave w ranks
+// its own contiguous quarter of the tile (16 rounds of 64 keys): per round,
+// lanes with the same digit find each other with DB ballots, read the wave's
+// running count of that digit from LDS and the round's leader advances it
+// (LDS operations of one wave complete in order).  Thread t owns digits t,
+// t + 256, ... for the tile's counts, the look-back and the global bases.
+template <int DB>
+__global__ __launch_bounds__(SORT_THREADS) void os_pass_kernel(const uint64_t *__restrict__ kin,
                                                               const uint32_t *__restrict__ vin,
                                                               uint64_t *__restrict__ kout,
-                                                              uint32_t *__restrict__ vout, uint64_t n,
-                                                              int shift, const uint32_t *__restrict__ offs,
-                                                              uint32_t ntiles, const uint32_t *__restrict__ totals)
+                                                              uint32_t *__restrict__ vout, uint64_t n, int shift,
+                                                              const uint32_t *__restrict__ dstart,
+                                                              uint64_t *__restrict__ status,
+                                                              uint32_t *__restrict__ ticket, uint32_t epoch,
+                                                              int *__restrict__ err)
 {
     constexpr int NWAVES = SORT_THREADS / 64;
+    constexpr int R = 1 << DB, E = R / SORT_THREADS;   // digits a thread owns
+    constexpr uint32_t MASK = (uint32_t)R - 1u;
     __shared__ uint64_t sk[SORT_TILE];
     __shared__ uint32_t sv[SORT_TILE];
-    __shared__ uint32_t wcnt[NWAVES][RADIX];
-    __shared__ uint32_t start[RADIX];
-    __shared__ uint32_t dbase[RADIX];
+    __shared__ uint16_t wcnt[NWAVES][R];   // per wave and digit: counts, then offsets (<= 4096)
+    __shared__ uint16_t start[R];          // the tile's digit counts, then its digit run starts
+    __shared__ uint32_t tile_s;
+    // the key tile's LDS is free until the keys are placed: the scan's sums
+    uint32_t *part = reinterpret_cast<uint32_t *>(sk);
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    dbase[t] = totals[t];
-    for (int w = 0; w < NWAVES; ++w) wcnt[w][t] = 0;
-    const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
+    if (t == 0) tile_s = atomicAdd(ticket, 1u);
+    for (int w = 0; w < NWAVES; ++w)
+        for (int e = 0; e < E; ++e) wcnt[w][t + e * SORT_THREADS] = 0;
+    __syncthreads();
+    const uint32_t tile = tile_s;
+    const uint64_t base = (uint64_t)tile * SORT_TILE;
     const uint64_t wbase = base + (uint64_t)wv * (SORT_ITEMS * 64);
     const uint64_t lane_lt = (1ull << lane) - 1ull;
     uint64_t key[SORT_ITEMS];
@@ -209,14 +245,13 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *_
         key[r] = x < n ? kin[x] : 0ull;
         val[r] = x < n ? vin[x] : 0u;
     }
-    __syncthreads();
 #pragma unroll
     for (int r = 0; r < SORT_ITEMS; ++r) {
         const bool valid = wbase + (uint64_t)r * 64 + lane < n;
-        const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & MASK;
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < DB; ++b) {
             const uint64_t bal = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
@@ -224,52 +259,106 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const uint64_t *_
         const uint32_t cur = wcnt[wv][d];
         rank[r] = cur + below;
         __builtin_amdgcn_wave_barrier();
-        if (valid && below == 0) wcnt[wv][d] = cur + (uint32_t)__popcll(peers);
+        if (valid && below == 0) wcnt[wv][d] = (uint16_t)(cur + (uint32_t)__popcll(peers));
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // thread t owns digit t: wave offsets within the tile and the tile count
-    uint32_t cnt = 0;
-    for (int w = 0; w < NWAVES; ++w) {
-        const uint32_t c = wcnt[w][t];
-        wcnt[w][t] = cnt;
-        cnt += c;
-    }
-    // exclusive scans of the tile's digit counts (start of each digit run in
-    // the tile) and of the global digit totals (start of each digit's output)
-    {
-        const uint32_t g = dbase[t];
-        start[t] = cnt;
-        __syncthreads();
-        for (int off = 1; off < RADIX; off <<= 1) {
-            const uint32_t v = t >= off ? start[t - off] : 0u;
-            const uint32_t w = t >= off ? dbase[t - off] : 0u;
-            __syncthreads();
-            start[t] += v;
-            dbase[t] += w;
-            __syncthreads();
+    // owned digits: wave offsets within the tile and the tile's counts, which
+    // are published at once (tile 0: its count is its inclusive prefix)
+    uint32_t cnt[E];
+    uint64_t *st = status + (uint64_t)tile * R;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int d = t + e * SORT_THREADS;
+        uint32_t c = 0;
+        for (int w = 0; w < NWAVES; ++w) {
+            const uint32_t x = wcnt[w][d];
+            wcnt[w][d] = (uint16_t)c;
+            c += x;
         }
-        const uint32_t incl = start[t], gincl = dbase[t];
+        cnt[e] = c;
+        __hip_atomic_store(&st[d], os_word(epoch, tile == 0 ? OS_FLAG_P : OS_FLAG_A, c), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // look-back: each owned digit's count over every preceding tile
+    uint32_t gb[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int d = t + e * SORT_THREADS;
+        uint64_t before = 0;
+        if (tile > 0) {
+            int64_t j = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            for (;;) {
+                const uint64_t w = __hip_atomic_load(&status[(uint64_t)j * R + d], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t flag = (w >> 38) & 3u;
+                if ((uint32_t)(w >> 40) == epoch && flag != 0) {
+                    before += w & OS_VALUE_MASK;
+                    if (flag == OS_FLAG_P || --j < 0) break;
+                    continue;
+                }
+                if (++spins > OS_SPIN_MAX) {
+                    atomicOr(err, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __hip_atomic_store(&st[d], os_word(epoch, OS_FLAG_P, (before + cnt[e]) & OS_VALUE_MASK),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        gb[e] = dstart[d] + (uint32_t)before;
+        start[d] = (uint16_t)cnt[e];
+    }
+    __syncthreads();
+    // exclusive scan of the tile's digit counts (the start of each digit run):
+    // thread t sums digits [t*E, t*E + E), the 256 sums are scanned, then each
+    // thread writes its digits' starts
+    uint32_t loc[E], tot = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        loc[e] = start[t * E + e];
+        tot += loc[e];
+    }
+    part[t] = tot;
+    __syncthreads();
+    for (int off = 1; off < SORT_THREADS; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
         __syncthreads();
-        start[t] = incl - cnt;
-        dbase[t] = gincl - g;
+        part[t] += v;
         __syncthreads();
     }
+    uint32_t run = part[t] - tot;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        start[t * E + e] = (uint16_t)run;
+        run += loc[e];
+    }
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < SORT_ITEMS; ++r) {
         if (wbase + (uint64_t)r * 64 + lane < n) {
-            const uint32_t d = (uint32_t)(key[r] >> shift) & 0xFFu;
+            const uint32_t d = (uint32_t)(key[r] >> shift) & MASK;
             const uint32_t p = start[d] + wcnt[wv][d] + rank[r];
             sk[p] = key[r];
             sv[p] = val[r];
         }
     }
     __syncthreads();
+    // the wave counts' LDS (no longer read) takes each digit's offset from a
+    // tile slot to its output position
+    uint32_t *off = reinterpret_cast<uint32_t *>(&wcnt[0][0]);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int d = t + e * SORT_THREADS;
+        off[d] = gb[e] - (uint32_t)start[d];
+    }
+    __syncthreads();
     const uint32_t tile_n = (uint32_t)min<uint64_t>(SORT_TILE, n - base);
     for (uint32_t p = t; p < tile_n; p += SORT_THREADS) {
         const uint64_t k = sk[p];
-        const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
-        const uint64_t g = (uint64_t)dbase[d] + offs[(uint64_t)d * ntiles + blockIdx.x] + (p - start[d]);
+        const uint32_t d = (uint32_t)(k >> shift) & MASK;
+        const uint64_t g = (uint32_t)(off[d] + p);   // below n < 2^32, so exact mod 2^32
         kout[g] = k;
         vout[g] = sv[p];
     }
@@ -451,7 +540,7 @@ int sml_build(mh_context *ctx, const void *d_ascii, uint64_t len, uint64_t seed,
     }
     MH_HIP(hipStreamSynchronize(ctx->stream));
     MH_CHECK(herr == 0, MH_ERR_GAP_CHAR, "Gap in genome sequence");
-    return MH_OK;
+    return sort_check(ctx);
 }
 
 int sml_alloc(mh_context *ctx, mh_sml *s) { return sml_take_buffers(ctx, s); }
@@ -488,24 +577,74 @@ int radix_sort_kv(mh_context *ctx, uint64_t *k0, uint32_t *v0, uint64_t *k1, uin
     *out_k = k0;
     *out_v = v0;
     if (n == 0) return MH_OK;
+    MH_CHECK(bits > 0 && bits <= 64, MH_ERR_INVALID, "radix sort: key bits out of range");
+    MH_CHECK(n < (1ull << 32), MH_ERR_UNSUPPORTED, "radix sort: more than 2^32 - 1 keys");
     const uint32_t ntiles = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
+    // the fewest passes of digits up to 10 bits, then the narrowest digit
+    // (8 at least) that covers the key in that many passes
+    int max_bits = OS_MAX_BITS;
+    if (const char *e = getenv("MH_SORT_MAX_DIGIT")) max_bits = std::min(OS_MAX_BITS, std::max(8, atoi(e)));
+    const int passes = (bits + max_bits - 1) / max_bits;
+    const int db = std::max(8, (bits + passes - 1) / passes);
+    const int radix = 1 << db;
+    MH_CHECK(passes * radix <= OS_HIST_LDS && db <= OS_MAX_BITS, MH_ERR_INVALID, "radix sort: digit table too large");
+    const uint32_t nparts = std::min<uint32_t>(OS_HIST_WGS, ntiles);
+    const uint64_t part_n = (uint64_t)nparts * passes * radix;
     int rc;
-    if ((rc = ctx->sort_tmp[2].ensure((uint64_t)RADIX * ntiles * 4 + RADIX * 4 + 64)) != MH_OK) return rc;
-    uint32_t *counts = ctx->sort_tmp[2].as<uint32_t>();
-    uint32_t *totals = counts + (uint64_t)RADIX * ntiles;
-    const int passes = (bits + 7) / 8;
+    if ((rc = ctx->sort_tmp[2].ensure((part_n + (uint64_t)passes * radix + OS_MAX_PASSES) * 4 + 64)) != MH_OK)
+        return rc;
+    uint32_t *part = ctx->sort_tmp[2].as<uint32_t>();
+    uint32_t *dstart = part + part_n;
+    uint32_t *tickets = dstart + (uint64_t)passes * radix;
+    // the look-back's status words (and the error word in front of them):
+    // zeroed when (re)allocated, then never cleared -- each pass has an epoch
+    const uint64_t status_bytes = 256 + (uint64_t)ntiles * radix * 8;
+    const void *old_ptr = ctx->sort_status.ptr;
+    if ((rc = ctx->sort_status.ensure(status_bytes)) != MH_OK) return rc;
+    if (ctx->sort_status.ptr != old_ptr) {
+        MH_HIP(hipMemsetAsync(ctx->sort_status.ptr, 0, ctx->sort_status.bytes, ctx->stream));
+        ctx->sort_epoch = 0;
+    }
+    int *err = ctx->sort_status.as<int>();
+    uint64_t *status = reinterpret_cast<uint64_t *>(ctx->sort_status.as<uint8_t>() + 256);
+    hipLaunchKernelGGL(os_hist_kernel, dim3(nparts), dim3(OS_HIST_THREADS), 0, ctx->stream, k0, n, db, passes, part,
+                       tickets);
+    hipLaunchKernelGGL(os_start_kernel, dim3(passes), dim3(radix), 0, ctx->stream, part, (int)nparts, passes, dstart);
     for (int pass = 0; pass < passes; ++pass) {
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(hist_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, n, shift, counts, ntiles);
-        hipLaunchKernelGGL(digit_scan_kernel, dim3(RADIX), dim3(SCAN_T), 0, ctx->stream, counts, ntiles, totals);
-        hipLaunchKernelGGL(scatter_kernel, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, v0, k1, v1, n, shift,
-                           counts, ntiles, totals);
+        if (++ctx->sort_epoch >= (1u << 24)) {
+            MH_HIP(hipMemsetAsync(ctx->sort_status.ptr, 0, ctx->sort_status.bytes, ctx->stream));
+            ctx->sort_epoch = 1;
+        }
+        const uint32_t *ds = dstart + (uint64_t)pass * radix;
+        const int shift = db * pass;
+        if (db == 8)
+            hipLaunchKernelGGL(os_pass_kernel<8>, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, v0, k1, v1, n,
+                               shift, ds, status, tickets + pass, ctx->sort_epoch, err);
+        else if (db == 9)
+            hipLaunchKernelGGL(os_pass_kernel<9>, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, v0, k1, v1, n,
+                               shift, ds, status, tickets + pass, ctx->sort_epoch, err);
+        else
+            hipLaunchKernelGGL(os_pass_kernel<10>, dim3(ntiles), dim3(SORT_THREADS), 0, ctx->stream, k0, v0, k1, v1,
+                               n, shift, ds, status, tickets + pass, ctx->sort_epoch, err);
         std::swap(k0, k1);
         std::swap(v0, v1);
     }
     MH_HIP(hipGetLastError());
     *out_k = k0;
     *out_v = v0;
+    return MH_OK;
+}
+
+int sort_check(mh_context *ctx)
+{
+    if (!ctx->sort_status.ptr) return MH_OK;
+    int err = 0;
+    MH_HIP(hipMemcpyAsync(&err, ctx->sort_status.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MH_HIP(hipStreamSynchronize(ctx->stream));
+    if (err) {
+        MH_HIP(hipMemsetAsync(ctx->sort_status.ptr, 0, 4, ctx->stream));
+        MH_CHECK(false, MH_ERR_DEVICE, "radix sort: a tile's look-back wait ran past its limit");
+    }
     return MH_OK;
 }
 
