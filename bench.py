@@ -39,11 +39,11 @@ MAX_GAP = 5000                               # max_gap_length, LM/ProgressiveAli
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
 # --pmc WRITE_SIZE on the recoded 4.6 Mb pair, scaled by the counters' reading
 # on a plain 1 GiB copy kernel (FETCH_SIZE x2, WRITE_SIZE x1): the
-# sorted-mer-list sort (hist, digit scan and scatter over 5 passes of
-# 4,599,984 keys), the join + solid-seed extension of the pair, and the
+# sorted-mer-list sort (one histogram kernel, then 4 onesweep passes of 9-bit
+# digits over 4,599,984 keys), the join + solid-seed extension of the pair, and the
 # whole path's NWSmall forward kernel (mean of the profile-pass and the
 # refinement-pass launches)
-MEASURED_TRAFFIC = {"sml_sort": 894084512, "extend": 782747872, "dp_forward_whole": 453269683}
+MEASURED_TRAFFIC = {"sml_sort": 628711488, "extend": 782747872, "dp_forward_whole": 453269683}
 
 
 def log(*a):
@@ -159,7 +159,7 @@ def main():
         # per-kernel device time over the timed region (HIP events on the library stream)
         N = La - mauvehip.get_seed_length(seed) + 1
         w = mauvehip.get_seed_weight(seed)
-        passes = (2 * w + 1 + 7) // 8
+        passes = (2 * w + 1 + 9) // 10        # digits of up to 10 bits, the fewest passes
         kern = {}
         for name in ("sml_pack", "sml_keys", "sml_sort", "join", "join_write", "extend", "partition", "dp_forward",
                      "dp_traceback", "host_wait_seeds", "host_partition", "host_replay", "host_list", "host_dp_plan", "host_dp_sort"):
@@ -167,7 +167,7 @@ def main():
             kern[name] = {"avg_ms": ms, "launches": n, "ms_per_step": ms * n / args.steps}
         nseeds = int(state["stats"].seed_tuples)
         # algorithmic work per launch
-        sort_bytes = 24 * N * passes + 4 * N                      # per genome
+        sort_bytes = 24 * N * passes + 8 * N                      # per genome: passes + one histogram read
         k3_bytes = 2 * 12 * N + 8 * nseeds + 2 * La // 4          # join + extension, per pair
         lines = {
             "sml_sort": ("hbm", sort_bytes / (kern["sml_sort"]["avg_ms"] * 1e-3) / 1e9 if kern["sml_sort"]["avg_ms"] else 0,
