@@ -361,15 +361,38 @@ def multi_leg(ctx, ref, seed, steps, warmup, cpu):
 UNIQUE_CPU_LENGTH = 2_000_000
 
 
+def pair_gap_jobs(starts, lengths, lens, b_base):
+    """The inter-anchor DP jobs of every genome pair (0, g) of a k-genome
+    match list: the matches holding genomes 0 and g, chained by the stand-in
+    chain (mh_chain_gap_jobs, gaps of 1..MAX_GAP on both sides); B offsets
+    shifted to genome g's place in the concatenated B codes.  The pairs chain
+    on host threads at once (the library call releases the interpreter)."""
+    import mauvehip
+    from concurrent.futures import ThreadPoolExecutor
+
+    def pair(g):
+        m = (starts[:, 0] != 0) & (starts[:, g] != 0)
+        _, jobs = mauvehip.chain_gap_jobs(np.ascontiguousarray(starts[m][:, [0, g]]), lengths[m], lens[0], lens[g],
+                                          MAX_GAP)
+        jobs = jobs.copy()
+        jobs["b_off"] += np.uint64(b_base[g])
+        return jobs
+
+    with ThreadPoolExecutor(max_workers=starts.shape[1] - 1) as ex:
+        return np.concatenate(list(ex.map(pair, range(1, starts.shape[1]))))
+
+
 def unique_leg(ctx, steps, warmup, cpu):
-    """Seed matching over eight 5 Mb genomes (45% GC) derived from one ancestor
-    along a random binary tree (1-5% SNPs, 0.2% indels, two inversions and a
-    2-20 kb deletion per branch): UniqueMatchFinder, the finder
-    progressiveMauve runs for more than four genomes, with the default seed of
-    weight 19 (rank 0, spaced).  Genomes resident in HBM; a step builds the
-    eight sorted mer lists and finds the matches.  The CPU baseline is the
-    oracle on the same generator at UNIQUE_CPU_LENGTH bases, whose list is
-    compared with the GPU's."""
+    """Eight 5 Mb genomes (45% GC) derived from one ancestor along a random
+    binary tree (1-5% SNPs, 0.2% indels, two inversions and a 2-20 kb deletion
+    per branch).  A step builds the eight sorted mer lists, finds the matches
+    with UniqueMatchFinder (the finder progressiveMauve runs for more than
+    four genomes) under the default seed of weight 19 (rank 0, spaced), chains
+    each genome pair (0, g) with the stand-in chain and runs the DP of every
+    pair's inter-anchor gaps as one batch.  Genomes resident in HBM.  The CPU
+    baseline is the oracle's matching on the same generator at
+    UNIQUE_CPU_LENGTH bases, whose list is compared with the GPU's; a sample
+    of the DP jobs is checked against the oracle's NWSmall."""
     import mauvehip
     from mauvehip import synth
     fam = synth.genome_family()
@@ -393,22 +416,72 @@ def unique_leg(ctx, steps, warmup, cpu):
         return dev
 
     dev = upload(fam)
-    for _ in range(max(1, warmup)):
+    # DP operands: genome 0's codes as A, genomes 1..7 concatenated as B
+    lens = [len(q) for q in fam]
+    b_base = [0] + list(np.cumsum([0] + lens[1:-1]))
+    cA = ctx.alloc(lens[0] + 16)
+    cB = ctx.alloc(sum(lens[1:]) + 16)
+    mauvehip.ascii_to_codes_device(ctx, dev[0], lens[0], cA)
+    for g in range(1, len(fam)):
+        mauvehip.ascii_to_codes_device(ctx, dev[g], lens[g], cB + int(b_base[g]))
+    st_, ln_, stats = run(fam, dev)
+    jobs = pair_gap_jobs(st_, ln_, lens, b_base)
+    d_paths = ctx.alloc(2 * int((jobs["a_len"].astype(np.int64) + jobs["b_len"]).sum()) + 16)
+    d_res = ctx.alloc(16 * 2 * len(jobs) + 16)
+    stage = {"match": 0.0, "chain": 0.0, "dp": 0.0}
+
+    def step():
+        t = [time.perf_counter()]
         st_, ln_, stats = run(fam, dev)
+        t.append(time.perf_counter())
+        jobs = pair_gap_jobs(st_, ln_, lens, b_base)
+        t.append(time.perf_counter())
+        mauvehip.global_align_device(ctx, cA, cB, jobs, d_paths, d_res)
+        ctx.synchronize()
+        t.append(time.perf_counter())
+        for k, name in enumerate(("match", "chain", "dp")):
+            stage[name] += t[k + 1] - t[k]
+        return st_, ln_, stats, jobs
+
+    for _ in range(max(1, warmup)):
+        step()
+    for k in stage:
+        stage[k] = 0.0
     ctx.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
-        st_, ln_, stats = run(fam, dev)
-        log(f"[bench] unique-matching step {k + 1}/{steps} done")
+        st_, ln_, stats, jobs = step()
+        log(f"[bench] eight-genome step {k + 1}/{steps} done")
     ctx.synchronize()
     el = (time.perf_counter() - t0) / steps
-    for d in dev:
+    cells = int((jobs["a_len"].astype(np.int64) * jobs["b_len"]).sum())
+    # a seeded sample of the DP jobs against the oracle's NWSmall
+    from oracle import pyoracle as po
+    res_dp = np.zeros(len(jobs), dtype=mauvehip.DP_RESULT_DTYPE)
+    ctx.d2h(res_dp, d_res)
+    codes = [np.frombuffer(q, dtype=np.uint8) if isinstance(q, bytes) else np.asarray(q) for q in fam]
+    lut = np.full(256, 4, dtype=np.uint8)
+    for c, v in zip(b"ACGTacgt", (0, 1, 2, 3, 0, 1, 2, 3)):
+        lut[c] = v
+    A0 = lut[codes[0]]
+    Bc = lut[np.concatenate(codes[1:])]
+    small = np.nonzero(jobs["a_len"].astype(np.int64) * jobs["b_len"] <= 1_000_000)[0]
+    pick = np.random.default_rng(5).choice(small, size=min(300, len(small)), replace=False)
+    same = all(int(po.nwsmall(A0[int(jobs["a_off"][j]):int(jobs["a_off"][j] + jobs["a_len"][j])],
+                              Bc[int(jobs["b_off"][j]):int(jobs["b_off"][j] + jobs["b_len"][j])])[1])
+               == int(res_dp["score"][j]) for j in pick)
+    for d in dev + [cA, cB, d_paths, d_res]:
         ctx.free(d)
-    res = {"genomes": len(fam), "genome_bp": [len(q) for q in fam], "seed": hex(seed),
+    res = {"genomes": len(fam), "genome_bp": lens, "seed": hex(seed),
            "value": round(total / 1e6 / el, 3), "unit": "Mb/s", "ms_per_step": round(1e3 * el, 3),
+           "stages_ms_per_step": {k: round(1e3 * v / steps, 2) for k, v in stage.items()},
            "seed_tuples": int(stats.seed_tuples), "matches": int(stats.matches),
-           "collisions": int(stats.collisions), "repeat_runs": int(stats.repeat_runs)}
-    log(f"[bench] eight-genome unique matching {res['value']:.1f} Mb/s, {res['ms_per_step']:.1f} ms/step")
+           "collisions": int(stats.collisions), "repeat_runs": int(stats.repeat_runs),
+           "dp": {"pairs": len(fam) - 1, "jobs": int(len(jobs)), "cells": cells,
+                  "gcups": round(cells / (stage["dp"] / steps) / 1e9, 1) if stage["dp"] else None,
+                  "oracle_sample_jobs": int(len(pick)), "oracle_sample_scores_identical": bool(same)}}
+    log(f"[bench] eight genomes: {res['value']:.1f} Mb/s, {res['ms_per_step']:.1f} ms/step "
+        f"({json.dumps(res['stages_ms_per_step'])}), DP {len(jobs)} jobs {cells / 1e9:.2f} Gcells")
     if cpu:
         from oracle import pyoracle as po
         small = synth.genome_family(length=UNIQUE_CPU_LENGTH)
