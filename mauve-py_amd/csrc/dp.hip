@@ -1673,21 +1673,33 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             std::copy(shaped.begin(), shaped.end(), items.begin() + (ptrdiff_t)first);
         }
         // A wavefront of 16-lane stripes runs four jobs whose stripe loops
-        // diverge: each stripe costs it the longest of the four, so the jobs
-        // of those shapes go in order of stripe count, then rows a stripe
-        // (largest first), and the four of a wavefront step alike.
+        // diverge: each stripe costs it the longest of the four, so banded
+        // jobs of those shapes go in order of stripe count, then rows a
+        // stripe (largest first), and the four of a wavefront step alike.
+        // (Full jobs of these shapes hold one stripe of LA rows: their
+        // largest-first order is that order already.)
         for (int q : {5, 7}) {   // 16-lane stripes of 8 and of 4 columns a lane
-            auto key = [&](const DpItem &it) {
+            const uint64_t b0 = first + rows_at[q], b1 = first + rows_at[q + 1];
+            bool banded = false;
+            for (uint64_t x = b0; x < b1 && !banded; ++x) banded = items[x].band != 0;
+            if (!banded) continue;
+            std::vector<std::pair<uint64_t, uint32_t>> keyed(b1 - b0);
+            for (uint64_t x = b0; x < b1; ++x) {
+                const DpItem &it = items[x];
                 const mh_dp_job &jb = jobs[it.job];
                 const uint64_t sw = 16ull * (it.cols - GROUP_TAG);
                 const uint64_t ns = (jb.b_len + sw - 1) / sw;
                 uint64_t rows = jb.a_len;
                 if (it.band) rows = std::min<uint64_t>(jb.a_len, jb.a_len * (sw + 2ull * it.band) / jb.b_len + 1);
-                return (ns << 32) | rows;
-            };
-            std::stable_sort(items.begin() + (ptrdiff_t)(first + rows_at[q]),
-                             items.begin() + (ptrdiff_t)(first + rows_at[q + 1]),
-                             [&](const DpItem &x, const DpItem &y) { return key(x) > key(y); });
+                keyed[x - b0] = {(ns << 32) | rows, (uint32_t)(x - b0)};
+            }
+            std::stable_sort(keyed.begin(), keyed.end(),
+                             [](const std::pair<uint64_t, uint32_t> &a, const std::pair<uint64_t, uint32_t> &b) {
+                                 return a.first > b.first;
+                             });
+            shaped.resize(b1 - b0);
+            for (uint64_t x = 0; x < b1 - b0; ++x) shaped[x] = items[b0 + keyed[x].second];
+            std::copy(shaped.begin(), shaped.end(), items.begin() + (ptrdiff_t)b0);
         }
         const uint64_t n64 = rows_at[1];   // rows_at[1..]: the row jobs, 16 columns a lane first
         DevJob *dj = nullptr;
