@@ -247,6 +247,7 @@ def main():
 
     if bandset is not None:
         out["learned_band"] = learned_band_leg(ctx, bandset)
+        out["learned_band"]["c2_gaps"] = c2_gap_band_leg(ctx, ref, mut, state["jobs"])
 
     if not args.whole_only:
         out["multi_genome"] = multi_leg(ctx, ref, seed, args.steps, args.warmup, not args.no_cpu_baseline)
@@ -671,6 +672,49 @@ def cpu_dp_batch(dp, gpu_scores):
 
 
 BAND_REPS = 3
+
+
+def c2_gap_band_leg(ctx, ref, mut, jobs):
+    """The learned band on the recoded 4.6 Mb pair's own inter-anchor gaps
+    (the core leg's chain gaps, A = the reference, B = the derivative): full DP
+    against learned band (features + network + banded DP + fallback), best of
+    three each, with mismatches against the full DP."""
+    import mauvehip
+    from mauvehip import band
+    lut = np.full(256, 4, dtype=np.uint8)
+    for c, v in zip(b"ACGTacgt", (0, 1, 2, 3, 0, 1, 2, 3)):
+        lut[c] = v
+    A = lut[np.frombuffer(np.asarray(ref).tobytes(), dtype=np.uint8)]
+    B = lut[np.frombuffer(np.asarray(mut).tobytes(), dtype=np.uint8)]
+    dj = band.DeviceJobs(ctx, A, B, jobs)
+    mlp, delta, min_band = band.load_default(ctx)
+    dj.align(None)
+    t_full = []
+    for _ in range(3):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        dj.align(None)
+        t_full.append(time.perf_counter() - t0)
+    res_f, _ = dj.results()
+    paths_f = dj.paths()
+    t_l = []
+    for _ in range(3):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        dj.features()
+        st = dj.align(mlp.predict_bands_device(dj.dF, dj.n, delta, min_band))
+        t_l.append(time.perf_counter() - t0)
+    res_l, _ = dj.results()
+    sm, pm = band.count_mismatches(res_f, paths_f, res_l, dj.paths())
+    out = {"jobs": int(dj.n), "full_dp_cells": int(dj.cells), "largest": [int(jobs["a_len"].max()), int(jobs["b_len"].max())],
+           "full_seconds": round(min(t_full), 5), "learned_seconds": round(min(t_l), 5),
+           "banded_jobs": int(st.banded), "learned_cells_computed": int(st.cells),
+           "reruns_2w": int(st.rerun_doubled), "reruns_full": int(st.rerun_full),
+           "score_mismatches": sm, "path_mismatches": pm}
+    log(f"[bench] learned band on the 4.6 Mb pair's gaps: {json.dumps(out)}")
+    mlp.free()
+    dj.free()
+    return out
 
 
 def learned_band_leg(ctx, bandset):
