@@ -792,6 +792,57 @@ __device__ __forceinline__ void traceback_thread_job(const DevJob *__restrict__ 
     bool ok = true;
     int64_t maxdev = 0;
     for (;;) {
+        if (stripes && i > 0 && j > 0) {
+            // Look ahead along the run that keeps the walk's state (M along the
+            // diagonal, D up, I along the row): the bytes of up to TB_AHEAD of
+            // its cells in this stripe are loaded at once, and the walk moves
+            // over all but the last cell that keeps the state (the step below
+            // takes that one), instead of one dependent load per cell.
+            constexpr int TB_AHEAD = 8;
+            uint32_t by[TB_AHEAD], sh[TB_AHEAD];
+            int cnt = 0, kl = k, cc = c;
+            bool go = true;
+#pragma unroll
+            for (int d = 0; d < TB_AHEAD; ++d) {
+                const int ci = i - (type == 'I' ? 0 : d);
+                if (d > 0 && type != 'D' && --cc < 0) {
+                    cc = cols - 1;
+                    if (--kl < 0) go = false;
+                }
+                if (ci < r0 || ci > r1 || ci < 1) go = false;
+                by[d] = 0;
+                sh[d] = 4u * (uint32_t)(cc & 1);
+                if (go) {
+                    by[d] = tj[base + ((uint64_t)(ci - r0 + kl) * G + kl) * lane_bytes + (cc >> 1)];
+                    cnt = d + 1;
+                }
+            }
+            // stay mask over the loaded cells; n = the leading run of ones
+            const uint32_t bit = type == 'M' ? 3u : (type == 'D' ? 4u : 8u);
+            uint32_t mask = 0;
+#pragma unroll
+            for (int d = 0; d < TB_AHEAD; ++d)
+                if (d < cnt && ((by[d] >> sh[d]) & bit) == 0u) mask |= 1u << d;
+            const int n = (int)__builtin_ctz(~mask);
+            if (n >= 2) {
+                const int adv = n - 1;
+                for (int d = 0; d < adv; ++d) slot[pos - 1 - d] = (uint8_t)type;
+                pos -= adv;
+                const int64_t st = type == 'M' ? (int64_t)la - lb : (type == 'D' ? -(int64_t)lb : (int64_t)la);
+                const int64_t d1 = dnum + st, d2 = dnum + (int64_t)adv * st;
+                maxdev = max(maxdev, max(d1 < 0 ? -d1 : d1, d2 < 0 ? -d2 : d2));
+                dnum = d2;
+                if (type != 'I') i -= adv;
+                if (type != 'D') {
+                    j -= adv;
+                    for (int d = 0; d < adv; ++d)
+                        if (--c < 0) {
+                            c = cols - 1;
+                            --k;
+                        }
+                }
+            }
+        }
         slot[--pos] = (uint8_t)type;
         uint32_t nib = 0;
         if (i > 0 && j > 0) {
