@@ -1183,14 +1183,15 @@ struct DpItem {
 
 // Relative cost per computed cell of the stripe shapes, used to pick a banded
 // job's shape (which decides how much of each stripe lies outside the band).
-// Measured on MI355X over a whole 100k-gap first pass, where a round's
-// shapes share one forward kernel: per shape alone (a class of B's width at a
-// time) 64-lane stripes of 16 columns evaluate cells fastest and 16-lane
-// stripes slowest (2-4x), but weights taken from those runs pick more
-// 16-column stripes, whose wide kernel then slows the round's other jobs, and
-// lose 6-15% on the whole pass; these weights gave the fastest pass.
-constexpr double COST_PER_CELL[3] = {1.30, 1.10, 1.00};   // 64-lane stripes of cols 4, 8, 16
-constexpr double COST_PER_CELL_G16[2] = {1.30, 1.10};      // 16-lane stripes of cols 4, 8
+// Per shape alone (one class of B's width at a time, MI355X) 64-lane stripes
+// of 16 columns evaluate cells fastest and 16-lane stripes slowest (2-4x),
+// but a round's shapes share one forward kernel, and weights taken from those
+// runs pick more 16-column stripes, whose wide kernel then slows the round's
+// other jobs.  These weights were chosen by timing whole banded runs (learned
+// and constant bands, edge fallback included) over 100k training gaps: the
+// fastest learned-band run of the sets tried.
+constexpr double COST_PER_CELL[3] = {1.34, 1.00, 1.30};   // 64-lane stripes of cols 4, 8, 16
+constexpr double COST_PER_CELL_G16[2] = {2.40, 1.50};      // 16-lane stripes of cols 4, 8
 
 
 }  // namespace
@@ -1225,6 +1226,30 @@ int dp_scratch_init(int device)
 }
 
 uint64_t dp_scratch_bytes() { return DP_SCRATCH_BYTES; }
+
+// The banded shapes' weights: COST_PER_CELL, COST_PER_CELL_G16, or five
+// comma-separated values in MH_DP_BAND_COST (read once per process, for sweeps)
+static const double *band_cost_weights()
+{
+    static double w[5];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (int q = 0; q < 5; ++q) w[q] = q < 3 ? COST_PER_CELL[q] : COST_PER_CELL_G16[q - 3];
+        const char *e = getenv("MH_DP_BAND_COST");
+        if (!e) return;
+        double v[5];
+        int n = 0;
+        for (const char *p = e; *p && n < 5;) {
+            char *end = nullptr;
+            const double x = strtod(p, &end);
+            if (end == p || !(x > 0.0)) return;
+            v[n++] = x;
+            p = *end == ',' ? end + 1 : end;
+        }
+        if (n == 5) std::memcpy(w, v, sizeof(v));
+    });
+    return w;
+}
 
 // MH_DP_COLS forces one shape: 4, 8 or 16 (64-lane stripes of that many
 // columns a lane) or 36, 40 (16-lane stripes of 4, 8 columns, four jobs a
@@ -1294,13 +1319,14 @@ uint32_t dp_cols_for(uint32_t la, uint32_t lb, uint32_t band, int force, bool wi
     double best_cost = 0;
     const char *g16e = getenv("MH_DP_BAND_G16");
     const int nshape = wide_only || (g16e && g16e[0] == '0') ? 3 : 5;
+    const double *wt = band_cost_weights();
     for (int q = 0; q < nshape; ++q) {
         const uint32_t g = q < 3 ? 64u : 16u;
         const uint32_t c = q < 3 ? 4u << q : 4u << (q - 3), sw = g * c;
         const double ns = (double)((lb + sw - 1) / sw);
         double rows = (double)la * (sw + 2.0 * band) / lb + 1.0;
         if (rows > la) rows = la;
-        const double cost = ns * (rows + (g - 1.0)) * sw * (q < 3 ? COST_PER_CELL[q] : COST_PER_CELL_G16[q - 3]);
+        const double cost = ns * (rows + (g - 1.0)) * sw * wt[q];
         if (q == 0 || cost < best_cost) {
             best = g == 64 ? c : GROUP_TAG + c;
             best_cost = cost;

@@ -23,7 +23,7 @@ from . import (MLP_IN, Mlp, band_cells, gap_features_device, global_align_banded
 MIN_BAND = 4
 DELTA_GRID = np.round(np.arange(0.0, 3.01, 0.05), 3)
 # grids measured by train_band.py on a calibration chunk
-CALIBRATION_DELTAS = (0.6, 0.75, 0.9, 1.05, 1.2, 1.4)
+CALIBRATION_DELTAS = (0.6, 0.75, 0.9, 1.05, 1.2, 1.4, 1.7, 2.0)
 CALIBRATION_CONSTANTS = (48, 64, 90, 117, 150, 200, 256)
 
 

@@ -35,6 +35,76 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def calibrate(ctx, mlp, chunk):
+    """delta and the best constant band measured on one chunk: the banded DP
+    with its fallback timed for every candidate, results compared with the
+    full DP's; among settings with no mismatch and within 5% of the fastest,
+    the widest."""
+    import mauvehip
+    from mauvehip import band
+    A, ao, al, B, bo, bl = chunk
+    cal = band.DeviceJobs(ctx, A, B, mauvehip.make_jobs(ao, al, bo, bl))
+    cal.features()
+    cal.align(None)
+    ref_res, _ = cal.results()
+    ref_paths = cal.paths()
+    ycal = band.predict_output(ctx, mlp, cal)
+
+    def measure(bands):
+        cal.align(bands)                        # warm (arena, caches)
+        t0 = time.perf_counter()
+        st = cal.align(bands)
+        el = time.perf_counter() - t0
+        r, _ = cal.results()
+        sm, pm = band.count_mismatches(ref_res, ref_paths, r, cal.paths())
+        return el, sm + pm, st
+    trials = {}
+    for d in band.CALIBRATION_DELTAS:
+        el, mm, st = measure(band.bands_from_output(ycal, d))
+        trials[float(d)] = {"seconds": el, "mismatches": mm, "cells": int(st.cells),
+                            "reruns": [int(st.rerun_doubled), int(st.rerun_full)]}
+        log(f"[train] delta {d}: {el:.3f}s, mismatches {mm}, reruns {st.rerun_doubled}/{st.rerun_full}")
+    # the rarer a rerun, the smaller the chance that a large gap's full-DP
+    # rerun (one wavefront, a few hundred ms) lands in a batch: among settings
+    # within 5% of the fastest, take the widest
+    ok = [d for d, v in trials.items() if v["mismatches"] == 0] or list(trials)
+    tbest = min(trials[d]["seconds"] for d in ok)
+    delta = max(d for d in ok if trials[d]["seconds"] <= 1.05 * tbest)
+    const = {}
+    for w in band.CALIBRATION_CONSTANTS:
+        el, mm, st = measure(np.full(cal.n, w, dtype=np.uint32))
+        const[int(w)] = {"seconds": el, "mismatches": mm, "cells": int(st.cells)}
+        log(f"[train] constant band {w}: {el:.3f}s, mismatches {mm}")
+    okc = [w for w, v in const.items() if v["mismatches"] == 0] or list(const)
+    tbc = min(const[w]["seconds"] for w in okc)
+    wc = max(w for w in okc if const[w]["seconds"] <= 1.05 * tbc)
+    cal.free()
+    return delta, wc, {"delta": delta, "calibration_seconds_learned": trials[delta]["seconds"],
+                       "best_constant_band": wc, "calibration_seconds_constant": const[wc]["seconds"],
+                       "calibration_trials": {str(k): v for k, v in trials.items()},
+                       "calibration_constants": {str(k): v for k, v in const.items()}}
+
+
+def recalibrate(ctx, chunk, args):
+    """--recalibrate: the model's parameters and training record kept, its
+    calibration measured again."""
+    import mauvehip
+    from mauvehip import band
+    params, _, _ = band.load_model(args.recalibrate)
+    with np.load(args.recalibrate, allow_pickle=False) as z:
+        info = json.loads(str(z["info_json"]))
+    mlp = mauvehip.Mlp(ctx, params=params)
+    delta, wc, cal = calibrate(ctx, mlp, chunk)
+    info.update(cal)
+    band.save_model(args.out, params, delta, {"info_json": json.dumps(info)})
+    log(f"[train] recalibrated: delta {delta}: {cal['calibration_seconds_learned']:.3f}s; best constant band {wc}: "
+        f"{cal['calibration_seconds_constant']:.3f}s (calibration chunk)")
+    print(json.dumps(cal), flush=True)
+    mlp.free()
+    ctx.close()
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--rows", type=int, default=1_000_000)
@@ -45,6 +115,9 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--recalibrate", default=None,
+                    help="keep this model's parameters and training record; measure delta and the best "
+                         "constant band again under the current kernels (first chunk only)")
     args = ap.parse_args(argv)
 
     from mauvehip import synth
@@ -56,6 +129,8 @@ def main(argv=None):
     chunks = []
     ss = np.random.SeedSequence(args.seed)
     kids = ss.spawn((args.rows + args.chunk - 1) // args.chunk)
+    if args.recalibrate:
+        kids = kids[:1]
     for c, kid in enumerate(kids):
         n = min(args.chunk, args.rows - c * args.chunk)
         cs = int(kid.generate_state(1)[0])
@@ -63,6 +138,8 @@ def main(argv=None):
         log(f"[train] generated chunk {c + 1}/{len(kids)} ({n} gaps) at {time.time() - t0:.0f}s")
 
     ctx = mauvehip.Context(0)
+    if args.recalibrate:
+        return recalibrate(ctx, chunks[0], args)
     X = ctx.alloc(4 * mauvehip.MLP_IN * args.rows + 16)
     labels = np.zeros(args.rows, dtype=np.uint32)
     la_all = np.zeros(args.rows, dtype=np.int64)
@@ -102,54 +179,15 @@ def main(argv=None):
     ctx.d2h(y, Y)
     resid = t - y
     # calibration by measurement on the first chunk
-    A, ao, al, B, bo, bl = chunks[0]
-    cal = band.DeviceJobs(ctx, A, B, mauvehip.make_jobs(ao, al, bo, bl))
-    cal.features()
-    cal.align(None)
-    ref_res, _ = cal.results()
-    ref_paths = cal.paths()
-    ycal = band.predict_output(ctx, mlp, cal)
-
-    def measure(bands):
-        cal.align(bands)                        # warm (arena, caches)
-        t0 = time.perf_counter()
-        st = cal.align(bands)
-        el = time.perf_counter() - t0
-        r, _ = cal.results()
-        sm, pm = band.count_mismatches(ref_res, ref_paths, r, cal.paths())
-        return el, sm + pm, st
-    trials = {}
-    for d in band.CALIBRATION_DELTAS:
-        el, mm, st = measure(band.bands_from_output(ycal, d))
-        trials[float(d)] = {"seconds": el, "mismatches": mm, "cells": int(st.cells),
-                            "reruns": [int(st.rerun_doubled), int(st.rerun_full)]}
-        log(f"[train] delta {d}: {el:.3f}s, mismatches {mm}, reruns {st.rerun_doubled}/{st.rerun_full}")
-    # the rarer a rerun, the smaller the chance that a large gap's full-DP
-    # rerun (one wavefront, a few hundred ms) lands in a batch: among settings
-    # within 5% of the fastest, take the widest
-    ok = [d for d, v in trials.items() if v["mismatches"] == 0] or list(trials)
-    tbest = min(trials[d]["seconds"] for d in ok)
-    delta = max(d for d in ok if trials[d]["seconds"] <= 1.05 * tbest)
-    const = {}
-    for w in band.CALIBRATION_CONSTANTS:
-        el, mm, st = measure(np.full(cal.n, w, dtype=np.uint32))
-        const[int(w)] = {"seconds": el, "mismatches": mm, "cells": int(st.cells)}
-        log(f"[train] constant band {w}: {el:.3f}s, mismatches {mm}")
-    okc = [w for w, v in const.items() if v["mismatches"] == 0] or list(const)
-    tbc = min(const[w]["seconds"] for w in okc)
-    wc = max(w for w in okc if const[w]["seconds"] <= 1.05 * tbc)
-    cal.free()
+    delta, wc, cal = calibrate(ctx, mlp, chunks[0])
     info = {"rows": args.rows, "seed": args.seed, "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
             "epoch_loss": [float(v) for v in curve], "train_seconds": round(train_s, 2),
-            "label_dp_cells": cells, "delta": delta, "calibration_seconds_learned": trials[delta]["seconds"],
-            "best_constant_band": wc, "calibration_seconds_constant": const[wc]["seconds"],
-            "calibration_trials": {str(k): v for k, v in trials.items()},
-            "calibration_constants": {str(k): v for k, v in const.items()},
+            "label_dp_cells": cells, **cal,
             "residual_std": float(resid.std()), "label_quantiles": {
                 str(q): float(np.percentile(labels, q)) for q in (50, 90, 99, 100)}}
     band.save_model(args.out, mlp.params(), delta, {"info_json": json.dumps(info)})
-    log(f"[train] delta {delta}: {trials[delta]['seconds']:.3f}s; best constant band {wc}: "
-        f"{const[wc]['seconds']:.3f}s (calibration chunk)")
+    log(f"[train] delta {delta}: {cal['calibration_seconds_learned']:.3f}s; best constant band {wc}: "
+        f"{cal['calibration_seconds_constant']:.3f}s (calibration chunk)")
     print(json.dumps(info), flush=True)
     for p in (X, T, Y):
         ctx.free(p)
