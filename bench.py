@@ -533,12 +533,14 @@ def full_leg(ctx, g, r, seed, steps, warmup):
     el = (time.perf_counter() - t0) / steps
     cells = int(rs.profile_cells + rs.window_cells)
     kern = {}
+    raw = {}
     for name in ("sml_pack", "sml_keys", "sml_sort", "join", "join_write", "extend", "partition", "dp_forward",
                  "dp_traceback", "host_replay", "host_dp_plan", "host_refine_prefix", "host_refine_pieces", "host_refine_plan",
                  "host_refine_dp", "host_refine_splice"):
         ms, n = ctx.kernel_time(name)
+        raw[name] = ms * n / steps
         kern[name] = {"avg_ms": round(ms, 4), "launches_per_step": n / steps, "ms_per_step": round(ms * n / steps, 3)}
-    fwd = kern["dp_forward"]["ms_per_step"]
+    fwd = raw["dp_forward"]
     gk = cells / (fwd * 1e-3) / 1e9 if fwd else 0.0
     roofline = {"kernel": "dp_forward", "bound": "valu", "achieved": round(gk, 2), "peak": round(DP_PEAK_GCUPS, 1),
                 "unit": "GCUPS", "frac": round(gk / DP_PEAK_GCUPS, 4),
