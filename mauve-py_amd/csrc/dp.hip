@@ -1399,6 +1399,7 @@ const FwdVariant FWD_VARIANTS[] = {
     {reinterpret_cast<const void *>(&dp_fwd<16, 3, true>), 16, 3, true},
     {reinterpret_cast<const void *>(&dp_fwd<16, 2, true>), 16, 2, true},
     {reinterpret_cast<const void *>(&dp_fwd<8, 4, true>), 8, 4, true},
+    {reinterpret_cast<const void *>(&dp_fwd<16, 4, false>), 16, 4, false},
 };
 constexpr int N_FWD_VARIANTS = (int)(sizeof(FWD_VARIANTS) / sizeof(FWD_VARIANTS[0]));
 static_assert(N_FWD_VARIANTS <= MH_DP_MAX_VARIANTS, "context graph slots");

@@ -51,7 +51,8 @@ SHAPES = {
 WIDE = {0: "16 cols, 3 waves/SIMD, one workgroup per item",
         2: "16 cols, 2 waves/SIMD, one workgroup per item",
         3: "16 cols, 3 waves/SIMD, persistent",
-        4: "16 cols, 2 waves/SIMD, persistent"}
+        4: "16 cols, 2 waves/SIMD, persistent",
+        6: "16 cols, 4 waves/SIMD, one workgroup per item (the 16-column shape spills)"}
 NARROW = {1: "8 cols, 4 waves/SIMD, one workgroup per item",
           5: "8 cols, 4 waves/SIMD, persistent",
           0: "16 cols, 3 waves/SIMD, one workgroup per item"}

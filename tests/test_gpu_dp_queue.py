@@ -111,7 +111,7 @@ def test_graph_replays_match_direct_launch(ctx):
     assert ref_p == want_p and ref_s.tolist() == want_s.tolist()
 
 
-@pytest.mark.parametrize("variant", [0, 3, 4])
+@pytest.mark.parametrize("variant", [0, 2, 3, 4, 6])
 def test_mixed_batch_on_every_wide_variant(ctx, variant):
     pairs, a, b, jobs = _batch(20 + variant, MIXED)
     want_p, want_s = _oracle(pairs)
