@@ -1924,15 +1924,66 @@ __global__ void uq_bucket_kernel(const int64_t *__restrict__ off, uint64_t ns, u
     atomicAdd(&cnt[b], 1u);
 }
 
-// One wavefront per bucket: the first lane walks std::lower_bound's probes
-// (the comparisons are the reference's, one at a time); the insertion's shift
-// of the vector's tail runs across the wave, 64 entries a step from the end
-// (every lane reads its entry before any lane writes).
+// std::lower_bound over the bucket's vector v[0, n) for q, with libstdc++'s
+// probe sequence (MheCompare is not a strict weak order, so the probes must
+// be exactly those), run by a whole wavefront: the entries of the next six
+// levels of the probe tree from (first, len) -- 63 positions, one a lane --
+// are loaded at once into LDS, then the first lane takes up to six probes
+// from LDS; repeated until the range is empty.  One round of dependent loads
+// serves six probes.
+__device__ uint32_t uq_lower_bound(const UqSeeds &q, const uint32_t *__restrict__ v, uint32_t n, const EntU &qe,
+                                   EntU *tree)
+{
+    const int lane = threadIdx.x;
+    uint32_t first = 0, len = n;
+    while (len > 0) {
+        const int node = lane + 1;                    // heap index of this lane's probe, 1..63
+        if (node < 64) {
+            const int lvl = 31 - __builtin_clz((unsigned)node);
+            uint32_t f = first, l = len;
+            for (int bt = lvl - 1; bt >= 0 && l > 0; --bt) {
+                const uint32_t half = l >> 1, mid = f + half;
+                if ((node >> bt) & 1) {                   // the probe was below q: right half
+                    f = mid + 1;
+                    l = l - half - 1;
+                } else {
+                    l = half;
+                }
+            }
+            if (l > 0) uq_entry(q, v[f + (l >> 1)], true, tree[node]);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            int node = 1;
+            for (int s = 0; s < 6 && len > 0; ++s) {
+                const uint32_t half = len >> 1, mid = first + half;
+                if (mhe_less_k(tree[node], qe, q.k)) {
+                    first = mid + 1;
+                    len = len - half - 1;
+                    node = 2 * node + 1;
+                } else {
+                    len = half;
+                    node = 2 * node;
+                }
+            }
+        }
+        first = (uint32_t)__builtin_amdgcn_readfirstlane((int)first);
+        len = (uint32_t)__builtin_amdgcn_readfirstlane((int)len);
+        __syncthreads();
+    }
+    return first;
+}
+
+// One wavefront per bucket: std::lower_bound's probes as above (the
+// comparisons are the reference's, one at a time, by the first lane); the
+// insertion's shift of the vector's tail runs across the wave, 64 entries a
+// step from the end (every lane reads its entry before any lane writes).
 __global__ __launch_bounds__(64) void uq_replay_kernel(UqSeeds q, const uint32_t *__restrict__ order,
                                                        const uint32_t *__restrict__ bstart, uint32_t *__restrict__ vec,
                                                        uint32_t *__restrict__ nent,
                                                        unsigned long long *__restrict__ coll)
 {
+    __shared__ EntU tree[64];
     const uint32_t b = blockIdx.x;
     const int lane = threadIdx.x;
     const uint32_t y0 = bstart[b], y1 = bstart[b + 1];
@@ -1942,22 +1993,22 @@ __global__ __launch_bounds__(64) void uq_replay_kernel(UqSeeds q, const uint32_t
     for (uint32_t y = y0; y < y1; ++y) {
         const uint32_t x = order[y];
         uint32_t pos = 0xFFFFFFFFu;     // insertion point, or none (a collision)
-        if (lane == 0) {
-            EntU cand;
-            uq_entry(q, x, false, cand);
-            auto at = [&](size_t i) {
-                EntU e;
-                uq_entry(q, v[i], true, e);
-                return e;
-            };
-            size_t it = lower_bound_at(ne, cand, q.k, at);
-            if (it != ne && !mhe_less_k(cand, at(it), q.k)) {   // !comp(*it, cand) holds at lower_bound
-                ++c;
-            } else {
-                EntU e;
-                uq_entry(q, x, true, e);
-                pos = (uint32_t)lower_bound_at(ne, e, q.k, at);
-            }
+        EntU cand;
+        uq_entry(q, x, false, cand);
+        const uint32_t it = uq_lower_bound(q, v, ne, cand, tree);
+        bool hit = false;
+        if (lane == 0 && it != ne) {
+            EntU at_it;
+            uq_entry(q, v[it], true, at_it);
+            hit = !mhe_less_k(cand, at_it, q.k);   // !comp(*it, cand) holds at lower_bound
+        }
+        hit = __builtin_amdgcn_readfirstlane((int)hit) != 0;
+        if (hit) {
+            if (lane == 0) ++c;
+        } else {
+            EntU e;
+            uq_entry(q, x, true, e);
+            pos = uq_lower_bound(q, v, ne, e, tree);
         }
         pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)pos);
         if (pos == 0xFFFFFFFFu) continue;
