@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--band-only", action="store_true",
                     help="run only the learned-band leg and print its record (for sweeps)")
+    ap.add_argument("--dp-only", action="store_true",
+                    help="run only the batched-DP leg and print its record (for profiling its kernels)")
     ap.add_argument("--whole-only", action="store_true",
                     help="run only the whole-path leg (for profiling its kernels on their own)")
     args = ap.parse_args()
@@ -76,6 +78,13 @@ def main():
         bandset = synth.dp_jobs(n=args.band_pairs, seed=402, workers=min(16, os.cpu_count() or 1))
         ctx = mauvehip.Context(0)
         print(json.dumps({"learned_band": learned_band_leg(ctx, bandset)}), flush=True)
+        ctx.close()
+        return 0
+    if args.dp_only:
+        import mauvehip
+        dp = synth.dp_jobs(n=args.dp_pairs, seed=301, workers=min(16, os.cpu_count() or 1))
+        ctx = mauvehip.Context(0)
+        print(json.dumps({"dp_batch": dp_batch_leg(ctx, dp, not args.no_cpu_baseline)}), flush=True)
         ctx.close()
         return 0
     t0 = time.time()
@@ -207,43 +216,7 @@ def main():
         }
 
     if dp is not None:
-        A, ao, al, B, bo, bl = dp
-        jobs_c3 = mauvehip.make_jobs(ao, al, bo, bl)
-        cells3 = int((al.astype(np.int64) * bl).sum())
-        dA3, dB3 = ctx.alloc(len(A) + 16), ctx.alloc(len(B) + 16)
-        ctx.h2d(dA3, A)
-        ctx.h2d(dB3, B)
-        dP3 = ctx.alloc(int((al + bl).sum()) + 16)
-        dR3 = ctx.alloc(16 * len(al) + 16)
-        # warm-up over the whole batch: the trace arena (sized once per call
-        # to the largest round, up to 70% of free HBM) is allocated here, not
-        # in the timed call
-        mauvehip.global_align_device(ctx, dA3, dB3, jobs_c3, dP3, dR3)
-        ctx.synchronize()
-        ctx.reset_kernel_times()
-        t0 = time.perf_counter()
-        mauvehip.global_align_device(ctx, dA3, dB3, jobs_c3, dP3, dR3)
-        ctx.synchronize()
-        el3 = time.perf_counter() - t0
-        fwd_ms, fwd_n = ctx.kernel_time("dp_forward")
-        tb_ms, tb_n = ctx.kernel_time("dp_traceback")
-        gc = cells3 / el3 / 1e9
-        gk = cells3 / (fwd_ms * fwd_n * 1e-3) / 1e9 if fwd_n else 0
-        out["dp_batch"] = {"pairs": len(al), "cells": cells3, "seconds": round(el3, 3), "gcups": round(gc, 1),
-                           "forward_kernel_gcups": round(gk, 1), "rounds": fwd_n,
-                           "traceback_ms": round(tb_ms * tb_n, 2),
-                           "roofline": {"kernel": "dp_forward", "bound": "valu", "achieved": round(gk, 1),
-                                        "peak": round(DP_PEAK_GCUPS, 1), "unit": "GCUPS",
-                                        "frac": round(gk / DP_PEAK_GCUPS, 4)}}
-        log(f"[bench] DP batch {len(al)} pairs {cells3 / 1e12:.2f} Tcells: {gc:.1f} GCUPS end-to-end, "
-            f"{gk:.1f} GCUPS forward kernel")
-        if not args.no_cpu_baseline:
-            res3 = np.zeros(len(al), dtype=mauvehip.DP_RESULT_DTYPE)
-            ctx.d2h(res3, dR3)
-            out["dp_batch"]["cpu_baseline"] = cpu_dp_batch(dp, res3["score"])
-            log(f"[bench] CPU NWSmall {json.dumps(out['dp_batch']['cpu_baseline'])}")
-        for p in (dA3, dB3, dP3, dR3):
-            ctx.free(p)
+        out["dp_batch"] = dp_batch_leg(ctx, dp, not args.no_cpu_baseline)
 
     if bandset is not None:
         out["learned_band"] = learned_band_leg(ctx, bandset)
@@ -674,6 +647,52 @@ def cpu_dp_batch(dp, gpu_scores):
 
 
 BAND_REPS = 3
+
+
+def dp_batch_leg(ctx, dp, cpu):
+    """The batched full DP over the 100k-pair distribution (C3): operands
+    resident, one warm-up call over the whole batch, then one timed call."""
+    import mauvehip
+    out = {}
+    A, ao, al, B, bo, bl = dp
+    jobs_c3 = mauvehip.make_jobs(ao, al, bo, bl)
+    cells3 = int((al.astype(np.int64) * bl).sum())
+    dA3, dB3 = ctx.alloc(len(A) + 16), ctx.alloc(len(B) + 16)
+    ctx.h2d(dA3, A)
+    ctx.h2d(dB3, B)
+    dP3 = ctx.alloc(int((al + bl).sum()) + 16)
+    dR3 = ctx.alloc(16 * len(al) + 16)
+    # warm-up over the whole batch: the trace arena (sized once per call
+    # to the largest round, up to 70% of free HBM) is allocated here, not
+    # in the timed call
+    mauvehip.global_align_device(ctx, dA3, dB3, jobs_c3, dP3, dR3)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    ctx.reset_kernel_times()
+    t0 = time.perf_counter()
+    mauvehip.global_align_device(ctx, dA3, dB3, jobs_c3, dP3, dR3)
+    ctx.synchronize()
+    el3 = time.perf_counter() - t0
+    fwd_ms, fwd_n = ctx.kernel_time("dp_forward")
+    tb_ms, tb_n = ctx.kernel_time("dp_traceback")
+    gc = cells3 / el3 / 1e9
+    gk = cells3 / (fwd_ms * fwd_n * 1e-3) / 1e9 if fwd_n else 0
+    out["dp_batch"] = {"pairs": len(al), "cells": cells3, "seconds": round(el3, 3), "gcups": round(gc, 1),
+                       "forward_kernel_gcups": round(gk, 1), "rounds": fwd_n,
+                       "traceback_ms": round(tb_ms * tb_n, 2),
+                       "roofline": {"kernel": "dp_forward", "bound": "valu", "achieved": round(gk, 1),
+                                    "peak": round(DP_PEAK_GCUPS, 1), "unit": "GCUPS",
+                                    "frac": round(gk / DP_PEAK_GCUPS, 4)}}
+    log(f"[bench] DP batch {len(al)} pairs {cells3 / 1e12:.2f} Tcells: {gc:.1f} GCUPS end-to-end, "
+        f"{gk:.1f} GCUPS forward kernel")
+    if cpu:
+        res3 = np.zeros(len(al), dtype=mauvehip.DP_RESULT_DTYPE)
+        ctx.d2h(res3, dR3)
+        out["dp_batch"]["cpu_baseline"] = cpu_dp_batch(dp, res3["score"])
+        log(f"[bench] CPU NWSmall {json.dumps(out['dp_batch']['cpu_baseline'])}")
+    for p in (dA3, dB3, dP3, dR3):
+        ctx.free(p)
+    return out["dp_batch"]
 
 
 def c2_gap_band_leg(ctx, ref, mut, jobs):
