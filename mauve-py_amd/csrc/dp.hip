@@ -151,12 +151,34 @@ __device__ __forceinline__ int max3i(int a, int b, int c)
 // bit 3 MI; each lane stores COLS/2 bytes per step, so a wavefront stores
 // 32*COLS contiguous bytes per step.  Cell (i, j) of lane k, column c is at
 // byte  stripe_base + ((i - r0 + k)*64 + k)*COLS/2 + c/2, nibble c%2.
-template <int COLS, int G>
+//
+// PIPE: the job's stripes are shared by P wavefronts, wavefront p0 taking
+// stripes p0, p0 + P, ...: stripe s reads the boundary column that stripe
+// s - 1 (another wavefront) writes.  Each boundary value travels as one
+// 64-bit word, the value and a tag (the writing stripe's index + 1), in one
+// of two buffers by stripe parity (zeroed by the host before the round):
+// written with atomic stores and read with atomic loads at device scope, a
+// word is either the one stripe s - 1 wrote or visibly not, so no fence is
+// needed.  A reader fetches a chunk of rows one chunk ahead and checks the
+// tags when it is about to use them, fetching again the rows not yet
+// written.  A wait is bounded (DP_SPIN_MAX polls), past which it sets *err
+// and goes on, and the wavefront waits no more (so a stuck job ends within a
+// few bounds, its result flagged by the host as an error).  A buffer is
+// rewritten by stripe s + 1 only at rows stripe s has used: it computes a row
+// after stripe s published it, and stripe s publishes it after reading it.
+// Deadlock-free: parts take tickets in the order they start running, so a
+// part waits only on parts that run or have run, except the parts of the one
+// job whose parts are not all taken, and P is below the resident wavefronts.
+// The result is the same as one wavefront's.
+constexpr uint32_t DP_SPIN_MAX = 1u << 22;
+
+template <int COLS, int G, bool PIPE = false>
 __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restrict__ A,
                                        const uint8_t *__restrict__ B, uint8_t *__restrict__ tb,
                                        int *__restrict__ bnd, DpOut *__restrict__ out, uint32_t sr0,
                                        uint32_t sr1, uint32_t sr2, uint32_t sr3, int goh, int (*lds_in)[64],
-                                       int (*lds_out)[64])
+                                       int (*lds_out)[64], int p0 = 0, int P = 1,
+                                       uint32_t *__restrict__ err = nullptr)
 {
     constexpr int STRIPE_W = G * COLS;
     constexpr int NW = (COLS + 7) / 8;               // 32-bit trace words per lane per step
@@ -178,15 +200,58 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
     int finM = NEG, finD = NEG, finI = NEG;
     int prev_r0 = 1, prev_r1 = 0;     // rows of the previous stripe's last column held in bnd
     uint64_t tbs = 0;                 // byte offset of the stripe's trace block
+    bool gave_up = false;             // PIPE: a wait ran past its bound
 
-    for (int s = 0; s < nstripes; ++s) {
+    for (int s = PIPE ? p0 : 0; s < nstripes; s += PIPE ? P : 1) {
         int r0, r1;
         stripe_rows(jb.la, jb.lb, jb.band, STRIPE_W, s, r0, r1);
+        // PIPE: the tagged boundary words read (written by stripe s - 1, tag s) and written (tag s + 1)
+        unsigned long long *pin = nullptr, *pout = nullptr;
+        if constexpr (PIPE) {
+            // stripe s - 1 runs on another wavefront: its rows, and this
+            // stripe's trace block after the blocks of the stripes before it
+            prev_r0 = 1;
+            prev_r1 = 0;
+            if (s > 0) stripe_rows(jb.la, jb.lb, jb.band, STRIPE_W, s - 1, prev_r0, prev_r1);
+            tbs = 0;
+            for (int q = 0; q < s; ++q) {
+                int a0, a1;
+                stripe_rows(jb.la, jb.lb, jb.band, STRIPE_W, q, a0, a1);
+                if (a1 >= a0) tbs += (uint64_t)(a1 - a0 + G) * G * LANE_BYTES;
+            }
+            unsigned long long *pb = reinterpret_cast<unsigned long long *>(bnd + jb.bnd_off);
+            pin = pb + (size_t)((s + 1) & 1) * 3 * nr;
+            pout = pb + (size_t)(s & 1) * 3 * nr;
+        }
         if (r1 < r0) {
             prev_r0 = 1;
             prev_r1 = 0;
             continue;
         }
+        // PIPE: until every lane with want set holds the words stripe s - 1
+        // wrote for row i (tag s), fetch them again
+        auto settle = [&](bool want, int i, unsigned long long &wM, unsigned long long &wD,
+                          unsigned long long &wI) {
+            if constexpr (PIPE) {
+                uint32_t spins = 0;
+                for (;;) {
+                    const bool bad = want && ((uint32_t)(wM >> 32) != (uint32_t)s ||
+                                              (uint32_t)(wD >> 32) != (uint32_t)s || (uint32_t)(wI >> 32) != (uint32_t)s);
+                    if (__ballot(bad) == 0 || gave_up) break;
+                    if (++spins > DP_SPIN_MAX) {
+                        if (lane == 0) atomicOr(err, 1u);
+                        gave_up = true;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                    if (bad) {
+                        wM = __hip_atomic_load(pin + i - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        wD = __hip_atomic_load(pin + nr + i - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        wI = __hip_atomic_load(pin + 2 * nr + i - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+            }
+        };
         const int nrows = r1 - r0 + 1;
         const int j0 = s * STRIPE_W + 1;
         const int jl = j0 + lane * COLS;
@@ -218,6 +283,21 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
             lpM = jl == 1 ? 0 : NEG;
             lpD = NEG + goh;
             lpI = jl == 1 ? NEG + goh : goh;
+        } else if constexpr (PIPE) {
+            const int i = r0 - 1;
+            const bool want = lane == 0 && s > 0 && i >= prev_r0 && i <= prev_r1;
+            unsigned long long wM = 0, wD = 0, wI = 0;
+            if (want) {
+                wM = __hip_atomic_load(pin + i - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                wD = __hip_atomic_load(pin + nr + i - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                wI = __hip_atomic_load(pin + 2 * nr + i - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            settle(want, i, wM, wD, wI);
+            if (want) {
+                lpM = (int)(uint32_t)wM;
+                lpD = (int)(uint32_t)wD;
+                lpI = (int)(uint32_t)wI;
+            }
         } else if (lane == 0 && s > 0 && r0 - 1 >= prev_r0 && r0 - 1 <= prev_r1) {
             lpM = bM[r0 - 2];
             lpD = bD[r0 - 2];
@@ -230,16 +310,44 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
         // where it computed that row (else NEG), or column 0 (M = NEG,
         // D' = 0 + goh, I' = NEG + goh); and the A codes.  The next chunk is
         // loaded one chunk ahead.
+        // (PIPE: the words are fetched here, for rows this stripe computes,
+        // and settled before use)
+        unsigned long long wM = 0, wD = 0, wI = 0;
+        bool want = false;
+        int wrow = 0;
         auto left_in = [&](int x, int &m, int &d, int &ii, int &ac) {
             const int i = r0 + x;
             if (s == 0) {
                 m = NEG; d = goh; ii = NEG + goh;
             } else if (i >= prev_r0 && i <= prev_r1) {
-                m = bM[i - 1]; d = bD[i - 1]; ii = bI[i - 1];
+                if constexpr (PIPE) {
+                    want = i <= r1;
+                    wrow = i;
+                    if (want) {
+                        wM = __hip_atomic_load(pin + i - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        wD = __hip_atomic_load(pin + nr + i - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        wI = __hip_atomic_load(pin + 2 * nr + i - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    m = NEG; d = NEG; ii = NEG;
+                } else {
+                    m = bM[i - 1]; d = bD[i - 1]; ii = bI[i - 1];
+                }
             } else {
+                if constexpr (PIPE) want = false;
                 m = NEG; d = NEG; ii = NEG;
             }
             ac = i <= r1 ? a[i - 1] : 0;
+        };
+        // PIPE: the fetched words, once settled, as the lane's inputs
+        auto take = [&](int &m, int &d, int &ii) {
+            if constexpr (PIPE) {
+                settle(want, wrow, wM, wD, wI);
+                if (want) {
+                    m = (int)(uint32_t)wM;
+                    d = (int)(uint32_t)wD;
+                    ii = (int)(uint32_t)wI;
+                }
+            }
         };
         int nM, nD, nI, nA;
         left_in(lane, nM, nD, nI, nA);
@@ -248,6 +356,7 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
         for (int t = 0; t < nsteps; ++t) {
             const int q = t & (G - 1);
             if (q == 0) {
+                take(nM, nD, nI);
                 lds_in[0][gb + lane] = nM;
                 lds_in[1][gb + lane] = nD;
                 lds_in[2][gb + lane] = nI;
@@ -345,9 +454,19 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
                     const int x = (xl & ~(G - 1)) + lane;
                     if (x < nrows) {
                         const int r = r0 - 1 + x;
-                        bM[r] = lds_out[0][gb + lane];
-                        bD[r] = lds_out[1][gb + lane];
-                        bI[r] = lds_out[2][gb + lane];
+                        if constexpr (PIPE) {
+                            const unsigned long long tag = (unsigned long long)(uint32_t)(s + 1) << 32;
+                            __hip_atomic_store(pout + r, tag | (uint32_t)lds_out[0][gb + lane], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                            __hip_atomic_store(pout + nr + r, tag | (uint32_t)lds_out[1][gb + lane], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                            __hip_atomic_store(pout + 2 * nr + r, tag | (uint32_t)lds_out[2][gb + lane],
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        } else {
+                            bM[r] = lds_out[0][gb + lane];
+                            bD[r] = lds_out[1][gb + lane];
+                            bI[r] = lds_out[2][gb + lane];
+                        }
                     }
                 }
             }
@@ -355,9 +474,10 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
         prev_r0 = r0;
         prev_r1 = r1;
         tbs += (uint64_t)(nrows + G - 1) * G * LANE_BYTES;
-        __threadfence();
+        if constexpr (!PIPE) __threadfence();
     }
-    if (lane == fin_lane) {
+    // (PIPE: the wavefront that ran the last stripe)
+    if (lane == fin_lane && (!PIPE || (p0 < nstripes && (nstripes - 1 - p0) % P == 0))) {
         // back to unshifted D and I for the final pick, MU/nwsmall.cpp:641-656
         finD -= goh;
         finI -= goh;
@@ -392,7 +512,10 @@ constexpr uint32_t ROWS_MAX_LB = 256;   // widest job run four to a wavefront
 // launches of a round take one pointer and are captured in a graph once per
 // context; a round only sets the grid sizes of the graph's nodes.
 constexpr int ROW_SHAPES = 7;
-enum : uint32_t { QK_STRIPE = 0, QK_ROWS = 1, QK_WINDOW = QK_ROWS + ROW_SHAPES, QK_TINY = QK_WINDOW + 1 };
+// QK_PIPE: 64-lane stripe jobs whose stripes pipe_p wavefronts share (an
+// item is one wavefront's part: job first + item / pipe_p, part item % pipe_p)
+enum : uint32_t { QK_STRIPE = 0, QK_ROWS = 1, QK_WINDOW = QK_ROWS + ROW_SHAPES, QK_TINY = QK_WINDOW + 1,
+                  QK_PIPE = QK_TINY + 1 };
 constexpr int DPQ_MAXB = 48;
 constexpr int DPQ_CURSORS = DPQ_MAXB;
 
@@ -405,7 +528,9 @@ struct DpRound {
     uint8_t *paths;
     mh_dp_result *res;
     uint32_t *dev;
-    uint32_t *cursor;          // DPQ_CURSORS counters, zeroed at the start of every round
+    uint32_t *cursor;          // DPQ_CURSORS counters, zeroed at the start of every round; the last
+                               // one is the round's wait-bound error flag
+    uint32_t pipe_p;           // wavefronts a QK_PIPE job
     uint32_t sr[4];
     int32_t goh;
     uint32_t nbins;
@@ -417,7 +542,7 @@ struct DpRound {
 
 __device__ __forceinline__ uint32_t qk_per_item(uint32_t kind)
 {
-    return kind == QK_STRIPE || kind == QK_WINDOW ? 1u : (kind == QK_TINY ? 64u : 4u);
+    return kind == QK_STRIPE || kind == QK_WINDOW || kind == QK_PIPE ? 1u : (kind == QK_TINY ? 64u : 4u);
 }
 
 // ---- sliding-window banded DP ---------------------------------------------
@@ -1051,7 +1176,26 @@ template <int MAXC>
 __device__ __forceinline__ void run_item(DP_FWD_PARAMS, uint32_t kind, uint32_t first, uint32_t count,
                                          uint32_t item, int (*lds_in)[64], int (*lds_out)[64])
 {
-    if (kind == QK_STRIPE) {
+    if (kind == QK_PIPE) {
+        const uint32_t P = rd->pipe_p;
+        const uint32_t x = first + item / P;
+        const int p0 = (int)(item % P);
+        const DevJob jb = jobs[x];
+        uint32_t *err = cursor + (DPQ_CURSORS - 1);
+        if constexpr (MAXC >= 16) {
+            if (jb.cols == 16) {
+                dp_job<16, 64, true>(jb, A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out, p0, (int)P,
+                                     err);
+                return;
+            }
+        }
+        if (jb.cols == 8)
+            dp_job<8, 64, true>(jb, A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out, p0, (int)P,
+                                err);
+        else
+            dp_job<4, 64, true>(jb, A, B, tb, bnd, &outs[x], sr0, sr1, sr2, sr3, goh, lds_in, lds_out, p0, (int)P,
+                                err);
+    } else if (kind == QK_STRIPE) {
         const uint32_t x = first + item;
         const DevJob jb = jobs[x];
         if constexpr (MAXC >= 16) {
@@ -1113,7 +1257,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         uint32_t b = 0;
         while (b + 1 < nb && w >= rd->item0[b + 1]) ++b;
         if (b >= nb || w >= rd->item0[nb]) return;
-        run_item<MAXC>(DP_FWD_ARGS, rd->kind[b], rd->first[b], rd->count[b], w - rd->item0[b], lds_in, lds_out);
+        uint32_t item = w - rd->item0[b];
+        if (rd->kind[b] == QK_PIPE) {
+            // a ticket, so that the parts of a job are taken in order by
+            // running wavefronts (a part waits only on parts taken before it)
+            uint32_t v = 0;
+            if (threadIdx.x == 0) v = atomicAdd(&cursor[b], 1u);
+            item = (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+        }
+        run_item<MAXC>(DP_FWD_ARGS, rd->kind[b], rd->first[b], rd->count[b], item, lds_in, lds_out);
     } else {
         uint32_t b = 0;
         for (;;) {
@@ -1706,9 +1858,19 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
     }
     uint32_t sr[4];
     packed_rows(sr);
+    // pipelined stripe jobs: wavefronts a job by the round's size (up to
+    // DP_PIPE_MAX), or MH_DP_PIPE (0 or 1 keeps one wavefront a job); the
+    // no-allocation path keeps one
+    constexpr uint32_t DP_PIPE_MAX = 8;
+    int pipe_env = -1;
+    if (const char *e = getenv("MH_DP_PIPE")) pipe_env = std::max(0, std::min((int)DP_PIPE_MAX, atoi(e)));
+    int ncu = 0;
+    if (!fx && pipe_env != 0 && pipe_env != 1)
+        MH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
 
     uint64_t first = 0;
     std::vector<DpItem> shaped;
+    std::vector<char> piped;
     while (first < nitems) {
         // one round: as many jobs (in schedule order) as fit the trace budget
         uint64_t tbb = 0, bi = 0, last = first;
@@ -1780,6 +1942,51 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             std::copy(shaped.begin(), shaped.end(), items.begin() + (ptrdiff_t)b0);
         }
         const uint64_t n64 = rows_at[1];   // rows_at[1..]: the row jobs, 16 columns a lane first
+        // the forward variant: the wide one when a shape wider than 8 columns a lane is in the round
+        bool wide = false;
+        for (uint64_t z = 0; z < n64 && !wide; ++z) wide = items[first + z].cols == 16;
+        for (int q = 0; q < 4; ++q)
+            if (rows_at[q + 2] > rows_at[q + 1]) wide = true;
+        const int variant = dp_variant_for(wide);
+        // Pipelined stripe jobs.  A round with fewer work items than the
+        // forward launch's wavefront slots ends on its longest jobs' latency
+        // with SIMDs idle: the first 64-lane stripe jobs (largest first) with
+        // at least P stripes then run on P wavefronts each, as many as the
+        // idle slots hold, moved to the front of the queue.  P: the slots
+        // over the items (at least 2, at most DP_PIPE_MAX).
+        uint32_t pipe_p = 1;
+        uint64_t npipe = 0;
+        if (!fx && ncu > 0) {
+            uint64_t items_est = n64 + (nbig - nstripe) + (n - nbig + 63) / 64;
+            for (int q = 0; q < ROW_SHAPES; ++q) items_est += (rows_at[q + 2] - rows_at[q + 1] + 3) / 4;
+            const uint64_t slots = (uint64_t)ncu * 4 * (uint64_t)FWD_VARIANTS[variant].wpe;
+            const uint32_t pipe_cfg =
+                pipe_env > 1 ? (uint32_t)pipe_env
+                             : (uint32_t)std::min<uint64_t>(DP_PIPE_MAX, std::max<uint64_t>(2, slots / std::max<uint64_t>(1, items_est)));
+            if (items_est < slots) {
+                const uint64_t kmax = (slots - items_est) / (pipe_cfg - 1);
+                piped.assign(n64, 0);
+                for (uint64_t x = 0; x < n64 && npipe < kmax; ++x) {
+                    const DpItem &it = items[first + x];
+                    const uint64_t sw = 64ull * it.cols;
+                    if ((jobs[it.job].b_len + sw - 1) / sw >= pipe_cfg) {
+                        piped[x] = 1;
+                        ++npipe;
+                    }
+                }
+                if (npipe) {
+                    pipe_p = pipe_cfg;
+                    shaped.clear();
+                    for (uint64_t x = 0; x < n64; ++x)
+                        if (piped[x]) shaped.push_back(items[first + x]);
+                    for (uint64_t x = 0; x < n64; ++x)
+                        if (!piped[x]) shaped.push_back(items[first + x]);
+                    std::copy(shaped.begin(), shaped.end(), items.begin() + (ptrdiff_t)first);
+                    // (a count under the timers' names: pipelined jobs a round)
+                    host_time(ctx, "dp_pipe_jobs", (double)npipe);
+                }
+            }
+        }
         DevJob *dj = nullptr;
         std::vector<DevJob> host_jobs;
         if (fx) {
@@ -1806,7 +2013,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             for (uint64_t x = 0; x < n; ++x)
                 if (!wave_walk(items[first + x])) tbi[t++] = (uint32_t)x;
         }
-        uint64_t tw = 0, bo = 0;
+        uint64_t tw = 0, bo = 0, bo_pipe = 0;
         for (uint64_t x = 0; x < n; ++x) {
             const DpItem &it = items[first + x];
             const uint32_t j = it.job;
@@ -1824,6 +2031,11 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
             d.pad_ = 0;
             tw += (it.tb_bytes + 15) & ~15ull;
             bo += bnd_ints(it);
+            if (x < npipe) {
+                // tagged 64-bit words, in two buffers (stripes alternate by parity)
+                bo += 3 * bnd_ints(it);
+                bo_pipe = bo;
+            }
             if (cells) *cells += 2 * it.tb_bytes;
         }
         DevJob *d_jobs;
@@ -1858,6 +2070,8 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         uint32_t *d_tbi = reinterpret_cast<uint32_t *>(d_outs + n);
         MH_HIP(hipMemcpyAsync(d_jobs, dj, n * sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream));
         MH_HIP(hipMemcpyAsync(d_tbi, tbi, n * 4, hipMemcpyHostToDevice, ctx->stream));
+        // the pipelined jobs' boundary words (the first jobs') start untagged
+        if (bo_pipe) MH_HIP(hipMemsetAsync(d_bnd, 0, bo_pipe * 4, ctx->stream));
         // the round's descriptor: pointers, the trace-back split and the queue's bins
         DpRound hr;
         std::memset(&hr, 0, sizeof(hr));
@@ -1875,7 +2089,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         hr.goh = -200;
         hr.nwave = (uint32_t)nwave;
         hr.n = (uint32_t)n;
-        bool wide = false;
+        hr.pipe_p = pipe_p;
         auto add_bin = [&](uint32_t kind, uint64_t b0, uint64_t cnt) {
             if (cnt == 0) return;
             hr.kind[hr.nbins] = kind;
@@ -1885,30 +2099,30 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         };
         {
             // stripe jobs: a new bin where the trace size falls below the next power of 4
+            // the pipelined jobs first, an item a part
+            add_bin(QK_PIPE, 0, npipe * pipe_p);
             auto cls = [](uint64_t v) { return v ? (63 - __builtin_clzll(v)) / 2 : 0; };
-            uint64_t x = 0;
+            uint64_t x = npipe;
             while (x < n64) {
                 const int c0 = cls(items[first + x].tb_bytes);
                 uint64_t y = x + 1;
                 while (y < n64 && cls(items[first + y].tb_bytes) == c0) ++y;
-                // keep ROW_SHAPES + 2 bins for the row, window and one-lane jobs
-                if (hr.nbins + 1 >= (uint32_t)(DPQ_MAXB - ROW_SHAPES - 2)) y = n64;
+                // keep ROW_SHAPES + 2 bins for the row, window and one-lane
+                // jobs, and the last cursor for the wait-bound error flag
+                if (hr.nbins + 1 >= (uint32_t)(DPQ_MAXB - ROW_SHAPES - 3)) y = n64;
                 add_bin(QK_STRIPE, x, y - x);
                 x = y;
             }
-            for (uint64_t z = 0; z < n64; ++z)
-                if (items[first + z].cols == 16) wide = true;
-            for (int q = 0; q < ROW_SHAPES; ++q) {
-                add_bin(QK_ROWS + q, rows_at[q + 1], rows_at[q + 2] - rows_at[q + 1]);
-                if (q < 4 && rows_at[q + 2] > rows_at[q + 1]) wide = true;
-            }
+            for (int q = 0; q < ROW_SHAPES; ++q) add_bin(QK_ROWS + q, rows_at[q + 1], rows_at[q + 2] - rows_at[q + 1]);
             add_bin(QK_WINDOW, nstripe, nbig - nstripe);
             add_bin(QK_TINY, nbig, n - nbig);
         }
-        const int variant = dp_variant_for(wide);
+        MH_CHECK(hr.nbins < (uint32_t)DPQ_CURSORS, MH_ERR_INVALID, "DP round: too many queue bins");
         uint64_t nitems_q = 0;
         for (uint32_t q = 0; q < hr.nbins; ++q) {
-            const uint32_t per = hr.kind[q] == QK_STRIPE || hr.kind[q] == QK_WINDOW ? 1u : (hr.kind[q] == QK_TINY ? 64u : 4u);
+            const uint32_t per =
+                hr.kind[q] == QK_STRIPE || hr.kind[q] == QK_WINDOW || hr.kind[q] == QK_PIPE ? 1u
+                                                                                          : (hr.kind[q] == QK_TINY ? 64u : 4u);
             hr.item0[q] = (uint32_t)nitems_q;
             nitems_q += (hr.count[q] + per - 1) / per;
         }
@@ -1956,6 +2170,12 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         }
         // the host job table is reused by the next round
         MH_HIP(hipStreamSynchronize(ctx->stream));
+        if (npipe) {
+            uint32_t err = 0;
+            MH_HIP(hipMemcpyAsync(&err, hr.cursor + (DPQ_CURSORS - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+            MH_HIP(hipStreamSynchronize(ctx->stream));
+            MH_CHECK(err == 0, MH_ERR_DEVICE, "DP: a pipelined stripe's wait ran past its limit");
+        }
         first = last;
     }
     return MH_OK;
