@@ -435,11 +435,6 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
 #pragma unroll
                 for (int y = 0; y < LANE_BYTES; ++y) dst[y] = (uint8_t)(word[y >> 2] >> (8 * (y & 3)));
             }
-            if (last_stripe && i == la && lane == fin_lane) {
-#pragma unroll
-                for (int c = 0; c < COLS; ++c)
-                    if (c == fin_col) { finM = Mp[c]; finD = Dp[c]; finI = Ip[c]; }
-            }
             }
             if (!last_stripe) {
                 // the last lane just finished relative row t-(G-1) of the stripe's last column
@@ -470,6 +465,21 @@ __device__ __forceinline__ void dp_job(const DevJob &jb, const uint8_t *__restri
                     }
                 }
             }
+        }
+        if (last_stripe && r1 == la && lane == fin_lane) {
+            // the lane's registers hold row LA, the last it computed: the
+            // final cell's values, picked by way of LDS (the last stripe
+            // stages no boundary there), so that no per-column select on
+            // fin_col is held through the step loop
+#pragma unroll
+            for (int c = 0; c < COLS; ++c) {
+                lds_out[0][gb + c] = Mp[c];
+                lds_out[1][gb + c] = Dp[c];
+                lds_out[2][gb + c] = Ip[c];
+            }
+            finM = lds_out[0][gb + fin_col];
+            finD = lds_out[1][gb + fin_col];
+            finI = lds_out[2][gb + fin_col];
         }
         prev_r0 = r0;
         prev_r1 = r1;
