@@ -30,6 +30,7 @@
 #include "mh_internal.hpp"
 #include "dp_tune_table.h"
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <cstdlib>
 #include <utility>
@@ -166,9 +167,10 @@ __device__ __forceinline__ int max3i(int a, int b, int c)
 // few bounds, its result flagged by the host as an error).  A buffer is
 // rewritten by stripe s + 1 only at rows stripe s has used: it computes a row
 // after stripe s published it, and stripe s publishes it after reading it.
-// Deadlock-free: parts take tickets in the order they start running, so a
-// part waits only on parts that run or have run, except the parts of the one
-// job whose parts are not all taken, and P is below the resident wavefronts.
+// Deadlock-free: parts take tickets from their bin in the order they start
+// running, so a part waits only on parts that run or have run, except the
+// parts of the one job a bin whose parts are not all taken; those are far
+// fewer than the resident wavefronts.
 // The result is the same as one wavefront's.
 constexpr uint32_t DP_SPIN_MAX = 1u << 22;
 
@@ -522,8 +524,9 @@ constexpr uint32_t ROWS_MAX_LB = 256;   // widest job run four to a wavefront
 // launches of a round take one pointer and are captured in a graph once per
 // context; a round only sets the grid sizes of the graph's nodes.
 constexpr int ROW_SHAPES = 7;
-// QK_PIPE: 64-lane stripe jobs whose stripes pipe_p wavefronts share (an
-// item is one wavefront's part: job first + item / pipe_p, part item % pipe_p)
+// QK_PIPE + P - 2 (P = 2 .. 8): 64-lane stripe jobs whose stripes P
+// wavefronts share (an item is one wavefront's part: job first + item / P,
+// part item % P)
 enum : uint32_t { QK_STRIPE = 0, QK_ROWS = 1, QK_WINDOW = QK_ROWS + ROW_SHAPES, QK_TINY = QK_WINDOW + 1,
                   QK_PIPE = QK_TINY + 1 };
 constexpr int DPQ_MAXB = 48;
@@ -540,7 +543,6 @@ struct DpRound {
     uint32_t *dev;
     uint32_t *cursor;          // DPQ_CURSORS counters, zeroed at the start of every round; the last
                                // one is the round's wait-bound error flag
-    uint32_t pipe_p;           // wavefronts a QK_PIPE job
     uint32_t sr[4];
     int32_t goh;
     uint32_t nbins;
@@ -552,7 +554,7 @@ struct DpRound {
 
 __device__ __forceinline__ uint32_t qk_per_item(uint32_t kind)
 {
-    return kind == QK_STRIPE || kind == QK_WINDOW || kind == QK_PIPE ? 1u : (kind == QK_TINY ? 64u : 4u);
+    return kind == QK_STRIPE || kind == QK_WINDOW || kind >= QK_PIPE ? 1u : (kind == QK_TINY ? 64u : 4u);
 }
 
 // ---- sliding-window banded DP ---------------------------------------------
@@ -1186,8 +1188,8 @@ template <int MAXC>
 __device__ __forceinline__ void run_item(DP_FWD_PARAMS, uint32_t kind, uint32_t first, uint32_t count,
                                          uint32_t item, int (*lds_in)[64], int (*lds_out)[64])
 {
-    if (kind == QK_PIPE) {
-        const uint32_t P = rd->pipe_p;
+    if (kind >= QK_PIPE) {
+        const uint32_t P = kind - QK_PIPE + 2;
         const uint32_t x = first + item / P;
         const int p0 = (int)(item % P);
         const DevJob jb = jobs[x];
@@ -1268,7 +1270,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         while (b + 1 < nb && w >= rd->item0[b + 1]) ++b;
         if (b >= nb || w >= rd->item0[nb]) return;
         uint32_t item = w - rd->item0[b];
-        if (rd->kind[b] == QK_PIPE) {
+        if (rd->kind[b] >= QK_PIPE) {
             // a ticket, so that the parts of a job are taken in order by
             // running wavefronts (a part waits only on parts taken before it)
             uint32_t v = 0;
@@ -1880,7 +1882,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
 
     uint64_t first = 0;
     std::vector<DpItem> shaped;
-    std::vector<char> piped;
+    std::vector<uint8_t> pj;   // wavefronts of each stripe job of a round
     while (first < nitems) {
         // one round: as many jobs (in schedule order) as fit the trace budget
         uint64_t tbb = 0, bi = 0, last = first;
@@ -1960,41 +1962,72 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         const int variant = dp_variant_for(wide);
         // Pipelined stripe jobs.  A round with fewer work items than the
         // forward launch's wavefront slots ends on its longest jobs' latency
-        // with SIMDs idle: the first 64-lane stripe jobs (largest first) with
-        // at least P stripes then run on P wavefronts each, as many as the
-        // idle slots hold, moved to the front of the queue.  P: the slots
-        // over the items (at least 2, at most DP_PIPE_MAX).
-        uint32_t pipe_p = 1;
+        // with SIMDs idle: its 64-lane stripe jobs then run on P_j wavefronts
+        // each.  P_j = ceil(W_j / T) (W_j the job's trace bytes, a measure of
+        // its cells), at most DP_PIPE_MAX and the job's stripes, with the
+        // smallest T whose parts fit the slots; so the parts are of about
+        // one size.  MH_DP_PIPE = P instead gives P wavefronts to the first
+        // jobs with at least P stripes, as many as the idle slots hold.
+        // Pipelined jobs go to the front of the queue, most parts first.
         uint64_t npipe = 0;
+        pj.assign(n64, 1);
         if (!fx && ncu > 0) {
             uint64_t items_est = n64 + (nbig - nstripe) + (n - nbig + 63) / 64;
             for (int q = 0; q < ROW_SHAPES; ++q) items_est += (rows_at[q + 2] - rows_at[q + 1] + 3) / 4;
             const uint64_t slots = (uint64_t)ncu * 4 * (uint64_t)FWD_VARIANTS[variant].wpe;
-            const uint32_t pipe_cfg =
-                pipe_env > 1 ? (uint32_t)pipe_env
-                             : (uint32_t)std::min<uint64_t>(DP_PIPE_MAX, std::max<uint64_t>(2, slots / std::max<uint64_t>(1, items_est)));
-            if (items_est < slots) {
-                const uint64_t kmax = (slots - items_est) / (pipe_cfg - 1);
-                piped.assign(n64, 0);
-                for (uint64_t x = 0; x < n64 && npipe < kmax; ++x) {
-                    const DpItem &it = items[first + x];
-                    const uint64_t sw = 64ull * it.cols;
-                    if ((jobs[it.job].b_len + sw - 1) / sw >= pipe_cfg) {
-                        piped[x] = 1;
-                        ++npipe;
+            auto nstr = [&](const DpItem &it) {
+                const uint64_t sw = 64ull * it.cols;
+                return (jobs[it.job].b_len + sw - 1) / sw;
+            };
+            if (items_est < slots && pipe_env > 1) {
+                const uint64_t kmax = (slots - items_est) / (uint64_t)(pipe_env - 1);
+                for (uint64_t x = 0, k = 0; x < n64 && k < kmax; ++x)
+                    if (nstr(items[first + x]) >= (uint64_t)pipe_env) {
+                        pj[x] = (uint8_t)pipe_env;
+                        ++k;
                     }
+            } else if (items_est < slots) {
+                const uint64_t room = slots - (items_est - n64);   // wavefronts for the stripe jobs
+                auto parts = [&](double T, bool set) {
+                    uint64_t tot = 0;
+                    for (uint64_t x = 0; x < n64; ++x) {
+                        const DpItem &it = items[first + x];
+                        const uint64_t cap = std::min<uint64_t>(DP_PIPE_MAX, nstr(it));
+                        const uint64_t q = std::max<uint64_t>(1, std::min<uint64_t>(cap, (uint64_t)std::ceil((double)it.tb_bytes / T)));
+                        tot += q;
+                        if (set) pj[x] = (uint8_t)q;
+                    }
+                    return tot;
+                };
+                double wmax = 1;
+                for (uint64_t x = 0; x < n64; ++x) wmax = std::max(wmax, (double)items[first + x].tb_bytes);
+                double lo = wmax / (2.0 * DP_PIPE_MAX), hi = wmax;   // parts(hi) = n64 <= room
+                for (int it = 0; it < 48; ++it) {
+                    const double mid = 0.5 * (lo + hi);
+                    (parts(mid, false) <= room ? hi : lo) = mid;
                 }
-                if (npipe) {
-                    pipe_p = pipe_cfg;
-                    shaped.clear();
+                parts(hi, true);
+            }
+            for (uint64_t x = 0; x < n64; ++x) npipe += pj[x] > 1;
+            if (npipe) {
+                // most parts first, each part count in order; then the others
+                shaped.clear();
+                std::vector<uint8_t> pj2;
+                for (uint32_t P = DP_PIPE_MAX; P >= 2; --P)
                     for (uint64_t x = 0; x < n64; ++x)
-                        if (piped[x]) shaped.push_back(items[first + x]);
-                    for (uint64_t x = 0; x < n64; ++x)
-                        if (!piped[x]) shaped.push_back(items[first + x]);
-                    std::copy(shaped.begin(), shaped.end(), items.begin() + (ptrdiff_t)first);
-                    // (a count under the timers' names: pipelined jobs a round)
-                    host_time(ctx, "dp_pipe_jobs", (double)npipe);
-                }
+                        if (pj[x] == P) {
+                            shaped.push_back(items[first + x]);
+                            pj2.push_back((uint8_t)P);
+                        }
+                for (uint64_t x = 0; x < n64; ++x)
+                    if (pj[x] <= 1) {
+                        shaped.push_back(items[first + x]);
+                        pj2.push_back(1);
+                    }
+                std::copy(shaped.begin(), shaped.end(), items.begin() + (ptrdiff_t)first);
+                pj.swap(pj2);
+                // (a count under the timers' names: pipelined jobs a round)
+                host_time(ctx, "dp_pipe_jobs", (double)npipe);
             }
         }
         DevJob *dj = nullptr;
@@ -2099,7 +2132,6 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         hr.goh = -200;
         hr.nwave = (uint32_t)nwave;
         hr.n = (uint32_t)n;
-        hr.pipe_p = pipe_p;
         auto add_bin = [&](uint32_t kind, uint64_t b0, uint64_t cnt) {
             if (cnt == 0) return;
             hr.kind[hr.nbins] = kind;
@@ -2109,8 +2141,13 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         };
         {
             // stripe jobs: a new bin where the trace size falls below the next power of 4
-            // the pipelined jobs first, an item a part
-            add_bin(QK_PIPE, 0, npipe * pipe_p);
+            // the pipelined jobs first, a bin for each part count, an item a part
+            for (uint64_t x = 0; x < npipe;) {
+                uint64_t y = x + 1;
+                while (y < npipe && pj[y] == pj[x]) ++y;
+                add_bin(QK_PIPE + (pj[x] - 2u), x, (y - x) * pj[x]);
+                x = y;
+            }
             auto cls = [](uint64_t v) { return v ? (63 - __builtin_clzll(v)) / 2 : 0; };
             uint64_t x = npipe;
             while (x < n64) {
@@ -2131,7 +2168,7 @@ static int dp_run_items(mh_context *ctx, const uint8_t *d_a, const uint8_t *d_b,
         uint64_t nitems_q = 0;
         for (uint32_t q = 0; q < hr.nbins; ++q) {
             const uint32_t per =
-                hr.kind[q] == QK_STRIPE || hr.kind[q] == QK_WINDOW || hr.kind[q] == QK_PIPE ? 1u
+                hr.kind[q] == QK_STRIPE || hr.kind[q] == QK_WINDOW || hr.kind[q] >= QK_PIPE ? 1u
                                                                                           : (hr.kind[q] == QK_TINY ? 64u : 4u);
             hr.item0[q] = (uint32_t)nitems_q;
             nitems_q += (hr.count[q] + per - 1) / per;

@@ -2,7 +2,8 @@
 wavefronts share, handing the boundary column over through device memory.
 
 A round with fewer work items than the forward launch's wavefront slots pipes
-its largest stripe jobs (MH_DP_PIPE sets P; 0 or 1 keeps one wavefront a job).
+its stripe jobs, P a job by its size (MH_DP_PIPE sets one P for the first
+jobs; 0 or 1 keeps one wavefront a job).
 Bar: bit-exact paths and scores against the oracle (full DP: NWSmall; banded:
 the region DP at the stripe width used), and identical output for every P.
 """
@@ -79,9 +80,8 @@ def test_pipelined_full_dp_matches_oracle(ctx, dp_env, cols):
         if P == "1":
             assert npiped == 0
         elif P == "auto":
-            # ten items: eight wavefronts a job with at least eight stripes
-            sw = 64 * int(cols)
-            assert npiped == sum((lb + sw - 1) // sw >= 8 for _, lb in SHAPES)
+            # ten items, far fewer than the slots: the large jobs get up to eight parts
+            assert npiped > 0
         else:
             # every job with at least P stripes of 64*cols columns runs pipelined
             sw = 64 * int(cols)
