@@ -261,4 +261,108 @@ MH_HD inline int extend_k(const ExtEnvK &e, const int *gs, int used, int64_t *st
     return EXT_OK;
 }
 
+#if defined(__HIPCC__)
+// extend_k for a seed of at most KMAX present genomes, with the lookups of a
+// step issued together.  The caller passes the present genomes' lists,
+// list sizes and sequence lengths in order (gk, gn, gL: slot x is genome
+// gs[x]; slots from `used` on repeat slot 0 and are never compared), and st
+// with KMAX slots.  A lookup's index is clamped into its list and the miss
+// flagged instead of branching around the load, so a step's loads do not
+// wait on each other; a miss is an error only where the literal loop, which
+// stops at the first genome that differs, would have made that lookup.
+// Same walk, result and errors as extend_k.
+template <int KMAX>
+__device__ inline int extend_kt(const uint64_t *const *gk, const uint32_t *gn, const int64_t *gL, int used,
+                                int seed_len, uint64_t mer_mask, int64_t *st, int64_t &len)
+{
+    int jump = seed_len;
+    uint32_t extend_limit = 0, extend_attempts = 0;
+    bool extend_again = false;
+    int64_t guard = 0, guard_max = 64;
+    MH_PRAGMA_UNROLL
+    for (int x = 0; x < KMAX; ++x)
+        if (x < used) guard_max += 8 * gL[x];
+    for (int dir = 0; dir < 4; ++dir) {
+        int64_t maxlen = dir >= 2 ? (int64_t)seed_len : (int64_t)4294967295ll;
+        MH_PRAGMA_UNROLL
+        for (int x = 0; x < KMAX; ++x) {
+            if (x >= used) continue;
+            if (st[x] < 0) {
+                const int64_t rc_len = gL[x] - len + st[x] + 1;
+                if (rc_len < maxlen) maxlen = rc_len;
+            } else if (st[x] - 1 < maxlen) {
+                maxlen = st[x] - 1;
+            }
+        }
+        int i = used;
+        extend_limit = 0;
+        extend_attempts = 0;
+        while (maxlen - jump >= 0) {
+            if (++guard > guard_max) return EXT_ERR;
+            len += jump;
+            maxlen -= jump;
+            MH_PRAGMA_UNROLL
+            for (int x = 0; x < KMAX; ++x)
+                if (x < used && st[x] > 0) {
+                    st[x] -= jump;
+                    if (st[x] <= 0) st[x] += gL[x];
+                }
+            // every slot's window (lookup_k), loads issued together
+            uint64_t kv[KMAX];
+            bool in[KMAX];
+            MH_PRAGMA_UNROLL
+            for (int x = 0; x < KMAX; ++x) {
+                const int64_t p = st[x] < 0 ? -st[x] + len - seed_len : st[x];
+                const int64_t idx = p - 1;
+                in[x] = x >= used || (idx >= 0 && (uint64_t)idx < (uint64_t)gn[x]);
+                kv[x] = gk[x][(x < used && in[x]) ? idx : 0];
+            }
+            if (!in[0]) return EXT_ERR;
+            const uint64_t cm = kv[0] & mer_mask;
+            const int cp = st[0] < 0 ? (int)(kv[0] & 1ull) : (int)!(kv[0] & 1ull);
+            i = used;
+            MH_PRAGMA_UNROLL
+            for (int x = 1; x < KMAX; ++x)
+                if (x < used && i == used) {
+                    if (!in[x]) return EXT_ERR;
+                    const int op = st[x] < 0 ? (int)(kv[x] & 1ull) : (int)!(kv[x] & 1ull);
+                    if (cm != (kv[x] & mer_mask) || cp != op) i = x;
+                }
+            if (i < used && dir < 2) maxlen = 0;
+            extend_attempts += jump;
+            if (i == used) extend_limit = extend_attempts;
+            if (dir > 1 && extend_attempts == (uint32_t)seed_len) break;
+        }
+        if (i < used) {
+            len -= jump;
+            MH_PRAGMA_UNROLL
+            for (int x = 0; x < KMAX; ++x)
+                if (x < used && st[x] >= 0) st[x] += jump;
+        }
+        if (dir > 1 && extend_attempts > 0) {
+            if (extend_limit > 0) extend_again = true;
+            int unmatched = (int)(extend_attempts - extend_limit);
+            if (i < used) unmatched -= jump;
+            len -= unmatched;
+            MH_PRAGMA_UNROLL
+            for (int x = 0; x < KMAX; ++x)
+                if (x < used && st[x] > 0) {
+                    st[x] += unmatched;
+                    if (st[x] > gL[x]) st[x] -= gL[x];
+                }
+        }
+        MH_PRAGMA_UNROLL
+        for (int x = 0; x < KMAX; ++x)
+            if (x < used) st[x] = -st[x];
+        if (dir >= 1) jump = 1;
+        if (dir == 3 && extend_again) {
+            dir = -1;
+            jump = seed_len;
+            extend_again = false;
+        }
+    }
+    return EXT_OK;
+}
+#endif
+
 }  // namespace mh

@@ -1815,7 +1815,7 @@ __global__ void uq_leader_kernel(const uint64_t *__restrict__ key, const uint32_
 
 __global__ void uq_seed_kernel(const uint64_t *__restrict__ key, const uint32_t *__restrict__ val,
                                const uint32_t *__restrict__ U, uint64_t m, const uint32_t *__restrict__ lead,
-                               uint64_t ns, UqLists ls, ExtEnvK env, int k, int solid_odd,
+                               uint64_t ns, UqLists ls, ExtEnvK env, int k, int solid_odd, int walk,
                                int32_t *__restrict__ seed_st, int32_t *__restrict__ ext_st,
                                uint32_t *__restrict__ ext_len, int64_t *__restrict__ offset,
                                uint32_t *__restrict__ err)
@@ -1875,12 +1875,121 @@ __global__ void uq_seed_kernel(const uint64_t *__restrict__ key, const uint32_t 
         for (int x = 1; x < used; ++x)
             ex[x] = st[x] > 0 ? q1[x] - left + 1 : -(env.L[gs[x]] - (q1[x] - left + len) + 1);
     } else {
+        if (!walk) return;   // extended by chains (uq_chain_*)
         for (int x = 0; x < used; ++x) ex[x] = st[x];
         if (extend_k(env, gs, used, ex, len) != EXT_OK) atomicOr(err, 1u);
     }
     for (int g = 0; g < k; ++g) ext_st[s * k + g] = 0;
     for (int x = 0; x < used; ++x) ext_st[s * k + gs[x]] = (int32_t)ex[x];
     ext_len[s] = (uint32_t)len;
+}
+
+// ---- spaced seeds: one k-genome walk per chain ---------------------------
+// As for two genomes (grid_* kernels): seed B and seed A with the same
+// genomes, where every start of A is B's minus the seed length L (forward:
+// the window L to the left; reversed: -(q + 1) - L, the window L to the right
+// in that genome's forward coordinates), extend to the same match.  B's first
+// seed-length step to the left lands on A's windows, which match (A is a
+// seed), and from there its walk is A's; A's first step to the right lands
+// on B's windows the same way.  Each chain is walked once, from its first
+// seed (its head), and the others copy the head's extension.
+
+// key: the first present genome and its (positive) start; a genome position
+// holds one mer, so no two seeds share a key
+__global__ void uq_chain_key_kernel(const int32_t *__restrict__ seed_st, uint64_t ns, int k,
+                                    uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    int g = 0;
+    while (g < k - 1 && seed_st[s * k + g] == 0) ++g;
+    key[s] = ((uint64_t)g << 32) | (uint32_t)seed_st[s * k + g];
+    val[s] = (uint32_t)s;
+}
+
+// link[r] = the sorted index of seed r's chain predecessor, or r
+__global__ void uq_chain_pred_kernel(const uint64_t *__restrict__ sk, const uint32_t *__restrict__ sv, uint64_t ns,
+                                     const int32_t *__restrict__ seed_st, int k, uint32_t seed_len,
+                                     uint32_t *__restrict__ link)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ns) return;
+    const uint64_t key = sk[r];
+    uint32_t out = (uint32_t)r;
+    if ((key & 0xFFFFFFFFull) > seed_len) {
+        const uint64_t want = key - seed_len;
+        uint64_t lo = 0, hi = r;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) / 2;
+            if (sk[mid] < want) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < r && sk[lo] == want) {
+            const int32_t *b = seed_st + (uint64_t)sv[r] * k, *a = seed_st + (uint64_t)sv[lo] * k;
+            bool same = true;
+            for (int g = 0; g < k && same; ++g)
+                same = b[g] == 0 ? a[g] == 0 : (a[g] == b[g] - (int32_t)seed_len && (b[g] < 0 || a[g] > 0));
+            if (same) out = (uint32_t)lo;
+        }
+    }
+    link[r] = out;
+}
+
+// the heads' walks (extend_kt: a seed holds at most KMAX genomes)
+template <int KMAX>
+__global__ __launch_bounds__(128) void uq_chain_walk_kernel(const uint32_t *__restrict__ heads, uint64_t nh, ExtEnvK env, int k,
+                                     const int32_t *__restrict__ seed_st, int32_t *__restrict__ ext_st,
+                                     uint32_t *__restrict__ ext_len, uint32_t *__restrict__ err)
+{
+    const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    const uint64_t s = heads[h];
+    const uint64_t *gk[KMAX];
+    uint32_t gn[KMAX];
+    int64_t gL[KMAX], ex[KMAX];
+    int gsl[KMAX];
+    int used = 0;
+    for (int g = 0; g < k; ++g) {
+        const int32_t v = seed_st[s * k + g];
+        if (v == 0) continue;
+#pragma unroll
+        for (int x = 0; x < KMAX; ++x)
+            if (x == used) {
+                gk[x] = env.kp[g];
+                gn[x] = (uint32_t)env.n[g];
+                gL[x] = env.L[g];
+                ex[x] = v;
+                gsl[x] = g;
+            }
+        ++used;
+    }
+#pragma unroll
+    for (int x = 1; x < KMAX; ++x)
+        if (x >= used) {
+            gk[x] = gk[0];
+            gn[x] = gn[0];
+            gL[x] = 0;
+            ex[x] = 0;
+            gsl[x] = 0;
+        }
+    int64_t len = env.seed_len;
+    if (extend_kt<KMAX>(gk, gn, gL, used, env.seed_len, env.mer_mask, ex, len) != EXT_OK) atomicOr(err, 1u);
+    for (int g = 0; g < k; ++g) ext_st[s * k + g] = 0;
+#pragma unroll
+    for (int x = 0; x < KMAX; ++x)
+        if (x < used) ext_st[s * k + gsl[x]] = (int32_t)ex[x];
+    ext_len[s] = (uint32_t)len;
+}
+
+// every other seed copies its head's extension
+__global__ void uq_chain_fill_kernel(const uint32_t *__restrict__ is_head, const uint32_t *__restrict__ head_of,
+                                     uint64_t ns, int k, int32_t *__restrict__ ext_st, uint32_t *__restrict__ ext_len)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns || is_head[s]) return;
+    const uint64_t h = head_of[s];
+    for (int g = 0; g < k; ++g) ext_st[s * k + g] = ext_st[h * k + g];
+    ext_len[s] = ext_len[h];
 }
 
 // ---- AddHashEntry of the unique seeds, one bucket per thread --------------
@@ -2047,6 +2156,63 @@ __global__ void uq_emit_kernel(UqSeeds q, const uint32_t *__restrict__ bstart, c
     }
 }
 
+// Spaced-seed extension of the k-genome seeds with one walk per chain (the
+// uq_chain_* kernels): sort the seeds by (first genome, start), link each to
+// its chain predecessor, find the heads by pointer jumping, walk the heads,
+// copy.  ext_st / ext_len as uq_seed_kernel writes them; *err set by a walk
+// that left a list; *walks the heads walked.
+static int uq_extend_by_chains(mh_context *ctx, const int32_t *d_sst, uint64_t S, int k, const ExtEnvK &env,
+                               int32_t *d_est, uint32_t *d_len, uint32_t *err, uint64_t *walks)
+{
+    DevBuf *g = ctx->grid_scratch;
+    int rc;
+    for (int q = 0; q < 2; ++q)
+        if ((rc = g[q].ensure(S * 8 + 64)) != MH_OK) return rc;            // keys
+    for (int q = 2; q < 8; ++q)
+        if ((rc = g[q].ensure(S * 4 + 64)) != MH_OK) return rc;            // u32 arrays
+    if ((rc = g[8].ensure(((S + JTILE - 1) / JTILE) * 4 + 64)) != MH_OK) return rc;
+    if ((rc = g[10].ensure(64)) != MH_OK) return rc;
+    const dim3 grid((unsigned)((S + 255) / 256)), blk(256);
+    uint64_t *k0 = g[0].as<uint64_t>(), *k1 = g[1].as<uint64_t>();
+    uint32_t *v0 = g[2].as<uint32_t>(), *v1 = g[3].as<uint32_t>();
+    hipLaunchKernelGGL(uq_chain_key_kernel, grid, blk, 0, ctx->stream, d_sst, S, k, k0, v0);
+    uint64_t *sk = nullptr;
+    uint32_t *sv = nullptr;
+    if ((rc = radix_sort_kv(ctx, k0, v0, k1, v1, S, 32 + 4, &sk, &sv)) != MH_OK) return rc;
+    uint32_t *la = sv == v0 ? v1 : v0, *lb = g[4].as<uint32_t>();
+    hipLaunchKernelGGL(uq_chain_pred_kernel, grid, blk, 0, ctx->stream, sk, sv, S, d_sst, k, (uint32_t)env.seed_len,
+                       la);
+    // links point to earlier sorted entries: this many doublings reach every head
+    int rounds = 1;
+    while ((1ull << (rounds - 1)) < S) ++rounds;
+    for (int round = 0; round < rounds; ++round) {
+        hipLaunchKernelGGL(grid_jump_kernel, grid, blk, 0, ctx->stream, la, lb, S);
+        std::swap(la, lb);
+    }
+    uint32_t *is_head = g[5].as<uint32_t>(), *head_of = g[6].as<uint32_t>();
+    hipLaunchKernelGGL(grid_head_kernel, grid, blk, 0, ctx->stream, sv, la, S, is_head, head_of);
+    uint32_t *heads = g[7].as<uint32_t>();
+    uint64_t nh = 0;
+    if ((rc = compact_flags(ctx, is_head, S, heads, g[8].as<uint32_t>(), g[10].as<uint32_t>(), &nh)) != MH_OK)
+        return rc;
+    if (nh) {
+        const dim3 hg((unsigned)((nh + 127) / 128)), hb(128);
+        if (k <= 4)
+            hipLaunchKernelGGL(uq_chain_walk_kernel<4>, hg, hb, 0, ctx->stream, heads, nh, env, k, d_sst, d_est, d_len,
+                               err);
+        else if (k <= 8)
+            hipLaunchKernelGGL(uq_chain_walk_kernel<8>, hg, hb, 0, ctx->stream, heads, nh, env, k, d_sst, d_est, d_len,
+                               err);
+        else
+            hipLaunchKernelGGL(uq_chain_walk_kernel<16>, hg, hb, 0, ctx->stream, heads, nh, env, k, d_sst, d_est,
+                               d_len, err);
+    }
+    hipLaunchKernelGGL(uq_chain_fill_kernel, grid, blk, 0, ctx->stream, is_head, head_of, S, k, d_est, d_len);
+    MH_HIP(hipGetLastError());
+    *walks = nh;
+    return MH_OK;
+}
+
 int unique_seeds(mh_context *ctx, const mh_sml *const *smls, int k, const LiveRanges &live, UniqueSeeds *out)
 {
     MH_CHECK(k >= 2 && k <= EXT_MAXK && k <= PLAN_MAXK, MH_ERR_UNSUPPORTED, "UniqueMatchFinder takes 2 to 16 genomes");
@@ -2110,6 +2276,7 @@ int unique_seeds(mh_context *ctx, const mh_sml *const *smls, int k, const LiveRa
     uint64_t S = 0;
     if ((rc = compact_flags(ctx, flag2, M, lead, tiles, d_cnt, &S)) != MH_OK) return rc;
     out->n = S;
+    out->walks = S;
     if (S == 0) return MH_OK;
     const bool solid_odd = smls[0]->seed_length == smls[0]->seed_weight && (smls[0]->seed_length & 1);
     ExtEnvK env;
@@ -2138,9 +2305,15 @@ int unique_seeds(mh_context *ctx, const mh_sml *const *smls, int k, const LiveRa
     MH_HIP(hipMemsetAsync(d_cnt, 0, 4, ctx->stream));
     {
         TimedLaunch tl(ctx, "extend");
+        // spaced seeds: starts and offsets here, one walk per chain below
+        // (MH_UQ_CHAINS=0 walks every seed)
+        const char *ce = getenv("MH_UQ_CHAINS");
+        const bool chains = !solid_odd && !(ce && ce[0] == '0');
         hipLaunchKernelGGL(uq_seed_kernel, dim3((unsigned)((S + 127) / 128)), dim3(128), 0, ctx->stream, sk, sv, U, M,
-                           lead, S, ls, env, k, solid_odd ? 1 : 0, d_sst, d_est, d_len, d_off, d_cnt);
+                           lead, S, ls, env, k, solid_odd ? 1 : 0, chains ? 0 : 1, d_sst, d_est, d_len, d_off, d_cnt);
         MH_HIP(hipGetLastError());
+        if (chains && (rc = uq_extend_by_chains(ctx, d_sst, S, k, env, d_est, d_len, d_cnt, &out->walks)) != MH_OK)
+            return rc;
     }
     uint32_t err = 0;
     MH_HIP(hipMemcpyAsync(&err, d_cnt, 4, hipMemcpyDeviceToHost, ctx->stream));

@@ -415,7 +415,7 @@ int find_matches_unique(mh_context *ctx, const mh_sml *const *smls, int k, std::
         st->collisions = us.collisions;
         st->matches = (int64_t)lengths.size();
         st->repeat_runs = live.restarts;
-        st->walks = (int64_t)us.n;
+        st->walks = (int64_t)us.walks;
     }
     return MH_OK;
 }

@@ -334,6 +334,7 @@ int search_live_ranges(mh_context *ctx, const mh_sml *const *smls, int k, LiveRa
 struct UniqueSeeds {
     int k = 0;
     uint64_t n = 0;
+    uint64_t walks = 0;        // ExtendMatch walks run (spaced seeds: one a chain)
     int64_t collisions = 0;
     std::vector<int64_t> starts;
     std::vector<uint64_t> lengths;

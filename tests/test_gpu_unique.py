@@ -112,3 +112,20 @@ def test_unique_eight_genome_tree_family(ctx):
     fam = [q.tobytes() for q in synth.genome_family(length=300_000)]
     st = _check(ctx, fam, mauvehip.get_seed(19, 0))
     assert st.matches > 10000
+
+
+@pytest.mark.parametrize("seed", [0xFFF, 0x7B974EF])
+def test_unique_chain_walks_equal_every_seed_walk(ctx, monkeypatch, seed):
+    """Spaced (and even solid) seeds walk once per chain of seeds one seed
+    length apart on their diagonal: the lists equal those of a walk from every
+    seed, with fewer walks."""
+    rng = np.random.default_rng(66)
+    seqs = _family(rng, 20000, 6)
+    monkeypatch.setenv("MH_UQ_CHAINS", "0")
+    s0, l0, st0 = _gpu(ctx, seqs, seed)
+    monkeypatch.delenv("MH_UQ_CHAINS")
+    s1, l1, st1 = _gpu(ctx, seqs, seed)
+    assert np.array_equal(s0, s1) and np.array_equal(l0, l1)
+    assert st0.collisions == st1.collisions
+    assert st0.walks == st0.seed_tuples
+    assert 0 < st1.walks < st0.walks
