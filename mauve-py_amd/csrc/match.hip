@@ -2037,16 +2037,20 @@ __global__ void uq_bucket_kernel(const int64_t *__restrict__ off, uint64_t ns, u
 // probe sequence (MheCompare is not a strict weak order, so the probes must
 // be exactly those), run by a whole wavefront: the entries of the next six
 // levels of the probe tree from (first, len) -- 63 positions, one a lane --
-// are loaded at once into LDS, then the first lane takes up to six probes
-// from LDS; repeated until the range is empty.  One round of dependent loads
-// serves six probes.
+// are loaded and compared with q at once (a ballot of the comparisons), then
+// the first lane takes up to six probes from the ballot; repeated until the
+// range is empty.  One round of dependent loads
+// serves six probes.  The end of the range is always n or the last probe that
+// was not below q, so when the result is not n, *at (first lane) is the entry
+// at the result: that probe's, kept from the LDS tree.
 __device__ uint32_t uq_lower_bound(const UqSeeds &q, const uint32_t *__restrict__ v, uint32_t n, const EntU &qe,
-                                   EntU *tree)
+                                   EntU *tree, EntU *at)
 {
     const int lane = threadIdx.x;
     uint32_t first = 0, len = n;
     while (len > 0) {
         const int node = lane + 1;                    // heap index of this lane's probe, 1..63
+        bool lt = false;
         if (node < 64) {
             const int lvl = 31 - __builtin_clz((unsigned)node);
             uint32_t f = first, l = len;
@@ -2059,22 +2063,31 @@ __device__ uint32_t uq_lower_bound(const UqSeeds &q, const uint32_t *__restrict_
                     l = half;
                 }
             }
-            if (l > 0) uq_entry(q, v[f + (l >> 1)], true, tree[node]);
+            if (l > 0) {
+                EntU mine;
+                uq_entry(q, v[f + (l >> 1)], true, mine);
+                lt = mhe_less_k(mine, qe, q.k);
+                tree[node] = mine;
+            }
         }
+        // every probe's comparison at once, one bit a node (bit node - 1)
+        const uint64_t below = __ballot(lt);
         __syncthreads();
         if (lane == 0) {
-            int node = 1;
+            int node = 1, keep = 0;
             for (int s = 0; s < 6 && len > 0; ++s) {
                 const uint32_t half = len >> 1, mid = first + half;
-                if (mhe_less_k(tree[node], qe, q.k)) {
+                if ((below >> (node - 1)) & 1ull) {
                     first = mid + 1;
                     len = len - half - 1;
                     node = 2 * node + 1;
                 } else {
                     len = half;
+                    keep = node;
                     node = 2 * node;
                 }
             }
+            if (keep) *at = tree[keep];
         }
         first = (uint32_t)__builtin_amdgcn_readfirstlane((int)first);
         len = (uint32_t)__builtin_amdgcn_readfirstlane((int)len);
@@ -2099,26 +2112,26 @@ __global__ __launch_bounds__(64) void uq_replay_kernel(UqSeeds q, const uint32_t
     uint32_t *v = vec + y0;
     uint32_t ne = 0;
     unsigned long long c = 0;
+    EntU cand, at_it;
+    if (y0 < y1) uq_entry(q, order[y0], false, cand);
     for (uint32_t y = y0; y < y1; ++y) {
         const uint32_t x = order[y];
         uint32_t pos = 0xFFFFFFFFu;     // insertion point, or none (a collision)
-        EntU cand;
-        uq_entry(q, x, false, cand);
-        const uint32_t it = uq_lower_bound(q, v, ne, cand, tree);
+        // the next seed's entry, loaded alongside this one's probes
+        EntU next = cand;
+        if (y + 1 < y1) uq_entry(q, order[y + 1], false, next);
+        const uint32_t it = uq_lower_bound(q, v, ne, cand, tree, &at_it);
         bool hit = false;
-        if (lane == 0 && it != ne) {
-            EntU at_it;
-            uq_entry(q, v[it], true, at_it);
-            hit = !mhe_less_k(cand, at_it, q.k);   // !comp(*it, cand) holds at lower_bound
-        }
+        if (lane == 0 && it != ne) hit = !mhe_less_k(cand, at_it, q.k);   // !comp(*it, cand) holds at lower_bound
         hit = __builtin_amdgcn_readfirstlane((int)hit) != 0;
         if (hit) {
             if (lane == 0) ++c;
         } else {
             EntU e;
             uq_entry(q, x, true, e);
-            pos = uq_lower_bound(q, v, ne, e, tree);
+            pos = uq_lower_bound(q, v, ne, e, tree, &at_it);
         }
+        cand = next;
         pos = (uint32_t)__builtin_amdgcn_readfirstlane((int)pos);
         if (pos == 0xFFFFFFFFu) continue;
         // shift [pos, ne) up by one, 64 entries a step from the end
