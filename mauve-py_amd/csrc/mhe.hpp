@@ -20,46 +20,72 @@ template <int MK, class T> struct EntT {
 using EntK = EntT<4, int64_t>;     // PairwiseMatchFinder, up to four genomes
 using EntU = EntT<16, int32_t>;    // UniqueMatchFinder, up to sixteen
 
+// The loops below run over all MK genome slots with the genome range as a
+// condition (unrolled), never from a run-time start index: on the GPU an
+// entry indexed at run time lives in scratch memory.
+
 // MatchHashEntry::Contains (a contains b), LM/MatchHashEntry.cpp:164-200
 template <class E> MH_HD inline bool contains_k(const E &a, const E &b, int k)
 {
+    constexpr int MK = (int)(sizeof(a.s) / sizeof(a.s[0]));
     if (a.off != b.off) return false;
-    int i = b.first;
-    const int64_t diff = (int64_t)b.s[i] - (int64_t)a.s[i];
-    if (a.s[i] == 0) return false;
+    const int f = b.first;
+    int64_t bf = 0, af = 0;
+    MH_PRAGMA_UNROLL
+    for (int g = 0; g < MK; ++g)
+        if (g == f) {
+            bf = b.s[g];
+            af = a.s[g];
+        }
+    const int64_t diff = bf - af;
+    if (af == 0) return false;
     if (diff < 0 || a.len < b.len + (uint64_t)diff) return false;
     const int64_t diff_rc = (int64_t)b.len - (int64_t)a.len + diff;
-    for (++i; i < k; ++i) {
-        const int64_t di = (int64_t)b.s[i] - (int64_t)a.s[i];
-        if (b.s[i] == 0 && a.s[i] == 0) continue;
-        if (b.s[i] < 0 && diff_rc == di) continue;
-        if (diff != di) return false;
+    bool ok = true;
+    MH_PRAGMA_UNROLL
+    for (int g = 0; g < MK; ++g) {
+        if (g <= f || g >= k) continue;
+        const int64_t di = (int64_t)b.s[g] - (int64_t)a.s[g];
+        if (b.s[g] == 0 && a.s[g] == 0) continue;
+        if (b.s[g] < 0 && diff_rc == di) continue;
+        if (diff != di) ok = false;
     }
-    return true;
+    return ok;
 }
 
 // strict_start_lessthan_ptr, LM/MatchHashEntry.cpp:48-67
 template <class E> MH_HD inline bool strict_less_k(const E &a, const E &b, int k)
 {
-    if (a.first == b.first)
-        for (int i = 0; i < k; ++i) {
-            int64_t as = a.s[i], bs = b.s[i];
+    constexpr int MK = (int)(sizeof(a.s) / sizeof(a.s[0]));
+    if (a.first == b.first) {
+        int res = 0;   // 1: a first, 2: b first
+        MH_PRAGMA_UNROLL
+        for (int g = 0; g < MK; ++g) {
+            if (g >= k || res) continue;
+            int64_t as = a.s[g], bs = b.s[g];
             if (as < 0) as = (int64_t)((uint64_t)(-as) + a.len - a.mersize);
             if (bs < 0) bs = (int64_t)((uint64_t)(-bs) + b.len - b.mersize);
-            if (as != bs) return as < bs;
+            if (as != bs) res = as < bs ? 1 : 2;
         }
+        if (res) return res == 1;
+    }
     return a.first < b.first;
 }
 
 // MheCompare, LM/MatchHashEntry.h:121-143
 template <class E> MH_HD inline bool mhe_less_k(const E &a, const E &b, int k)
 {
+    constexpr int MK = (int)(sizeof(a.s) / sizeof(a.s[0]));
     if (a.first > b.first) return true;
     if (a.first != b.first) return false;
-    for (int i = a.first; i < k; ++i) {
-        if (a.s[i] == 0 && b.s[i] != 0) return true;
-        if (a.s[i] != 0 && b.s[i] == 0) return false;
+    int res = 0;   // the first genome from a.first on present in one only: 1 in b, 2 in a
+    MH_PRAGMA_UNROLL
+    for (int g = 0; g < MK; ++g) {
+        if (g < a.first || g >= k || res) continue;
+        if (a.s[g] == 0 && b.s[g] != 0) res = 1;
+        else if (a.s[g] != 0 && b.s[g] == 0) res = 2;
     }
+    if (res) return res == 1;
     if (contains_k(a, b, k) || contains_k(b, a, k)) return false;
     return strict_less_k(a, b, k);
 }
