@@ -439,6 +439,22 @@ __global__ void grid_jump_kernel(const uint32_t *__restrict__ in, uint32_t *__re
     out[r] = in[in[r]];
 }
 
+// flag[r]: sorted seed r heads its chain
+__global__ void chain_root_flag_kernel(const uint32_t *__restrict__ root, uint64_t ns, uint32_t *__restrict__ flag)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ns) return;
+    flag[r] = root[r] == (uint32_t)r;
+}
+
+// sorted positions of the heads to their seed indices, in place
+__global__ void chain_gather_kernel(const uint32_t *__restrict__ sv, uint64_t nh, uint32_t *__restrict__ heads)
+{
+    const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    heads[h] = sv[heads[h]];
+}
+
 // per seed (original order): is it a chain head, and which seed heads its chain
 __global__ void grid_head_kernel(const uint32_t *__restrict__ sv, const uint32_t *__restrict__ root, uint64_t ns,
                                  uint32_t *__restrict__ is_head, uint32_t *__restrict__ head_of)
@@ -1308,9 +1324,15 @@ static int extend_by_chains(mh_context *ctx, const mh_sml *a, const mh_sml *b, c
     }
     uint32_t *is_head = g[5].as<uint32_t>(), *head_of = g[6].as<uint32_t>();
     hipLaunchKernelGGL(grid_head_kernel, grid, blk, 0, ctx->stream, sv, la, ns, is_head, head_of);
+    // the heads in genome-1 order, so that neighbouring threads walk
+    // neighbouring stretches and share cache lines
     uint32_t *heads = g[7].as<uint32_t>();
+    hipLaunchKernelGGL(chain_root_flag_kernel, grid, blk, 0, ctx->stream, la, ns, lb);
     uint64_t nh = 0;
-    if ((rc = compact_flags(ctx, is_head, ns, heads, g[8].as<uint32_t>(), cnt + 3, &nh)) != MH_OK) return rc;
+    if ((rc = compact_flags(ctx, lb, ns, heads, g[8].as<uint32_t>(), cnt + 3, &nh)) != MH_OK) return rc;
+    if (nh)
+        hipLaunchKernelGGL(chain_gather_kernel, dim3((unsigned)((nh + 255) / 256)), blk, 0, ctx->stream, sv, nh,
+                           heads);
     // head operands reuse the key buffer the sort left spare (8 ns + 64 bytes)
     uint32_t *hoi = reinterpret_cast<uint32_t *>(sk == k0 ? k1 : k0), *hoj = hoi + ns + 16;
     uint32_t *headpos = lb;
@@ -1981,22 +2003,6 @@ __global__ __launch_bounds__(128) void uq_chain_walk_kernel(const uint32_t *__re
     ext_len[s] = (uint32_t)len;
 }
 
-// flag[r]: sorted seed r heads its chain
-__global__ void uq_chain_root_flag_kernel(const uint32_t *__restrict__ root, uint64_t ns, uint32_t *__restrict__ flag)
-{
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ns) return;
-    flag[r] = root[r] == (uint32_t)r;
-}
-
-// sorted positions of the heads to their seed indices, in place
-__global__ void uq_chain_gather_kernel(const uint32_t *__restrict__ sv, uint64_t nh, uint32_t *__restrict__ heads)
-{
-    const uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nh) return;
-    heads[h] = sv[heads[h]];
-}
-
 // every other seed copies its head's extension
 __global__ void uq_chain_fill_kernel(const uint32_t *__restrict__ is_head, const uint32_t *__restrict__ head_of,
                                      uint64_t ns, int k, int32_t *__restrict__ ext_st, uint32_t *__restrict__ ext_len)
@@ -2223,12 +2229,12 @@ static int uq_extend_by_chains(mh_context *ctx, const int32_t *d_sst, uint64_t S
     // the heads in sorted order (first genome, start), so that neighbouring
     // threads walk neighbouring stretches of the genomes and share cache lines
     uint32_t *sflag = lb, *heads = g[7].as<uint32_t>();
-    hipLaunchKernelGGL(uq_chain_root_flag_kernel, grid, blk, 0, ctx->stream, la, S, sflag);
+    hipLaunchKernelGGL(chain_root_flag_kernel, grid, blk, 0, ctx->stream, la, S, sflag);
     uint64_t nh = 0;
     if ((rc = compact_flags(ctx, sflag, S, heads, g[8].as<uint32_t>(), g[10].as<uint32_t>(), &nh)) != MH_OK)
         return rc;
     if (nh) {
-        hipLaunchKernelGGL(uq_chain_gather_kernel, dim3((unsigned)((nh + 255) / 256)), blk, 0, ctx->stream, sv, nh,
+        hipLaunchKernelGGL(chain_gather_kernel, dim3((unsigned)((nh + 255) / 256)), blk, 0, ctx->stream, sv, nh,
                            heads);
         const dim3 hg((unsigned)((nh + 127) / 128)), hb(128);
         if (k <= 4)
