@@ -2030,11 +2030,15 @@ struct UqSeeds {
     int k, sl;
 };
 
-__device__ __forceinline__ void uq_entry(const UqSeeds &q, uint32_t x, bool ext, EntU &e)
+// E: EntU, or EntT<8, int32_t> for seeds of at most eight genomes (fewer
+// slots for the comparisons to run over)
+template <class E> __device__ __forceinline__ void uq_entry(const UqSeeds &q, uint32_t x, bool ext, E &e)
 {
+    constexpr int MK = (int)(sizeof(e.s) / sizeof(e.s[0]));
     const int32_t *s = (ext ? q.est : q.sst) + (uint64_t)x * q.k;
     e.first = -1;
-    for (int g = 0; g < EXT_MAXK; ++g) {
+#pragma unroll
+    for (int g = 0; g < MK; ++g) {
         e.s[g] = g < q.k ? s[g] : 0;
         if (e.first < 0 && e.s[g] != 0) e.first = g;
     }
@@ -2065,8 +2069,9 @@ __global__ void uq_bucket_kernel(const int64_t *__restrict__ off, uint64_t ns, u
 // serves six probes.  The end of the range is always n or the last probe that
 // was not below q, so when the result is not n, *at (first lane) is the entry
 // at the result: that probe's, kept from the LDS tree.
-__device__ uint32_t uq_lower_bound(const UqSeeds &q, const uint32_t *__restrict__ v, uint32_t n, const EntU &qe,
-                                   EntU *tree, EntU *at)
+template <class E>
+__device__ uint32_t uq_lower_bound(const UqSeeds &q, const uint32_t *__restrict__ v, uint32_t n, const E &qe,
+                                   E *tree, E *at)
 {
     const int lane = threadIdx.x;
     uint32_t first = 0, len = n;
@@ -2086,7 +2091,7 @@ __device__ uint32_t uq_lower_bound(const UqSeeds &q, const uint32_t *__restrict_
                 }
             }
             if (l > 0) {
-                EntU mine;
+                E mine;
                 uq_entry(q, v[f + (l >> 1)], true, mine);
                 lt = mhe_less_k(mine, qe, q.k);
                 tree[node] = mine;
@@ -2122,25 +2127,26 @@ __device__ uint32_t uq_lower_bound(const UqSeeds &q, const uint32_t *__restrict_
 // comparisons are the reference's, one at a time, by the first lane); the
 // insertion's shift of the vector's tail runs across the wave, 64 entries a
 // step from the end (every lane reads its entry before any lane writes).
+template <class E>
 __global__ __launch_bounds__(64) void uq_replay_kernel(UqSeeds q, const uint32_t *__restrict__ order,
                                                        const uint32_t *__restrict__ bstart, uint32_t *__restrict__ vec,
                                                        uint32_t *__restrict__ nent,
                                                        unsigned long long *__restrict__ coll)
 {
-    __shared__ EntU tree[64];
+    __shared__ E tree[64];
     const uint32_t b = blockIdx.x;
     const int lane = threadIdx.x;
     const uint32_t y0 = bstart[b], y1 = bstart[b + 1];
     uint32_t *v = vec + y0;
     uint32_t ne = 0;
     unsigned long long c = 0;
-    EntU cand, at_it;
+    E cand, at_it;
     if (y0 < y1) uq_entry(q, order[y0], false, cand);
     for (uint32_t y = y0; y < y1; ++y) {
         const uint32_t x = order[y];
         uint32_t pos = 0xFFFFFFFFu;     // insertion point, or none (a collision)
         // the next seed's entry, loaded alongside this one's probes
-        EntU next = cand;
+        E next = cand;
         if (y + 1 < y1) uq_entry(q, order[y + 1], false, next);
         const uint32_t it = uq_lower_bound(q, v, ne, cand, tree, &at_it);
         bool hit = false;
@@ -2149,7 +2155,7 @@ __global__ __launch_bounds__(64) void uq_replay_kernel(UqSeeds q, const uint32_t
         if (hit) {
             if (lane == 0) ++c;
         } else {
-            EntU e;
+            E e;
             uq_entry(q, x, true, e);
             pos = uq_lower_bound(q, v, ne, e, tree, &at_it);
         }
@@ -2392,7 +2398,12 @@ int unique_seeds(mh_context *ctx, const mh_sml *const *smls, int k, const LiveRa
     qs.sl = sl;
     {
         TimedLaunch tl(ctx, "partition");
-        hipLaunchKernelGGL(uq_replay_kernel, dim3(NB), dim3(64), 0, ctx->stream, qs, order, bst, vec, nent, d_coll);
+        if (k <= 8)
+            hipLaunchKernelGGL((uq_replay_kernel<EntT<8, int32_t>>), dim3(NB), dim3(64), 0, ctx->stream, qs, order, bst, vec,
+                               nent, d_coll);
+        else
+            hipLaunchKernelGGL(uq_replay_kernel<EntU>, dim3(NB), dim3(64), 0, ctx->stream, qs, order, bst, vec, nent,
+                               d_coll);
         MH_HIP(hipGetLastError());
     }
     std::vector<uint32_t> hn(NB);
