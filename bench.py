@@ -65,6 +65,8 @@ def main():
                     help="run only the batched-DP leg and print its record (for profiling its kernels)")
     ap.add_argument("--whole-only", action="store_true",
                     help="run only the whole-path leg (for profiling its kernels on their own)")
+    ap.add_argument("--unique-only", action="store_true",
+                    help="run only the eight-genome UniqueMatchFinder leg and print its record (for profiling)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -78,6 +80,13 @@ def main():
         bandset = synth.dp_jobs(n=args.band_pairs, seed=402, workers=min(16, os.cpu_count() or 1))
         ctx = mauvehip.Context(0)
         print(json.dumps({"learned_band": learned_band_leg(ctx, bandset)}), flush=True)
+        ctx.close()
+        return 0
+    if args.unique_only:
+        import mauvehip
+        ctx = mauvehip.Context(0)
+        print(json.dumps({"multi_genome_unique": unique_leg(ctx, args.steps, args.warmup, not args.no_cpu_baseline)}),
+              flush=True)
         ctx.close()
         return 0
     if args.dp_only:
